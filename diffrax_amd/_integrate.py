@@ -1,0 +1,289 @@
+"""diffeqsolve (mirrors diffrax/_integrate.py:888-1543) on MI355X kernels.
+
+The Python layer does what the reference does at trace time -- argument checking
+(_integrate.py:1036-1235), dtype promotion, output allocation (1272-1305) -- and then
+hands the whole integration loop to one persistent HIP kernel through the C ABI.  The
+batch axis that jax.vmap would add is explicit: y0 of shape [B, d] solves B instances,
+with t1, SaveAt(ts) and vector-field args either shared or given per instance."""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Any, Optional
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._adjoint import AbstractAdjoint, RecursiveCheckpointAdjoint
+from ._saveat import SaveAt, save_y
+from ._solution import RESULTS, Solution
+from ._solver import AbstractERK, AbstractSolver
+from ._step_size_controller import ConstantStepSize, PIDController
+from ._term import ODETerm
+from ._vector_fields import RegisteredVectorField
+
+_SMALL_VFS = (_lib.VF_LV, _lib.VF_VDP, _lib.VF_LINEAR)
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("diffrax_amd runs on a ROCm GPU (MI355X); no GPU is visible and "
+                           "there is no CPU fallback.")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _nan_if_none(x):
+    return float("nan") if x is None else float(x)
+
+
+def _scalar_time(x, name):
+    if isinstance(x, torch.Tensor):
+        if x.numel() != 1:
+            return None
+        return float(x.reshape(()).item())
+    if isinstance(x, np.ndarray):
+        if x.size != 1:
+            return None
+        return float(x.reshape(()))
+    return float(x)
+
+
+class _Workspace:
+    """Per-device scratch buffers owned by the caller side (torch allocator)."""
+
+    def __init__(self):
+        self._buf = {}
+
+    def get(self, nbytes, device):
+        key = (device.index, "ws")
+        buf = self._buf.get(key)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=device)
+            self._buf[key] = buf
+        return buf
+
+
+_WS = _Workspace()
+
+
+class ErkPlan:
+    """A fully prepared batched ERK solve: inputs, outputs and the C structs.
+
+    ``launch()`` enqueues the solve on the current stream (no host sync), so a plan can be
+    replayed or captured into a CUDA/HIP graph by the caller."""
+
+    def __init__(self, problem, tensors, n_out, batched, direction_sign, launch_cfg=None):
+        self.problem = problem
+        self.t = tensors
+        self.n_out = n_out
+        self.batched = batched
+        self.launch_cfg = launch_cfg
+        self.direction_sign = direction_sign
+        io = _lib.ErkIO()
+        for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid"):
+            v = tensors.get(name)
+            setattr(io, name, v.data_ptr() if v is not None else None)
+        self.io = io
+        nbytes = ctypes.c_size_t(0)
+        lib = _lib.load()
+        _lib.check(lib.dfx_erk_workspace_bytes(ctypes.byref(problem), ctypes.byref(nbytes)),
+                   "dfx_erk_workspace_bytes")
+        self.ws_bytes = int(nbytes.value)
+        self.ws = torch.empty(max(self.ws_bytes, 256), dtype=torch.uint8,
+                              device=tensors["y0"].device)
+
+    def launch(self, stream=None):
+        lib = _lib.load()
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        lp = None
+        if self.launch_cfg is not None:
+            lp = ctypes.byref(_lib.Launch(int(self.launch_cfg[0]), int(self.launch_cfg[1])))
+        rc = lib.dfx_erk_solve(ctypes.byref(self.problem), ctypes.byref(self.io),
+                               ctypes.c_void_p(self.ws.data_ptr()), ctypes.c_size_t(self.ws.numel()),
+                               lp, ctypes.c_void_p(stream))
+        _lib.check(rc, "dfx_erk_solve")
+
+
+def _check_ts(ts, t0, t1_vals, directions):
+    """saveat.ts must be increasing and lie in [t0, t1] (_integrate.py:1222-1235).
+    ts: [n] or [B, n]; t1_vals, directions: [1] or [B] (float64 numpy)."""
+    ts = np.asarray(ts, dtype=np.float64)
+    dirs = np.asarray(directions, dtype=np.float64).reshape(-1, 1)
+    tn = np.atleast_2d(ts) * dirs
+    if tn.shape[-1] > 1 and bool((tn[:, 1:] < tn[:, :-1]).any()):
+        raise RuntimeError("saveat.ts must be increasing or decreasing.")
+    t0n = t0 * dirs
+    t1n = np.asarray(t1_vals, dtype=np.float64).reshape(-1, 1) * dirs
+    if bool(((tn > t1n) | (tn < t0n)).any()):
+        raise RuntimeError("saveat.ts must lie between t0 and t1.")
+
+
+def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
+                 max_steps, launch, record_grid=False):
+    device = _require_gpu()
+    if not isinstance(terms, ODETerm) or not isinstance(terms.vector_field, RegisteredVectorField):
+        raise NotImplementedError(
+            "diffrax_amd evaluates registered device vector fields only: wrap one of "
+            "LotkaVolterra, VanDerPol, LinearField, MLPField in ODETerm.")
+    vfo = terms.vector_field
+    if not isinstance(solver, AbstractERK):
+        raise NotImplementedError(f"solver {solver!r} is not an explicit Runge-Kutta method")
+    y0 = torch.as_tensor(y0)
+    if y0.dtype not in (torch.float64, torch.float32):
+        raise ValueError(f"y0 must be float64 or float32, got {y0.dtype}")
+    batched = y0.ndim == 2
+    y0b = y0 if batched else y0.reshape(1, -1)
+    y0b = y0b.to(device).contiguous()
+    B, d = y0b.shape
+    if not vfo.supports(d):
+        raise NotImplementedError(f"{type(vfo).__name__} does not support state dimension {d}")
+    precision = _lib.F64 if y0b.dtype == torch.float64 else _lib.F32S_F64T
+
+    t0f = _scalar_time(t0, "t0")
+    if t0f is None:
+        raise ValueError("t0 must be a scalar")
+    t1f = _scalar_time(t1, "t1")
+    if t1f is not None:
+        t1_t = torch.tensor([t1f], dtype=torch.float64, device=device)
+        t1_stride = 0
+        t1_host = np.array([t1f])
+    else:
+        t1_t = torch.as_tensor(t1, dtype=torch.float64).reshape(-1).to(device).contiguous()
+        if t1_t.numel() != B:
+            raise ValueError(f"t1 must be a scalar or have shape [{B}]")
+        t1_stride = 1
+        t1_host = t1_t.cpu().numpy()
+    directions = np.where(t0f < t1_host, 1.0, -1.0)
+
+    if dt0 is not None:
+        dt0f = _scalar_time(dt0, "dt0")
+        if dt0f is None:
+            raise NotImplementedError("dt0 must be a scalar")
+        if bool(np.any((t1_host - t0f) * dt0f < 0)):
+            raise RuntimeError(
+                f"Must have (t1 - t0) * dt0 >= 0, we instead got t1 with value {t1}, t0 "
+                f"with value {t0}, dt0 with value {dt0}")
+    else:
+        dt0f = None
+
+    if saveat.dense:
+        raise NotImplementedError("SaveAt(dense=True) is not provided by diffrax_amd")
+    if saveat.solver_state or saveat.controller_state or saveat.made_jump:
+        raise NotImplementedError("saving solver/controller state is not provided by diffrax_amd")
+    sub = saveat.subs
+    if sub is None:
+        raise ValueError("SaveAt must save something")
+    if not hasattr(sub, "fn"):
+        raise NotImplementedError("only a single SubSaveAt is supported")
+    if sub.fn is not save_y:
+        raise NotImplementedError("SaveAt(fn=...) is not provided by diffrax_amd")
+    if sub.steps not in (0, 1):
+        raise NotImplementedError("SaveAt(steps=n) with n > 1 is not provided by diffrax_amd")
+    if max_steps is None:
+        raise NotImplementedError("max_steps=None is not supported; pass an integer")
+
+    ts_t = None
+    ts_stride = 0
+    n_ts = 0
+    if sub.ts is not None:
+        ts_t = torch.as_tensor(sub.ts, dtype=torch.float64)
+        if ts_t.ndim == 1:
+            n_ts = ts_t.shape[0]
+        elif ts_t.ndim == 2 and ts_t.shape[0] == B:
+            n_ts = ts_t.shape[1]
+            ts_stride = n_ts
+        else:
+            raise ValueError(f"SaveAt(ts) must be [n] or [{B}, n]")
+        _check_ts(ts_t.cpu().numpy(), t0f, t1_host, directions)
+        ts_t = ts_t.to(device).contiguous()
+
+    params, param_stride = vfo.pack(args, B, d, y0b.dtype, device)
+
+    if isinstance(stepsize_controller, PIDController):
+        c = stepsize_controller
+        controller = _lib.PID
+        pcoeff, icoeff, dcoeff = float(c.pcoeff), float(c.icoeff), float(c.dcoeff)
+        rtol, atol = c.rtol, c.atol
+        extra = dict(dtmin=_nan_if_none(c.dtmin), dtmax=_nan_if_none(c.dtmax),
+                     factormin=c.factormin, factormax=c.factormax, safety=c.safety,
+                     error_order=_nan_if_none(c.error_order), force_dtmin=int(c.force_dtmin))
+    elif isinstance(stepsize_controller, ConstantStepSize):
+        controller = _lib.CONSTANT
+        if dt0f is None:
+            raise ValueError("Constant step size solvers cannot select step size "
+                             "automatically; please pass a value for `dt0`.")
+        pcoeff, icoeff, dcoeff, rtol, atol = 0.0, 1.0, 0.0, 0.0, 0.0
+        extra = dict(dtmin=float("nan"), dtmax=float("nan"), factormin=0.2, factormax=10.0,
+                     safety=0.9, error_order=float("nan"), force_dtmin=1)
+    else:
+        raise NotImplementedError(f"stepsize controller {stepsize_controller!r}")
+
+    prob = _lib.ErkProblem(
+        solver=solver.solver_id, controller=controller, vf=vfo.vf_id, precision=precision,
+        d=d, hidden=0, n_ts=n_ts, save_t0=int(sub.t0), save_t1=int(sub.t1),
+        save_steps=int(sub.steps), max_steps=int(max_steps), batch=B, t1_stride=t1_stride,
+        ts_stride=ts_stride, param_stride=param_stride, t0=t0f,
+        dt0=_nan_if_none(dt0f), rtol=rtol, atol=atol, pcoeff=pcoeff, icoeff=icoeff,
+        dcoeff=dcoeff, **extra)
+    lib = _lib.load()
+    n_out = lib.dfx_erk_n_out(ctypes.byref(prob))
+    tensors = dict(
+        y0=y0b, t1=t1_t, ts=ts_t, params=params,
+        ys=torch.empty((B, n_out, d), dtype=y0b.dtype, device=device),
+        ts_out=torch.empty((B, n_out), dtype=torch.float64, device=device),
+        stats=torch.empty((B, 3), dtype=torch.int32, device=device),
+        result=torch.empty((B,), dtype=torch.int32, device=device),
+        grid=(torch.empty((B, int(max_steps), 2), dtype=torch.float64, device=device)
+              if record_grid else None),
+    )
+    return ErkPlan(prob, tensors, n_out, batched, directions, launch)
+
+
+def _finish(plan, t0, t1, max_steps, throw):
+    T = plan.t
+    stats = {
+        "num_steps": T["stats"][:, 0],
+        "num_accepted_steps": T["stats"][:, 1],
+        "num_rejected_steps": T["stats"][:, 2],
+        "max_steps": max_steps,
+    }
+    ys, ts, result = T["ys"], T["ts_out"], T["result"]
+    if not plan.batched:
+        ys, ts, result = ys[0], ts[0], result[0]
+        stats = {k: (v[0] if isinstance(v, torch.Tensor) else v) for k, v in stats.items()}
+    if throw:
+        codes = T["result"]
+        bad = codes != 0
+        if bool(bad.any()):
+            first = int(codes[bad][0].item())
+            raise RuntimeError(RESULTS[first])
+    res = result
+    sol = Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=res)
+    return sol
+
+
+def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None, *,
+                saveat: SaveAt = SaveAt(t1=True), stepsize_controller=ConstantStepSize(),
+                adjoint: AbstractAdjoint = RecursiveCheckpointAdjoint(), event=None,
+                max_steps: Optional[int] = 4096, throw: bool = True, progress_meter=None,
+                solver_state=None, controller_state=None, made_jump=None,
+                launch: Optional[tuple] = None) -> Solution:
+    """Solve an initial value problem on the GPU (signature of _integrate.py:890-911).
+
+    Extra keyword ``launch=(block, lanes)`` overrides the persistent kernel's launch shape.
+    Batched use: y0 of shape [B, d] solves B independent instances in one launch."""
+    if event is not None:
+        raise NotImplementedError("events are not provided by diffrax_amd")
+    if solver_state is not None or controller_state is not None or made_jump is not None:
+        raise NotImplementedError("resuming from solver/controller state is not provided")
+    if progress_meter is not None and type(progress_meter).__name__ != "NoProgressMeter":
+        raise NotImplementedError("progress meters are not provided by diffrax_amd")
+    plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
+                        max_steps, launch)
+    plan.launch()
+    return _finish(plan, t0, t1, max_steps, throw)
+
+
+_ = math

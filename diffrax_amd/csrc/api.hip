@@ -1,0 +1,180 @@
+// api.hip -- C ABI of libdiffrax_amd (declared in include/diffrax_amd.h).
+//
+// Host-side argument checking and kernel selection.  No device memory is allocated
+// here; every buffer, including scratch, comes from the caller.
+#include <math.h>
+#include <string.h>
+
+#include "../../include/diffrax_amd.h"
+#include "erk_small.hpp"
+
+namespace dfx {
+int erk_small_solve_f64(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
+                        hipStream_t s);
+int erk_small_solve_f32(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
+                        hipStream_t s);
+}  // namespace dfx
+
+using namespace dfx;
+
+namespace {
+
+int solver_order(int solver) { return solver == DFX_HEUN ? 2 : 5; }
+
+bool small_vf_supported(int vf, int d) {
+  if (vf == DFX_VF_LV || vf == DFX_VF_VDP) return d == 2;
+  if (vf == DFX_VF_LINEAR) return d == 1 || d == 2 || d == 3 || d == 4 || d == 10;
+  return false;
+}
+
+// Fill PidParams the way PIDController.adapt_step_size derives its exponents
+// (pid.py:505-510): coeff1 = (i + p + d)/order, coeff2 = -(p + 2d)/order, coeff3 = d/order.
+PidParams make_pid(const dfx_erk_problem* p, double error_order) {
+  PidParams q;
+  memset(&q, 0, sizeof(q));
+  q.rtol = p->rtol;
+  q.atol = p->atol;
+  q.coeff1 = (p->icoeff + p->pcoeff + p->dcoeff) / error_order;
+  q.coeff2 = -(p->pcoeff + 2.0 * p->dcoeff) / error_order;
+  q.coeff3 = p->dcoeff / error_order;
+  q.dtmin = p->dtmin;
+  q.dtmax = p->dtmax;
+  q.has_dtmin = !isnan(p->dtmin);
+  q.has_dtmax = !isnan(p->dtmax);
+  q.force_dtmin = p->force_dtmin;
+  q.factormin = p->factormin;
+  q.factormax = p->factormax;
+  q.safety = p->safety;
+  return q;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dfx_version(void) { return 100; }
+
+const char* dfx_strerror(int code) {
+  switch (code) {
+    case DFX_OK: return "success";
+    case DFX_E_ARG: return "invalid argument or unsupported solver/vector-field/precision combination";
+    case DFX_E_HIP: return "HIP runtime error";
+    case DFX_E_WORKSPACE: return "workspace missing or too small";
+    default: return "unknown error";
+  }
+}
+
+int dfx_erk_n_out(const dfx_erk_problem* p) {
+  if (!p) return DFX_E_ARG;
+  int n = 0;
+  if (p->save_t0) n += 1;
+  n += p->n_ts;
+  if (p->save_steps) n += p->max_steps;
+  if (p->save_t1 && !p->save_steps) n += 1;
+  return n;
+}
+
+int64_t dfx_vf_param_count(int32_t vf, int32_t d, int32_t hidden) {
+  const int64_t D = d, H = hidden;
+  switch (vf) {
+    case DFX_VF_LV: return 4;
+    case DFX_VF_VDP: return 1;
+    case DFX_VF_LINEAR: return D * D;
+    case DFX_VF_MLP: return H * D + H + H * H + H + D * H + D;
+    case DFX_VF_OU: return 3 * D;
+    default: return DFX_E_ARG;
+  }
+}
+
+int dfx_tableau(int32_t solver, double* a, double* b, double* b_err, double* c, int32_t* info) {
+  if (!a || !b || !b_err || !c || !info) return DFX_E_ARG;
+  for (int i = 0; i < 49; ++i) a[i] = 0.0;
+  for (int i = 0; i < 7; ++i) b[i] = b_err[i] = c[i] = 0.0;
+  auto fill = [&](auto tab) {
+    using Tab = decltype(tab);
+    for (int i = 0; i < Tab::S; ++i) {
+      for (int j = 0; j < Tab::S; ++j) a[i * 7 + j] = Tab::a[i][j];
+      b[i] = Tab::b[i];
+      b_err[i] = Tab::berr[i];
+      c[i] = Tab::c[i];
+    }
+    info[0] = Tab::S;
+    info[1] = Tab::fsal;
+    info[2] = Tab::ssal;
+    info[3] = Tab::order;
+  };
+  switch (solver) {
+    case DFX_TSIT5: fill(Tsit5Tab{}); return DFX_OK;
+    case DFX_DOPRI5: fill(Dopri5Tab{}); return DFX_OK;
+    case DFX_HEUN: fill(HeunTab{}); return DFX_OK;
+    default: return DFX_E_ARG;
+  }
+}
+
+int dfx_erk_workspace_bytes(const dfx_erk_problem* p, size_t* bytes) {
+  if (!p || !bytes) return DFX_E_ARG;
+  *bytes = 256;  // work counter of the persistent kernel
+  return DFX_OK;
+}
+
+int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspace,
+                  size_t workspace_bytes, const dfx_launch* launch, void* stream) {
+  if (!p || !io) return DFX_E_ARG;
+  if (p->solver < DFX_TSIT5 || p->solver > DFX_HEUN) return DFX_E_ARG;
+  if (p->controller != DFX_PID && p->controller != DFX_CONSTANT) return DFX_E_ARG;
+  if (p->precision != DFX_F64 && p->precision != DFX_F32S_F64T) return DFX_E_ARG;
+  if (!small_vf_supported(p->vf, p->d)) return DFX_E_ARG;
+  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0) return DFX_E_ARG;
+  if (p->t1_stride < 0 || p->ts_stride < 0 || p->param_stride < 0) return DFX_E_ARG;
+  if (p->controller == DFX_CONSTANT && isnan(p->dt0)) return DFX_E_ARG;
+  if (p->batch == 0) return DFX_OK;
+  if (!io->y0 || !io->t1 || !io->params || !io->stats || !io->result) return DFX_E_ARG;
+  const int n_out = dfx_erk_n_out(p);
+  if (n_out > 0 && (!io->ys || !io->ts_out)) return DFX_E_ARG;
+  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
+  if (!workspace || workspace_bytes < 256) return DFX_E_WORKSPACE;
+
+  const double error_order = isnan(p->error_order) ? (double)solver_order(p->solver) : p->error_order;
+  ErkKArgs a;
+  memset(&a, 0, sizeof(a));
+  a.y0 = io->y0;
+  a.t1 = io->t1;
+  a.ts = io->ts;
+  a.params = io->params;
+  a.ys = io->ys;
+  a.ts_out = io->ts_out;
+  a.stats = io->stats;
+  a.result = io->result;
+  a.grid = io->grid;
+  a.counter = reinterpret_cast<unsigned int*>(workspace);
+  a.B = p->batch;
+  a.t1_stride = p->t1_stride;
+  a.ts_stride = p->ts_stride;
+  a.param_stride = p->param_stride;
+  a.n_ts = p->n_ts;
+  a.n_out = n_out;
+  a.save_t0 = p->save_t0;
+  a.save_t1 = p->save_t1;
+  a.save_steps = p->save_steps;
+  a.max_steps = p->max_steps;
+  a.has_dt0 = !isnan(p->dt0);
+  a.controller = p->controller;
+  a.t0 = p->t0;
+  a.dt0 = p->dt0;
+  a.error_order = error_order;
+  a.pid = make_pid(p, error_order);
+
+  int block = 64;
+  int64_t lanes = p->batch;
+  if (launch) {
+    if (launch->block > 0) block = launch->block;
+    if (launch->lanes > 0) lanes = launch->lanes;
+  }
+  if (block % 64 != 0 || block > 256) return DFX_E_ARG;
+  if (lanes > p->batch) lanes = p->batch;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (p->precision == DFX_F64) return erk_small_solve_f64(p->solver, p->vf, p->d, a, block, lanes, s);
+  return erk_small_solve_f32(p->solver, p->vf, p->d, a, block, lanes, s);
+}
+
+}  // extern "C"

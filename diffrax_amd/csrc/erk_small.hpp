@@ -1,0 +1,386 @@
+// erk_small.hpp -- persistent one-lane-per-trajectory ERK kernel for small systems.
+//
+// Each lane owns one trajectory and runs the whole diffrax integration loop for it
+// (_integrate.py:302-885) in registers: FSAL/SSAL stages, embedded error, PID
+// accept/reject with end clipping, SaveAt(ts) dense-output saves and the final
+// bookkeeping.  A lane whose trajectory finishes writes its results and claims the next
+// trajectory id from a work counter (one atomic per wave), so a launch with fewer lanes
+// than instances keeps every wave busy until the queue drains.  Lanes of a wave stay in
+// lock-step only at the instruction level: accept/reject, save counts and refills are
+// per lane.
+#pragma once
+#include "erk_common.hpp"
+#include "vector_fields.hpp"
+#include "../../include/diffrax_amd.h"
+
+namespace dfx {
+
+struct ErkKArgs {
+  const void* y0;
+  const double* t1;
+  const double* ts;
+  const void* params;
+  void* ys;
+  double* ts_out;
+  int32_t* stats;
+  int32_t* result;
+  double* grid;
+  unsigned int* counter;
+  int64_t B, t1_stride, ts_stride, param_stride;
+  int32_t n_ts, n_out, save_t0, save_t1, save_steps, max_steps, has_dt0, controller;
+  double t0, dt0, error_order;
+  PidParams pid;
+};
+
+// Sum_j coef[j] * k[j][comp] over the nonzero coefficients, in ascending j, starting
+// from the first nonzero product (the order the reference tensordot is restated with).
+template <class T, int S, int N>
+__device__ __forceinline__ T lincomb(const double (&coef)[N], const T (&k)[S], int upto) {
+  T acc = T(0);
+  bool first = true;
+#pragma unroll
+  for (int m = 0; m < S; ++m) {
+    if (m < upto && coef[m] != 0.0) {
+      const T term = T(coef[m]) * k[m];
+      acc = first ? term : acc + term;
+      first = false;
+    }
+  }
+  return acc;
+}
+
+__device__ __forceinline__ unsigned int lane_rank(unsigned long long mask) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned int)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
+}
+
+template <class Tab, class VF, class T>
+struct ErkLane {
+  static constexpr int D = VF::D;
+  static constexpr int P = VF::P;
+  static constexpr int S = Tab::S;
+
+  // ---- per-trajectory state -------------------------------------------------------
+  int64_t traj;
+  int dir;
+  double T1, floor_t1, tprev, tnext;
+  T y[D];
+  T fsal_f[D];
+  T q[P];
+  bool first;
+  PidState pid;
+  int32_t cs_completed, cs_num;
+  double cs_t0, cs_t1;
+  int32_t num_steps, num_acc, num_rej, result;
+  int32_t save_index, ts_index;
+  double next_ts;  // normalised next save time, +inf when none left
+  const double* ts_row;
+
+  __device__ __forceinline__ void load_next_ts(const ErkKArgs& a) {
+    next_ts = (ts_index < a.n_ts) ? ts_row[ts_index] * (double)dir : __builtin_inf();
+  }
+
+  __device__ __forceinline__ void eval_f(double t, const T (&z)[D], T (&f)[D]) const {
+    VF::eval(t, z, q, f);
+  }
+
+  // _select_initial_step (pid.py:33-81) in normalised time.
+  __device__ double initial_step(const ErkKArgs& a, double T0) const {
+    const double rtol = a.pid.rtol, atol = a.pid.atol;
+    double scale[D], e[D];
+    T f0[D], f1[D], y1[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) scale[j] = atol + fabs((double)y[j]) * rtol;
+    eval_f(T0 * dir, y, f0);
+#pragma unroll
+    for (int j = 0; j < D; ++j) e[j] = (double)y[j] / scale[j];
+    const double d0 = rms_norm<D>(e);
+#pragma unroll
+    for (int j = 0; j < D; ++j) e[j] = (double)f0[j] / scale[j];
+    const double d1 = rms_norm<D>(e);
+    const bool cond = (d0 < 1e-5) || (d1 < 1e-5);
+    const double d1_ = cond ? 1.0 : d1;
+    const double h0 = cond ? 1e-6 : 0.01 * (d0 / d1_);
+#pragma unroll
+    for (int j = 0; j < D; ++j) y1[j] = y[j] + T(h0) * f0[j];
+    eval_f((T0 + h0) * dir, y1, f1);
+#pragma unroll
+    for (int j = 0; j < D; ++j) e[j] = ((double)f1[j] - (double)f0[j]) / scale[j];
+    const double d2 = rms_norm<D>(e) / h0;
+    const double max_d = jmax(d1, d2);
+    const double h1 = (max_d <= 1e-15) ? jmax(1e-6, h0 * 1e-3)
+                                       : pow_ctl(0.01 / max_d, 1.0 / a.error_order);
+    return jmin(100.0 * h0, h1);
+  }
+
+  // diffeqsolve set-up for one instance (_integrate.py:1156-1305, pid.py:316-392,
+  // constant.py:30-54).
+  __device__ void start(const ErkKArgs& a, int64_t id) {
+    traj = id;
+    const T* y0p = reinterpret_cast<const T*>(a.y0) + id * D;
+#pragma unroll
+    for (int j = 0; j < D; ++j) { y[j] = y0p[j]; fsal_f[j] = T(0); }
+    const T* qp = reinterpret_cast<const T*>(a.params) + id * a.param_stride;
+#pragma unroll
+    for (int j = 0; j < P; ++j) q[j] = qp[j];
+    const double t1r = a.t1[id * a.t1_stride];
+    dir = (a.t0 < t1r) ? 1 : -1;
+    const double T0 = a.t0 * dir;
+    T1 = t1r * dir;
+    first = true;
+    pid.inv = 1.0; pid.pinv = 1.0; pid.at_dtmin = 0;
+    if (a.controller == DFX_PID) {
+      double dt0 = a.has_dt0 ? a.dt0 * dir : initial_step(a, T0);
+      if (a.pid.has_dtmax) dt0 = jmin(dt0, a.pid.dtmax);
+      if (a.pid.has_dtmin) dt0 = jmax(dt0, a.pid.dtmin);
+      tnext = T0 + dt0;
+      cs_completed = 0; cs_num = 0; cs_t0 = T0; cs_t1 = T1;
+    } else {
+      const double DT0 = a.dt0 * dir;
+      const double qn = ceil((T1 - T0) / nextup(DT0));
+      cs_num = __builtin_isfinite(T1) ? (int32_t)qn : -1;
+      cs_t1 = __builtin_isfinite(T1) ? T1 : DT0;
+      cs_completed = 1;
+      cs_t0 = T0;
+      tnext = T0 + DT0;
+    }
+    tnext = jmin(tnext, T1);
+    double fl = T1;
+    for (int r = 0; r < 100; ++r) fl = prevbefore(fl);
+    floor_t1 = fl;
+    tprev = T0;
+    num_steps = num_acc = num_rej = 0;
+    result = DFX_SUCCESSFUL;
+    save_index = 0;
+    ts_index = 0;
+    ts_row = a.ts ? a.ts + id * a.ts_stride : nullptr;
+    load_next_ts(a);
+    if (a.save_t0) {
+      write_save(a, T0, y);
+    }
+  }
+
+  __device__ __forceinline__ void write_save(const ErkKArgs& a, double t_norm, const T (&v)[D]) {
+    T* row = reinterpret_cast<T*>(a.ys) + (traj * a.n_out + save_index) * D;
+#pragma unroll
+    for (int j = 0; j < D; ++j) row[j] = v[j];
+    a.ts_out[traj * a.n_out + save_index] = t_norm * (double)dir;
+    ++save_index;
+  }
+
+  __device__ __forceinline__ bool running(const ErkKArgs& a) const {
+    return tprev < T1 && result == DFX_SUCCESSFUL && num_steps < a.max_steps;
+  }
+
+  // One pass of body_fun_aux (_integrate.py:365-662).
+  __device__ void step(const ErkKArgs& a) {
+    const double ts0 = tprev, ts1 = tnext;
+    const double dt = ts1 - ts0;
+    const T control = T((double)dir * dt);
+    T k[S][D];
+    T z[D], f[D];
+    if constexpr (Tab::fsal) {
+      if (first) {
+        // First step: stage 0 is evaluated (z0 = y0 + 0, t = t0 + 0*dt).
+#pragma unroll
+        for (int j = 0; j < D; ++j) z[j] = y[j] + T(0);
+        eval_f((ts0 + 0.0 * dt) * dir, z, fsal_f);
+      }
+#pragma unroll
+      for (int j = 0; j < D; ++j) k[0][j] = control * fsal_f[j];
+    }
+    constexpr int start_stage = Tab::fsal ? 1 : 0;
+#pragma unroll
+    for (int i = start_stage; i < S; ++i) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        T kc[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : T(0);
+        z[j] = y[j] + lincomb<T, S, S>(Tab::a[i], kc, i);
+      }
+      const double ti = (Tab::c[i] == 1.0) ? ts1 : ts0 + Tab::c[i] * dt;
+      eval_f(ti * dir, z, f);
+#pragma unroll
+      for (int j = 0; j < D; ++j) k[i][j] = control * f[j];
+    }
+    T y1[D], yerr[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      T kc[S];
+#pragma unroll
+      for (int m = 0; m < S; ++m) kc[m] = k[m][j];
+      if constexpr (Tab::ssal) {
+        y1[j] = z[j];
+      } else {
+        y1[j] = y[j] + lincomb<T, S, S>(Tab::b, kc, S);
+      }
+      yerr[j] = lincomb<T, S, S>(Tab::berr, kc, S);
+      if (yerr[j] != yerr[j]) yerr[j] = T(__builtin_inf());
+    }
+
+    // ---- step-size control ----
+    bool keep;
+    double ntprev, ntnext;
+    int code = 0;
+    if (a.controller == DFX_PID) {
+      bool anynan = false;
+#pragma unroll
+      for (int j = 0; j < D; ++j) anynan = anynan || (y1[j] != y1[j]);
+      double e[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const double y1c = anynan ? (double)y[j] : (double)y1[j];
+        const double yy = jmax(fabs((double)y[j]), fabs(y1c));
+        e[j] = (double)yerr[j] / (a.pid.atol + yy * a.pid.rtol);
+      }
+      const double scaled = rms_norm<D>(e);
+      keep = pid_adapt(a.pid, scaled, ts0, ts1, pid, ntprev, ntnext, code);
+    } else {
+      keep = true;
+      const int32_t sc = cs_completed + 1;
+      double tn = cs_t0 + (cs_t1 - cs_t0) * ((double)sc / (double)cs_num);
+      if (sc == cs_num) tn = cs_t1;
+      if (!(cs_num >= 0)) tn = ts1 + cs_t1;
+      cs_completed = sc;
+      ntprev = ts1;
+      ntnext = tn;
+    }
+    ntprev = jmin(ntprev, T1);
+    if (ntnext > floor_t1) ntnext = keep ? T1 : ntprev + 0.5 * (T1 - ntprev);
+
+    ++num_steps;
+    if (keep) ++num_acc; else ++num_rej;
+    if (result == DFX_SUCCESSFUL) result = code;
+
+    if (keep) {
+      if (a.grid) {
+        double* g = a.grid + (traj * a.max_steps + (num_acc - 1)) * 2;
+        g[0] = ts0;
+        g[1] = ts1;
+      }
+      // SaveAt(ts): interpolate inside [ts0, ts1] (_integrate.py:456-491).
+      while (next_ts <= ts1) {
+        T v[D];
+        interp(ts0, ts1, y, y1, k, next_ts, v);
+        write_save(a, next_ts, v);
+        ++ts_index;
+        load_next_ts(a);
+      }
+#pragma unroll
+      for (int j = 0; j < D; ++j) y[j] = y1[j];
+      if constexpr (Tab::fsal) {
+        // The last stage's f becomes the next step's first stage (FSAL).
+        first = false;
+#pragma unroll
+        for (int j = 0; j < D; ++j) fsal_f[j] = f[j];
+      }
+      if (a.save_steps) write_save(a, ntprev, y);
+    }
+    tprev = ntprev;
+    tnext = ntnext;
+  }
+
+  __device__ __forceinline__ void interp(double ts0, double ts1, const T (&y0)[D],
+                                         const T (&y1)[D], const T (&k)[S][D], double t,
+                                         T (&out)[D]) const {
+    const double th = linear_rescale(ts0, t, ts1);
+    if constexpr (Tab::id == SOLVER_TSIT5) {
+      // Tsit5
+      double bb[7];
+      tsit5_weights(th, bb);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        T acc = T(bb[0]) * k[0][j];
+#pragma unroll
+        for (int m = 1; m < 7; ++m) acc = acc + T(bb[m]) * k[m][j];
+        out[j] = y0[j] + acc;
+      }
+    } else if constexpr (Tab::id == SOLVER_DOPRI5) {
+      // Dopri5: quartic through y0, ymid, y1 with end slopes k0, k6.
+      const T tt = T(th);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        T kc[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) kc[m] = k[m][j];
+        const T ymid = y0[j] + lincomb<T, S, S>(Dopri5Tab::cmid, kc, S);
+        const T f0 = k[0][j], f1 = k[6][j];
+        const T ca = T(2) * (f1 - f0) - T(8) * (y1[j] + y0[j]) + T(16) * ymid;
+        const T cb = T(5) * f0 - T(3) * f1 + T(18) * y0[j] + T(14) * y1[j] - T(32) * ymid;
+        const T cc = f1 - T(4) * f0 - T(11) * y0[j] - T(5) * y1[j] + T(16) * ymid;
+        out[j] = (((ca * tt + cb) * tt + cc) * tt + f0) * tt + y0[j];
+      }
+    } else {
+      // Heun: cubic Hermite from k0, k_last.
+      const T tt = T(th);
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const T k0 = k[0][j], k1 = k[S - 1][j];
+        const T ca = k0 + k1 + T(2) * y0[j] - T(2) * y1[j];
+        const T cb = T(-2) * k0 - k1 - T(3) * y0[j] + T(3) * y1[j];
+        out[j] = ((ca * tt + cb) * tt + k0) * tt + y0[j];
+      }
+    }
+  }
+
+  // Loop exit bookkeeping (_integrate.py:823-883) and output of counters.
+  __device__ void finish(const ErkKArgs& a) {
+    const double T0 = a.t0 * dir;
+    if (T0 == T1) {
+      for (int r = 0; r < a.n_ts; ++r) write_save(a, T0, y);
+    }
+    if (a.save_t1 && !a.save_steps) write_save(a, tprev, y);
+    if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
+    // Unfilled rows keep the reference's preset values (inf / direction*inf).
+    T* rows = reinterpret_cast<T*>(a.ys) + traj * a.n_out * D;
+    for (int s = save_index; s < a.n_out; ++s) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) rows[s * D + j] = T(__builtin_inf());
+      a.ts_out[traj * a.n_out + s] = (double)dir * __builtin_inf();
+    }
+    a.stats[traj * 3 + 0] = num_steps;
+    a.stats[traj * 3 + 1] = num_acc;
+    a.stats[traj * 3 + 2] = num_rej;
+    a.result[traj] = result;
+  }
+};
+
+template <class Tab, class VF, class T>
+__global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
+  ErkLane<Tab, VF, T> L;
+  bool active = false;
+  bool drained = false;  // wave-uniform: the work queue is exhausted
+  while (true) {
+    const bool need = !active && !drained;
+    const unsigned long long need_mask = __ballot(need);
+    if (need_mask) {
+      const int n = __popcll(need_mask);
+      const int leader = __ffsll((long long)need_mask) - 1;
+      unsigned int base = 0;
+      if ((int)__lane_id() == leader) base = atomicAdd(a.counter, (unsigned int)n);
+      base = __shfl(base, leader);
+      if ((int64_t)base + n >= a.B) drained = true;
+      if (need) {
+        const int64_t id = (int64_t)base + lane_rank(need_mask);
+        if (id < a.B) {
+          L.start(a, id);
+          active = true;
+        }
+      }
+    }
+    if (!__any(active)) break;
+    if (active) {
+      if (L.running(a)) {
+        L.step(a);
+      } else {
+        L.finish(a);
+        active = false;
+      }
+    }
+  }
+}
+
+static __global__ void zero_counter_kernel(unsigned int* counter) { *counter = 0u; }
+
+}  // namespace dfx

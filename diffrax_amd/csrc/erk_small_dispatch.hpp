@@ -1,0 +1,51 @@
+// erk_small_dispatch.hpp -- instantiates the small-system kernel for one state type and
+// maps (solver, vector field, d) to a kernel.  Included once per precision build.
+#pragma once
+#include "erk_small.hpp"
+
+namespace dfx {
+
+template <class Tab, class VF, class T>
+static int launch_small(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
+  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
+  const int64_t nblocks = (lanes + block - 1) / block;
+  hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T>), dim3((unsigned)nblocks), dim3(block), 0, s, a);
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+template <class Tab, class T>
+static int dispatch_vf(int vf, int d, const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
+  switch (vf) {
+    case VF_LV:
+      if (d == 2) return launch_small<Tab, LotkaVolterraVF<T>, T>(a, block, lanes, s);
+      break;
+    case VF_VDP:
+      if (d == 2) return launch_small<Tab, VanDerPolVF<T>, T>(a, block, lanes, s);
+      break;
+    case VF_LINEAR:
+      switch (d) {
+        case 1: return launch_small<Tab, LinearVF<T, 1>, T>(a, block, lanes, s);
+        case 2: return launch_small<Tab, LinearVF<T, 2>, T>(a, block, lanes, s);
+        case 3: return launch_small<Tab, LinearVF<T, 3>, T>(a, block, lanes, s);
+        case 4: return launch_small<Tab, LinearVF<T, 4>, T>(a, block, lanes, s);
+        case 10: return launch_small<Tab, LinearVF<T, 10>, T>(a, block, lanes, s);
+        default: break;
+      }
+      break;
+    default: break;
+  }
+  return DFX_E_ARG;
+}
+
+template <class T>
+static int dispatch_small(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
+                          hipStream_t s) {
+  switch (solver) {
+    case SOLVER_TSIT5: return dispatch_vf<Tsit5Tab, T>(vf, d, a, block, lanes, s);
+    case SOLVER_DOPRI5: return dispatch_vf<Dopri5Tab, T>(vf, d, a, block, lanes, s);
+    case SOLVER_HEUN: return dispatch_vf<HeunTab, T>(vf, d, a, block, lanes, s);
+    default: return DFX_E_ARG;
+  }
+}
+
+}  // namespace dfx

@@ -1,0 +1,11 @@
+// erk_small_f32.hip -- float32-state / float64-time build of the small-system kernel.
+// Stage algebra runs in float32 (contraction into FMA allowed); time, step size, the
+// error norm and the controller stay in float64.
+#include "erk_small_dispatch.hpp"
+
+namespace dfx {
+int erk_small_solve_f32(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
+                        hipStream_t s) {
+  return dispatch_small<float>(solver, vf, d, a, block, lanes, s);
+}
+}  // namespace dfx

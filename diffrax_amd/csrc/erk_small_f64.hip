@@ -1,0 +1,11 @@
+// erk_small_f64.hip -- float64 parity build of the small-system kernel.  Compiled with
+// -ffp-contract=off so every a*b+c is the rounded product then the rounded sum, as the
+// reference formulas are written.
+#include "erk_small_dispatch.hpp"
+
+namespace dfx {
+int erk_small_solve_f64(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
+                        hipStream_t s) {
+  return dispatch_small<double>(solver, vf, d, a, block, lanes, s);
+}
+}  // namespace dfx

@@ -1,0 +1,121 @@
+/*
+ * diffrax_amd.h -- C ABI of the MI355X explicit Runge-Kutta integrator.
+ *
+ * The library replaces the body of one diffrax call,
+ *     diffeqsolve(terms, solver, t0, t1, dt0, y0, args, saveat=SaveAt(...),
+ *                 stepsize_controller=PIDController(...)|ConstantStepSize(),
+ *                 adjoint=RecursiveCheckpointAdjoint())
+ * for a batch of independent instances (what jax.vmap over diffeqsolve computes),
+ * with the vector field chosen from a registered set of device functions.
+ *
+ * Conventions.
+ *  - Plain C: pointers are device pointers (hipMalloc / torch CUDA tensors), sizes are
+ *    element counts, `stream` is a hipStream_t passed as void*.
+ *  - The library allocates no device memory.  Scratch memory the kernels need is a
+ *    caller-provided workspace sized by the matching *_workspace_bytes call.
+ *  - Every call is stream ordered, makes no host synchronisation and may be captured
+ *    into a hipGraph.
+ *  - Return value: 0 on success, a negative DFX_E* code otherwise (dfx_strerror).
+ *    Per-instance numerical outcomes go to int32 result[B] with the codes below,
+ *    whose names follow diffrax RESULTS (_solution.py:13-31; successful == 0).
+ */
+#ifndef DIFFRAX_AMD_H
+#define DIFFRAX_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums ---------------------------------------------------------------------- */
+enum { DFX_TSIT5 = 0, DFX_DOPRI5 = 1, DFX_HEUN = 2 };          /* solver              */
+enum { DFX_PID = 0, DFX_CONSTANT = 1 };                         /* stepsize controller */
+enum { DFX_VF_LV = 0, DFX_VF_VDP = 1, DFX_VF_LINEAR = 2,       /* vector field        */
+       DFX_VF_MLP = 3, DFX_VF_OU = 4 };
+enum { DFX_F64 = 0,          /* state, stages, time and controller in float64            */
+       DFX_F32S_F64T = 1,    /* float32 state and stages; float64 time and controller    */
+       DFX_F32 = 2,          /* MLP: float32 matmuls (MFMA f32)                          */
+       DFX_BF16_F32 = 3 };   /* MLP: bf16 MFMA matmuls, f32 accumulate, f32 RK algebra   */
+enum { DFX_SUCCESSFUL = 0, DFX_MAX_STEPS_REACHED = 1, DFX_DT_MIN_REACHED = 2 };
+
+#define DFX_OK 0
+#define DFX_E_ARG (-1)        /* invalid argument / unsupported combination */
+#define DFX_E_HIP (-2)        /* a HIP runtime call failed                  */
+#define DFX_E_WORKSPACE (-3)  /* workspace missing or too small             */
+
+/* ---- problem description (host memory) ------------------------------------------- */
+/* Fields mirror diffeqsolve's arguments (_integrate.py:890-911), SaveAt
+ * (_saveat.py:65-105) and PIDController (_step_size_controller/pid.py:299-311).
+ * NaN in dt0 / dtmin / dtmax / error_order means None. */
+typedef struct dfx_erk_problem {
+  int32_t solver, controller, vf, precision;
+  int32_t d;            /* state dimension                                          */
+  int32_t hidden;       /* MLP width (vf == DFX_VF_MLP), else 0                       */
+  int32_t n_ts;         /* SaveAt(ts=...) length                                      */
+  int32_t save_t0, save_t1, save_steps;
+  int32_t max_steps, force_dtmin;
+  int64_t batch;        /* B instances                                                */
+  int64_t t1_stride;    /* 0: one t1 for all; 1: t1[B]                                */
+  int64_t ts_stride;    /* 0: ts[n_ts] shared; n_ts: ts[B, n_ts]                       */
+  int64_t param_stride; /* 0: shared parameters; P: params[B, P]                        */
+  double t0, dt0;
+  double rtol, atol, pcoeff, icoeff, dcoeff, dtmin, dtmax;
+  double factormin, factormax, safety, error_order;
+} dfx_erk_problem;
+
+/* ---- device buffers of one solve --------------------------------------------------- */
+typedef struct dfx_erk_io {
+  const void* y0;       /* [B, d]  state dtype (double for F64, float otherwise)       */
+  const double* t1;     /* [B * t1_stride] (at least one element)                      */
+  const double* ts;     /* [B * ts_stride + n_ts] or NULL when n_ts == 0               */
+  const void* params;   /* vector-field parameters, state dtype (MLP: see below)      */
+  void* ys;             /* out [B, n_out, d] state dtype; unfilled entries = inf       */
+  double* ts_out;       /* out [B, n_out]; unfilled entries = direction * inf         */
+  int32_t* stats;       /* out [B, 3] = num_steps, num_accepted_steps, num_rejected   */
+  int32_t* result;      /* out [B] DFX_SUCCESSFUL / DFX_MAX_STEPS_REACHED / ...      */
+  double* grid;         /* optional out [B, max_steps, 2]: accepted (tprev, tnext) in
+                           direction-normalised time; NULL to skip                  */
+} dfx_erk_io;
+
+/* Launch shape override (NULL = library default / tuned). */
+typedef struct dfx_launch {
+  int32_t block;        /* threads per workgroup (multiple of 64)                     */
+  int32_t lanes;        /* total persistent lanes (trajectory slots); 0 = batch      */
+} dfx_launch;
+
+/* Library version: major*10000 + minor*100 + patch. */
+int dfx_version(void);
+const char* dfx_strerror(int code);
+/* Number of saved rows per instance: t0 + n_ts + (steps ? max_steps : 0) + t1
+ * (the allocation rule of _integrate.py:1272-1305 for one SubSaveAt). */
+int dfx_erk_n_out(const dfx_erk_problem* p);
+/* Number of vector-field parameters per instance (LV 4, VdP 1, linear d*d,
+ * MLP h*d + h + h*h + h + d*h + d, OU 3d). */
+int64_t dfx_vf_param_count(int32_t vf, int32_t d, int32_t hidden);
+/* Copy a solver's tableau: a[7*7] row-major (zero padded), b[7], b_err[7], c[7],
+ * info[4] = (stages, fsal, ssal, order).  Host memory.
+ * Replaces: ButcherTableau (_solver/runge_kutta.py:48-166) of Tsit5/Dopri5/Heun. */
+int dfx_tableau(int32_t solver, double* a, double* b, double* b_err, double* c,
+                int32_t* info);
+
+/* Workspace needed by dfx_erk_solve (bytes, 256-aligned). */
+int dfx_erk_workspace_bytes(const dfx_erk_problem* p, size_t* bytes);
+
+/* Batched adaptive ODE solve of small systems (LV, VdP, linear; d <= 16).
+ * One persistent kernel: each lane owns one trajectory and steps, rejects, saves and
+ * refills independently.
+ * Replaces: the whole integration loop that AbstractAdjoint.loop runs
+ * (_adjoint.py:118-144 -> _integrate.py:302-885), i.e. AbstractRungeKutta.step
+ * (_solver/runge_kutta.py:446-1203), PIDController.adapt_step_size
+ * (_step_size_controller/pid.py:394-567) / ConstantStepSize.adapt_step_size
+ * (_step_size_controller/constant.py:56-104), the interpolation behind SaveAt(ts)
+ * (_solver/tsit5.py:103-156, _local_interpolation.py:50-139) and, when dt0 is NaN,
+ * _select_initial_step (_step_size_controller/pid.py:33-81). */
+int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspace,
+                  size_t workspace_bytes, const dfx_launch* launch, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DIFFRAX_AMD_H */

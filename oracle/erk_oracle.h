@@ -1,0 +1,41 @@
+/*
+ * erk_oracle.h -- interface of the FP64 CPU oracle.  TEST INFRASTRUCTURE ONLY:
+ * imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ */
+#ifndef ERK_ORACLE_H
+#define ERK_ORACLE_H
+#include <stdint.h>
+
+#define ORC_MAXD 64
+
+enum { ORC_TSIT5 = 0, ORC_DOPRI5 = 1, ORC_HEUN = 2 };
+enum { ORC_PID = 0, ORC_CONSTANT = 1 };
+enum { ORC_VF_LV = 0, ORC_VF_VDP = 1, ORC_VF_LINEAR = 2, ORC_VF_MLP = 3, ORC_VF_OU = 4 };
+/* Result codes: names follow diffrax RESULTS (_solution.py:13-31); successful == 0. */
+enum { ORC_SUCCESSFUL = 0, ORC_MAX_STEPS_REACHED = 1, ORC_DT_MIN_REACHED = 2 };
+
+typedef struct {
+    int32_t solver, controller, vf, d, hidden, n_ts;
+    int32_t save_t0, save_t1, save_steps, max_steps, force_dtmin, pad_;
+    double t0, dt0;                       /* dt0 = NaN means None */
+    double rtol, atol, pcoeff, icoeff, dcoeff;
+    double dtmin, dtmax;                  /* NaN means None */
+    double factormin, factormax, safety;
+    double error_order;                   /* NaN means the solver's own */
+} orc_problem;
+
+int orc_n_out(const orc_problem *p);
+long orc_param_count(const orc_problem *p);
+int orc_tableau(int solver, double *a49, double *b7, double *berr7, double *c7, int *info4);
+/* Batched solve.  y0 [B,d]; t1 [B*t1_stride]; ts [B*ts_stride + n_ts]; params
+ * [B*param_stride + count]; dW [steps, B, d] or NULL; ys [B, n_out, d];
+ * ts_out [B, n_out]; stats [B,3] = (num_steps, accepted, rejected); result [B];
+ * grid [B, max_steps, 2] accepted (tprev, tnext) in normalised time, or NULL. */
+int orc_solve(const orc_problem *p, long B, const double *y0, const double *t1,
+              long t1_stride, const double *ts, long ts_stride, const double *params,
+              long param_stride, const double *dW, double *ys, double *ts_out,
+              int32_t *stats, int32_t *result, double *grid, int nthreads);
+double orc_pow(double x, double c);
+int orc_single_step(const orc_problem *p, const double *params, double ts0, double ts1,
+                    const double *y0, double *y1, double *yerr, double *k);
+#endif
