@@ -18,6 +18,8 @@ from ._step_size_controller import (AbstractAdaptiveStepSizeController,
                                     PIDController, rms_norm)
 from ._term import (AbstractPath, AbstractTerm, ControlTerm, IncrementPath, MultiTerm,
                     ODETerm)
-from ._vector_fields import LinearField, LotkaVolterra, RegisteredVectorField, VanDerPol
+from ._vector_fields import (DiagonalNoise, LinearField, LotkaVolterra, OUDrift,
+                             RegisteredVectorField, VanDerPol)
+from . import _sde  # noqa: F401  (registers the SDE C-ABI signatures)
 
 __version__ = "0.1.0"

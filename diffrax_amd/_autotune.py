@@ -1,0 +1,80 @@
+"""Launch-shape autotuner for the persistent ERK kernels.
+
+For a prepared plan it times a few candidate launch shapes (workgroup size, number of
+persistent trajectory slots) with HIP events and keeps the fastest.  Results are cached
+in memory and, if DIFFRAX_AMD_TUNE_CACHE names a file, as JSON keyed by
+(kernel signature, batch, device name)."""
+from __future__ import annotations
+
+import json
+import os
+
+import torch
+
+_CACHE: dict = {}
+
+
+def _key(plan):
+    p = plan.problem
+    name = torch.cuda.get_device_name(torch.cuda.current_device())
+    return f"erk:{p.solver}:{p.vf}:{p.precision}:{p.d}:{p.controller}:{p.batch}:{name}"
+
+
+def _load_file():
+    path = os.environ.get("DIFFRAX_AMD_TUNE_CACHE")
+    if path and os.path.exists(path):
+        with open(path) as f:
+            _CACHE.update(json.load(f))
+
+
+def _save_file():
+    path = os.environ.get("DIFFRAX_AMD_TUNE_CACHE")
+    if path:
+        with open(path, "w") as f:
+            json.dump(_CACHE, f, indent=1)
+
+
+def candidates(batch):
+    out = []
+    for block in (64, 128, 256):
+        for frac in (1, 2):
+            lanes = max(block, (batch // frac) // block * block)
+            if frac > 1 and lanes >= batch:
+                continue
+            out.append((block, lanes))
+    return out
+
+
+def time_plan(plan, launch_cfg, reps=3):
+    plan.launch_cfg = launch_cfg
+    plan.launch()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        plan.launch()
+    end.record()
+    end.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+def tune(plan, verbose=False):
+    """Pick the fastest launch shape for this plan; returns (block, lanes) and timings."""
+    if not _CACHE:
+        _load_file()
+    key = _key(plan)
+    if key in _CACHE:
+        plan.launch_cfg = tuple(_CACHE[key]["best"])
+        return plan.launch_cfg, _CACHE[key]["ms"]
+    timings = {}
+    for cfg in candidates(int(plan.problem.batch)):
+        timings[f"{cfg[0]}x{cfg[1]}"] = time_plan(plan, cfg)
+        if verbose:
+            print(f"  tune {cfg}: {timings[f'{cfg[0]}x{cfg[1]}']:.3f} ms", flush=True)
+    best = min(timings, key=timings.get)
+    b, l = (int(x) for x in best.split("x"))
+    plan.launch_cfg = (b, l)
+    _CACHE[key] = {"best": [b, l], "ms": timings}
+    _save_file()
+    return plan.launch_cfg, timings

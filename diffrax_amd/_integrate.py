@@ -20,8 +20,8 @@ from ._saveat import SaveAt, save_y
 from ._solution import RESULTS, Solution
 from ._solver import AbstractERK, AbstractSolver
 from ._step_size_controller import ConstantStepSize, PIDController
-from ._term import ODETerm
-from ._vector_fields import RegisteredVectorField
+from ._term import ControlTerm, IncrementPath, MultiTerm, ODETerm
+from ._vector_fields import DiagonalNoise, OUDrift, RegisteredVectorField
 
 _SMALL_VFS = (_lib.VF_LV, _lib.VF_VDP, _lib.VF_LINEAR)
 
@@ -280,10 +280,33 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
         raise NotImplementedError("resuming from solver/controller state is not provided")
     if progress_meter is not None and type(progress_meter).__name__ != "NoProgressMeter":
         raise NotImplementedError("progress meters are not provided by diffrax_amd")
+    if _is_ou_sde(terms):
+        from ._sde import solve_sde
+        device = _require_gpu()
+        ts, ys, n = solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat,
+                              stepsize_controller, adjoint, device)
+        if n > max_steps:
+            if throw:
+                raise RuntimeError(RESULTS[RESULTS.max_steps_reached])
+        shape = ys.shape[:-2]
+        full = lambda v: torch.full(shape, v, dtype=torch.int32, device=ys.device)
+        stats = {"num_steps": full(n), "num_accepted_steps": full(n),
+                 "num_rejected_steps": full(0), "max_steps": max_steps}
+        return Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats,
+                        result=full(0))
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                         max_steps, launch)
     plan.launch()
     return _finish(plan, t0, t1, max_steps, throw)
+
+
+def _is_ou_sde(terms):
+    if not isinstance(terms, MultiTerm) or len(terms.terms) != 2:
+        return False
+    a, b = terms.terms
+    return (isinstance(a, ODETerm) and isinstance(a.vector_field, OUDrift)
+            and isinstance(b, ControlTerm) and isinstance(b.vector_field, DiagonalNoise)
+            and isinstance(b.control, IncrementPath))
 
 
 _ = math

@@ -114,3 +114,31 @@ class LinearField(RegisteredVectorField):
 
 
 _ = math
+
+
+class OUDrift(RegisteredVectorField):
+    """Ornstein-Uhlenbeck drift theta * (mu - y); args = (theta, mu, sigma), each [d]."""
+
+    vf_id = _lib.VF_OU
+    name = "ou_drift"
+
+    def supports(self, d):
+        return True
+
+    def __call__(self, t, y, args):
+        theta, mu, _ = args
+        return theta * (mu - y)
+
+
+class DiagonalNoise(RegisteredVectorField):
+    """Diagonal diffusion g(t, y) = diag(sigma); args = (theta, mu, sigma).  Used as the
+    vector field of a ControlTerm, whose product with dW is sigma * dW."""
+
+    vf_id = _lib.VF_OU
+    name = "ou_diagonal_noise"
+
+    def supports(self, d):
+        return True
+
+    def __call__(self, t, y, args):
+        return args[2] * torch.ones_like(y)

@@ -177,17 +177,20 @@ __host__ __device__ inline double cr_pow_fifth(double x) {
   const int j = E - 5 * k;  // 0..4
   const double f = bits_to_double((bx & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);  // [1,2)
   const double m = f * (double)(1 << j);  // x = m * 2^(5k), m in [1, 32)
-  // initial guess: 2^(j/5) * (quadratic fit of f^(1/5) on [1,2))
-  const double tab[5] = {1.0, 1.148698354997035, 1.3195079107728942, 1.515716566510398,
-                         1.7411011265922482};
-  const double u = f - 1.0;
-  double r = tab[j] * (1.0 + u * (0.19425 - 0.04555 * u));
-  for (int it = 0; it < 3; ++it) {
-    const double r2 = r * r;
-    const double r4 = r2 * r2;
-    r = (4.0 * r + m / r4) * 0.2;
+  // w ~ m^(-1/5): float guess, then two division-free Newton steps w += w (1 - m w^5) / 5
+#if defined(__HIP_DEVICE_COMPILE__)
+  double w = (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)m));
+#else
+  double w = (double)exp2f(-0.2f * log2f((float)m));
+#endif
+  for (int it = 0; it < 2; ++it) {
+    const double w2 = w * w;
+    const double w5 = w2 * w2 * w;
+    w = w + w * ((1.0 - m * w5) * 0.2);
   }
-  // double-double residual m - r^5 and Newton correction
+  const double w4 = (w * w) * (w * w);
+  const double r = m * w4;  // m^(1/5) to ~1 ulp
+  // double-double residual m - r^5, Newton correction delta = res / (5 r^4) ~ res * w^4 / 5
   const double h2 = r * r;
   const double l2 = fma(r, r, -h2);
   const double h4 = h2 * h2;
@@ -195,12 +198,15 @@ __host__ __device__ inline double cr_pow_fifth(double x) {
   const double h5 = h4 * r;
   const double l5 = fma(h4, r, -h5) + l4 * r;
   const double res = (m - h5) - l5;
-  const double delta = res / (5.0 * h4);
+  const double delta = res * (w4 * 0.2);
   // ln of the unscaled input: ln(f) + (E + 5*adj) ln 2, ln(f) from the atanh series
-  const double s = u / (f + 1.0);
+  // (f folded into [sqrt(1/2), sqrt(2)) so the truncated series is good to ~1e-9)
+  const bool hi = f > 1.4142135623730951;
+  const double fl = hi ? f * 0.5 : f;
+  const double s = (fl - 1.0) / (fl + 1.0);
   const double s2 = s * s;
   const double lnf = 2.0 * s * (1.0 + s2 * (1.0 / 3.0 + s2 * (0.2 + s2 * (1.0 / 7.0 + s2 * (1.0 / 9.0)))));
-  const double lnx = lnf + (double)(E + 5 * adj) * 0.6931471805599453;
+  const double lnx = lnf + (double)(E + (hi ? 1 : 0) + 5 * adj) * 0.6931471805599453;
   const double eps = 1.1102230246251565e-17;  // 0.2 (float64) - 1/5
   const double corr = r * (eps * lnx);
   const double out = r + (delta + corr);

@@ -346,8 +346,23 @@ struct ErkLane {
   }
 };
 
+// One trajectory per lane, no work queue: used when the launch has a lane for every
+// instance.  Lanes whose trajectory has finished idle (masked) until the wave's last lane
+// is done.
 template <class Tab, class VF, class T>
 __global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= a.B) return;
+  ErkLane<Tab, VF, T> L;
+  L.start(a, id);
+  while (L.running(a)) L.step(a);
+  L.finish(a);
+}
+
+// Persistent variant with refill: a lane whose trajectory finishes claims the next
+// instance id from the work counter (one atomic per wave per claim round).
+template <class Tab, class VF, class T>
+__global__ void __launch_bounds__(256) erk_small_queue_kernel(const ErkKArgs a) {
   ErkLane<Tab, VF, T> L;
   bool active = false;
   bool drained = false;  // wave-uniform: the work queue is exhausted

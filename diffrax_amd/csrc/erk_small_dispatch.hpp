@@ -7,9 +7,14 @@ namespace dfx {
 
 template <class Tab, class VF, class T>
 static int launch_small(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
-  hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
   const int64_t nblocks = (lanes + block - 1) / block;
-  hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T>), dim3((unsigned)nblocks), dim3(block), 0, s, a);
+  if (lanes >= a.B) {
+    hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T>), dim3((unsigned)nblocks), dim3(block), 0, s, a);
+  } else {
+    hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
+    hipLaunchKernelGGL((erk_small_queue_kernel<Tab, VF, T>), dim3((unsigned)nblocks), dim3(block), 0,
+                       s, a);
+  }
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }
 

@@ -115,6 +115,54 @@ int dfx_erk_workspace_bytes(const dfx_erk_problem* p, size_t* bytes);
 int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspace,
                   size_t workspace_bytes, const dfx_launch* launch, void* stream);
 
+/* ---- SDE: Heun with diagonal noise on the registered OU terms ----------------------- */
+/* dy = theta (mu - y) dt + sigma dW, theta/mu/sigma of length d shared by all paths. */
+typedef struct dfx_sde_problem {
+  int32_t precision;        /* DFX_F64 (float64 state) or DFX_F32 (float32 state)        */
+  int32_t d;                /* components per path (multiple of 4 for F32, of 2 for F64) */
+  int64_t paths;            /* P                                                        */
+  int32_t num_steps;        /* N steps of the ConstantStepSize grid                      */
+  int32_t checkpoint_every; /* CK: states stored every CK steps for the adjoint (8/16/32),
+                               0 when no gradient will be taken                          */
+  int32_t save_t0, save_t1; /* SaveAt(t0=..., t1=...)                                    */
+} dfx_sde_problem;
+
+typedef struct dfx_sde_io {
+  const void* y0;           /* [P, d] state dtype                                        */
+  const void* theta;        /* [d] state dtype                                           */
+  const void* mu;           /* [d]                                                       */
+  const void* sigma;        /* [d]                                                       */
+  const float* dW;          /* [N, P, d] float32 Brownian increments (step-major)         */
+  const double* grid;       /* [N + 1] step times of ConstantStepSize (host-computed)     */
+  void* ys;                 /* out [P, save_t0 + save_t1, d]                              */
+  void* ckpt;               /* out [dfx_sde_n_ckpt, P, d] states at steps CK, 2CK, ...
+                               (NULL when no gradient is needed)                          */
+} dfx_sde_io;
+
+typedef struct dfx_sde_grad_io {
+  const void* ys_bar;       /* [P, save_t0 + save_t1, d] cotangent of ys                  */
+  void* y0_bar;             /* out [P, d] or NULL                                          */
+  double* theta_bar;        /* out [d] float64                                             */
+  double* mu_bar;           /* out [d]                                                     */
+  double* sigma_bar;        /* out [d]                                                     */
+} dfx_sde_grad_io;
+
+/* Number of stored checkpoints: ceil(N / CK) - 1. */
+int dfx_sde_n_ckpt(const dfx_sde_problem* p);
+/* Workspace of dfx_sde_heun_adjoint (per-block partial sums), bytes. */
+int dfx_sde_workspace_bytes(const dfx_sde_problem* p, size_t* bytes);
+/* Forward Heun solve of P independent paths; writes ys and (if ckpt) the checkpoints.
+ * Replaces: _integrate.loop (_integrate.py:302-885) with Heun (_solver/heun.py:12-45,
+ * _solver/runge_kutta.py:847-1203), MultiTerm(ODETerm, ControlTerm).vf_prod
+ * (_term.py:711-722, 420-427) and ConstantStepSize (_step_size_controller/constant.py:30-104). */
+int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* stream);
+/* Discrete adjoint of dfx_sde_heun_solve: from ys_bar, the cotangents of y0, theta, mu,
+ * sigma, replaying each CK-step segment from its checkpoint.
+ * Replaces: reverse-mode through RecursiveCheckpointAdjoint.loop (_adjoint.py:264-312). */
+int dfx_sde_heun_adjoint(const dfx_sde_problem* p, const dfx_sde_io* io,
+                         const dfx_sde_grad_io* g, void* workspace, size_t workspace_bytes,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

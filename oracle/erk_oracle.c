@@ -213,9 +213,11 @@ static double cr_pow_fifth(double x) {
     double h5 = h4 * r, l5 = fma(h4, r, -h5) + l4 * r;
     double res = (m - h5) - l5;
     double delta = res / (5.0 * h4);
-    double s = u / (f + 1.0), s2 = s * s;
+    int hi = f > 1.4142135623730951;
+    double fl = hi ? f * 0.5 : f;
+    double s = (fl - 1.0) / (fl + 1.0), s2 = s * s;
     double lnf = 2.0 * s * (1.0 + s2 * (1.0 / 3.0 + s2 * (0.2 + s2 * (1.0 / 7.0 + s2 * (1.0 / 9.0)))));
-    double lnx = lnf + (double)(E + 5 * adj) * 0.6931471805599453;
+    double lnx = lnf + (double)(E + hi + 5 * adj) * 0.6931471805599453;
     double corr = r * (1.1102230246251565e-17 * lnx);
     double out = r + (delta + corr);
     return out * bits2d((uint64_t)(1023 + k + adj) << 52);
