@@ -214,6 +214,9 @@ __host__ __device__ inline double cr_pow_fifth(double x) {
   return out * scale;
 }
 
+// Exponents other than 0.2 / 0.5 / 1 (non-default PID coefficients or error orders) use
+// the math library's pow (float64, within an ulp or so of correct rounding).
+template <bool Generic = true>
 __host__ __device__ inline double pow_ctl(double x, double c) {
   if (c == 0.2) return cr_pow_fifth(x);
   if (c == 1.0) return x;
@@ -221,7 +224,8 @@ __host__ __device__ inline double pow_ctl(double x, double c) {
     if (x == 0.0) return 0.0;
     return (x < 0.0) ? __builtin_nan("") : sqrt(x);
   }
-  return pow(x, c);
+  if constexpr (Generic) return pow(x, c);
+  return __builtin_nan("");
 }
 
 // ----------------------------------------------------------------------------------
@@ -242,6 +246,9 @@ struct PidState {
 
 // PIDController.adapt_step_size after the scaled error is known.  Returns keep and
 // writes next (t0, t1), the new state and an error code (2 = dt_min_reached).
+// Generic = false is the integral-only controller (pcoeff = dcoeff = 0, the default) with
+// exponent 0.2, 0.5 or 1: factor2 = factor3 = 1 and no library pow is compiled in.
+template <bool Generic>
 __device__ __forceinline__ bool pid_adapt(const PidParams& p, double scaled, double tprev,
                                           double tnext, PidState& st, double& ntprev,
                                           double& ntnext, int& code) {
@@ -249,9 +256,14 @@ __device__ __forceinline__ bool pid_adapt(const PidParams& p, double scaled, dou
   bool keep = scaled < 1.0;
   if (p.has_dtmin) keep = keep || st.at_dtmin;
   double inv = 1.0 / scaled;
-  const double f1 = (p.coeff1 == 0.0) ? 1.0 : pow_ctl(inv, p.coeff1);
-  const double f2 = (p.coeff2 == 0.0) ? 1.0 : pow_ctl(st.inv, p.coeff2);
-  const double f3 = (p.coeff3 == 0.0) ? 1.0 : pow_ctl(st.pinv, p.coeff3);
+  double f1, f2 = 1.0, f3 = 1.0;
+  if constexpr (Generic) {
+    f1 = (p.coeff1 == 0.0) ? 1.0 : pow_ctl<true>(inv, p.coeff1);
+    f2 = (p.coeff2 == 0.0) ? 1.0 : pow_ctl<true>(st.inv, p.coeff2);
+    f3 = (p.coeff3 == 0.0) ? 1.0 : pow_ctl<true>(st.pinv, p.coeff3);
+  } else {
+    f1 = pow_ctl<false>(inv, p.coeff1);
+  }
   const double fmin = keep ? 1.0 : p.factormin;
   const double fmax = keep ? p.factormax : p.safety;
   const double factor = jclip(p.safety * f1 * f2 * f3, fmin, fmax);
@@ -272,6 +284,12 @@ __device__ __forceinline__ bool pid_adapt(const PidParams& p, double scaled, dou
   st.inv = new_inv;
   st.pinv = new_pinv;
   return keep;
+}
+
+// Whether the fast (non-generic) controller path reproduces these coefficients.
+inline bool pid_is_integral(const PidParams& p) {
+  return p.coeff2 == 0.0 && p.coeff3 == 0.0 &&
+         (p.coeff1 == 0.2 || p.coeff1 == 0.5 || p.coeff1 == 1.0);
 }
 
 // optimistix.rms_norm restated: sqrt(sum x^2)/sqrt(n), |x| when n == 1.

@@ -9,6 +9,8 @@
 // lock-step only at the instruction level: accept/reject, save counts and refills are
 // per lane.
 #pragma once
+#include <type_traits>
+
 #include "erk_common.hpp"
 #include "vector_fields.hpp"
 #include "../../include/diffrax_amd.h"
@@ -32,21 +34,73 @@ struct ErkKArgs {
   PidParams pid;
 };
 
-// Sum_j coef[j] * k[j][comp] over the nonzero coefficients, in ascending j, starting
-// from the first nonzero product (the order the reference tensordot is restated with).
-template <class T, int S, int N>
-__device__ __forceinline__ T lincomb(const double (&coef)[N], const T (&k)[S], int upto) {
-  T acc = T(0);
-  bool first = true;
+// A float64 value the compiler cannot constant-fold: x + (a VGPR zero written by inline
+// asm).  Tableau coefficients pass through it once per trajectory so they live in VGPRs
+// (plentiful at one wave per SIMD) instead of being hoisted into SGPRs, which overflow
+// and spill through v_readlane on every use.  x + 0 == x for every nonzero x.
+__device__ __forceinline__ double in_vgpr(double x) {
+  double z;
+  asm volatile("v_mov_b64 %0, 0" : "=v"(z));
+  return x + z;
+}
+
+// Per-lane copies of a tableau's nonzero coefficients (zeros stay compile-time zeros).
+template <class Tab>
+struct TabRegs {
+  static constexpr int S = Tab::S;
+  double a[S][S];
+  double b[S];
+  double berr[S];
+  __device__ __forceinline__ void load() {
 #pragma unroll
-  for (int m = 0; m < S; ++m) {
-    if (m < upto && coef[m] != 0.0) {
-      const T term = T(coef[m]) * k[m];
-      acc = first ? term : acc + term;
-      first = false;
+    for (int i = 0; i < S; ++i) {
+#pragma unroll
+      for (int m = 0; m < S; ++m) a[i][m] = (Tab::a[i][m] != 0.0) ? in_vgpr(Tab::a[i][m]) : 0.0;
+      b[i] = (Tab::b[i] != 0.0) ? in_vgpr(Tab::b[i]) : 0.0;
+      berr[i] = (Tab::berr[i] != 0.0) ? in_vgpr(Tab::berr[i]) : 0.0;
     }
   }
-  return acc;
+};
+
+// Compile-time tableau access: WHICH 0 = a[ROW], 1 = b, 2 = b_error, 3 = Dopri5 c_mid.
+template <class Tab, int WHICH, int ROW, int M>
+constexpr double tab_coef() {
+  if constexpr (WHICH == 0) return Tab::a[ROW][M];
+  else if constexpr (WHICH == 1) return Tab::b[M];
+  else if constexpr (WHICH == 2) return Tab::berr[M];
+  else return Dopri5Tab::cmid[M];
+}
+
+// Sum over m < END of coef[m] * k[m] for the coefficients that are nonzero in the tableau,
+// in ascending m, starting from the first nonzero product (the order the reference
+// tensordot is restated with).  Zero entries, and the choice of the first term, are
+// resolved at compile time.  `vals` holds the coefficient values (in VGPRs).
+template <class Tab, int WHICH, int ROW, int M, int END, bool STARTED, class T, int S>
+__device__ __forceinline__ T comb(const double* vals, const T (&k)[S], T acc) {
+  if constexpr (M >= END) {
+    return acc;
+  } else {
+    constexpr double c = tab_coef<Tab, WHICH, ROW, M>();
+    if constexpr (c != 0.0) {
+      const T term = T(vals[M]) * k[M];
+      if constexpr (STARTED) {
+        return comb<Tab, WHICH, ROW, M + 1, END, true>(vals, k, acc + term);
+      } else {
+        return comb<Tab, WHICH, ROW, M + 1, END, true>(vals, k, term);
+      }
+    } else {
+      return comb<Tab, WHICH, ROW, M + 1, END, STARTED>(vals, k, acc);
+    }
+  }
+}
+
+// static_for<B, E>(f): f(integral_constant<int, i>) for i in [B, E), unrolled at compile time.
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
 }
 
 __device__ __forceinline__ unsigned int lane_rank(unsigned long long mask) {
@@ -54,13 +108,14 @@ __device__ __forceinline__ unsigned int lane_rank(unsigned long long mask) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
 }
 
-template <class Tab, class VF, class T>
+template <class Tab, class VF, class T, bool GenericPid>
 struct ErkLane {
   static constexpr int D = VF::D;
   static constexpr int P = VF::P;
   static constexpr int S = Tab::S;
 
   // ---- per-trajectory state -------------------------------------------------------
+  TabRegs<Tab> tr;
   int64_t traj;
   int dir;
   double T1, floor_t1, tprev, tnext;
@@ -109,13 +164,14 @@ struct ErkLane {
     const double d2 = rms_norm<D>(e) / h0;
     const double max_d = jmax(d1, d2);
     const double h1 = (max_d <= 1e-15) ? jmax(1e-6, h0 * 1e-3)
-                                       : pow_ctl(0.01 / max_d, 1.0 / a.error_order);
+                                       : pow_ctl<true>(0.01 / max_d, 1.0 / a.error_order);
     return jmin(100.0 * h0, h1);
   }
 
   // diffeqsolve set-up for one instance (_integrate.py:1156-1305, pid.py:316-392,
   // constant.py:30-54).
   __device__ void start(const ErkKArgs& a, int64_t id) {
+    tr.load();
     traj = id;
     const T* y0p = reinterpret_cast<const T*>(a.y0) + id * D;
 #pragma unroll
@@ -190,20 +246,21 @@ struct ErkLane {
       for (int j = 0; j < D; ++j) k[0][j] = control * fsal_f[j];
     }
     constexpr int start_stage = Tab::fsal ? 1 : 0;
-#pragma unroll
-    for (int i = start_stage; i < S; ++i) {
+    static_for<start_stage, S>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
 #pragma unroll
       for (int j = 0; j < D; ++j) {
         T kc[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : T(0);
-        z[j] = y[j] + lincomb<T, S, S>(Tab::a[i], kc, i);
+        z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, T(0));
       }
-      const double ti = (Tab::c[i] == 1.0) ? ts1 : ts0 + Tab::c[i] * dt;
+      constexpr double ci = Tab::c[i];
+      const double ti = (ci == 1.0) ? ts1 : ts0 + ci * dt;
       eval_f(ti * dir, z, f);
 #pragma unroll
       for (int j = 0; j < D; ++j) k[i][j] = control * f[j];
-    }
+    });
     T y1[D], yerr[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) {
@@ -213,9 +270,9 @@ struct ErkLane {
       if constexpr (Tab::ssal) {
         y1[j] = z[j];
       } else {
-        y1[j] = y[j] + lincomb<T, S, S>(Tab::b, kc, S);
+        y1[j] = y[j] + comb<Tab, 1, 0, 0, S, false>(tr.b, kc, T(0));
       }
-      yerr[j] = lincomb<T, S, S>(Tab::berr, kc, S);
+      yerr[j] = comb<Tab, 2, 0, 0, S, false>(tr.berr, kc, T(0));
       if (yerr[j] != yerr[j]) yerr[j] = T(__builtin_inf());
     }
 
@@ -235,7 +292,7 @@ struct ErkLane {
         e[j] = (double)yerr[j] / (a.pid.atol + yy * a.pid.rtol);
       }
       const double scaled = rms_norm<D>(e);
-      keep = pid_adapt(a.pid, scaled, ts0, ts1, pid, ntprev, ntnext, code);
+      keep = pid_adapt<GenericPid>(a.pid, scaled, ts0, ts1, pid, ntprev, ntnext, code);
     } else {
       keep = true;
       const int32_t sc = cs_completed + 1;
@@ -304,7 +361,7 @@ struct ErkLane {
         T kc[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) kc[m] = k[m][j];
-        const T ymid = y0[j] + lincomb<T, S, S>(Dopri5Tab::cmid, kc, S);
+        const T ymid = y0[j] + comb<Tab, 3, 0, 0, S, false>(Dopri5Tab::cmid, kc, T(0));
         const T f0 = k[0][j], f1 = k[6][j];
         const T ca = T(2) * (f1 - f0) - T(8) * (y1[j] + y0[j]) + T(16) * ymid;
         const T cb = T(5) * f0 - T(3) * f1 + T(18) * y0[j] + T(14) * y1[j] - T(32) * ymid;
@@ -349,11 +406,11 @@ struct ErkLane {
 // One trajectory per lane, no work queue: used when the launch has a lane for every
 // instance.  Lanes whose trajectory has finished idle (masked) until the wave's last lane
 // is done.
-template <class Tab, class VF, class T>
+template <class Tab, class VF, class T, bool GenericPid>
 __global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= a.B) return;
-  ErkLane<Tab, VF, T> L;
+  ErkLane<Tab, VF, T, GenericPid> L;
   L.start(a, id);
   while (L.running(a)) L.step(a);
   L.finish(a);
@@ -361,9 +418,9 @@ __global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
 
 // Persistent variant with refill: a lane whose trajectory finishes claims the next
 // instance id from the work counter (one atomic per wave per claim round).
-template <class Tab, class VF, class T>
+template <class Tab, class VF, class T, bool GenericPid>
 __global__ void __launch_bounds__(256) erk_small_queue_kernel(const ErkKArgs a) {
-  ErkLane<Tab, VF, T> L;
+  ErkLane<Tab, VF, T, GenericPid> L;
   bool active = false;
   bool drained = false;  // wave-uniform: the work queue is exhausted
   while (true) {

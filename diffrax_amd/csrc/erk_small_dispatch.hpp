@@ -5,16 +5,24 @@
 
 namespace dfx {
 
-template <class Tab, class VF, class T>
-static int launch_small(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
+template <class Tab, class VF, class T, bool G>
+static void launch_small_g(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
   const int64_t nblocks = (lanes + block - 1) / block;
   if (lanes >= a.B) {
-    hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T>), dim3((unsigned)nblocks), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T, G>), dim3((unsigned)nblocks), dim3(block), 0, s, a);
   } else {
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-    hipLaunchKernelGGL((erk_small_queue_kernel<Tab, VF, T>), dim3((unsigned)nblocks), dim3(block), 0,
-                       s, a);
+    hipLaunchKernelGGL((erk_small_queue_kernel<Tab, VF, T, G>), dim3((unsigned)nblocks), dim3(block),
+                       0, s, a);
   }
+}
+
+template <class Tab, class VF, class T>
+static int launch_small(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
+  if (a.controller == DFX_PID && !pid_is_integral(a.pid))
+    launch_small_g<Tab, VF, T, true>(a, block, lanes, s);
+  else
+    launch_small_g<Tab, VF, T, false>(a, block, lanes, s);
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }
 
