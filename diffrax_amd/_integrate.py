@@ -122,7 +122,7 @@ def _check_ts(ts, t0, t1_vals, directions):
 
 def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                  max_steps, launch, record_grid=False):
-    device = _require_gpu()
+    # Host-side checks first (they need no GPU); device placement at the end.
     if not isinstance(terms, ODETerm) or not isinstance(terms.vector_field, RegisteredVectorField):
         raise NotImplementedError(
             "diffrax_amd evaluates registered device vector fields only: wrap one of "
@@ -135,7 +135,6 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         raise ValueError(f"y0 must be float64 or float32, got {y0.dtype}")
     batched = y0.ndim == 2
     y0b = y0 if batched else y0.reshape(1, -1)
-    y0b = y0b.to(device).contiguous()
     B, d = y0b.shape
     if not vfo.supports(d):
         raise NotImplementedError(f"{type(vfo).__name__} does not support state dimension {d}")
@@ -146,11 +145,11 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         raise ValueError("t0 must be a scalar")
     t1f = _scalar_time(t1, "t1")
     if t1f is not None:
-        t1_t = torch.tensor([t1f], dtype=torch.float64, device=device)
+        t1_t = torch.tensor([t1f], dtype=torch.float64)
         t1_stride = 0
         t1_host = np.array([t1f])
     else:
-        t1_t = torch.as_tensor(t1, dtype=torch.float64).reshape(-1).to(device).contiguous()
+        t1_t = torch.as_tensor(t1, dtype=torch.float64).reshape(-1)
         if t1_t.numel() != B:
             raise ValueError(f"t1 must be a scalar or have shape [{B}]")
         t1_stride = 1
@@ -197,9 +196,6 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         else:
             raise ValueError(f"SaveAt(ts) must be [n] or [{B}, n]")
         _check_ts(ts_t.cpu().numpy(), t0f, t1_host, directions)
-        ts_t = ts_t.to(device).contiguous()
-
-    params, param_stride = vfo.pack(args, B, d, y0b.dtype, device)
 
     if isinstance(stepsize_controller, PIDController):
         c = stepsize_controller
@@ -220,6 +216,12 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
     else:
         raise NotImplementedError(f"stepsize controller {stepsize_controller!r}")
 
+    device = _require_gpu()
+    y0b = y0b.to(device).contiguous()
+    t1_t = t1_t.to(device).contiguous()
+    if ts_t is not None:
+        ts_t = ts_t.to(device).contiguous()
+    params, param_stride = vfo.pack(args, B, d, y0b.dtype, device)
     prob = _lib.ErkProblem(
         solver=solver.solver_id, controller=controller, vf=vfo.vf_id, precision=precision,
         d=d, hidden=0, n_ts=n_ts, save_t0=int(sub.t0), save_t1=int(sub.t1),

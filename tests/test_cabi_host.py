@@ -1,0 +1,120 @@
+"""C ABI and host-side logic (CPU only: no kernel is launched).
+
+- every function declared in include/diffrax_amd.h is exported by libdiffrax_amd.so and
+  bound in the Python loader;
+- the product's device tableaux (read back through dfx_tableau) are bit-identical to the
+  oracle's, so the order-condition pins of the oracle cover them too;
+- argument errors are reported as the reference reports them (RuntimeError for the
+  error_if checks of _integrate.py:1036-1045, 1222-1235) before any GPU work."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+import torch
+
+import diffrax_amd as dfx
+import oracle
+from diffrax_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = []
+    for h in os.listdir(os.path.join(ROOT, "include")):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*(?:const\s+)?[A-Za-z_0-9]+\s*\*?\s*(dfx_[a-z0-9_]+)\s*\(",
+                            text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_header_symbols_exported_and_bound():
+    names = _declared_functions()
+    assert len(names) >= 10
+    lib = _lib.load()
+    for n in names:
+        assert hasattr(lib, n), f"{n} not exported by {_lib.LIB_PATH}"
+        assert n in _lib._SIGS, f"{n} has no ctypes signature in diffrax_amd._lib"
+
+
+@pytest.mark.parametrize("solver,sid", [("tsit5", 0), ("dopri5", 1), ("heun", 2)])
+def test_device_tableau_matches_oracle(solver, sid):
+    lib = _lib.load()
+    a = (ctypes.c_double * 49)(); b = (ctypes.c_double * 7)(); be = (ctypes.c_double * 7)()
+    c = (ctypes.c_double * 7)(); info = (ctypes.c_int32 * 4)()
+    assert lib.dfx_tableau(sid, a, b, be, c, info) == 0
+    oa, ob, obe, oc, oinfo = oracle.tableau(solver)
+    assert np.array_equal(np.array(a[:]).reshape(7, 7), oa)
+    assert np.array_equal(np.array(b[:]), ob) and np.array_equal(np.array(be[:]), obe)
+    assert np.array_equal(np.array(c[:]), oc) and tuple(info) == oinfo
+    t = getattr(dfx, {"tsit5": "Tsit5", "dopri5": "Dopri5", "heun": "Heun"}[solver])().tableau
+    assert t.num_stages == oinfo[0] and t.fsal == bool(oinfo[1])
+
+
+def test_n_out_and_param_counts():
+    lib = _lib.load()
+    p = _lib.ErkProblem(n_ts=64, save_t0=1, save_t1=1, save_steps=0, max_steps=100)
+    assert lib.dfx_erk_n_out(ctypes.byref(p)) == 66
+    p.save_steps = 1
+    assert lib.dfx_erk_n_out(ctypes.byref(p)) == 165
+    assert lib.dfx_vf_param_count(_lib.VF_MLP, 64, 256) == 98880
+    assert lib.dfx_vf_param_count(_lib.VF_OU, 16, 0) == 48
+
+
+def test_solve_rejects_bad_arguments_without_launching():
+    lib = _lib.load()
+    p = _lib.ErkProblem(solver=7, controller=0, vf=_lib.VF_LV, precision=0, d=2, batch=4,
+                        max_steps=10)
+    io = _lib.ErkIO()
+    rc = lib.dfx_erk_solve(ctypes.byref(p), ctypes.byref(io), None, 0, None, None)
+    assert rc == -1
+    p.solver = 0
+    p.d = 3   # LV needs d == 2
+    assert lib.dfx_erk_solve(ctypes.byref(p), ctypes.byref(io), None, 0, None, None) == -1
+    assert lib.dfx_strerror(-3).decode().startswith("workspace")
+
+
+def _solve(**kw):
+    base = dict(terms=dfx.ODETerm(dfx.LotkaVolterra()), solver=dfx.Tsit5(), t0=0.0, t1=1.0,
+                dt0=0.1, y0=torch.ones(2, dtype=torch.float64), args=(1.5, 1.0, 3.0, 1.0),
+                saveat=dfx.SaveAt(t1=True),
+                stepsize_controller=dfx.PIDController(rtol=1e-6, atol=1e-8))
+    base.update(kw)
+    return dfx.diffeqsolve(**base)
+
+
+def test_host_validation_errors():
+    with pytest.raises(RuntimeError, match="dt0"):
+        _solve(dt0=-0.1)
+    with pytest.raises(RuntimeError, match="between t0 and t1"):
+        _solve(saveat=dfx.SaveAt(ts=[0.5, 3.0]))
+    with pytest.raises(RuntimeError, match="increasing"):
+        _solve(saveat=dfx.SaveAt(ts=[0.5, 0.2]))
+    with pytest.raises(NotImplementedError, match="registered"):
+        _solve(terms=dfx.ODETerm(lambda t, y, args: -y))
+    with pytest.raises(ValueError):
+        dfx.PIDController(rtol=None, atol=1e-6)
+    with pytest.raises(ValueError):
+        dfx.SaveAt(ts=[0.1], subs=dfx.SubSaveAt(t1=True))
+    assert dfx.RESULTS.successful._value == 0
+    assert isinstance(dfx.RESULTS[dfx.RESULTS.max_steps_reached], str)
+
+
+def test_no_cpu_fallback():
+    """Without a GPU a well-formed solve raises instead of computing on the CPU."""
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _solve()
+
+
+def test_registered_fields_evaluate_like_formula():
+    y = torch.tensor([[1.2, 0.7]], dtype=torch.float64)
+    f = dfx.LotkaVolterra()(0.0, y, (1.5, 1.0, 3.0, 1.0))
+    assert torch.allclose(f, torch.tensor([[1.5 * 1.2 - 1.2 * 0.7, 1.2 * 0.7 - 3 * 0.7]],
+                                          dtype=torch.float64))
+    g = dfx.VanDerPol()(0.0, y, 2.0)
+    assert torch.allclose(g, torch.tensor([[0.7, 2.0 * (1 - 1.44) * 0.7 - 1.2]], dtype=torch.float64))

@@ -1,0 +1,348 @@
+"""Pins of the CPU oracle (CPU only).
+
+The reference holds no stored arrays for this path, so the oracle is pinned the way the
+reference pins itself -- closed forms, scipy cross-checks, invariants -- plus checks the
+mathematics fixes: exact-rational one-step results, Runge-Kutta order conditions, a
+correctly rounded power against 60-digit decimal arithmetic, and agreement of the two
+independently written restatements (C and Python)."""
+import math
+from decimal import Decimal, localcontext
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.integrate as integrate
+import scipy.linalg
+import scipy.stats
+
+import oracle
+from oracle import pyoracle
+
+
+# ---------------------------------------------------------------------------------------
+# Tableaux
+# ---------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("solver", ["tsit5", "dopri5", "heun"])
+def test_tableau_invariants_and_sources_agree(solver):
+    """ButcherTableau.__post_init__ invariants (runge_kutta.py:116-165); the C oracle and
+    the Python restatement hold bit-identical coefficients."""
+    a, b, be, c, (S, fsal, ssal, order) = oracle.tableau(solver)
+    A = a[:S, :S]
+    for i in range(1, S):
+        assert math.isclose(A[i, :i].sum(), c[i], rel_tol=0, abs_tol=1e-14)
+    assert math.isclose(b[:S].sum(), 1.0, abs_tol=1e-14)
+    assert abs(be[:S].sum()) < 1e-14
+    tab = pyoracle.TABLEAUX[solver]
+    assert (fsal, ssal) == (tab.fsal, tab.ssal) == ((True, True) if solver != "heun" else (False, False))
+    for i in range(1, S):
+        assert list(A[i, :i]) == [float(x) for x in tab.a[i]]
+    assert list(b[:S]) == [float(x) for x in tab.b]
+    assert list(be[:S]) == [float(x) for x in tab.berr]
+    assert list(c[:S]) == [float(x) for x in tab.c]
+    assert order == tab.order
+
+
+def _order_conditions(A, b, c):
+    """Residuals of the Runge-Kutta order conditions up to order 5 (Butcher trees)."""
+    Ac = A @ c
+    Ac2 = A @ c**2
+    AAc = A @ Ac
+    res = {
+        1: [b.sum() - 1],
+        2: [b @ c - 1 / 2],
+        3: [b @ c**2 - 1 / 3, b @ Ac - 1 / 6],
+        4: [b @ c**3 - 1 / 4, b @ (c * Ac) - 1 / 8, b @ Ac2 - 1 / 12, b @ AAc - 1 / 24],
+        5: [b @ c**4 - 1 / 5, b @ (c**2 * Ac) - 1 / 10, b @ (c * Ac2) - 1 / 15,
+            b @ (c * AAc) - 1 / 30, b @ (Ac * Ac) - 1 / 20, b @ (A @ c**3) - 1 / 20,
+            b @ (A @ (c * Ac)) - 1 / 40, b @ (A @ Ac2) - 1 / 60, b @ (A @ AAc) - 1 / 120],
+    }
+    return res
+
+
+@pytest.mark.parametrize("solver,order,emb", [("tsit5", 5, 4), ("dopri5", 5, 4), ("heun", 2, 1)])
+def test_order_conditions(solver, order, emb):
+    """b satisfies every condition up to the method's order, b - b_err (the embedded
+    method) up to its order -- any mistyped digit breaks one of them."""
+    a, b, be, c, (S, *_rest) = oracle.tableau(solver)
+    A, b, be, c = a[:S, :S], b[:S], be[:S], c[:S]
+    for p, rs in _order_conditions(A, b, c).items():
+        if p <= order:
+            assert max(abs(r) for r in rs) < 1e-13, (solver, p, rs)
+    for p, rs in _order_conditions(A, b - be, c).items():
+        if p <= emb:
+            assert max(abs(r) for r in rs) < 1e-13, (solver, "embedded", p)
+    # and b genuinely has no higher order than claimed
+    if order + 1 <= 5:
+        assert max(abs(r) for r in _order_conditions(A, b, c)[order + 1]) > 1e-6
+
+
+# ---------------------------------------------------------------------------------------
+# Exact-rational one-step KAT
+# ---------------------------------------------------------------------------------------
+
+def _exact_step(solver, y0, dt, q):
+    """One ERK step on Lotka-Volterra in exact rational arithmetic, using the exact values
+    of the float64 tableau entries (polynomial right-hand side: no rounding at all)."""
+    a, b, be, c, (S, *_r) = oracle.tableau(solver)
+    F = Fraction
+    A = [[F(float(a[i, j])) for j in range(S)] for i in range(S)]
+    al, be_, ga, de = (F(float(x)) for x in q)
+
+    def f(y):
+        x, v = y
+        return [al * x - be_ * x * v, de * x * v - ga * v]
+
+    y = [F(float(v)) for v in y0]
+    dtF = F(float(dt))
+    ks = []
+    for i in range(S):
+        z = [y[j] + sum(A[i][m] * ks[m][j] for m in range(i)) for j in range(2)]
+        fz = f(z)
+        ks.append([dtF * fz[j] for j in range(2)])
+    y1 = [y[j] + sum(F(float(b[m])) * ks[m][j] for m in range(S)) for j in range(2)]
+    err = [sum(F(float(be[m])) * ks[m][j] for m in range(S)) for j in range(2)]
+    return y1, err
+
+
+@pytest.mark.parametrize("solver", ["dopri5", "tsit5", "heun"])
+def test_one_step_exact_rational(solver):
+    q = [1.5, 1.0, 3.0, 1.0]
+    y0 = np.array([1.25, 0.75])
+    dt = 0.125
+    y1, yerr, _ = oracle.single_step(solver, "lv", y0, 0.0, dt, q)
+    ex1, exe = _exact_step(solver, y0, dt, q)
+    for j in range(2):
+        assert abs(y1[j] - float(ex1[j])) <= 4 * math.ulp(float(ex1[j])), (j, y1[j], float(ex1[j]))
+        assert abs(yerr[j] - float(exe[j])) <= 1e-15 * max(1.0, abs(float(ex1[j])))
+
+
+# ---------------------------------------------------------------------------------------
+# Convergence order against expm (test_integrate.py:144-190 recipe)
+# ---------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("solver,order", [("tsit5", 5), ("dopri5", 5), ("heun", 2)])
+def test_convergence_order_linear(solver, order):
+    rng = np.random.default_rng(5678)
+    A = rng.normal(size=(10, 10)) * 0.5
+    y0 = rng.normal(size=10)
+    true = scipy.linalg.expm(4.0 * A) @ y0
+    xs, es = [], []
+    for e in [0, -1, -2, -3, -4, -6, -8]:
+        r = oracle.solve(y0[None], 4.0, A.ravel(), solver=solver, vf="linear",
+                         controller="constant", dt0=2.0**e, save_t1=True, max_steps=10**6)
+        err = np.sum(np.abs(r["ys"][0, -1] - true))
+        if err < 2**-28:
+            break
+        xs.append(e)
+        es.append(math.log2(err))
+    slope = scipy.stats.linregress(xs, es).slope
+    assert -0.9 < slope - order < 0.9, slope
+
+
+# ---------------------------------------------------------------------------------------
+# DETEST (Hull et al. 1972) against scipy DOP853 (test_detest.py:390-469 recipe)
+# ---------------------------------------------------------------------------------------
+
+def _detest_problems():
+    P = {}
+    P["a1"] = (lambda t, y: -y, [1.0])
+    P["a2"] = (lambda t, y: -0.5 * y**3, [1.0])
+    P["a3"] = (lambda t, y: y * np.cos(t), [1.0])
+    P["a4"] = (lambda t, y: 0.25 * y * (1 - 0.05 * y), [1.0])
+    P["a5"] = (lambda t, y: (y - t) / (y + t), [4.0])
+    P["b1"] = (lambda t, y: np.array([2 * (y[0] - y[0] * y[1]), -y[1] + y[0] * y[1]]), [1.0, 3.0])
+    P["b2"] = (lambda t, y: np.array([-y[0] + y[1], y[0] - 2 * y[1] + y[2], y[1] - y[2]]),
+               [2.0, 0.0, 1.0])
+    P["b3"] = (lambda t, y: np.array([-y[0], y[0] - y[1]**2, y[1]**2]), [1.0, 0.0, 0.0])
+
+    def b4(t, y):
+        r = np.sqrt(y[0]**2 + y[1]**2)
+        return np.array([-y[1] - y[0] * y[2] / r, y[0] - y[1] * y[2] / r, y[0] / r])
+    P["b4"] = (b4, [3.0, 0.0, 0.0])
+    P["b5"] = (lambda t, y: np.array([y[1] * y[2], -y[0] * y[2], -0.51 * y[0] * y[1]]),
+               [0.0, 1.0, 1.0])
+    C1 = np.diag(-np.ones(10)) + np.diag(np.ones(9), -1)
+    C1[9, 9] = 0.0
+    P["c1"] = (lambda t, y: C1 @ y, np.eye(10)[0])
+    C2 = np.zeros((10, 10))
+    C2[np.arange(9), np.arange(9)] = -np.arange(1.0, 10.0)
+    C2[np.arange(1, 10), np.arange(9)] = np.arange(1.0, 10.0)
+    P["c2"] = (lambda t, y: C2 @ y, np.eye(10)[0])
+    for name, eps in (("d1", 0.1), ("d3", 0.5), ("d5", 0.9)):
+        def d(t, y):
+            r3 = (y[0]**2 + y[1]**2) ** 1.5
+            return np.array([y[2], y[3], -y[0] / r3, -y[1] / r3])
+        P[name] = (d, [1 - eps, 0.0, 0.0, math.sqrt((1 + eps) / (1 - eps))])
+    return P
+
+
+@pytest.mark.parametrize("name", sorted(_detest_problems()))
+@pytest.mark.parametrize("solver", ["tsit5", "dopri5"])
+def test_detest_against_scipy(name, solver):
+    f, init = _detest_problems()[name]
+    y0 = np.asarray(init, dtype=np.float64)
+    sol = pyoracle.solve(f, solver, 0.0, 20.0, None, y0, [], rtol=1e-8, atol=1e-8,
+                         save_t1=True, max_steps=16**4)
+    assert sol["result"] == 0
+    ref = integrate.solve_ivp(lambda t, y: np.atleast_1d(f(t, y)), (0, 20), y0,
+                              method="DOP853", rtol=1e-8, atol=1e-8, t_eval=[20])
+    np.testing.assert_allclose(sol["ys"][-1], ref.y[:, 0], rtol=4e-5, atol=4e-5)
+
+
+# ---------------------------------------------------------------------------------------
+# SaveAt semantics: closed-form exponential decay (test_saveat_solution.py:29-111)
+# ---------------------------------------------------------------------------------------
+
+def test_saveat_exp_decay_closed_form():
+    y0 = np.array([[2.1]])
+    r = oracle.solve(y0, 1.1, [-0.5], ts=[0.5, 0.8], solver="dopri5", vf="linear", t0=0.1,
+                     dt0=None, rtol=1e-8, atol=1e-8, save_t1=True)
+    assert r["result"][0] == 0
+    assert list(r["ts"][0]) == [0.5, 0.8, 1.1]
+    want = [2.1 * math.exp(-0.2), 2.1 * math.exp(-0.35), 2.1 * math.exp(-0.5)]
+    np.testing.assert_allclose(r["ys"][0, :, 0], want, rtol=1e-5, atol=1e-8)
+    p = pyoracle.solve(lambda t, y: -0.5 * y, "dopri5", 0.1, 1.1, None, np.array([2.1]),
+                       [0.5, 0.8], rtol=1e-8, atol=1e-8, save_t1=True)
+    assert np.array_equal(p["ys"], r["ys"][0])
+    assert tuple(p["stats"]) == tuple(r["stats"][0])
+
+
+def test_unfilled_outputs_and_t0_equals_t1():
+    """Preset inf buffers (_integrate.py:1272-1305) and the t0 == t1 repeat save (823-845)."""
+    y0 = np.array([[1.0, 2.0]])
+    r = oracle.solve(y0, 1.0, [1.5, 1.0, 3.0, 1.0], ts=[1.0, 1.0], solver="tsit5", vf="lv",
+                     t0=1.0, dt0=0.1, save_t1=True)
+    assert r["stats"][0, 0] == 0 and r["result"][0] == 0
+    assert np.array_equal(r["ys"][0], np.repeat(y0, 3, axis=0))
+    r = oracle.solve(y0, 50.0, [1.5, 1.0, 3.0, 1.0], ts=np.linspace(0, 50, 5), solver="tsit5",
+                     vf="lv", dt0=0.1, max_steps=20)
+    assert r["result"][0] == 1  # max_steps_reached
+    assert np.isinf(r["ys"][0, -1]).all() and np.isinf(r["ts"][0, -1])
+
+
+# ---------------------------------------------------------------------------------------
+# Interpolation identities (test_local_interpolation.py:46-60, endpoints)
+# ---------------------------------------------------------------------------------------
+
+def test_hermite_reproduces_cubic():
+    t0, t1 = 2.0, 3.9
+    y = lambda t: 0.4 + 0.7 * t - 1.1 * t**2 + 0.4 * t**3
+    dy = lambda t: 0.7 - 2.2 * t + 1.2 * t**2
+    h = t1 - t0
+    ks = [np.array([dy(t0) * h]), np.array([dy(t1) * h])]
+    tab = pyoracle.TABLEAUX["heun"]
+    v = pyoracle.interpolate(tab, t0, t1, np.array([y(t0)]), np.array([y(t1)]), ks, 2.6)
+    assert abs(v[0] - y(2.6)) < 1e-12
+
+
+@pytest.mark.parametrize("solver", ["tsit5", "dopri5", "heun"])
+def test_interpolant_endpoints(solver):
+    q = [1.5, 1.0, 3.0, 1.0]
+    y0 = np.array([1.25, 0.75])
+    y1, _, k = oracle.single_step(solver, "lv", y0, 0.0, 0.1, q)
+    tab = pyoracle.TABLEAUX[solver]
+    ks = [k[m] for m in range(tab.S)]
+    v0 = pyoracle.interpolate(tab, 0.0, 0.1, y0, y1, ks, 0.0)
+    v1 = pyoracle.interpolate(tab, 0.0, 0.1, y0, y1, ks, 0.1)
+    assert np.array_equal(v0, y0)
+    np.testing.assert_allclose(v1, y1, rtol=0, atol=4e-15)
+
+
+# ---------------------------------------------------------------------------------------
+# PID controller formula (pid.py:394-567), derived independently with numpy
+# ---------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dt0", [0.05, 2.0])
+def test_pid_second_step_size(dt0):
+    """Accepted (dt0 small) and rejected (dt0 large) first steps: the second interval equals
+    dt0 * clip(0.9 * scaled^-0.2, [1, 10] or [0.2, 0.9])."""
+    q = [1.5, 1.0, 3.0, 1.0]
+    y0 = np.array([1.25, 0.75])
+    rtol, atol = 1e-6, 1e-8
+    y1, yerr, _ = oracle.single_step("tsit5", "lv", y0, 0.0, dt0, q)
+    e = yerr / (atol + np.maximum(np.abs(y0), np.abs(y1)) * rtol)
+    scaled = math.sqrt(float(np.sum(e * e))) / math.sqrt(2.0)
+    keep = scaled < 1
+    with localcontext() as ctx:
+        ctx.prec = 60
+        f1 = float(Decimal(1.0 / scaled) ** Decimal(0.2))
+    lo, hi = (1.0, 10.0) if keep else (0.2, 0.9)
+    factor = min(max(0.9 * f1, lo), hi)
+    r = oracle.solve(y0[None], 10.0, q, solver="tsit5", vf="lv", dt0=dt0, rtol=rtol, atol=atol,
+                     max_steps=2, record_grid=True, save_t1=True)
+    g = r["grid"][0]
+    if keep:
+        assert g[0, 0] == 0.0 and g[0, 1] == dt0
+        assert g[1, 1] - g[1, 0] == dt0 * factor
+    else:
+        assert r["stats"][0, 2] >= 1
+        # after the rejection the retry starts at 0 with dt0 * factor
+        r1 = oracle.solve(y0[None], 10.0, q, solver="tsit5", vf="lv", dt0=dt0 * factor,
+                          rtol=rtol, atol=atol, max_steps=1, save_t1=True)
+        assert factor < 0.9 + 1e-15 and factor >= 0.2
+        assert r1["stats"][0, 0] == 1
+
+
+def test_pow_correctly_rounded_against_decimal():
+    import ctypes
+    L = oracle.lib()
+    L.orc_pow.argtypes = [ctypes.c_double, ctypes.c_double]
+    L.orc_pow.restype = ctypes.c_double
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([10 ** rng.uniform(-300, 300, 3000), rng.uniform(0.5, 2, 3000),
+                         [5e-324, 1e-310, 1.0, 2.0, 32.0, 1e308]])
+    with localcontext() as ctx:
+        ctx.prec = 60
+        for x in xs:
+            assert L.orc_pow(float(x), 0.2) == float(Decimal(float(x)) ** Decimal(0.2)), x
+    assert L.orc_pow(0.0, 0.2) == 0.0 and math.isinf(L.orc_pow(math.inf, 0.2))
+    assert L.orc_pow(4.0, 0.5) == 2.0 and L.orc_pow(3.0, 1.0) == 3.0
+
+
+# ---------------------------------------------------------------------------------------
+# The two restatements agree; reverse time is the mirrored forward problem
+# ---------------------------------------------------------------------------------------
+
+@pytest.mark.parametrize("solver", ["tsit5", "dopri5", "heun"])
+def test_c_and_python_oracles_agree(solver):
+    rng = np.random.default_rng(11)
+    B = 6
+    mu = np.exp(rng.uniform(np.log(0.1), np.log(5.0), size=B))
+    y0 = rng.uniform(-2, 2, size=(B, 2))
+    ts = np.linspace(0.0, 6.0, 13)
+    r = oracle.solve(y0, 6.0, mu[:, None], ts=ts, solver=solver, vf="vdp", dt0=0.1,
+                     rtol=1e-6, atol=1e-9, max_steps=20000, save_t0=True)
+    for b in range(B):
+        f = lambda t, y, m=mu[b]: np.array([y[1], m * (1 - y[0] * y[0]) * y[1] - y[0]])
+        p = pyoracle.solve(f, solver, 0.0, 6.0, 0.1, y0[b], ts, rtol=1e-6, atol=1e-9,
+                           max_steps=20000, save_t0=True)
+        assert tuple(p["stats"]) == tuple(r["stats"][b])
+        assert np.array_equal(p["ys"], r["ys"][b])
+        assert np.array_equal(p["ts"], r["ts"][b])
+
+
+def test_reverse_time_is_mirrored_forward_problem():
+    """Solving f from t0 = 2 down to t1 = 0 equals solving g(s, y) = -f(-s, y) from -2 to 0
+    (WrapTerm, _term.py:734-764): same bits, times negated."""
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(3, 3)) * 0.7
+    y0 = rng.normal(size=(4, 3))
+    ts = np.array([1.5, 1.0, 0.25])
+    rev = oracle.solve(y0, 0.0, A.ravel(), ts=ts, solver="tsit5", vf="linear", t0=2.0, dt0=-0.1,
+                       rtol=1e-7, atol=1e-9, save_t1=True)
+    fwd = oracle.solve(y0, 0.0, (-A).ravel(), ts=-ts, solver="tsit5", vf="linear", t0=-2.0,
+                       dt0=0.1, rtol=1e-7, atol=1e-9, save_t1=True)
+    assert np.array_equal(rev["ys"], fwd["ys"])
+    assert np.array_equal(rev["ts"], -fwd["ts"])
+    assert np.array_equal(rev["stats"], fwd["stats"])
+
+
+def test_sde_step_grid_matches_constant_controller():
+    from diffrax_amd._sde import constant_grid
+    g = constant_grid(0.0, 1.0, 1e-3)
+    r = oracle.solve(np.zeros((1, 4)), 1.0, np.ones(12), dW=np.zeros((1000, 1, 4)),
+                     solver="heun", vf="ou", controller="constant", dt0=1e-3, save_t1=True,
+                     max_steps=2000, record_grid=True)
+    grid = r["grid"][0, :1000]
+    assert np.array_equal(grid[:, 0], g[:-1]) and np.array_equal(grid[:, 1], g[1:])
