@@ -18,8 +18,9 @@ from ._step_size_controller import (AbstractAdaptiveStepSizeController,
                                     PIDController, rms_norm)
 from ._term import (AbstractPath, AbstractTerm, ControlTerm, IncrementPath, MultiTerm,
                     ODETerm)
-from ._vector_fields import (DiagonalNoise, LinearField, LotkaVolterra, OUDrift,
+from ._vector_fields import (DiagonalNoise, LinearField, LotkaVolterra, MLPField, OUDrift,
                              RegisteredVectorField, VanDerPol)
+from . import _mlp  # noqa: F401  (registers the neural-ODE C-ABI signatures)
 from . import _sde  # noqa: F401  (registers the SDE C-ABI signatures)
 
 __version__ = "0.1.0"

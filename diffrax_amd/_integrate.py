@@ -21,7 +21,7 @@ from ._solution import RESULTS, Solution
 from ._solver import AbstractERK, AbstractSolver
 from ._step_size_controller import ConstantStepSize, PIDController
 from ._term import ControlTerm, IncrementPath, MultiTerm, ODETerm
-from ._vector_fields import DiagonalNoise, OUDrift, RegisteredVectorField
+from ._vector_fields import DiagonalNoise, MLPField, OUDrift, RegisteredVectorField
 
 _SMALL_VFS = (_lib.VF_LV, _lib.VF_VDP, _lib.VF_LINEAR)
 
@@ -296,6 +296,11 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
                  "num_rejected_steps": full(0), "max_steps": max_steps}
         return Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats,
                         result=full(0))
+    if isinstance(terms, ODETerm) and isinstance(terms.vector_field, MLPField):
+        from . import _mlp
+        return _mlp.diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat,
+                                    stepsize_controller, adjoint, max_steps, throw,
+                                    _require_gpu())
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                         max_steps, launch)
     plan.launch()

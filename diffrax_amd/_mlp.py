@@ -1,0 +1,325 @@
+"""Neural-ODE path: ODETerm(MLPField(...)) with Tsit5 + PIDController on the MFMA tile
+kernel (erk_mlp.hip), and its discrete adjoint (mlp_adj.hip)."""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class MlpProblem(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("d", ctypes.c_int32), ("hidden", ctypes.c_int32), ("n_ts", ctypes.c_int32),
+                ("save_t0", ctypes.c_int32), ("save_t1", ctypes.c_int32),
+                ("max_steps", ctypes.c_int32), ("ckpt_every", ctypes.c_int32),
+                ("force_dtmin", ctypes.c_int32), ("batch", ctypes.c_int64),
+                ("t0", ctypes.c_double), ("t1", ctypes.c_double), ("dt0", ctypes.c_double),
+                ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+                ("pcoeff", ctypes.c_double), ("icoeff", ctypes.c_double),
+                ("dcoeff", ctypes.c_double), ("dtmin", ctypes.c_double),
+                ("dtmax", ctypes.c_double), ("factormin", ctypes.c_double),
+                ("factormax", ctypes.c_double), ("safety", ctypes.c_double),
+                ("error_order", ctypes.c_double)]
+
+
+class MlpAdjIO(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("grid", "ckpt", "stats", "ts", "ys_bar", "ybar", "zs", "rows", "partial", "dw")]
+
+
+class MlpIO(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("y0", "ts", "ys", "ts_out", "stats", "result", "grid", "ckpt")]
+
+
+_P = ctypes.POINTER
+_SIGS = {
+    "dfx_mlp_pack_bytes": ([ctypes.c_int32, _P(ctypes.c_size_t)], ctypes.c_int),
+    "dfx_mlp_pack": ([ctypes.c_int32] + [ctypes.c_void_p] * 5, ctypes.c_int),
+    "dfx_mlp_n_ckpt": ([_P(MlpProblem)], ctypes.c_int),
+    "dfx_mlp_eval": ([ctypes.c_int32] + [ctypes.c_void_p] * 6 + [ctypes.c_int64, ctypes.c_void_p],
+                     ctypes.c_int),
+    "dfx_mlp_solve": ([_P(MlpProblem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                       ctypes.c_void_p, _P(MlpIO), ctypes.c_void_p], ctypes.c_int),
+}
+_SIGS.update({
+    "dfx_mlp_adj_workspace": ([_P(MlpProblem), ctypes.c_int32, _P(ctypes.c_size_t),
+                               _P(ctypes.c_size_t), _P(ctypes.c_size_t)], ctypes.c_int),
+    "dfx_mlp_adj_segment": ([_P(MlpProblem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                             ctypes.c_void_p, _P(MlpAdjIO), ctypes.c_int32, ctypes.c_int32,
+                             ctypes.c_void_p], ctypes.c_int),
+})
+_lib._SIGS.update(_SIGS)
+
+PRECISION = {"f32": _lib.F32, "bf16": _lib.BF16_F32}
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def pack_weights(W1, W2, W3, matmul, device):
+    """MFMA fragment packing of the three weight matrices (device kernel)."""
+    lib = _lib.load()
+    prec = PRECISION[matmul]
+    nbytes = ctypes.c_size_t(0)
+    _lib.check(lib.dfx_mlp_pack_bytes(prec, ctypes.byref(nbytes)), "dfx_mlp_pack_bytes")
+    packed = torch.empty(int(nbytes.value), dtype=torch.uint8, device=device)
+    ws = [w.detach().to(device=device, dtype=torch.float32).contiguous() for w in (W1, W2, W3)]
+    _lib.check(lib.dfx_mlp_pack(prec, *(ctypes.c_void_p(w.data_ptr()) for w in ws),
+                                ctypes.c_void_p(packed.data_ptr()), _stream()), "dfx_mlp_pack")
+    return packed, ws
+
+
+class MlpPlan:
+    """Prepared batched neural-ODE solve (launch() enqueues without host sync)."""
+
+    def __init__(self, vfo, t0, t1, dt0, y0, ts, ctrl, sub, max_steps, ckpt_every, device):
+        lib = _lib.load()
+        self.vfo = vfo
+        B = y0.shape[0]
+        self.y0 = y0.to(device=device, dtype=torch.float32).contiguous()
+        W1, b1, W2, b2, W3, b3 = vfo.params
+        self.packed, self.wf = pack_weights(W1, W2, W3, vfo.matmul, device)
+        self.b = [b.detach().to(device=device, dtype=torch.float32).contiguous() for b in (b1, b2, b3)]
+        n_ts = 0 if ts is None else int(ts.numel())
+        self.ts = (ts.to(device=device, dtype=torch.float64).contiguous() if ts is not None else None)
+        nan = float("nan")
+        self.prob = MlpProblem(
+            solver=_lib.TSIT5, precision=PRECISION[vfo.matmul], d=64, hidden=256, n_ts=n_ts,
+            save_t0=int(sub.t0), save_t1=int(sub.t1), max_steps=int(max_steps),
+            ckpt_every=int(ckpt_every), force_dtmin=int(ctrl.force_dtmin), batch=B,
+            t0=float(t0), t1=float(t1), dt0=float(dt0), rtol=ctrl.rtol, atol=ctrl.atol,
+            pcoeff=float(ctrl.pcoeff), icoeff=float(ctrl.icoeff), dcoeff=float(ctrl.dcoeff),
+            dtmin=nan if ctrl.dtmin is None else float(ctrl.dtmin),
+            dtmax=nan if ctrl.dtmax is None else float(ctrl.dtmax),
+            factormin=ctrl.factormin, factormax=ctrl.factormax, safety=ctrl.safety,
+            error_order=nan if ctrl.error_order is None else float(ctrl.error_order))
+        self.n_out = int(sub.t0) + n_ts + int(sub.t1)
+        self.ys = torch.empty((B, self.n_out, 64), dtype=torch.float32, device=device)
+        self.ts_out = torch.empty((B, self.n_out), dtype=torch.float64, device=device)
+        self.stats = torch.empty((B, 3), dtype=torch.int32, device=device)
+        self.result = torch.empty((B,), dtype=torch.int32, device=device)
+        if ckpt_every > 0:
+            n_ck = lib.dfx_mlp_n_ckpt(ctypes.byref(self.prob))
+            self.grid = torch.empty((B, int(max_steps), 2), dtype=torch.float64, device=device)
+            self.ckpt = torch.empty((B, n_ck, 64), dtype=torch.float32, device=device)
+        else:
+            self.grid = self.ckpt = None
+        io = MlpIO()
+        io.y0 = self.y0.data_ptr()
+        io.ts = self.ts.data_ptr() if self.ts is not None else None
+        io.ys, io.ts_out = self.ys.data_ptr(), self.ts_out.data_ptr()
+        io.stats, io.result = self.stats.data_ptr(), self.result.data_ptr()
+        io.grid = self.grid.data_ptr() if self.grid is not None else None
+        io.ckpt = self.ckpt.data_ptr() if self.ckpt is not None else None
+        self.io = io
+
+    def launch(self):
+        lib = _lib.load()
+        rc = lib.dfx_mlp_solve(ctypes.byref(self.prob), ctypes.c_void_p(self.packed.data_ptr()),
+                               *(ctypes.c_void_p(b.data_ptr()) for b in self.b),
+                               ctypes.byref(self.io), _stream())
+        _lib.check(rc, "dfx_mlp_solve")
+
+
+class _Probe:
+    pass
+
+
+def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max_steps,
+                device, ckpt_every=0, allocate=True):
+    from ._solver import Tsit5
+    from ._step_size_controller import PIDController
+    if not isinstance(solver, Tsit5):
+        raise NotImplementedError("the neural-ODE kernel implements Tsit5")
+    if not isinstance(stepsize_controller, PIDController):
+        raise NotImplementedError("the neural-ODE kernel implements PIDController")
+    c = stepsize_controller
+    if float(c.pcoeff) != 0.0 or float(c.dcoeff) != 0.0:
+        raise NotImplementedError("the neural-ODE kernel implements the integral controller "
+                                  "(pcoeff = dcoeff = 0)")
+    if dt0 is None:
+        raise NotImplementedError("pass dt0 for the neural-ODE kernel")
+    sub = saveat.subs
+    if sub is None or sub.steps or saveat.dense:
+        raise NotImplementedError("the neural-ODE kernel saves SaveAt(t0, t1, ts)")
+    y0 = torch.as_tensor(y0)
+    batched = y0.ndim == 2
+    y0b = y0 if batched else y0.reshape(1, -1)
+    ts = None
+    if sub.ts is not None:
+        ts = torch.as_tensor(sub.ts, dtype=torch.float64).reshape(-1)
+        t0f, t1f = float(t0), float(t1)
+        dirn = 1.0 if t0f < t1f else -1.0
+        tn = ts.numpy() * dirn
+        if len(tn) > 1 and (tn[1:] < tn[:-1]).any():
+            raise RuntimeError("saveat.ts must be increasing or decreasing.")
+        if ((tn > t1f * dirn) | (tn < t0f * dirn)).any():
+            raise RuntimeError("saveat.ts must lie between t0 and t1.")
+    if not allocate:
+        probe = _Probe()
+        probe.ts_host = ts
+        return probe
+    plan = MlpPlan(terms.vector_field, t0, t1, dt0, y0b, ts, c, sub, max_steps, ckpt_every, device)
+    plan.batched = batched
+    return plan
+
+
+_ = (math, np)
+
+
+def _solution(plan, t0, t1, max_steps, throw, ys):
+    from ._solution import RESULTS, Solution
+    st = plan.stats
+    stats = {"num_steps": st[:, 0], "num_accepted_steps": st[:, 1],
+             "num_rejected_steps": st[:, 2], "max_steps": max_steps}
+    ts, result = plan.ts_out, plan.result
+    if not plan.batched:
+        ys, ts, result = ys[0], ts[0], result[0]
+        stats = {k: (v[0] if isinstance(v, torch.Tensor) else v) for k, v in stats.items()}
+    if throw:
+        bad = plan.result != 0
+        if bool(bad.any()):
+            raise RuntimeError(RESULTS[int(plan.result[bad][0].item())])
+    return Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=result)
+
+
+class _Params:
+    """The parameter set MlpPlan reads (an MLPField or the detached autograd inputs)."""
+
+    def __init__(self, params, matmul):
+        self.params = params
+        self.matmul = matmul
+
+
+def backward_mlp(plan, ys_bar):
+    """Gradients (y0, W1, b1, W2, b2, W3, b3) of sum(ys * ys_bar) through the recorded solve."""
+    lib = _lib.load()
+    dev = plan.ys.device
+    B = plan.ys.shape[0]
+    p = plan.prob
+    nacc = plan.stats[:, 1]
+    n_max = int(nacc.max().item()) if B else 0
+    ys_bar = ys_bar.to(device=dev, dtype=torch.float32).contiguous()
+    n_ts = p.n_ts
+    ybar = (ys_bar[:, -1, :].clone() if p.save_t1 else torch.zeros((B, 64), dtype=torch.float32, device=dev))
+    dw = torch.zeros(98880, dtype=torch.float32, device=dev)
+    C = p.ckpt_every
+    if n_max > 0:
+        Cs = min(C, n_max)
+        zs_b, rows_b, part_b = (ctypes.c_size_t(0) for _ in range(3))
+        _lib.check(lib.dfx_mlp_adj_workspace(ctypes.byref(p), Cs, ctypes.byref(zs_b),
+                                             ctypes.byref(rows_b), ctypes.byref(part_b)),
+                   "dfx_mlp_adj_workspace")
+        zs = torch.empty(int(zs_b.value) // 4, dtype=torch.float32, device=dev)
+        rows = torch.zeros(int(rows_b.value), dtype=torch.uint8, device=dev)
+        part = torch.empty(int(part_b.value) // 4, dtype=torch.float32, device=dev)
+        io = MlpAdjIO(grid=plan.grid.data_ptr(), ckpt=plan.ckpt.data_ptr(),
+                      stats=plan.stats.data_ptr(),
+                      ts=plan.ts.data_ptr() if plan.ts is not None else None,
+                      ys_bar=ys_bar.data_ptr(), ybar=ybar.data_ptr(), zs=zs.data_ptr(),
+                      rows=rows.data_ptr(), partial=part.data_ptr(), dw=dw.data_ptr())
+        nseg = (n_max + C - 1) // C
+        for s in reversed(range(nseg)):
+            _lib.check(lib.dfx_mlp_adj_segment(ctypes.byref(p), ctypes.c_void_p(plan.packed.data_ptr()),
+                                               *(ctypes.c_void_p(b.data_ptr()) for b in plan.b),
+                                               ctypes.byref(io), s, Cs, _stream()),
+                       "dfx_mlp_adj_segment")
+    y0_bar = ybar
+    if p.save_t0:
+        y0_bar = y0_bar + ys_bar[:, 0, :]
+    if float(p.t0) == float(p.t1) and n_ts > 0:
+        off = int(p.save_t0)
+        y0_bar = y0_bar + ys_bar[:, off:off + n_ts, :].sum(1)
+    o = 0
+    shapes = [(256, 64), (256, 256), (64, 256)]
+    Ws = []
+    for s_ in shapes:
+        n = s_[0] * s_[1]
+        Ws.append(dw[o:o + n].view(*s_))
+        o += n
+    db1, db2, db3 = dw[o:o + 256], dw[o + 256:o + 512], dw[o + 512:o + 576]
+    return y0_bar, Ws[0], db1, Ws[1], db2, Ws[2], db3
+
+
+class _MlpSolve(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, y0, W1, b1, W2, b2, W3, b3, cfg):
+        params = _Params([t.detach() for t in (W1, b1, W2, b2, W3, b3)], cfg["matmul"])
+        need = any(ctx.needs_input_grad[:7])
+        plan = MlpPlan(params, cfg["t0"], cfg["t1"], cfg["dt0"], y0.detach(), cfg["ts"],
+                       cfg["ctrl"], cfg["sub"], cfg["max_steps"],
+                       cfg["ckpt_every"] if need else 0, cfg["device"])
+        plan.launch()
+        ctx.plan = plan
+        ctx.dtypes = [t.dtype for t in (y0, W1, b1, W2, b2, W3, b3)]
+        ctx.mark_non_differentiable(plan.stats, plan.result, plan.ts_out)
+        return plan.ys, plan.stats, plan.result, plan.ts_out
+
+    @staticmethod
+    def backward(ctx, ys_bar, *_):
+        grads = backward_mlp(ctx.plan, ys_bar)
+        out = [g.to(dt) if need else None
+               for g, dt, need in zip(grads, ctx.dtypes, ctx.needs_input_grad[:7])]
+        return (*out, None)
+
+
+def checkpoint_every(adjoint, max_steps):
+    """Steps between stored states: ceil(max_steps / checkpoints), or sqrt(max_steps) when
+    RecursiveCheckpointAdjoint(checkpoints=None)."""
+    k = getattr(adjoint, "checkpoints", None)
+    if k is None:
+        return max(1, int(math.ceil(math.sqrt(max_steps))))
+    return max(1, int(math.ceil(max_steps / int(k))))
+
+
+def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, adjoint,
+                    max_steps, throw, device):
+    vfo = terms.vector_field
+    y0t = torch.as_tensor(y0)
+    needs_grad = torch.is_grad_enabled() and (
+        y0t.requires_grad or any(p.requires_grad for p in vfo.params))
+    if not needs_grad:
+        plan = prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
+                           max_steps, device)
+        plan.launch()
+        return _solution(plan, t0, t1, max_steps, throw, plan.ys)
+    from ._adjoint import RecursiveCheckpointAdjoint
+    if not isinstance(adjoint, RecursiveCheckpointAdjoint):
+        raise NotImplementedError("gradients are provided through RecursiveCheckpointAdjoint")
+    # validation and shapes via a throwaway-free preparation of the configuration
+    probe = prepare_mlp(terms, solver, t0, t1, dt0, y0t.detach(), saveat, stepsize_controller,
+                        max_steps, device, ckpt_every=0, allocate=False)
+    cfg = dict(matmul=vfo.matmul, t0=t0, t1=t1, dt0=dt0, ts=probe.ts_host, ctrl=stepsize_controller,
+               sub=saveat.subs, max_steps=max_steps, ckpt_every=checkpoint_every(adjoint, max_steps),
+               device=device)
+    batched = y0t.ndim == 2
+    y0b = y0t if batched else y0t.reshape(1, -1)
+    ys, stats, result, ts_out = _MlpSolve.apply(y0b.to(device), *vfo.params, cfg)
+
+    class _P:
+        pass
+    pl = _P()
+    pl.stats, pl.result, pl.ts_out, pl.batched = stats, result, ts_out, batched
+    return _solution(pl, t0, t1, max_steps, throw, ys)
+
+
+def mlp_eval(vfo, y):
+    """Evaluate the registered MLP vector field on the GPU for states y [R, 64]."""
+    lib = _lib.load()
+    dev = y.device
+    W1, b1, W2, b2, W3, b3 = vfo.params
+    packed, _ = pack_weights(W1, W2, W3, vfo.matmul, dev)
+    bs = [b.detach().to(device=dev, dtype=torch.float32).contiguous() for b in (b1, b2, b3)]
+    y = y.to(torch.float32).contiguous()
+    f = torch.empty_like(y)
+    _lib.check(lib.dfx_mlp_eval(PRECISION[vfo.matmul], ctypes.c_void_p(packed.data_ptr()),
+                                *(ctypes.c_void_p(b.data_ptr()) for b in bs),
+                                ctypes.c_void_p(y.data_ptr()), ctypes.c_void_p(f.data_ptr()),
+                                y.shape[0], _stream()), "dfx_mlp_eval")
+    return f

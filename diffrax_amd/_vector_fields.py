@@ -142,3 +142,33 @@ class DiagonalNoise(RegisteredVectorField):
 
     def __call__(self, t, y, args):
         return args[2] * torch.ones_like(y)
+
+
+class MLPField(RegisteredVectorField):
+    """Neural vector field f(t, y) = tanh(W3 softplus(W2 softplus(W1 y + b1) + b2) + b3)
+    with d = 64, width 256, depth 2 -- what eqx.nn.MLP(64, 64, 256, 2, activation=softplus,
+    final_activation=tanh) computes.  W1 [256, 64], W2 [256, 256], W3 [64, 256] (torch
+    Linear layout: out x in).  ``matmul`` selects the device form: "f32" (float32 MFMA) or
+    "bf16" (bfloat16 MFMA operands, float32 accumulate); the RK algebra is float32 in both
+    and time / step control float64."""
+
+    vf_id = _lib.VF_MLP
+    name = "mlp"
+
+    def __init__(self, W1, b1, W2, b2, W3, b3, matmul="f32"):
+        if matmul not in ("f32", "bf16"):
+            raise ValueError("matmul must be 'f32' or 'bf16'")
+        self.params = [torch.as_tensor(p) for p in (W1, b1, W2, b2, W3, b3)]
+        shapes = [(256, 64), (256,), (256, 256), (256,), (64, 256), (64,)]
+        for p, s in zip(self.params, shapes):
+            if tuple(p.shape) != s:
+                raise ValueError(f"MLP parameter of shape {tuple(p.shape)}, expected {s}")
+        self.matmul = matmul
+
+    def supports(self, d):
+        return d == 64
+
+    def __call__(self, t, y, args=None):
+        W1, b1, W2, b2, W3, b3 = (p.to(y) for p in self.params)
+        sp = torch.nn.functional.softplus
+        return torch.tanh(sp(sp(y @ W1.T + b1) @ W2.T + b2) @ W3.T + b3)

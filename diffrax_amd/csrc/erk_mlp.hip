@@ -1,0 +1,409 @@
+// erk_mlp.hip -- persistent Runge-Kutta solve of the neural ODE dy/dt = MLP(y) (d = 64,
+// hidden 256) for batches of trajectories, and weight packing for the MFMA fragments.
+//
+// One 256-thread workgroup integrates a tile of 32 trajectories: every stage of every step
+// is one MLP evaluation of the whole tile on MFMA (mlp_tile.hpp), while time, step size,
+// error norm, accept/reject, saves and counters stay per trajectory (owner threads, float64
+// controller as in erk_common.hpp).  Reference behaviour: the same loop as the small-system
+// kernel (_integrate.py:302-885, _solver/runge_kutta.py:446-1203,
+// _step_size_controller/pid.py:394-567, _solver/tsit5.py:103-156) with the vector field an
+// eqx.nn.MLP(64, 64, width 256, depth 2, softplus, final tanh).
+//
+// Gradient recording: with ckpt_every = C > 0 the kernel also writes the accepted step grid
+// and the state at every C-th accepted step; the adjoint (mlp_adj.hip) replays from there.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include "../../include/diffrax_amd.h"
+#include "erk_small.hpp"
+#include "mlp_tile.hpp"
+
+namespace dfx {
+
+struct MlpArgs {
+  const float* y0;
+  const double* ts;
+  float* ys;
+  double* ts_out;
+  int32_t* stats;
+  int32_t* result;
+  double* grid;
+  float* ckpt;
+  int64_t B;
+  int32_t n_ts, n_out, save_t0, save_t1, max_steps, ckpt_every, n_ckpt, has_dt0;
+  double t0, t1, dt0, error_order;
+  PidParams pid;
+};
+
+template <class P, class Tab, bool Grad>
+__global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a, const MlpWeights<P> W) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
+  constexpr int S = Tab::S;
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int orow = tid >> 3;          // trajectory within the tile
+  const int c0 = (tid & 7) * 8;       // owner's first component
+  const int gbase = lane & ~7;        // first lane of this trajectory's owner group
+  const int64_t traj = (int64_t)blockIdx.x * MLP_TILE + orow;
+  const bool valid = traj < a.B;
+
+  TabRegs<Tab> tr;
+  tr.load();
+
+  // ---- set-up (diffeqsolve / PIDController.init) ----
+  float y[8], fsal[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) y[j] = valid ? a.y0[traj * MLP_D + c0 + j] : 0.0f;
+  const int dir = (a.t0 < a.t1) ? 1 : -1;
+  const double T0 = a.t0 * dir, T1 = a.t1 * dir;
+  double dt0v = a.dt0 * dir;
+  if (a.pid.has_dtmax) dt0v = jmin(dt0v, a.pid.dtmax);
+  if (a.pid.has_dtmin) dt0v = jmax(dt0v, a.pid.dtmin);
+  double tnext = jmin(T0 + dt0v, T1);
+  double floor_t1 = T1;
+  for (int r = 0; r < 100; ++r) floor_t1 = prevbefore(floor_t1);
+  double tprev = T0;
+  PidState pid;
+  pid.inv = 1.0; pid.pinv = 1.0; pid.at_dtmin = 0;
+  int32_t num_steps = 0, num_acc = 0, num_rej = 0, result = DFX_SUCCESSFUL;
+  int32_t save_index = 0, ts_index = 0;
+  double next_ts = (a.n_ts > 0) ? a.ts[0] * dir : __builtin_inf();
+  float* ys_row = valid ? a.ys + traj * (int64_t)a.n_out * MLP_D : nullptr;
+  double* tso_row = valid ? a.ts_out + traj * (int64_t)a.n_out : nullptr;
+  auto write_save = [&](double tn, const float (&v)[8]) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ys_row[save_index * MLP_D + c0 + j] = v[j];
+    if ((tid & 7) == 0) tso_row[save_index] = tn * dir;
+    ++save_index;
+  };
+  if (valid && a.save_t0) write_save(T0, y);
+
+  // ---- first stage of the first step (FSAL seed): f(t0, y0) ----
+#pragma unroll
+  for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, y[j]);
+  __syncthreads();
+  mlp_forward<P>(W, L, wave, lane, orow, c0, fsal);
+
+  auto running = [&]() {
+    return valid && tprev < T1 && result == DFX_SUCCESSFUL && num_steps < a.max_steps;
+  };
+
+  while (__syncthreads_or(running())) {
+    const bool run = running();
+    const double ts0 = tprev, ts1 = tnext;
+    const double dt = ts1 - ts0;
+    const float control = (float)((double)dir * dt);
+    float k[S][8];
+    float z[8], f[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) k[0][j] = control * fsal[j];
+    static_for<1, S>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float kc[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
+        z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, 0.0f);
+        store_input<P>(L, orow, c0 + j, z[j]);
+      }
+      __syncthreads();
+      mlp_forward<P>(W, L, wave, lane, orow, c0, f);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) k[i][j] = control * f[j];
+    });
+    // y1 = last stage (SSAL); embedded error
+    float yerr[8];
+    double part = 0.0;
+    bool anynan = false;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      float kc[S];
+#pragma unroll
+      for (int m = 0; m < S; ++m) kc[m] = k[m][j];
+      yerr[j] = comb<Tab, 2, 0, 0, S, false>(tr.berr, kc, 0.0f);
+      anynan = anynan || (z[j] != z[j]);
+    }
+    // group reduction in owner order (identical in all 8 owners)
+    unsigned long long nanmask = __ballot(anynan);
+    const bool group_nan = ((nanmask >> gbase) & 0xffull) != 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const double y1c = group_nan ? (double)y[j] : (double)z[j];
+      const double yy = jmax(fabs((double)y[j]), fabs(y1c));
+      double e = (double)yerr[j];
+      if (e != e) e = __builtin_inf();
+      e = e / (a.pid.atol + yy * a.pid.rtol);
+      part += e * e;
+    }
+    double ssum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ssum += __shfl(part, gbase + q);
+    const double scaled = sqrt(ssum) / 8.0;  // rms over d = 64 components
+    double ntprev, ntnext;
+    int code = 0;
+    PidState npid = pid;
+    bool keep = pid_adapt<false>(a.pid, scaled, ts0, ts1, npid, ntprev, ntnext, code);
+    ntprev = jmin(ntprev, T1);
+    if (ntnext > floor_t1) ntnext = keep ? T1 : ntprev + 0.5 * (T1 - ntprev);
+    if (run) {
+      pid = npid;
+      ++num_steps;
+      if (keep) ++num_acc; else ++num_rej;
+      if (result == DFX_SUCCESSFUL) result = code;
+      if (keep) {
+        if constexpr (Grad) {
+          const int n = num_acc - 1;
+          if ((tid & 7) == 0) {
+            a.grid[(traj * a.max_steps + n) * 2 + 0] = ts0;
+            a.grid[(traj * a.max_steps + n) * 2 + 1] = ts1;
+          }
+          if (n % a.ckpt_every == 0 && n / a.ckpt_every < a.n_ckpt) {
+            float* c = a.ckpt + (traj * a.n_ckpt + n / a.ckpt_every) * MLP_D + c0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) c[j] = y[j];
+          }
+        }
+        while (next_ts <= ts1) {
+          const double th = linear_rescale(ts0, next_ts, ts1);
+          double bb[7];
+          tsit5_weights(th, bb);
+          float v[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            float acc = (float)bb[0] * k[0][j];
+#pragma unroll
+            for (int m = 1; m < 7; ++m) acc = acc + (float)bb[m] * k[m][j];
+            v[j] = y[j] + acc;
+          }
+          write_save(next_ts, v);
+          ++ts_index;
+          next_ts = (ts_index < a.n_ts) ? a.ts[ts_index] * dir : __builtin_inf();
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) { y[j] = z[j]; fsal[j] = f[j]; }
+      }
+      tprev = ntprev;
+      tnext = ntnext;
+    }
+  }
+  if (!valid) return;
+  if (T0 == T1) {
+    for (int r = 0; r < a.n_ts; ++r) write_save(T0, y);
+  }
+  if (a.save_t1) write_save(tprev, y);
+  if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
+  for (int s = save_index; s < a.n_out; ++s) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ys_row[s * MLP_D + c0 + j] = __builtin_inff();
+    if ((tid & 7) == 0) tso_row[s] = (double)dir * __builtin_inf();
+  }
+  if ((tid & 7) == 0) {
+    a.stats[traj * 3 + 0] = num_steps;
+    a.stats[traj * 3 + 1] = num_acc;
+    a.stats[traj * 3 + 2] = num_rej;
+    a.result[traj] = result;
+  }
+}
+
+// f(y) for R independent rows (the registered MLP vector field on its own; ODETerm.vf).
+template <class P>
+__global__ void __launch_bounds__(MLP_THREADS, 1) mlp_eval_kernel(const float* __restrict__ Y,
+                                                                  float* __restrict__ F, int64_t R,
+                                                                  const MlpWeights<P> W) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int orow = tid >> 3, c0 = (tid & 7) * 8;
+  const int64_t row = (int64_t)blockIdx.x * MLP_TILE + orow;
+  const bool valid = row < R;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, valid ? Y[row * MLP_D + c0 + j] : 0.0f);
+  __syncthreads();
+  float f[8];
+  mlp_forward<P>(W, L, wave, lane, orow, c0, f);
+  if (valid) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) F[row * MLP_D + c0 + j] = f[j];
+  }
+}
+
+// Pack a row-major float32 matrix into MFMA B fragments.  forward: B[k][n] = W[n][k]
+// (W is [N][K]); otherwise B[k][n] = W[k][n] (W is [K][N]).
+template <class P>
+__global__ void mlp_pack_kernel(const float* __restrict__ W, typename P::act_t* __restrict__ out,
+                                int N, int K, int forward) {
+  const int64_t total = (int64_t)N * K;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int NKS = K / P::KSTEP;
+  const int j = (int)(e % P::KA);
+  const int64_t fr = e / P::KA;
+  const int lane = (int)(fr % 64);
+  const int64_t f2 = fr / 64;
+  const int ks = (int)(f2 % NKS);
+  const int nt = (int)(f2 / NKS);
+  const int n = nt * 32 + (lane & 31);
+  const int k = ks * P::KSTEP + (lane >> 5) * P::KA + j;
+  const float v = forward ? W[(int64_t)n * K + k] : W[(int64_t)k * N + n];
+  out[e] = P::to_act(v);
+}
+
+}  // namespace dfx
+
+using namespace dfx;
+
+namespace {
+
+size_t act_bytes(int32_t precision) { return precision == DFX_BF16_F32 ? 2 : 4; }
+
+// element offsets of the six packed matrices inside one packed buffer
+void pack_offsets(int64_t (&off)[7]) {
+  const int64_t D = MLP_D, H = MLP_H;
+  const int64_t sz[6] = {H * D, H * H, D * H, D * H, H * H, H * D};
+  off[0] = 0;
+  for (int i = 0; i < 6; ++i) off[i + 1] = off[i] + sz[i];
+}
+
+int mlp_check(const dfx_mlp_problem* p) {
+  if (!p) return DFX_E_ARG;
+  if (p->d != MLP_D || p->hidden != MLP_H) return DFX_E_ARG;
+  if (p->precision != DFX_F32 && p->precision != DFX_BF16_F32) return DFX_E_ARG;
+  if (p->solver != DFX_TSIT5) return DFX_E_ARG;  // dense output implemented for Tsit5
+  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0 || p->ckpt_every < 0) return DFX_E_ARG;
+  return DFX_OK;
+}
+
+template <class P>
+MlpWeights<P> weights_from(const void* packed, const float* b1, const float* b2, const float* b3) {
+  int64_t off[7];
+  pack_offsets(off);
+  const typename P::act_t* base = reinterpret_cast<const typename P::act_t*>(packed);
+  MlpWeights<P> W;
+  W.w1 = base + off[0]; W.w2 = base + off[1]; W.w3 = base + off[2];
+  W.w1t = base + off[3]; W.w2t = base + off[4]; W.w3t = base + off[5];
+  W.b1 = b1; W.b2 = b2; W.b3 = b3;
+  return W;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dfx_mlp_pack_bytes(int32_t precision, size_t* bytes) {
+  if (!bytes || (precision != DFX_F32 && precision != DFX_BF16_F32)) return DFX_E_ARG;
+  int64_t off[7];
+  pack_offsets(off);
+  *bytes = (size_t)off[6] * act_bytes(precision);
+  return DFX_OK;
+}
+
+int dfx_mlp_pack(int32_t precision, const float* w1, const float* w2, const float* w3, void* packed,
+                 void* stream) {
+  if (!w1 || !w2 || !w3 || !packed) return DFX_E_ARG;
+  if (precision != DFX_F32 && precision != DFX_BF16_F32) return DFX_E_ARG;
+  int64_t off[7];
+  pack_offsets(off);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  struct Job { const float* W; int N, K, fwd; };
+  const Job jobs[6] = {{w1, MLP_H, MLP_D, 1}, {w2, MLP_H, MLP_H, 1}, {w3, MLP_D, MLP_H, 1},
+                       {w1, MLP_D, MLP_H, 0}, {w2, MLP_H, MLP_H, 0}, {w3, MLP_H, MLP_D, 0}};
+  for (int i = 0; i < 6; ++i) {
+    const int64_t n = (int64_t)jobs[i].N * jobs[i].K;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    if (precision == DFX_BF16_F32) {
+      __bf16* o = reinterpret_cast<__bf16*>(packed) + off[i];
+      hipLaunchKernelGGL(mlp_pack_kernel<PolicyBF16>, dim3(nb), dim3(256), 0, s, jobs[i].W, o,
+                         jobs[i].N, jobs[i].K, jobs[i].fwd);
+    } else {
+      float* o = reinterpret_cast<float*>(packed) + off[i];
+      hipLaunchKernelGGL(mlp_pack_kernel<PolicyF32>, dim3(nb), dim3(256), 0, s, jobs[i].W, o,
+                         jobs[i].N, jobs[i].K, jobs[i].fwd);
+    }
+  }
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+int dfx_mlp_eval(int32_t precision, const void* packed, const float* b1, const float* b2,
+                 const float* b3, const float* y, float* f, int64_t rows, void* stream) {
+  if (precision != DFX_F32 && precision != DFX_BF16_F32) return DFX_E_ARG;
+  if (!packed || !b1 || !b2 || !b3 || rows < 0) return DFX_E_ARG;
+  if (rows == 0) return DFX_OK;
+  if (!y || !f) return DFX_E_ARG;
+  const unsigned nb = (unsigned)((rows + MLP_TILE - 1) / MLP_TILE);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (precision == DFX_BF16_F32) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_eval_kernel<PolicyBF16>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<PolicyBF16>));
+    hipLaunchKernelGGL(mlp_eval_kernel<PolicyBF16>, dim3(nb), dim3(MLP_THREADS),
+                       sizeof(MlpLds<PolicyBF16>), s, y, f, rows,
+                       weights_from<PolicyBF16>(packed, b1, b2, b3));
+  } else {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_eval_kernel<PolicyF32>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<PolicyF32>));
+    hipLaunchKernelGGL(mlp_eval_kernel<PolicyF32>, dim3(nb), dim3(MLP_THREADS),
+                       sizeof(MlpLds<PolicyF32>), s, y, f, rows,
+                       weights_from<PolicyF32>(packed, b1, b2, b3));
+  }
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+int dfx_mlp_n_ckpt(const dfx_mlp_problem* p) {
+  if (!p || p->ckpt_every <= 0) return 0;
+  return (p->max_steps + p->ckpt_every - 1) / p->ckpt_every;
+}
+
+int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1, const float* b2,
+                  const float* b3, const dfx_mlp_io* io, void* stream) {
+  int rc = mlp_check(p);
+  if (rc != DFX_OK) return rc;
+  if (!packed || !b1 || !b2 || !b3 || !io) return DFX_E_ARG;
+  if (p->batch == 0) return DFX_OK;
+  const int n_out = (p->save_t0 ? 1 : 0) + p->n_ts + (p->save_t1 ? 1 : 0);
+  if (!io->y0 || !io->stats || !io->result) return DFX_E_ARG;
+  if (n_out > 0 && (!io->ys || !io->ts_out)) return DFX_E_ARG;
+  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
+  if (p->ckpt_every > 0 && (!io->grid || !io->ckpt)) return DFX_E_ARG;
+  const double error_order = isnan(p->error_order) ? 5.0 : p->error_order;
+  MlpArgs a;
+  memset(&a, 0, sizeof(a));
+  a.y0 = io->y0; a.ts = io->ts; a.ys = io->ys; a.ts_out = io->ts_out; a.stats = io->stats;
+  a.result = io->result; a.grid = io->grid; a.ckpt = io->ckpt;
+  a.B = p->batch; a.n_ts = p->n_ts; a.n_out = n_out; a.save_t0 = p->save_t0;
+  a.save_t1 = p->save_t1; a.max_steps = p->max_steps; a.ckpt_every = p->ckpt_every;
+  a.n_ckpt = dfx_mlp_n_ckpt(p);
+  a.has_dt0 = !isnan(p->dt0);
+  a.t0 = p->t0; a.t1 = p->t1; a.dt0 = p->dt0; a.error_order = error_order;
+  PidParams& q = a.pid;
+  q.rtol = p->rtol; q.atol = p->atol;
+  q.coeff1 = (p->icoeff + p->pcoeff + p->dcoeff) / error_order;
+  q.coeff2 = -(p->pcoeff + 2.0 * p->dcoeff) / error_order;
+  q.coeff3 = p->dcoeff / error_order;
+  q.dtmin = p->dtmin; q.dtmax = p->dtmax;
+  q.has_dtmin = !isnan(p->dtmin); q.has_dtmax = !isnan(p->dtmax);
+  q.force_dtmin = p->force_dtmin; q.factormin = p->factormin; q.factormax = p->factormax;
+  q.safety = p->safety;
+  if (!pid_is_integral(q)) return DFX_E_ARG;  // the neural-ODE kernel implements the I controller
+  if (!a.has_dt0) return DFX_E_ARG;           // dt0 must be given
+  const unsigned nblocks = (unsigned)((p->batch + MLP_TILE - 1) / MLP_TILE);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+#define DFX_MLP_LAUNCH(POL, G)                                                               \
+  do {                                                                                       \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&erk_mlp_kernel<POL, Tsit5Tab, G>), \
+                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<POL>)); \
+    hipLaunchKernelGGL((erk_mlp_kernel<POL, Tsit5Tab, G>), dim3(nblocks), dim3(MLP_THREADS),   \
+                       sizeof(MlpLds<POL>), s, a, weights_from<POL>(packed, b1, b2, b3));      \
+  } while (0)
+  const bool g = p->ckpt_every > 0;
+  if (p->precision == DFX_BF16_F32) {
+    if (g) DFX_MLP_LAUNCH(PolicyBF16, true); else DFX_MLP_LAUNCH(PolicyBF16, false);
+  } else {
+    if (g) DFX_MLP_LAUNCH(PolicyF32, true); else DFX_MLP_LAUNCH(PolicyF32, false);
+  }
+#undef DFX_MLP_LAUNCH
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+}  // extern "C"

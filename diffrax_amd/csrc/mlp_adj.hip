@@ -1,0 +1,471 @@
+// mlp_adj.hip -- discrete adjoint of the neural-ODE solve (erk_mlp.hip): gradients of a
+// loss on the saved states with respect to y0 and all MLP weights and biases.
+//
+// Semantics (RecursiveCheckpointAdjoint, _adjoint.py:264-312): reverse mode through the
+// accepted step sequence with the step sizes held fixed -- the controller's factor is
+// stop_gradient (_step_size_controller/pid.py:498-530) and rejected steps leave y and the
+// FSAL state unchanged (_integrate.py:422-426), so they carry no gradient.  The forward
+// pass (gradient mode) recorded the accepted (t_n, t_{n+1}) and the state at every C-th
+// accepted step.  The backward walks segments of C steps from the last one; for each
+// segment and each tile of 32 trajectories this kernel
+//   1. recomputes the segment forward from its checkpoint, keeping the stage inputs z_i of
+//      every step (identical arithmetic to the forward kernel);
+//   2. runs the exact reverse sweep: cotangents of the SaveAt(ts) values enter through the
+//      Tsit5 interpolant (k_i gets b_i(theta) ybar_s), the SSAL output y_{n+1} = z_6, and
+//      each stage k_i = dt f(z_i) is pulled back through the MLP on MFMA
+//      (tanh' = 1 - f^2, softplus' = 1 - exp(-softplus));
+//   3. stages the per-row factors (delta_l, layer input) of every stage in blocked row
+//      buffers.
+// A split-K MFMA GEMM then forms dW_l = sum_rows delta_l^T input_l for the segment, and a
+// fixed-order reduction adds it to the float32 accumulators (deterministic result).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include "../../include/diffrax_amd.h"
+#include "erk_small.hpp"
+#include "mlp_tile.hpp"
+
+namespace dfx {
+
+constexpr int ADJ_CHUNK = 4096;  // rows per split-K chunk of the weight-gradient GEMM
+
+// Six row buffers: D1 (width 256), Z (64), D2 (256), H1 (256), D3 (64), H2 (256).
+
+template <class P>
+struct RowBufs {
+  typename P::act_t* b[6];
+};
+
+struct AdjArgs {
+  const double* grid;
+  const float* ckpt;
+  const int32_t* stats;
+  const double* ts;
+  const float* ys_bar;
+  float* ybar;
+  float* zs;
+  int64_t B, B_pad;
+  int32_t max_steps, n_ckpt, C, Cs, seg, n_ts, n_out, save_t0;
+  double t0, t1;
+};
+
+// Copy 32 tile rows of an LDS buffer [32][ld] into a blocked global row buffer of width
+// Wd: element (row, col) at ((row / KA) * Wd + col) * KA + row % KA.
+template <class P>
+__device__ __forceinline__ void rows_out(const typename P::act_t* lds, int ld, int Wd,
+                                         typename P::act_t* dst, int64_t row0) {
+  constexpr int KA = P::KA;
+  const int pairs = (MLP_TILE / KA) * Wd;
+  for (int e = threadIdx.x; e < pairs; e += MLP_THREADS) {
+    const int rb = e / Wd, col = e - rb * Wd;
+    typename P::act_t* o = dst + ((row0 / KA + rb) * Wd + col) * KA;
+#pragma unroll
+    for (int j = 0; j < KA; ++j) o[j] = lds[(rb * KA + j) * ld + col];
+  }
+}
+
+template <class P>
+__global__ void __launch_bounds__(MLP_THREADS, 1)
+mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> R) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
+  constexpr bool F = P::fast_math;
+  using Tab = Tsit5Tab;
+  constexpr int S = Tab::S;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int orow = tid >> 3, c0 = (tid & 7) * 8;
+  const int tile = blockIdx.x;
+  const int64_t traj = (int64_t)tile * MLP_TILE + orow;
+  const bool valid = traj < a.B;
+  const int nacc = valid ? a.stats[traj * 3 + 1] : 0;
+  const int n0 = a.seg * a.C;
+  int steps_here = nacc - n0;
+  steps_here = steps_here < 0 ? 0 : (steps_here > a.Cs ? a.Cs : steps_here);
+  const int dir = (a.t0 < a.t1) ? 1 : -1;
+  TabRegs<Tab> tr;
+  tr.load();
+  float* zs_tile = a.zs + (int64_t)tile * a.Cs * S * MLP_TILE * MLP_D;
+  auto zslot = [&](int q, int i) { return zs_tile + (((int64_t)q * S + i) * MLP_TILE + orow) * MLP_D + c0; };
+
+  // ---- 1. recompute the segment's stage inputs ----
+  float y[8], fl[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) y[j] = (steps_here > 0) ? a.ckpt[(traj * a.n_ckpt + a.seg) * MLP_D + c0 + j] : 0.0f;
+  for (int q = 0; q < a.Cs; ++q) {
+    const bool act = q < steps_here;
+    const int n = n0 + q;
+    const double ta = act ? a.grid[(traj * a.max_steps + n) * 2 + 0] : 0.0;
+    const double tb = act ? a.grid[(traj * a.max_steps + n) * 2 + 1] : 0.0;
+    const float control = (float)((double)dir * (tb - ta));
+    float k[S][8], z[8], f[8];
+    if (q == 0) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, y[j]);
+      __syncthreads();
+      mlp_forward<P>(W, L, wave, lane, orow, c0, fl);
+    }
+    {
+      float* zp = zslot(q, 0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) { zp[j] = y[j]; k[0][j] = control * fl[j]; }
+    }
+    static_for<1, S>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      float* zp = zslot(q, i);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float kc[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
+        z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, 0.0f);
+        zp[j] = z[j];
+        store_input<P>(L, orow, c0 + j, z[j]);
+      }
+      __syncthreads();
+      mlp_forward<P>(W, L, wave, lane, orow, c0, f);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) k[i][j] = control * f[j];
+    });
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (act) y[j] = z[j];
+      fl[j] = f[j];
+    }
+  }
+
+  // ---- 2. reverse sweep ----
+  float yb[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) yb[j] = valid ? a.ybar[traj * MLP_D + c0 + j] : 0.0f;
+  for (int q = a.Cs - 1; q >= 0; --q) {
+    const bool act = q < steps_here;
+    const int n = n0 + q;
+    const double ta = act ? a.grid[(traj * a.max_steps + n) * 2 + 0] : 0.0;
+    const double tb = act ? a.grid[(traj * a.max_steps + n) * 2 + 1] : 0.0;
+    const float control = (float)((double)dir * (tb - ta));
+    float kb[S][8], ybn[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      ybn[j] = yb[j];
+#pragma unroll
+      for (int m = 0; m < S; ++m) kb[m][j] = 0.0f;
+    }
+    if (act) {
+      // y_{n+1} = z_6 = y_n + sum_j a_6j k_j
+#pragma unroll
+      for (int m = 0; m < S - 1; ++m) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) kb[m][j] += (float)tr.a[S - 1][m] * yb[j];
+      }
+      // SaveAt(ts) values interpolated inside this step
+      int jlo = 0, jhi = 0;
+      for (int s = 0; s < a.n_ts; ++s) {
+        const double tsn = a.ts[s] * dir;
+        if (n > 0 && tsn <= ta) jlo = s + 1;
+        if (tsn <= tb) jhi = s + 1;
+      }
+      for (int s = jlo; s < jhi; ++s) {
+        const double th = linear_rescale(ta, a.ts[s] * dir, tb);
+        double bb[7];
+        tsit5_weights(th, bb);
+        const float* g = a.ys_bar + (traj * a.n_out + (a.save_t0 ? 1 : 0) + s) * MLP_D + c0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const float gv = g[j];
+          ybn[j] += gv;
+#pragma unroll
+          for (int m = 0; m < 7; ++m) kb[m][j] += (float)bb[m] * gv;
+        }
+      }
+    }
+    const int64_t rowbase_q = (int64_t)q * S;
+    // stages in reverse
+    static_for<0, S>([&](auto ic) {
+      constexpr int i = S - 1 - decltype(ic)::value;
+      const int64_t row0 = (rowbase_q + i) * a.B_pad + (int64_t)tile * MLP_TILE;
+      const float* zp = zslot(q, i);
+      float fb[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        fb[j] = act ? control * kb[i][j] : 0.0f;
+        store_input<P>(L, orow, c0 + j, act ? zp[j] : 0.0f);
+      }
+      __syncthreads();
+      float f[8];
+      mlp_forward<P>(W, L, wave, lane, orow, c0, f);
+      // delta3 = fbar * (1 - f^2)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) L.d3[orow * L.LDX + c0 + j] = P::to_act(fb[j] * (1.0f - f[j] * f[j]));
+      __syncthreads();
+      rows_out<P>(L.d3, L.LDX, MLP_D, R.b[4], row0);
+      rows_out<P>(L.h2, L.LDH, MLP_H, R.b[5], row0);
+      // h2bar = d3 . W3 ; delta2 = h2bar * softplus'(.)
+      floatx16_t acc[2];
+      wave_gemm<P, MLP_D / P::KSTEP, 2>(L.d3, L.LDX, W.w3t, 2 * wave, 0, MLP_D / P::KSTEP, acc, lane);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 64 * wave + 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = acc_row(lane, v);
+          const float h = (float)L.h2[r * L.LDH + col];
+          L.d[r * L.LDH + col] = P::to_act(acc[t][v] * dsoftplus_from_value<F>(h));
+        }
+      }
+      __syncthreads();
+      // h1bar = d2 . W2 ; delta1 = h1bar * softplus'(.), stored over h2 (no longer needed)
+      wave_gemm<P, MLP_H / P::KSTEP, 2>(L.d, L.LDH, W.w2t, 2 * wave, 0, MLP_H / P::KSTEP, acc, lane);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 64 * wave + 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = acc_row(lane, v);
+          const float h = (float)L.h1[r * L.LDH + col];
+          L.h2[r * L.LDH + col] = P::to_act(acc[t][v] * dsoftplus_from_value<F>(h));
+        }
+      }
+      __syncthreads();
+      rows_out<P>(L.d, L.LDH, MLP_H, R.b[2], row0);
+      rows_out<P>(L.h1, L.LDH, MLP_H, R.b[3], row0);
+      rows_out<P>(L.h2, L.LDH, MLP_H, R.b[0], row0);
+      rows_out<P>(L.x, L.LDX, MLP_D, R.b[1], row0);
+      // zbar = d1 . W1 (split-K over the 256 hidden units)
+      constexpr int KS1 = MLP_H / P::KSTEP;
+      wave_gemm<P, KS1, 2>(L.h2, L.LDH, W.w1t, 0, wave * (KS1 / 4), (wave + 1) * (KS1 / 4), acc, lane);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) L.part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
+      }
+      __syncthreads();
+      float zb[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int c = c0 + j;
+        float s = L.part[(0 * MLP_TILE + orow) * L.LDP + c];
+        s += L.part[(1 * MLP_TILE + orow) * L.LDP + c];
+        s += L.part[(2 * MLP_TILE + orow) * L.LDP + c];
+        s += L.part[(3 * MLP_TILE + orow) * L.LDP + c];
+        zb[j] = act ? s : 0.0f;
+      }
+      // z_i = y_n + sum_{m<i} a_im k_m
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        ybn[j] += zb[j];
+#pragma unroll
+        for (int m = 0; m < i; ++m) kb[m][j] += (float)tr.a[i][m] * zb[j];
+      }
+      __syncthreads();
+    });
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) yb[j] = ybn[j];
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a.ybar[traj * MLP_D + c0 + j] = yb[j];
+  }
+}
+
+// ---- weight gradients of one segment ---------------------------------------------------
+// Output tiles: matrix 0 = dW1 [256 x 64] (D1, Z), 1 = dW2 [256 x 256] (D2, H1),
+// 2 = dW3 [64 x 256] (D3, H2); 32x32 tiles numbered matrix by matrix, row-major.
+struct DwLayout {
+  int n1[3], n2[3], first[4];
+};
+__host__ __device__ inline DwLayout dw_layout() {
+  DwLayout l{{MLP_H, MLP_H, MLP_D}, {MLP_D, MLP_H, MLP_H}, {0, 0, 0, 0}};
+  for (int m = 0; m < 3; ++m) l.first[m + 1] = l.first[m] + (l.n1[m] / 32) * (l.n2[m] / 32);
+  return l;
+}
+
+template <class P>
+__global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R, int64_t rows,
+                                                         float* __restrict__ partial) {
+  const DwLayout lay = dw_layout();
+  const int tg = blockIdx.x;
+  const int chunk = blockIdx.y;
+  int m = 0;
+  while (tg >= lay.first[m + 1]) ++m;
+  const int t = tg - lay.first[m];
+  const int ntj = lay.n2[m] / 32;
+  const int ti = t / ntj, tj = t - ti * ntj;
+  const typename P::act_t* A = R.b[2 * m];       // deltas (width n1)
+  const typename P::act_t* Bm = R.b[2 * m + 1];  // layer inputs (width n2)
+  const int WA = lay.n1[m], WB = lay.n2[m];
+  const int lane = threadIdx.x;
+  constexpr int KA = P::KA, KS = P::KSTEP;
+  const int64_t r_begin = (int64_t)chunk * ADJ_CHUNK;
+  int64_t r_end = r_begin + ADJ_CHUNK;
+  if (r_end > rows) r_end = rows;
+  floatx16_t acc = floatx16_t{};
+  const int ca = ti * 32 + (lane & 31), cb = tj * 32 + (lane & 31);
+  for (int64_t r0 = r_begin; r0 < r_end; r0 += KS) {
+    const int64_t rb = r0 / KA + (lane >> 5);
+    const typename P::frag_t fa = P::load_a(A + (rb * WA + ca) * KA);
+    const typename P::frag_t fb = P::load_a(Bm + (rb * WB + cb) * KA);
+    acc = P::mfma(fa, fb, acc);
+  }
+  float* o = partial + ((int64_t)chunk * lay.first[3] + tg) * 1024 + lane * 16;
+#pragma unroll
+  for (int v = 0; v < 16; ++v) o[v] = acc[v];
+}
+
+// Column sums of the three delta buffers for one chunk: bias gradients.
+template <class P>
+__global__ void mlp_db_kernel(const RowBufs<P> R, int64_t rows, float* __restrict__ partial_db) {
+  const int chunk = blockIdx.x;
+  constexpr int KA = P::KA;
+  const int64_t r_begin = (int64_t)chunk * ADJ_CHUNK;
+  int64_t r_end = r_begin + ADJ_CHUNK;
+  if (r_end > rows) r_end = rows;
+  for (int col = threadIdx.x; col < MLP_H + MLP_H + MLP_D; col += blockDim.x) {
+    const int which = col < MLP_H ? 0 : (col < 2 * MLP_H ? 2 : 4);
+    const int c = col < MLP_H ? col : (col < 2 * MLP_H ? col - MLP_H : col - 2 * MLP_H);
+    const int Wd = (which == 4) ? MLP_D : MLP_H;
+    const typename P::act_t* D = R.b[which];
+    float s = 0.0f;
+    for (int64_t r = r_begin; r < r_end; ++r) s += (float)D[((r / KA) * Wd + c) * KA + r % KA];
+    partial_db[(int64_t)chunk * (2 * MLP_H + MLP_D) + col] = s;
+  }
+}
+
+// dw_acc (dW1 | dW2 | dW3 | db1 | db2 | db3, row-major) += partials summed in chunk order.
+__global__ void mlp_dw_reduce_kernel(const float* __restrict__ partial,
+                                     const float* __restrict__ partial_db, int nchunks,
+                                     float* __restrict__ dw_acc) {
+  const DwLayout lay = dw_layout();
+  const int nW = MLP_H * MLP_D + MLP_H * MLP_H + MLP_D * MLP_H;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nW + 2 * MLP_H + MLP_D) return;
+  float s = 0.0f;
+  if (e < nW) {
+    int m = 0, off = e;
+    const int sz[3] = {MLP_H * MLP_D, MLP_H * MLP_H, MLP_D * MLP_H};
+    while (off >= sz[m]) { off -= sz[m]; ++m; }
+    const int n1 = off / lay.n2[m], n2 = off - (off / lay.n2[m]) * lay.n2[m];
+    const int tg = lay.first[m] + (n1 / 32) * (lay.n2[m] / 32) + n2 / 32;
+    const int r = n1 % 32, c = n2 % 32;
+    const int lane = c + 32 * ((r % 8) / 4);
+    const int v = 4 * (r / 8) + (r % 4);
+    for (int ch = 0; ch < nchunks; ++ch) s += partial[((int64_t)ch * lay.first[3] + tg) * 1024 + lane * 16 + v];
+  } else {
+    const int col = e - nW;
+    for (int ch = 0; ch < nchunks; ++ch) s += partial_db[(int64_t)ch * (2 * MLP_H + MLP_D) + col];
+  }
+  dw_acc[e] += s;
+}
+
+}  // namespace dfx
+
+using namespace dfx;
+
+namespace {
+
+size_t act_size(int32_t precision) { return precision == DFX_BF16_F32 ? 2 : 4; }
+
+int64_t padded_batch(int64_t B) { return (B + MLP_TILE - 1) / MLP_TILE * MLP_TILE; }
+
+int64_t seg_rows(int64_t B, int32_t seg_steps) { return (int64_t)seg_steps * 7 * padded_batch(B); }
+
+int64_t seg_chunks(int64_t B, int32_t seg_steps) {
+  return (seg_rows(B, seg_steps) + ADJ_CHUNK - 1) / ADJ_CHUNK;
+}
+
+template <class P>
+MlpWeights<P> weights_of(const void* packed, const float* b1, const float* b2, const float* b3) {
+  const int64_t D = MLP_D, H = MLP_H;
+  const int64_t sz[6] = {H * D, H * H, D * H, D * H, H * H, H * D};
+  const typename P::act_t* base = reinterpret_cast<const typename P::act_t*>(packed);
+  MlpWeights<P> W;
+  int64_t o = 0;
+  W.w1 = base + o; o += sz[0];
+  W.w2 = base + o; o += sz[1];
+  W.w3 = base + o; o += sz[2];
+  W.w1t = base + o; o += sz[3];
+  W.w2t = base + o; o += sz[4];
+  W.w3t = base + o;
+  W.b1 = b1; W.b2 = b2; W.b3 = b3;
+  return W;
+}
+
+template <class P>
+int run_segment(const dfx_mlp_problem* p, const void* packed, const float* b1, const float* b2,
+                const float* b3, const dfx_mlp_adj_io* io, int32_t seg, int32_t seg_steps,
+                hipStream_t s) {
+  AdjArgs a;
+  memset(&a, 0, sizeof(a));
+  a.grid = io->grid; a.ckpt = io->ckpt; a.stats = io->stats; a.ts = io->ts; a.ys_bar = io->ys_bar;
+  a.ybar = io->ybar; a.zs = io->zs;
+  a.B = p->batch; a.B_pad = padded_batch(p->batch);
+  a.max_steps = p->max_steps; a.n_ckpt = dfx_mlp_n_ckpt(p); a.C = p->ckpt_every;
+  a.Cs = seg_steps; a.seg = seg; a.n_ts = p->n_ts;
+  a.n_out = (p->save_t0 ? 1 : 0) + p->n_ts + (p->save_t1 ? 1 : 0);
+  a.save_t0 = p->save_t0; a.t0 = p->t0; a.t1 = p->t1;
+  const MlpWeights<P> W = weights_of<P>(packed, b1, b2, b3);
+  RowBufs<P> R;
+  const int64_t rows_alloc = seg_chunks(p->batch, seg_steps) * ADJ_CHUNK;
+  typename P::act_t* base = reinterpret_cast<typename P::act_t*>(io->rows);
+  int64_t off = 0;
+  const int widths[6] = {MLP_H, MLP_D, MLP_H, MLP_H, MLP_D, MLP_H};
+  for (int i = 0; i < 6; ++i) { R.b[i] = base + off; off += rows_alloc * widths[i]; }
+  const unsigned ntiles = (unsigned)(a.B_pad / MLP_TILE);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_adj_segment_kernel<P>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<P>));
+  hipLaunchKernelGGL(mlp_adj_segment_kernel<P>, dim3(ntiles), dim3(MLP_THREADS), sizeof(MlpLds<P>),
+                     s, a, W, R);
+  const int64_t rows = seg_rows(p->batch, seg_steps);
+  const int nch = (int)seg_chunks(p->batch, seg_steps);
+  const DwLayout lay = dw_layout();
+  float* partial = io->partial;
+  float* partial_db = io->partial + (int64_t)nch * lay.first[3] * 1024;
+  hipLaunchKernelGGL(mlp_dw_gemm_kernel<P>, dim3((unsigned)lay.first[3], (unsigned)nch), dim3(64), 0,
+                     s, R, rows, partial);
+  hipLaunchKernelGGL(mlp_db_kernel<P>, dim3((unsigned)nch), dim3(256), 0, s, R, rows, partial_db);
+  const int nout = MLP_H * MLP_D + MLP_H * MLP_H + MLP_D * MLP_H + 2 * MLP_H + MLP_D;
+  hipLaunchKernelGGL(mlp_dw_reduce_kernel, dim3((nout + 255) / 256), dim3(256), 0, s, partial,
+                     partial_db, nch, io->dw);
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dfx_mlp_adj_workspace(const dfx_mlp_problem* p, int32_t seg_steps, size_t* zs_bytes,
+                          size_t* rows_bytes, size_t* partial_bytes) {
+  if (!p || seg_steps <= 0 || !zs_bytes || !rows_bytes || !partial_bytes) return DFX_E_ARG;
+  const int64_t Bp = padded_batch(p->batch);
+  *zs_bytes = (size_t)(Bp * seg_steps * 7 * MLP_D) * sizeof(float);
+  const int64_t rows_alloc = seg_chunks(p->batch, seg_steps) * ADJ_CHUNK;
+  int64_t wsum = 0;
+  const int widths[6] = {MLP_H, MLP_D, MLP_H, MLP_H, MLP_D, MLP_H};
+  for (int i = 0; i < 6; ++i) wsum += widths[i];
+  *rows_bytes = (size_t)(rows_alloc * wsum) * act_size(p->precision);
+  const DwLayout lay = dw_layout();
+  *partial_bytes = (size_t)(seg_chunks(p->batch, seg_steps) * ((int64_t)lay.first[3] * 1024 + 2 * MLP_H + MLP_D)) * sizeof(float);
+  return DFX_OK;
+}
+
+int dfx_mlp_adj_segment(const dfx_mlp_problem* p, const void* packed, const float* b1,
+                        const float* b2, const float* b3, const dfx_mlp_adj_io* io,
+                        int32_t segment, int32_t seg_steps, void* stream) {
+  if (!p || !packed || !b1 || !b2 || !b3 || !io) return DFX_E_ARG;
+  if (p->d != MLP_D || p->hidden != MLP_H || p->solver != DFX_TSIT5) return DFX_E_ARG;
+  if (p->ckpt_every <= 0 || seg_steps <= 0 || seg_steps > p->ckpt_every || segment < 0) return DFX_E_ARG;
+  if (!io->grid || !io->ckpt || !io->stats || !io->ys_bar || !io->ybar || !io->zs || !io->rows ||
+      !io->partial || !io->dw)
+    return DFX_E_ARG;
+  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
+  if (p->batch == 0) return DFX_OK;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (p->precision == DFX_BF16_F32) return run_segment<PolicyBF16>(p, packed, b1, b2, b3, io, segment, seg_steps, s);
+  if (p->precision == DFX_F32) return run_segment<PolicyF32>(p, packed, b1, b2, b3, io, segment, seg_steps, s);
+  return DFX_E_ARG;
+}
+
+}  // extern "C"

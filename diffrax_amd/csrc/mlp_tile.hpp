@@ -1,0 +1,238 @@
+// mlp_tile.hpp -- the registered neural vector field f(y) = tanh(W3 sp(W2 sp(W1 y + b1) + b2) + b3)
+// (sp = softplus; d = 64 -> 256 -> 256 -> 64) evaluated for a tile of 32 trajectories by one
+// 256-thread workgroup on MFMA, plus its vector-Jacobian product.
+//
+// Layout of one tile (4 waves):
+//  - owner threads: thread t owns trajectory t/8 of the tile and state components
+//    (t%8)*8 .. +8, so a trajectory's 8 owners are adjacent lanes of one wave;
+//  - activations live in LDS, row-major [32 rows][K + PAD]; every MFMA reads its A fragment
+//    (32 rows x KSTEP) straight from there;
+//  - weights are read as MFMA B fragments from a host-packed copy (fragment (n_tile, k_step)
+//    is 64 lanes x KA contiguous elements, one coalesced load per lane);
+//  - layers 1 and 2: wave w produces output columns [64w, 64w + 64) (two 32x32 tiles);
+//    layer 3 (64 outputs, K = 256) is split over K: wave w reduces K in [64w, 64w + 64) and
+//    the four partial sums meet in LDS, where the owner threads add them in wave order.
+// Precision policies: BF16 operands with f32 accumulate (v_mfma_f32_32x32x16_bf16), or
+// f32 throughout (v_mfma_f32_32x32x2f32).  Activation functions are evaluated in f32.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dfx {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+
+constexpr int MLP_D = 64;
+constexpr int MLP_H = 256;
+constexpr int MLP_TILE = 32;
+constexpr int MLP_THREADS = 256;
+
+struct PolicyBF16 {
+  using act_t = __bf16;
+  using frag_t = bf16x8_t;
+  static constexpr int KSTEP = 16;  // K per MFMA
+  static constexpr int KA = 8;      // elements per lane per fragment
+  static constexpr int PAD = 8;     // LDS row padding (elements); keeps 16-byte alignment
+  static constexpr bool fast_math = true;
+  // Every layer input (the RK stage value and both hidden activations) is fed as hi + lo
+  // bfloat16 terms, ~16 significant bits.  A single bfloat16 (8 bits) turns f into a step
+  // function at the scale of the O(dt f) differences between stages, and the embedded error
+  // estimate then measures rounding noise instead of truncation error.  Each layer costs two
+  // bfloat16 MFMA passes (float32 accumulate); the weights themselves are bfloat16.
+  static constexpr bool split_input = true;
+  __device__ __forceinline__ static frag_t load_a(const act_t* p) {
+    return *reinterpret_cast<const frag_t*>(p);
+  }
+  __device__ __forceinline__ static frag_t load_b(const act_t* packed, int frag) {
+    return reinterpret_cast<const frag_t*>(packed)[frag];
+  }
+  __device__ __forceinline__ static floatx16_t mfma(frag_t a, frag_t b, floatx16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+  __device__ __forceinline__ static act_t to_act(float x) { return (act_t)x; }
+};
+
+struct PolicyF32 {
+  using act_t = float;
+  using frag_t = float;
+  static constexpr int KSTEP = 2;
+  static constexpr int KA = 1;
+  static constexpr int PAD = 1;     // odd row stride: conflict-free column reads
+  static constexpr bool fast_math = false;
+  static constexpr bool split_input = false;
+  __device__ __forceinline__ static frag_t load_a(const act_t* p) { return *p; }
+  __device__ __forceinline__ static frag_t load_b(const act_t* packed, int frag) {
+    return packed[frag];
+  }
+  __device__ __forceinline__ static floatx16_t mfma(frag_t a, frag_t b, floatx16_t c) {
+    return __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, c, 0, 0, 0);
+  }
+  __device__ __forceinline__ static act_t to_act(float x) { return x; }
+};
+
+// Packed weights of one solve.  Forward fragments: B[k][n] = W[n][k].  Backward fragments
+// (transposed products): B[k][n] = W[k][n].
+template <class P>
+struct MlpWeights {
+  const typename P::act_t* w1;   // layer 1 forward fragments (N = 256, K = 64)
+  const typename P::act_t* w2;   // layer 2 forward (N = 256, K = 256)
+  const typename P::act_t* w3;   // layer 3 forward (N = 64, K = 256)
+  const typename P::act_t* w1t;  // z-bar = d1 . W1   (N = 64, K = 256)
+  const typename P::act_t* w2t;  // h1-bar = d2 . W2  (N = 256, K = 256)
+  const typename P::act_t* w3t;  // h2-bar = d3 . W3  (N = 256, K = 64)
+  const float* b1;
+  const float* b2;
+  const float* b3;
+};
+
+template <class P>
+struct MlpLds {
+  static constexpr int LDX = MLP_D + P::PAD;
+  static constexpr int LDH = MLP_H + P::PAD;
+  static constexpr int LDP = MLP_D + 4;
+  typename P::act_t x[MLP_TILE * LDX];    // layer-1 input (stage value), high part
+  typename P::act_t xlo[P::split_input ? MLP_TILE * LDX : 1];  // layer-1 input, low part
+  typename P::act_t h1[MLP_TILE * LDH];   // softplus(layer 1) (high part)
+  typename P::act_t h2[MLP_TILE * LDH];   // softplus(layer 2) (high part)
+  typename P::act_t h1lo[P::split_input ? MLP_TILE * LDH : 1];  // low parts
+  typename P::act_t h2lo[P::split_input ? MLP_TILE * LDH : 1];
+  typename P::act_t d[MLP_TILE * LDH];    // backward: delta of layer 2
+  typename P::act_t d3[MLP_TILE * LDX];   // backward: delta of layer 3
+  float part[4 * MLP_TILE * LDP];         // split-K partial sums of the 64-wide outputs
+};
+
+// Store one float value of the layer-1 input (row r, column c) in its policy's form.
+template <class P>
+__device__ __forceinline__ void store_input(MlpLds<P>& L, int r, int c, float v) {
+  const typename P::act_t hi = P::to_act(v);
+  L.x[r * L.LDX + c] = hi;
+  if constexpr (P::split_input) L.xlo[r * L.LDX + c] = P::to_act(v - (float)hi);
+}
+
+// Store a hidden activation (hi, and lo for split policies).
+template <class P>
+__device__ __forceinline__ void store_hidden(typename P::act_t* hi_buf, typename P::act_t* lo_buf,
+                                             int idx, float v) {
+  const typename P::act_t hi = P::to_act(v);
+  hi_buf[idx] = hi;
+  if constexpr (P::split_input) lo_buf[idx] = P::to_act(v - (float)hi);
+}
+
+// ---- activations ------------------------------------------------------------------------
+template <bool Fast>
+__device__ __forceinline__ float softplus_f(float x) {
+  // jax.nn.softplus(x) = logaddexp(x, 0) = max(x, 0) + log1p(exp(-|x|))
+  if constexpr (Fast) {
+    const float e = __builtin_amdgcn_exp2f(-fabsf(x) * 1.4426950408889634f);
+    return fmaxf(x, 0.0f) + 0.6931471805599453f * __builtin_amdgcn_logf(1.0f + e);
+  } else {
+    return fmaxf(x, 0.0f) + log1pf(expf(-fabsf(x)));
+  }
+}
+template <bool Fast>
+__device__ __forceinline__ float tanh_f(float x) {
+  if constexpr (Fast) {
+    // tanh(x) = sign(x) (1 - 2 / (exp(2|x|) + 1))
+    const float e = __builtin_amdgcn_exp2f(fminf(2.0f * fabsf(x), 60.0f) * 1.4426950408889634f);
+    const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
+    return copysignf(t, x);
+  } else {
+    return tanhf(x);
+  }
+}
+// derivative of softplus from its value h = softplus(s): sigmoid(s) = 1 - exp(-h)
+template <bool Fast>
+__device__ __forceinline__ float dsoftplus_from_value(float h) {
+  if constexpr (Fast) return 1.0f - __builtin_amdgcn_exp2f(-h * 1.4426950408889634f);
+  else return -expm1f(-h);
+}
+
+// ---- one wave's share of a layer --------------------------------------------------------
+// acc[nt] (32x32 f32) = sum over k-steps [ks0, ks1) of A(lds rows, k) * B(fragment (nt0+nt, ks)).
+template <class P, int NKS, int NT, bool Accumulate = false>
+__device__ __forceinline__ void wave_gemm(const typename P::act_t* a_lds, int lda,
+                                          const typename P::act_t* wp, int nt0, int ks0,
+                                          int ks1, floatx16_t (&acc)[NT], int lane) {
+  const int row = lane & 31;
+  const int kh = (lane >> 5) * P::KA;
+  if constexpr (!Accumulate) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = floatx16_t{};
+  }
+#pragma unroll 4
+  for (int ks = ks0; ks < ks1; ++ks) {
+    const typename P::frag_t a = P::load_a(a_lds + row * lda + ks * P::KSTEP + kh);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const typename P::frag_t b = P::load_b(wp, ((nt0 + t) * NKS + ks) * 64 + lane);
+      acc[t] = P::mfma(a, b, acc[t]);
+    }
+  }
+}
+
+// C-layout helpers: accumulator element v (= 4i + j) of lane l sits at
+// row 8i + (l/32)*4 + j, column l%32 of the 32x32 tile.
+__device__ __forceinline__ int acc_row(int lane, int v) { return 8 * (v >> 2) + (lane >> 5) * 4 + (v & 3); }
+
+// Forward of the MLP on the tile: reads L.x (stage values), leaves softplus activations in
+// L.h1 / L.h2 and returns f for the owner's 8 components in fout.
+template <class P>
+__device__ __forceinline__ void mlp_forward(const MlpWeights<P>& W, MlpLds<P>& L, int wave,
+                                            int lane, int owner_row, int owner_c0,
+                                            float (&fout)[8]) {
+  constexpr bool F = P::fast_math;
+  floatx16_t acc[2];
+  // layer 1: [32 x 64] . W1^T -> [32 x 256], wave's columns [64w, 64w + 64)
+  wave_gemm<P, MLP_D / P::KSTEP, 2>(L.x, L.LDX, W.w1, 2 * wave, 0, MLP_D / P::KSTEP, acc, lane);
+  if constexpr (P::split_input)
+    wave_gemm<P, MLP_D / P::KSTEP, 2, true>(L.xlo, L.LDX, W.w1, 2 * wave, 0, MLP_D / P::KSTEP, acc,
+                                            lane);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int col = 64 * wave + 32 * t + (lane & 31);
+    const float bias = W.b1[col];
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      store_hidden<P>(L.h1, L.h1lo, acc_row(lane, v) * L.LDH + col, softplus_f<F>(acc[t][v] + bias));
+  }
+  __syncthreads();
+  // layer 2: [32 x 256] . W2^T -> [32 x 256]
+  wave_gemm<P, MLP_H / P::KSTEP, 2>(L.h1, L.LDH, W.w2, 2 * wave, 0, MLP_H / P::KSTEP, acc, lane);
+  if constexpr (P::split_input)
+    wave_gemm<P, MLP_H / P::KSTEP, 2, true>(L.h1lo, L.LDH, W.w2, 2 * wave, 0, MLP_H / P::KSTEP, acc,
+                                            lane);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int col = 64 * wave + 32 * t + (lane & 31);
+    const float bias = W.b2[col];
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      store_hidden<P>(L.h2, L.h2lo, acc_row(lane, v) * L.LDH + col, softplus_f<F>(acc[t][v] + bias));
+  }
+  __syncthreads();
+  // layer 3 (split-K): wave w sums K in [64w, 64w + 64) for all 64 outputs
+  constexpr int KS3 = MLP_H / P::KSTEP;
+  wave_gemm<P, KS3, 2>(L.h2, L.LDH, W.w3, 0, wave * (KS3 / 4), (wave + 1) * (KS3 / 4), acc, lane);
+  if constexpr (P::split_input)
+    wave_gemm<P, KS3, 2, true>(L.h2lo, L.LDH, W.w3, 0, wave * (KS3 / 4), (wave + 1) * (KS3 / 4), acc,
+                               lane);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int col = 32 * t + (lane & 31);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) L.part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = owner_c0 + j;
+    float s = L.part[(0 * MLP_TILE + owner_row) * L.LDP + c];
+    s += L.part[(1 * MLP_TILE + owner_row) * L.LDP + c];
+    s += L.part[(2 * MLP_TILE + owner_row) * L.LDP + c];
+    s += L.part[(3 * MLP_TILE + owner_row) * L.LDP + c];
+    fout[j] = tanh_f<F>(s + W.b3[c]);
+  }
+}
+
+}  // namespace dfx

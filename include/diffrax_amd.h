@@ -163,6 +163,87 @@ int dfx_sde_heun_adjoint(const dfx_sde_problem* p, const dfx_sde_io* io,
                          const dfx_sde_grad_io* g, void* workspace, size_t workspace_bytes,
                          void* stream);
 
+/* ---- neural ODE: dy/dt = MLP(y), d = 64, width 256, depth 2 ------------------------- */
+/* f(y) = tanh(W3 softplus(W2 softplus(W1 y + b1) + b2) + b3), W1 [256,64], W2 [256,256],
+ * W3 [64,256] (eqx.nn.MLP(64, 64, 256, 2, softplus, final_activation=tanh)).
+ * Weights are packed once per set of values into MFMA fragments (dfx_mlp_pack). */
+typedef struct dfx_mlp_problem {
+  int32_t solver;           /* DFX_TSIT5                                                 */
+  int32_t precision;        /* DFX_F32 or DFX_BF16_F32                                   */
+  int32_t d, hidden;        /* 64, 256                                                   */
+  int32_t n_ts;             /* SaveAt(ts=...) length (ts shared by all instances)        */
+  int32_t save_t0, save_t1;
+  int32_t max_steps;
+  int32_t ckpt_every;       /* C > 0: record accepted grid and every C-th state (adjoint) */
+  int32_t force_dtmin;
+  int64_t batch;
+  double t0, t1, dt0;
+  double rtol, atol, pcoeff, icoeff, dcoeff, dtmin, dtmax;
+  double factormin, factormax, safety, error_order;
+} dfx_mlp_problem;
+
+typedef struct dfx_mlp_io {
+  const float* y0;          /* [B, 64]                                                   */
+  const double* ts;         /* [n_ts]                                                    */
+  float* ys;                /* out [B, n_out, 64]                                        */
+  double* ts_out;           /* out [B, n_out]                                            */
+  int32_t* stats;           /* out [B, 3]                                                */
+  int32_t* result;          /* out [B]                                                   */
+  double* grid;             /* out [B, max_steps, 2] when ckpt_every > 0                  */
+  float* ckpt;              /* out [B, dfx_mlp_n_ckpt, 64] when ckpt_every > 0            */
+} dfx_mlp_io;
+
+/* Bytes of the packed-weight buffer for a precision. */
+int dfx_mlp_pack_bytes(int32_t precision, size_t* bytes);
+/* Pack float32 row-major W1, W2, W3 (device) into MFMA fragments (forward and transposed
+ * products) in `packed` (device, dfx_mlp_pack_bytes). */
+int dfx_mlp_pack(int32_t precision, const float* w1, const float* w2, const float* w3,
+                 void* packed, void* stream);
+/* f(y) for `rows` states y [rows, 64] -> f [rows, 64] (device).
+ * Replaces: ODETerm.vf (_term.py:193-211) with the MLP vector field. */
+int dfx_mlp_eval(int32_t precision, const void* packed, const float* b1, const float* b2,
+                 const float* b3, const float* y, float* f, int64_t rows, void* stream);
+/* Checkpoint slots per instance: ceil(max_steps / ckpt_every). */
+int dfx_mlp_n_ckpt(const dfx_mlp_problem* p);
+/* Batched adaptive Tsit5 solve of the neural ODE: one workgroup per 32 trajectories,
+ * MLP stages on MFMA, per-trajectory float64 step control.
+ * Replaces: the integration loop (_integrate.py:302-885) with ODETerm(MLP) and Tsit5
+ * (_solver/tsit5.py, _solver/runge_kutta.py:446-1203) under PIDController
+ * (_step_size_controller/pid.py:394-567) and SaveAt(ts) (_integrate.py:456-491). */
+int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
+                  const float* b2, const float* b3, const dfx_mlp_io* io, void* stream);
+
+/* ---- neural ODE adjoint ---------------------------------------------------------- */
+typedef struct dfx_mlp_adj_io {
+  const double* grid;       /* [B, max_steps, 2] from dfx_mlp_solve (ckpt_every > 0)        */
+  const float* ckpt;        /* [B, dfx_mlp_n_ckpt, 64] from dfx_mlp_solve                   */
+  const int32_t* stats;     /* [B, 3] from dfx_mlp_solve                                    */
+  const double* ts;         /* [n_ts] save times                                           */
+  const float* ys_bar;      /* [B, n_out, 64] cotangent of the saved states                 */
+  float* ybar;              /* in/out [B, 64]: cotangent of the state at the segment's end
+                               (in), at its start (out); set to ys_bar of SaveAt(t1) before
+                               the last segment; after segment 0 it is dL/dy0 (minus t0 save) */
+  float* zs;                /* scratch, dfx_mlp_adj_workspace zs_bytes                      */
+  void* rows;               /* scratch, rows_bytes                                          */
+  float* partial;           /* scratch, partial_bytes                                       */
+  float* dw;                /* in/out accumulator: dW1 | dW2 | dW3 | db1 | db2 | db3 (float32,
+                               row-major, 98880 values); zero it before the first segment   */
+} dfx_mlp_adj_io;
+
+/* Scratch sizes for a segment of seg_steps (<= ckpt_every) accepted steps. */
+int dfx_mlp_adj_workspace(const dfx_mlp_problem* p, int32_t seg_steps, size_t* zs_bytes,
+                          size_t* rows_bytes, size_t* partial_bytes);
+/* Reverse sweep over accepted steps [segment*C, segment*C + seg_steps) of every
+ * instance (C = ckpt_every): recompute from the checkpoint, pull the cotangents back
+ * through the saves, the stages and the MLP, and add the segment's weight gradients to
+ * io->dw.  Call for segment = ceil(max accepted steps / C) - 1 down to 0.
+ * Replaces: reverse-mode differentiation of diffeqsolve through
+ * RecursiveCheckpointAdjoint (_adjoint.py:264-312), whose step-size factors are
+ * stop_gradient (_step_size_controller/pid.py:498-530). */
+int dfx_mlp_adj_segment(const dfx_mlp_problem* p, const void* packed, const float* b1,
+                        const float* b2, const float* b3, const dfx_mlp_adj_io* io,
+                        int32_t segment, int32_t seg_steps, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
