@@ -14,6 +14,18 @@ max relative error against the FP64 reference is <= 1e-6.  The CPU baseline is t
 C oracle (OpenMP over the host cores) on the same inputs.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build measures one GPU.
+
+The same line carries the other workloads of the path under "workloads" (not the
+headline; each with its own timing, error and CPU-baseline fields):
+  neural_ode_forward: MLP 64 -> 256 -> 256 -> 64 (softplus, tanh output), Xavier-normal
+      weights, zero biases, batch 8,192, y0 ~ N(0, 1), t in [0, 1], Tsit5 + PID(1e-5, 1e-7),
+      dt0 = 0.1, 32 saves; float32 and bfloat16-MFMA forms.
+  neural_ode_grad: the same solve plus the checkpointed discrete adjoint of
+      loss = mean((ys - targets)^2), targets ~ N(0, 0.5^2), gradients for all 98,880 MLP
+      parameters and y0.
+  ou_sde_heun: 262,144 paths of a 16-dim Ornstein-Uhlenbeck SDE, Heun, 1,000 steps on
+      [0, 1], float32 Brownian increments [1000, 262144, 16] (16.8 GB) resident in HBM;
+      forward, and forward + adjoint of loss = mean y(1)^2 for theta, mu, sigma.
 """
 from __future__ import annotations
 
@@ -116,6 +128,198 @@ def cpu_baseline(mu, y0, t1, ts, reps=3):
                        f"host CPU: {cpu_model}; JAX-CPU diffrax: unavailable"), out
 
 
+MLP_FLOP_PER_EVAL = 2 * (64 * 256 + 256 * 256 + 256 * 64)   # 196,608 MFMA flop per trajectory
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bfloat16 MFMA
+F32_MFMA_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _event_time(fn, steps, warmup):
+    """Mean and median event time (ms) of fn() on the current stream, and wall ms/step."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    per = [a.elapsed_time(b) for a, b in evs]
+    return statistics.fmean(per), statistics.median(per), wall * 1e3 / steps
+
+
+def _graphed(launch):
+    import torch
+    g = torch.cuda.CUDAGraph()
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        launch()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        launch()
+    return g
+
+
+def mlp_inputs(seed=0, B=8192, n_ts=32):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    xav = lambda o, i: rng.normal(0.0, np.sqrt(2.0 / (i + o)), size=(o, i))
+    W1, W2, W3 = xav(256, 64), xav(256, 256), xav(64, 256)
+    y0 = rng.normal(size=(B, 64))
+    targets = rng.normal(0.0, 0.5, size=(B, n_ts, 64))
+    ts = np.linspace(0.0, 1.0, n_ts)
+    return [W1, np.zeros(256), W2, np.zeros(256), W3, np.zeros(64)], y0, ts, targets
+
+
+def bench_mlp(steps, warmup, with_cpu, check=256):
+    """Neural-ODE forward and forward+gradient, float32 and bfloat16-MFMA forms."""
+    import numpy as np
+    import torch
+    import diffrax_amd as dfx
+    from diffrax_amd import _mlp
+    import oracle
+    dev = torch.device("cuda", 0)
+    W, y0, ts, targets = mlp_inputs()
+    B = y0.shape[0]
+    ctrl = dfx.PIDController(rtol=1e-5, atol=1e-7)
+    y0d = torch.tensor(y0, dtype=torch.float32, device=dev)
+    tgt = torch.tensor(targets, dtype=torch.float32, device=dev)
+    q = oracle.mlp_pack(*W)
+    y0c = y0[:check].astype(np.float32).astype(np.float64)
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    t = time.perf_counter()
+    ref = oracle.solve(y0c, 1.0, q, ts=ts, solver="tsit5", vf="mlp", hidden=256, dt0=0.1,
+                       rtol=1e-5, atol=1e-7, nthreads=cores)
+    cpu_s = time.perf_counter() - t
+    true = oracle.solve(y0c, 1.0, q, ts=ts, solver="tsit5", vf="mlp", hidden=256, dt0=0.1,
+                        rtol=1e-12, atol=1e-12, max_steps=100000, nthreads=cores)["ys"]
+    fwd, grad = {}, {}
+    for matmul in ("f32", "bf16"):
+        vf = dfx.MLPField(*(torch.tensor(w, dtype=torch.float32, device=dev) for w in W),
+                          matmul=matmul)
+        term = dfx.ODETerm(vf)
+        plan = _mlp.prepare_mlp(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0d, dfx.SaveAt(ts=ts), ctrl,
+                                4096, dev)
+        g = _graphed(plan.launch)
+        k_mean, k_med, wall = _event_time(g.replay, steps, warmup)
+        st = plan.stats.cpu().numpy()
+        ys = plan.ys[:check].double().cpu().numpy()
+        flop = float(((6 * st[:, 0] + 1) * MLP_FLOP_PER_EVAL).sum())
+        peak = F32_MFMA_PEAK_TFLOPS if matmul == "f32" else BF16_PEAK_TFLOPS
+        tf = flop / (k_mean * 1e-3) / 1e12
+        fwd[matmul] = dict(
+            solves_per_s=round(B / (wall * 1e-3), 1), ms_per_solve=round(wall, 4),
+            kernel_ms_mean=round(k_mean, 4), kernel_ms_median=round(k_med, 4),
+            attempted_steps_mean=float(st[:, 0].mean()), attempted_steps_max=int(st[:, 0].max()),
+            algorithmic_tflops=round(tf, 3), mfma_peak_tflops=peak, frac=round(tf / peak, 5),
+            max_rel_err_vs_fp64=oracle.max_rel_err(ys, ref["ys"], 1e-7),
+            max_abs_err_vs_tight_fp64=float(np.max(np.abs(ys - true))),
+            fp64_oracle_abs_err_vs_tight=float(np.max(np.abs(ref["ys"] - true))),
+            all_successful=bool((plan.result == 0).all().item()))
+        # forward + gradient (checkpointed discrete adjoint), loss = mean((ys - targets)^2)
+        gplan = _mlp.prepare_mlp(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0d, dfx.SaveAt(ts=ts), ctrl,
+                                 4096, dev, ckpt_every=_mlp.checkpoint_every(
+                                     dfx.RecursiveCheckpointAdjoint(), 4096))
+        out = {}
+
+        def fwd_grad():
+            gplan.launch()
+            ys_bar = (gplan.ys - tgt) * (2.0 / gplan.ys.numel())
+            out["g"] = _mlp.backward_mlp(gplan, ys_bar)
+
+        k_mean, k_med, wall = _event_time(fwd_grad, steps, warmup)
+        grad[matmul] = dict(solves_per_s=round(B / (wall * 1e-3), 1), ms_per_solve=round(wall, 4),
+                            event_ms_mean=round(k_mean, 4),
+                            checkpoint_every=int(gplan.prob.ckpt_every),
+                            grad_finite=bool(all(torch.isfinite(x).all().item() for x in out["g"])))
+        del plan, gplan, g
+    cpu = dict(value=round(check / cpu_s, 2), unit="solves/s", cores=cores, kind="port",
+               sample=f"{check} of the {B} trajectories, FP64 C oracle forward, OpenMP {cores} "
+                      f"threads, one run ({cpu_s:.3f} s); host CPU: {_cpu_model()}")
+    return dict(neural_ode_forward=dict(batch=B, forms=fwd, cpu_baseline=cpu,
+                                        error_subset=check),
+                neural_ode_grad=dict(batch=B, forms=grad,
+                                     loss="mean((ys - targets)^2), targets ~ N(0, 0.25)"))
+
+
+def bench_sde(steps, warmup, with_cpu, P=262144, d=16, N=1000, cpu_paths=8192):
+    """OU SDE, Heun, 1000 steps: forward and forward + adjoint (float32)."""
+    import numpy as np
+    import torch
+    from diffrax_amd import _sde
+    import oracle
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(0)
+    theta = rng.uniform(0.5, 2.0, size=d)
+    mu = rng.normal(0.0, 1.0, size=d)
+    sigma = rng.uniform(0.1, 0.5, size=d)
+    dt = 1.0 / N
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    dW = torch.randn((N, P, d), generator=gen, dtype=torch.float32, device=dev)
+    dW.mul_(float(np.sqrt(dt)))
+    y0 = torch.zeros((P, d), dtype=torch.float32, device=dev)
+    prm = [torch.tensor(a, dtype=torch.float32, device=dev) for a in (theta, mu, sigma)]
+    grid = torch.tensor(_sde.constant_grid(0.0, 1.0, dt), dtype=torch.float64, device=dev)
+    fwd = _sde._Sde(y0, *prm, dW, grid, False, True, False, None)
+    k_mean, k_med, wall = _event_time(fwd.forward, steps, warmup)
+    alg_bytes = P * d * N * 4 + 2 * P * d * 4
+    out = dict(paths=P, d=d, steps=N,
+               forward=dict(paths_per_s=round(P / (k_mean * 1e-3), 1),
+                            kernel_ms_mean=round(k_mean, 4), kernel_ms_median=round(k_med, 4),
+                            ms_per_solve=round(wall, 4),
+                            algorithmic_bytes=alg_bytes,
+                            achieved_gbs=round(alg_bytes / (k_mean * 1e-3) / 1e9, 1),
+                            frac=round(alg_bytes / (k_mean * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)))
+    gs = _sde._Sde(y0, *prm, dW, grid, False, True, True, None)
+
+    def fwd_grad():
+        gs.forward()
+        ys_bar = gs.ys * (2.0 / gs.ys.numel())
+        out["g"] = gs.adjoint(ys_bar, False)
+
+    k_mean, k_med, wall = _event_time(fwd_grad, steps, warmup)
+    out["forward_grad"] = dict(paths_per_s=round(P / (wall * 1e-3), 1),
+                               ms_per_solve=round(wall, 4), event_ms_mean=round(k_mean, 4),
+                               checkpoint_every=int(gs.prob.checkpoint_every),
+                               algorithmic_bytes=2 * alg_bytes,
+                               achieved_gbs=round(2 * alg_bytes / (wall * 1e-3) / 1e9, 1))
+    out.pop("g", None)
+    if with_cpu:
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        dWc = dW[:, :cpu_paths].double().cpu().numpy()
+        t = time.perf_counter()
+        oracle.solve(np.zeros((cpu_paths, d)), 1.0, np.concatenate([theta, mu, sigma]), dW=dWc,
+                     solver="heun", vf="ou", controller="constant", dt0=dt, save_t1=True,
+                     max_steps=100000, nthreads=cores)
+        cpu_s = time.perf_counter() - t
+        out["cpu_baseline"] = dict(value=round(cpu_paths / cpu_s, 1), unit="paths/s (forward)",
+                                   cores=cores, kind="port",
+                                   sample=f"{cpu_paths} of the {P} paths, FP64 C oracle, OpenMP "
+                                          f"{cores} threads ({cpu_s:.3f} s)")
+    del dW, fwd, gs
+    torch.cuda.empty_cache()
+    return {"ou_sde_heun": out}
+
+
 def read_traffic(kernel_tag):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.json")
@@ -132,6 +336,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="f64,f32", help="device forms to time")
+    ap.add_argument("--workloads", default="mlp,sde",
+                    help="extra workloads to time besides the headline (mlp, sde; '' for none)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -222,6 +428,18 @@ def main():
         "cpu_baseline": cpu,
         "forms": {f: {k: v for k, v in forms[f].items() if k != "tune_ms"} for f in forms},
     }
+    for p in plans.values():
+        p.t.clear()
+    plans.clear()
+    torch.cuda.empty_cache()
+    extra = {}
+    wl = [w for w in args.workloads.split(",") if w]
+    if "mlp" in wl:
+        extra.update(bench_mlp(args.steps, args.warmup, not args.no_cpu))
+    if "sde" in wl:
+        extra.update(bench_sde(max(3, args.steps // 4), 1, not args.no_cpu))
+    if extra:
+        line["workloads"] = extra
     print(json.dumps(line))
 
 

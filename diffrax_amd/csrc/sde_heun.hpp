@@ -1,0 +1,243 @@
+// sde_heun.hpp -- kernels of the Heun (Stratonovich) SDE path with diagonal noise for the
+// registered Ornstein-Uhlenbeck terms
+//     dy = theta (mu - y) dt + sigma dW      (theta, mu, sigma per component).
+//
+// Replaces, for MultiTerm(ODETerm(drift), ControlTerm(diag, path)) + Heun +
+// ConstantStepSize, the reference loop (_integrate.py:302-885) running
+// AbstractRungeKutta.step with the Heun tableau (_solver/heun.py:12-17,
+// _solver/runge_kutta.py:847-1203): k = f dt + g dW (MultiTerm.vf_prod, _term.py:711-722;
+// diagonal ControlTerm.prod, _term.py:423-427), z = y + k0, y1 = y + (k0/2 + k1/2).
+// The ConstantStepSize grid (constant.py:30-104, with _clip_to_end) is computed on the
+// host and passed in, so the kernels only read dt_n = t_{n+1} - t_n.
+//
+// Layout: increments dW[N, P, d] float32 (step-major, so one step of all paths is a
+// contiguous slab); states [P, d].  Forward: each thread owns V consecutive components of
+// one path and walks all N steps, streaming its dW slice with U steps of loads in flight.
+// The solve is HBM-bound: 4 bytes of dW per component-step.
+//
+// Gradient (RecursiveCheckpointAdjoint semantics, _adjoint.py:264-312): the forward pass
+// stores the state every CK steps; the adjoint pass gives one component of one path to a
+// thread, walks the segments backwards, recomputes a segment from its checkpoint keeping
+// states and increments in registers, then runs the exact reverse-mode recursion of the
+// Heun step through it.  The step sizes of the current segment sit in LDS (shared by the
+// block).  theta/mu/sigma cotangents are summed per segment in the kernel's precision,
+// folded into float64 per thread, per block in LDS, then across blocks in a fixed order
+// (deterministic).
+//
+// The float64 instantiations are compiled without FMA contraction (bit parity with the
+// FP64 oracle); the float32 ones with contraction, in their own translation unit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dfx {
+
+struct SdeArgs {
+  const void* y0;         // [P, d]
+  const void* theta;      // [d]
+  const void* mu;         // [d]
+  const void* sigma;      // [d]
+  const float* dW;        // [N, P, d]
+  const double* grid;     // [N + 1]
+  void* ys;               // [P, n_out, d]
+  void* ckpt;             // [n_ckpt, P, d] (states at steps CK, 2CK, ...) or null
+  const void* ys_bar;     // [P, n_out, d]
+  void* y0_bar;           // [P, d] or null
+  double* partial;        // [n_blocks, 3, d]
+  int64_t P;
+  int32_t d, N, CK, n_out, save_t0, save_t1;
+};
+
+template <class T, int V>
+struct Vec {
+  T v[V];
+};
+
+template <class T, int V>
+__device__ __forceinline__ Vec<T, V> load_dw(const float* __restrict__ p) {
+  Vec<T, V> r;
+  if constexpr (V == 4) {
+    const float4 x = *reinterpret_cast<const float4*>(p);
+    r.v[0] = (T)x.x; r.v[1] = (T)x.y; r.v[2] = (T)x.z; r.v[3] = (T)x.w;
+  } else if constexpr (V == 2) {
+    const float2 x = *reinterpret_cast<const float2*>(p);
+    r.v[0] = (T)x.x; r.v[1] = (T)x.y;
+  } else {
+    r.v[0] = (T)p[0];
+  }
+  return r;
+}
+
+// One Heun step of one component: k0 = dt*(th*(mu - y)) + sg*w ; z = y + 1*k0 ;
+// k1 = dt*(th*(mu - z)) + sg*w ; y1 = y + (0.5*k0 + 0.5*k1).
+template <class T>
+__device__ __forceinline__ T heun_ou(T y, T dt, T th, T mu, T sg, T w) {
+  const T k0 = dt * (th * (mu - y)) + sg * w;
+  const T z = y + k0;
+  const T k1 = dt * (th * (mu - z)) + sg * w;
+  return y + (T(0.5) * k0 + T(0.5) * k1);
+}
+
+template <class T, int V, int U>
+__global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
+  const int groups = a.d / V;
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= a.P * groups) return;
+  const int64_t p = tid / groups;
+  const int j0 = (int)(tid - p * groups) * V;
+  const T* th_p = reinterpret_cast<const T*>(a.theta) + j0;
+  const T* mu_p = reinterpret_cast<const T*>(a.mu) + j0;
+  const T* sg_p = reinterpret_cast<const T*>(a.sigma) + j0;
+  T th[V], mu[V], sg[V], y[V];
+  const T* y0 = reinterpret_cast<const T*>(a.y0) + p * a.d + j0;
+#pragma unroll
+  for (int v = 0; v < V; ++v) { th[v] = th_p[v]; mu[v] = mu_p[v]; sg[v] = sg_p[v]; y[v] = y0[v]; }
+  T* ys = reinterpret_cast<T*>(a.ys) + p * (int64_t)a.n_out * a.d + j0;
+  if (a.save_t0) {
+#pragma unroll
+    for (int v = 0; v < V; ++v) ys[v] = y[v];
+  }
+  const int64_t slab = a.P * a.d;  // elements per step
+  const float* dwp = a.dW + p * a.d + j0;
+  T* ck = reinterpret_cast<T*>(a.ckpt);
+  int next_ck = a.CK;  // step count at which the next checkpoint is due
+  for (int n0 = 0; n0 < a.N; n0 += U) {
+    Vec<T, V> w[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (n0 + u < a.N) w[u] = load_dw<T, V>(dwp + (int64_t)(n0 + u) * slab);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int n = n0 + u;
+      if (n < a.N) {
+        const T dt = (T)(a.grid[n + 1] - a.grid[n]);
+#pragma unroll
+        for (int v = 0; v < V; ++v) y[v] = heun_ou<T>(y[v], dt, th[v], mu[v], sg[v], w[u].v[v]);
+        if (ck && n + 1 == next_ck) {
+          if (n + 1 < a.N) {
+            T* c = ck + ((int64_t)(next_ck / a.CK - 1) * a.P + p) * a.d + j0;
+#pragma unroll
+            for (int v = 0; v < V; ++v) c[v] = y[v];
+          }
+          next_ck += a.CK;
+        }
+      }
+    }
+  }
+  if (a.save_t1) {
+    T* o = ys + (a.save_t0 ? a.d : 0);
+#pragma unroll
+    for (int v = 0; v < V; ++v) o[v] = y[v];
+  }
+}
+
+// Reverse-mode of one Heun step for one component (see heun_ou).  Given the state y at the
+// start of the step and the cotangent yb of y1, returns the cotangent of y and adds the
+// step's parameter cotangents: sth += d/dtheta, smu += dt (k0b + k1b) (times theta gives
+// d/dmu), ssg += d/dsigma.
+template <class T>
+__device__ __forceinline__ T heun_ou_vjp(T y, T dt, T th, T mu, T sg, T w, T yb, T& sth,
+                                         T& smu, T& ssg) {
+  const T k0 = dt * (th * (mu - y)) + sg * w;
+  const T z = y + k0;
+  const T c = dt * th;
+  const T k1b = T(0.5) * yb;     // y1 = y + (k0/2 + k1/2)
+  const T zb = -(c * k1b);       // k1 = dt*(th*(mu - z)) + sg*w
+  const T k0b = k1b + zb;        // z = y + k0
+  const T s = k0b + k1b;
+  sth += dt * ((mu - z) * k1b + (mu - y) * k0b);
+  smu += dt * s;
+  ssg += w * s;
+  return (yb + zb) - c * k0b;    // k0 = dt*(th*(mu - y)) + sg*w
+}
+
+// One checkpoint segment of the adjoint for one component: recompute the cnt (<= CK) steps
+// from the segment's initial state y keeping every step's start state and increment, then
+// walk them backwards.  FULL = (cnt == CK) drops the per-step bounds tests.
+template <class T, int CK, bool FULL>
+__device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, int64_t slab,
+                                            int cnt, const T* dts, T th, T mu, T sg, T& yb,
+                                            double& thb, double& mub, double& sgb) {
+  T yh[CK], w[CK];
+#pragma unroll
+  for (int c = 0; c < CK; ++c)
+    if (FULL || c < cnt) w[c] = (T)dw[(int64_t)c * slab];
+#pragma unroll
+  for (int c = 0; c < CK; ++c) {
+    yh[c] = y;
+    if (FULL || c < cnt) y = heun_ou<T>(y, dts[c], th, mu, sg, w[c]);
+  }
+  T sth = T(0), smu = T(0), ssg = T(0);
+#pragma unroll
+  for (int c = CK - 1; c >= 0; --c)
+    if (FULL || c < cnt) yb = heun_ou_vjp<T>(yh[c], dts[c], th, mu, sg, w[c], yb, sth, smu, ssg);
+  thb += (double)sth;
+  mub += (double)th * (double)smu;
+  sgb += (double)ssg;
+}
+
+template <class T, int CK>
+__global__ void __launch_bounds__(256) sde_heun_adj_kernel(const SdeArgs a) {
+  __shared__ T dts[CK];
+  extern __shared__ double red[];
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = tid < a.P * a.d;
+  const int64_t p = live ? tid / a.d : 0;
+  const int j = live ? (int)(tid - p * a.d) : 0;
+  double thb = 0.0, mub = 0.0, sgb = 0.0;
+  const T th = reinterpret_cast<const T*>(a.theta)[j];
+  const T mu = reinterpret_cast<const T*>(a.mu)[j];
+  const T sg = reinterpret_cast<const T*>(a.sigma)[j];
+  const T* ysb = reinterpret_cast<const T*>(a.ys_bar) + p * (int64_t)a.n_out * a.d + j;
+  T yb = (live && a.save_t1) ? ysb[a.save_t0 ? a.d : 0] : T(0);
+  const int64_t slab = a.P * a.d;
+  const float* dwp = a.dW + (live ? tid : 0);
+  const T* ck = reinterpret_cast<const T*>(a.ckpt);
+  const T* y0 = reinterpret_cast<const T*>(a.y0);
+  const int nseg = (a.N + CK - 1) / CK;
+  for (int s = nseg - 1; s >= 0; --s) {
+    const int nbeg = s * CK;
+    const int cnt = min(CK, a.N - nbeg);
+    __syncthreads();  // the previous segment's reads of dts are done
+    if (threadIdx.x < (unsigned)cnt)
+      dts[threadIdx.x] = (T)(a.grid[nbeg + threadIdx.x + 1] - a.grid[nbeg + threadIdx.x]);
+    __syncthreads();
+    if (live) {
+      const T y = (s == 0) ? y0[tid] : ck[(int64_t)(s - 1) * slab + tid];
+      if (cnt == CK)
+        adj_segment<T, CK, true>(y, dwp + (int64_t)nbeg * slab, slab, CK, dts, th, mu, sg, yb,
+                                 thb, mub, sgb);
+      else
+        adj_segment<T, CK, false>(y, dwp + (int64_t)nbeg * slab, slab, cnt, dts, th, mu, sg,
+                                  yb, thb, mub, sgb);
+    }
+  }
+  if (live) {
+    if (a.save_t0) yb = yb + ysb[0];
+    if (a.y0_bar) reinterpret_cast<T*>(a.y0_bar)[tid] = yb;
+  }
+  // Block reduction of the parameter cotangents over the paths of this block.
+  // LDS layout: [3][d][paths_per_block] (blockDim is a multiple of d).
+  const int ppb = blockDim.x / a.d;
+  const int lp = threadIdx.x / a.d;
+  const int lj = threadIdx.x % a.d;
+  red[(0 * a.d + lj) * ppb + lp] = thb;
+  red[(1 * a.d + lj) * ppb + lp] = mub;
+  red[(2 * a.d + lj) * ppb + lp] = sgb;
+  __syncthreads();
+  for (int q = threadIdx.x; q < 3 * a.d; q += blockDim.x) {
+    double acc = 0.0;
+    for (int i = 0; i < ppb; ++i) acc += red[q * ppb + i];
+    a.partial[(int64_t)blockIdx.x * 3 * a.d + q] = acc;
+  }
+}
+
+// Launchers (one translation unit per precision).
+int sde_launch_fwd_f32(const SdeArgs& a, hipStream_t s);
+int sde_launch_fwd_f64(const SdeArgs& a, hipStream_t s);
+int sde_launch_adj_f32(const SdeArgs& a, int64_t blocks, size_t lds, hipStream_t s);
+int sde_launch_adj_f64(const SdeArgs& a, int64_t blocks, size_t lds, hipStream_t s);
+constexpr int kSdeBlock = 256;
+
+}  // namespace dfx
