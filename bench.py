@@ -398,6 +398,21 @@ def main():
             forms[form]["step_counts_equal_fp64_oracle"] = bool(
                 np.array_equal(results[form]["stats"], ref["stats"]))
 
+    # Matched-error view on a subset: each form's error against a tight FP64 reference
+    # (rtol = atol = 1e-12), next to the FP64 oracle's own error at the workload tolerance.
+    sub = 512
+    tight = oracle.solve(y0[:sub], t1[:sub], mu[:sub, None], ts=ts[:sub], solver="tsit5",
+                         vf="vdp", dt0=0.1, rtol=1e-12, atol=1e-12, max_steps=200000,
+                         nthreads=min(len(os.sched_getaffinity(0)), 16))["ys"]
+    scale = np.max(np.abs(tight))
+    for form in forms:
+        forms[form]["normwise_err_vs_tight_fp64_subset"] = float(
+            np.max(np.abs(results[form]["ys"][:sub] - tight)) / scale)
+    if ref is not None:
+        ref_err = float(np.max(np.abs(ref["ys"][:sub] - tight)) / scale)
+        for form in forms:
+            forms[form]["fp64_oracle_normwise_err_vs_tight_subset"] = ref_err
+
     eligible = [f for f in forms if forms[f]["max_rel_err_vs_fp64"] <= 1e-6]
     head = max(eligible, key=lambda f: forms[f]["solves_per_s"]) if eligible else "f64"
     h = forms[head]

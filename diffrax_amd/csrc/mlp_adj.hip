@@ -8,14 +8,15 @@
 // pass (gradient mode) recorded the accepted (t_n, t_{n+1}) and the state at every C-th
 // accepted step.  The backward walks segments of C steps from the last one; for each
 // segment and each tile of 32 trajectories this kernel
-//   1. recomputes the segment forward from its checkpoint, keeping the stage inputs z_i of
-//      every step (identical arithmetic to the forward kernel);
+//   1. recomputes the segment forward from its checkpoint (identical arithmetic to the
+//      forward kernel), keeping every stage's f(z_i) in scratch and its layer inputs
+//      (z_i, h1, h2) in blocked row buffers;
 //   2. runs the exact reverse sweep: cotangents of the SaveAt(ts) values enter through the
 //      Tsit5 interpolant (k_i gets b_i(theta) ybar_s), the SSAL output y_{n+1} = z_6, and
 //      each stage k_i = dt f(z_i) is pulled back through the MLP on MFMA
-//      (tanh' = 1 - f^2, softplus' = 1 - exp(-softplus));
-//   3. stages the per-row factors (delta_l, layer input) of every stage in blocked row
-//      buffers.
+//      (tanh' = 1 - f^2 from the kept f, softplus' = 1 - exp(-h) from the kept h), with no
+//      forward re-evaluation;
+//   3. stages the deltas of every stage in blocked row buffers beside the layer inputs.
 // A split-K MFMA GEMM then forms dW_l = sum_rows delta_l^T input_l for the segment, and a
 // fixed-order reduction adds it to the float32 accumulators (deterministic result).
 #include <hip/hip_runtime.h>
@@ -65,6 +66,14 @@ __device__ __forceinline__ void rows_out(const typename P::act_t* lds, int ld, i
   }
 }
 
+// Element (row, col) of a blocked row buffer of width Wd.
+template <class P>
+__device__ __forceinline__ float row_val(const typename P::act_t* buf, int Wd, int64_t row,
+                                         int col) {
+  constexpr int KA = P::KA;
+  return (float)buf[((row / KA) * Wd + col) * KA + row % KA];
+}
+
 template <class P>
 __global__ void __launch_bounds__(MLP_THREADS, 1)
 mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> R) {
@@ -85,10 +94,21 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
   const int dir = (a.t0 < a.t1) ? 1 : -1;
   TabRegs<Tab> tr;
   tr.load();
-  float* zs_tile = a.zs + (int64_t)tile * a.Cs * S * MLP_TILE * MLP_D;
-  auto zslot = [&](int q, int i) { return zs_tile + (((int64_t)q * S + i) * MLP_TILE + orow) * MLP_D + c0; };
+  float* fs_tile = a.zs + (int64_t)tile * a.Cs * S * MLP_TILE * MLP_D;
+  auto fslot = [&](int q, int i) { return fs_tile + (((int64_t)q * S + i) * MLP_TILE + orow) * MLP_D + c0; };
+  // Rows of stage (q, i) in the segment's row buffers.
+  auto stage_row0 = [&](int q, int i) { return ((int64_t)q * S + i) * a.B_pad + (int64_t)tile * MLP_TILE; };
+  // The forward factors of the weight gradients (layer inputs z, h1, h2) straight from LDS
+  // after the stage's MLP evaluation.
+  auto stash_rows = [&](int q, int i) {
+    const int64_t r0 = stage_row0(q, i);
+    rows_out<P>(L.h1, L.LDH, MLP_H, R.b[3], r0);
+    rows_out<P>(L.h2, L.LDH, MLP_H, R.b[5], r0);
+    rows_out<P>(L.x, L.LDX, MLP_D, R.b[1], r0);
+  };
 
-  // ---- 1. recompute the segment's stage inputs ----
+  // ---- 1. recompute the segment: keep every stage's f (scratch) and its layer inputs (row
+  //      buffers), so that the reverse sweep needs no forward re-evaluation ----
   float y[8], fl[8];
 #pragma unroll
   for (int j = 0; j < 8; ++j) y[j] = (steps_here > 0) ? a.ckpt[(traj * a.n_ckpt + a.seg) * MLP_D + c0 + j] : 0.0f;
@@ -104,28 +124,36 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, y[j]);
       __syncthreads();
       mlp_forward<P>(W, L, wave, lane, orow, c0, fl);
+      stash_rows(0, 0);
+      __syncthreads();
     }
     {
-      float* zp = zslot(q, 0);
+      // stage 0 is the FSAL evaluation: f(y_n), whose rows the previous step's last stage
+      // (same input, z_6 = y_{n+1}) or the evaluation above wrote
+      float* fp = fslot(q, 0);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) { zp[j] = y[j]; k[0][j] = control * fl[j]; }
+      for (int j = 0; j < 8; ++j) { fp[j] = fl[j]; k[0][j] = control * fl[j]; }
     }
     static_for<1, S>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      float* zp = zslot(q, i);
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         float kc[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
         z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, 0.0f);
-        zp[j] = z[j];
         store_input<P>(L, orow, c0 + j, z[j]);
       }
       __syncthreads();
       mlp_forward<P>(W, L, wave, lane, orow, c0, f);
+      stash_rows(q, i);
+      if constexpr (i == S - 1) {
+        if (q + 1 < a.Cs) stash_rows(q + 1, 0);
+      }
+      __syncthreads();
+      float* fp = fslot(q, i);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) k[i][j] = control * f[j];
+      for (int j = 0; j < 8; ++j) { fp[j] = f[j]; k[i][j] = control * f[j]; }
     });
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
@@ -179,61 +207,52 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
         }
       }
     }
-    const int64_t rowbase_q = (int64_t)q * S;
     // stages in reverse
     static_for<0, S>([&](auto ic) {
       constexpr int i = S - 1 - decltype(ic)::value;
-      const int64_t row0 = (rowbase_q + i) * a.B_pad + (int64_t)tile * MLP_TILE;
-      const float* zp = zslot(q, i);
-      float fb[8];
+      const int64_t row0 = stage_row0(q, i);
+      const float* fp = fslot(q, i);
+      // delta3 = fbar * (1 - f^2), fbar = control * kbar_i
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
-        fb[j] = act ? control * kb[i][j] : 0.0f;
-        store_input<P>(L, orow, c0 + j, act ? zp[j] : 0.0f);
+        const float fb = act ? control * kb[i][j] : 0.0f;
+        const float fv = fp[j];
+        L.d3[orow * L.LDX + c0 + j] = P::to_act(fb * (1.0f - fv * fv));
       }
       __syncthreads();
-      float f[8];
-      mlp_forward<P>(W, L, wave, lane, orow, c0, f);
-      // delta3 = fbar * (1 - f^2)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) L.d3[orow * L.LDX + c0 + j] = P::to_act(fb[j] * (1.0f - f[j] * f[j]));
-      __syncthreads();
       rows_out<P>(L.d3, L.LDX, MLP_D, R.b[4], row0);
-      rows_out<P>(L.h2, L.LDH, MLP_H, R.b[5], row0);
-      // h2bar = d3 . W3 ; delta2 = h2bar * softplus'(.)
+      // h2bar = d3 . W3 ; delta2 = h2bar * softplus'(.) (h2 from its row buffer)
       floatx16_t acc[2];
-      wave_gemm<P, MLP_D / P::KSTEP, 2>(L.d3, L.LDX, W.w3t, 2 * wave, 0, MLP_D / P::KSTEP, acc, lane);
+      wave_gemm<P, MLP_D / P::KSTEP, 2>(L.d3, L.d3, L.LDX, W.w3t, 2 * wave, 0, MLP_D / P::KSTEP, acc, lane);
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const int col = 64 * wave + 32 * t + (lane & 31);
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const int r = acc_row(lane, v);
-          const float h = (float)L.h2[r * L.LDH + col];
+          const float h = row_val<P>(R.b[5], MLP_H, row0 + r, col);
           L.d[r * L.LDH + col] = P::to_act(acc[t][v] * dsoftplus_from_value<F>(h));
         }
       }
       __syncthreads();
-      // h1bar = d2 . W2 ; delta1 = h1bar * softplus'(.), stored over h2 (no longer needed)
-      wave_gemm<P, MLP_H / P::KSTEP, 2>(L.d, L.LDH, W.w2t, 2 * wave, 0, MLP_H / P::KSTEP, acc, lane);
+      // h1bar = d2 . W2 ; delta1 = h1bar * softplus'(.), kept in L.h2
+      wave_gemm<P, MLP_H / P::KSTEP, 2>(L.d, L.d, L.LDH, W.w2t, 2 * wave, 0, MLP_H / P::KSTEP, acc, lane);
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const int col = 64 * wave + 32 * t + (lane & 31);
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const int r = acc_row(lane, v);
-          const float h = (float)L.h1[r * L.LDH + col];
+          const float h = row_val<P>(R.b[3], MLP_H, row0 + r, col);
           L.h2[r * L.LDH + col] = P::to_act(acc[t][v] * dsoftplus_from_value<F>(h));
         }
       }
       __syncthreads();
       rows_out<P>(L.d, L.LDH, MLP_H, R.b[2], row0);
-      rows_out<P>(L.h1, L.LDH, MLP_H, R.b[3], row0);
       rows_out<P>(L.h2, L.LDH, MLP_H, R.b[0], row0);
-      rows_out<P>(L.x, L.LDX, MLP_D, R.b[1], row0);
       // zbar = d1 . W1 (split-K over the 256 hidden units)
       constexpr int KS1 = MLP_H / P::KSTEP;
-      wave_gemm<P, KS1, 2>(L.h2, L.LDH, W.w1t, 0, wave * (KS1 / 4), (wave + 1) * (KS1 / 4), acc, lane);
+      wave_gemm<P, KS1, 2>(L.h2, L.h2, L.LDH, W.w1t, 0, wave * (KS1 / 4), (wave + 1) * (KS1 / 4), acc, lane);
 #pragma unroll
       for (int t = 0; t < 2; ++t) {
         const int col = 32 * t + (lane & 31);
@@ -283,9 +302,16 @@ __host__ __device__ inline DwLayout dw_layout() {
   return l;
 }
 
+// One 32x32 output tile of one chunk of ADJ_CHUNK rows: partial[chunk][tile] = A^T B over
+// the chunk (A = deltas, B = layer inputs), 8 k-steps of fragments loaded ahead of their
+// MFMAs.  The row buffers are allocated in whole chunks and rows past the segment's last are
+// zero, so every chunk runs its full length.  Tiles in the first tile column of a matrix
+// also produce the chunk's column sums of A (the bias gradient partials, partial_db:
+// [chunk][db1 | db2 | db3]).
 template <class P>
-__global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R, int64_t rows,
-                                                         float* __restrict__ partial) {
+__global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R,
+                                                         float* __restrict__ partial,
+                                                         float* __restrict__ partial_db) {
   const DwLayout lay = dw_layout();
   const int tg = blockIdx.x;
   const int chunk = blockIdx.y;
@@ -298,39 +324,41 @@ __global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R, int
   const typename P::act_t* Bm = R.b[2 * m + 1];  // layer inputs (width n2)
   const int WA = lay.n1[m], WB = lay.n2[m];
   const int lane = threadIdx.x;
-  constexpr int KA = P::KA, KS = P::KSTEP;
+  constexpr int KA = P::KA, KS = P::KSTEP, U = 8;
+  static_assert(ADJ_CHUNK % (KS * U) == 0, "chunk must hold whole k-step groups");
+  const bool col_sums = (tj == 0);
   const int64_t r_begin = (int64_t)chunk * ADJ_CHUNK;
-  int64_t r_end = r_begin + ADJ_CHUNK;
-  if (r_end > rows) r_end = rows;
   floatx16_t acc = floatx16_t{};
+  float csum = 0.0f;
   const int ca = ti * 32 + (lane & 31), cb = tj * 32 + (lane & 31);
-  for (int64_t r0 = r_begin; r0 < r_end; r0 += KS) {
-    const int64_t rb = r0 / KA + (lane >> 5);
-    const typename P::frag_t fa = P::load_a(A + (rb * WA + ca) * KA);
-    const typename P::frag_t fb = P::load_a(Bm + (rb * WB + cb) * KA);
-    acc = P::mfma(fa, fb, acc);
+  for (int64_t r0 = r_begin; r0 < r_begin + ADJ_CHUNK; r0 += KS * U) {
+    typename P::frag_t fa[U], fb[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t rb = (r0 + u * KS) / KA + (lane >> 5);
+      fa[u] = P::load_a(A + (rb * WA + ca) * KA);
+      fb[u] = P::load_a(Bm + (rb * WB + cb) * KA);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      acc = P::mfma(fa[u], fb[u], acc);
+      if (col_sums) {
+        if constexpr (KA == 1) {
+          csum += fa[u];
+        } else {
+#pragma unroll
+          for (int j = 0; j < KA; ++j) csum += (float)fa[u][j];
+        }
+      }
+    }
   }
   float* o = partial + ((int64_t)chunk * lay.first[3] + tg) * 1024 + lane * 16;
 #pragma unroll
   for (int v = 0; v < 16; ++v) o[v] = acc[v];
-}
-
-// Column sums of the three delta buffers for one chunk: bias gradients.
-template <class P>
-__global__ void mlp_db_kernel(const RowBufs<P> R, int64_t rows, float* __restrict__ partial_db) {
-  const int chunk = blockIdx.x;
-  constexpr int KA = P::KA;
-  const int64_t r_begin = (int64_t)chunk * ADJ_CHUNK;
-  int64_t r_end = r_begin + ADJ_CHUNK;
-  if (r_end > rows) r_end = rows;
-  for (int col = threadIdx.x; col < MLP_H + MLP_H + MLP_D; col += blockDim.x) {
-    const int which = col < MLP_H ? 0 : (col < 2 * MLP_H ? 2 : 4);
-    const int c = col < MLP_H ? col : (col < 2 * MLP_H ? col - MLP_H : col - 2 * MLP_H);
-    const int Wd = (which == 4) ? MLP_D : MLP_H;
-    const typename P::act_t* D = R.b[which];
-    float s = 0.0f;
-    for (int64_t r = r_begin; r < r_end; ++r) s += (float)D[((r / KA) * Wd + c) * KA + r % KA];
-    partial_db[(int64_t)chunk * (2 * MLP_H + MLP_D) + col] = s;
+  if (col_sums) {
+    csum += __shfl_xor(csum, 32);
+    const int off = (m == 0) ? 0 : (m == 1 ? MLP_H : 2 * MLP_H);
+    if (lane < 32) partial_db[(int64_t)chunk * (2 * MLP_H + MLP_D) + off + ca] = csum;
   }
 }
 
@@ -418,14 +446,12 @@ int run_segment(const dfx_mlp_problem* p, const void* packed, const float* b1, c
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<P>));
   hipLaunchKernelGGL(mlp_adj_segment_kernel<P>, dim3(ntiles), dim3(MLP_THREADS), sizeof(MlpLds<P>),
                      s, a, W, R);
-  const int64_t rows = seg_rows(p->batch, seg_steps);
   const int nch = (int)seg_chunks(p->batch, seg_steps);
   const DwLayout lay = dw_layout();
   float* partial = io->partial;
   float* partial_db = io->partial + (int64_t)nch * lay.first[3] * 1024;
   hipLaunchKernelGGL(mlp_dw_gemm_kernel<P>, dim3((unsigned)lay.first[3], (unsigned)nch), dim3(64), 0,
-                     s, R, rows, partial);
-  hipLaunchKernelGGL(mlp_db_kernel<P>, dim3((unsigned)nch), dim3(256), 0, s, R, rows, partial_db);
+                     s, R, partial, partial_db);
   const int nout = MLP_H * MLP_D + MLP_H * MLP_H + MLP_D * MLP_H + 2 * MLP_H + MLP_D;
   hipLaunchKernelGGL(mlp_dw_reduce_kernel, dim3((nout + 255) / 256), dim3(256), 0, s, partial,
                      partial_db, nch, io->dw);
@@ -456,7 +482,10 @@ int dfx_mlp_adj_segment(const dfx_mlp_problem* p, const void* packed, const floa
                         int32_t segment, int32_t seg_steps, void* stream) {
   if (!p || !packed || !b1 || !b2 || !b3 || !io) return DFX_E_ARG;
   if (p->d != MLP_D || p->hidden != MLP_H || p->solver != DFX_TSIT5) return DFX_E_ARG;
-  if (p->ckpt_every <= 0 || seg_steps <= 0 || seg_steps > p->ckpt_every || segment < 0) return DFX_E_ARG;
+  if (p->precision != DFX_F32 && p->precision != DFX_BF16_F32) return DFX_E_ARG;
+  if (p->ckpt_every <= 0 || seg_steps <= 0 || seg_steps > p->ckpt_every || segment < 0 ||
+      segment >= dfx_mlp_n_ckpt(p))
+    return DFX_E_ARG;
   if (!io->grid || !io->ckpt || !io->stats || !io->ys_bar || !io->ybar || !io->zs || !io->rows ||
       !io->partial || !io->dw)
     return DFX_E_ARG;

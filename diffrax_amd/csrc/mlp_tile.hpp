@@ -149,24 +149,39 @@ __device__ __forceinline__ float dsoftplus_from_value(float h) {
 }
 
 // ---- one wave's share of a layer --------------------------------------------------------
-// acc[nt] (32x32 f32) = sum over k-steps [ks0, ks1) of A(lds rows, k) * B(fragment (nt0+nt, ks)).
-template <class P, int NKS, int NT, bool Accumulate = false>
-__device__ __forceinline__ void wave_gemm(const typename P::act_t* a_lds, int lda,
+// acc[nt] (32x32 f32) = sum over k-steps [ks0, ks1) of A(lds rows, k) * B(fragment (nt0+nt, ks)),
+// where A = a_lds, or a_lds + a_lo (LO: the hi + lo halves of a split operand, two MFMAs per
+// B fragment).  Each B fragment is loaded from global memory (L2-resident) once, one k-step
+// ahead of its MFMAs.
+template <class P, int NKS, int NT, bool LO = false>
+__device__ __forceinline__ void wave_gemm(const typename P::act_t* a_lds,
+                                          const typename P::act_t* a_lo, int lda,
                                           const typename P::act_t* wp, int nt0, int ks0,
                                           int ks1, floatx16_t (&acc)[NT], int lane) {
+  using frag_t = typename P::frag_t;
   const int row = lane & 31;
   const int kh = (lane >> 5) * P::KA;
-  if constexpr (!Accumulate) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = floatx16_t{};
-  }
-#pragma unroll 4
+  for (int t = 0; t < NT; ++t) acc[t] = floatx16_t{};
+  frag_t bn[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) bn[t] = P::load_b(wp, ((nt0 + t) * NKS + ks0) * 64 + lane);
+#pragma unroll 2
   for (int ks = ks0; ks < ks1; ++ks) {
-    const typename P::frag_t a = P::load_a(a_lds + row * lda + ks * P::KSTEP + kh);
+    frag_t b[NT];
+    const int kn = (ks + 1 < ks1) ? ks + 1 : ks;  // last step re-reads its own fragment
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      const typename P::frag_t b = P::load_b(wp, ((nt0 + t) * NKS + ks) * 64 + lane);
-      acc[t] = P::mfma(a, b, acc[t]);
+      b[t] = bn[t];
+      bn[t] = P::load_b(wp, ((nt0 + t) * NKS + kn) * 64 + lane);
+    }
+    const frag_t a = P::load_a(a_lds + row * lda + ks * P::KSTEP + kh);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = P::mfma(a, b[t], acc[t]);
+    if constexpr (LO) {
+      const frag_t al = P::load_a(a_lo + row * lda + ks * P::KSTEP + kh);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = P::mfma(al, b[t], acc[t]);
     }
   }
 }
@@ -184,10 +199,9 @@ __device__ __forceinline__ void mlp_forward(const MlpWeights<P>& W, MlpLds<P>& L
   constexpr bool F = P::fast_math;
   floatx16_t acc[2];
   // layer 1: [32 x 64] . W1^T -> [32 x 256], wave's columns [64w, 64w + 64)
-  wave_gemm<P, MLP_D / P::KSTEP, 2>(L.x, L.LDX, W.w1, 2 * wave, 0, MLP_D / P::KSTEP, acc, lane);
-  if constexpr (P::split_input)
-    wave_gemm<P, MLP_D / P::KSTEP, 2, true>(L.xlo, L.LDX, W.w1, 2 * wave, 0, MLP_D / P::KSTEP, acc,
-                                            lane);
+  constexpr bool S = P::split_input;
+  wave_gemm<P, MLP_D / P::KSTEP, 2, S>(L.x, L.xlo, L.LDX, W.w1, 2 * wave, 0, MLP_D / P::KSTEP, acc,
+                                       lane);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int col = 64 * wave + 32 * t + (lane & 31);
@@ -198,10 +212,8 @@ __device__ __forceinline__ void mlp_forward(const MlpWeights<P>& W, MlpLds<P>& L
   }
   __syncthreads();
   // layer 2: [32 x 256] . W2^T -> [32 x 256]
-  wave_gemm<P, MLP_H / P::KSTEP, 2>(L.h1, L.LDH, W.w2, 2 * wave, 0, MLP_H / P::KSTEP, acc, lane);
-  if constexpr (P::split_input)
-    wave_gemm<P, MLP_H / P::KSTEP, 2, true>(L.h1lo, L.LDH, W.w2, 2 * wave, 0, MLP_H / P::KSTEP, acc,
-                                            lane);
+  wave_gemm<P, MLP_H / P::KSTEP, 2, S>(L.h1, L.h1lo, L.LDH, W.w2, 2 * wave, 0, MLP_H / P::KSTEP,
+                                       acc, lane);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int col = 64 * wave + 32 * t + (lane & 31);
@@ -213,10 +225,8 @@ __device__ __forceinline__ void mlp_forward(const MlpWeights<P>& W, MlpLds<P>& L
   __syncthreads();
   // layer 3 (split-K): wave w sums K in [64w, 64w + 64) for all 64 outputs
   constexpr int KS3 = MLP_H / P::KSTEP;
-  wave_gemm<P, KS3, 2>(L.h2, L.LDH, W.w3, 0, wave * (KS3 / 4), (wave + 1) * (KS3 / 4), acc, lane);
-  if constexpr (P::split_input)
-    wave_gemm<P, KS3, 2, true>(L.h2lo, L.LDH, W.w3, 0, wave * (KS3 / 4), (wave + 1) * (KS3 / 4), acc,
-                               lane);
+  wave_gemm<P, KS3, 2, S>(L.h2, L.h2lo, L.LDH, W.w3, 0, wave * (KS3 / 4), (wave + 1) * (KS3 / 4),
+                          acc, lane);
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int col = 32 * t + (lane & 31);

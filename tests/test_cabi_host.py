@@ -118,3 +118,58 @@ def test_registered_fields_evaluate_like_formula():
                                           dtype=torch.float64))
     g = dfx.VanDerPol()(0.0, y, 2.0)
     assert torch.allclose(g, torch.tensor([[0.7, 2.0 * (1 - 1.44) * 0.7 - 1.2]], dtype=torch.float64))
+
+
+def test_sde_sizes_and_argument_checks():
+    from diffrax_amd import _sde
+    lib = _lib.load()
+    p = _sde.SdeProblem(precision=_lib.F32, d=16, paths=1000, num_steps=1000,
+                        checkpoint_every=32, save_t0=0, save_t1=1)
+    assert lib.dfx_sde_n_ckpt(ctypes.byref(p)) == 31          # ceil(1000 / 32) - 1
+    nb = ctypes.c_size_t(0)
+    assert lib.dfx_sde_workspace_bytes(ctypes.byref(p), ctypes.byref(nb)) == 0
+    assert nb.value == ((1000 * 16 + 255) // 256) * 3 * 16 * 8 + 256
+    io, g = _sde.SdeIO(), _sde.SdeGradIO()
+    # missing device pointers, a dimension the float32 kernel cannot split (d % 4),
+    # an unsupported precision and an adjoint segment length outside {8, 16, 32}
+    assert lib.dfx_sde_heun_solve(ctypes.byref(p), ctypes.byref(io), None) == -1
+    for field, bad in (("d", 6), ("precision", 7)):
+        q = _sde.SdeProblem(precision=p.precision, d=p.d, paths=p.paths, num_steps=p.num_steps,
+                            checkpoint_every=p.checkpoint_every, save_t0=0, save_t1=1)
+        setattr(q, field, bad)
+        assert lib.dfx_sde_heun_solve(ctypes.byref(q), ctypes.byref(io), None) == -1
+    p.checkpoint_every = 12
+    assert lib.dfx_sde_heun_adjoint(ctypes.byref(p), ctypes.byref(io), ctypes.byref(g), None, 0,
+                                    None) == -1
+
+
+def test_mlp_sizes_and_argument_checks():
+    from diffrax_amd import _mlp
+    lib = _lib.load()
+    nb = ctypes.c_size_t(0)
+    assert lib.dfx_mlp_pack_bytes(_lib.F32, ctypes.byref(nb)) == 0
+    assert nb.value == 2 * (256 * 64 + 256 * 256 + 64 * 256) * 4   # forward + transposed
+    assert lib.dfx_mlp_pack_bytes(_lib.BF16_F32, ctypes.byref(nb)) == 0
+    assert nb.value == 2 * (256 * 64 + 256 * 256 + 64 * 256) * 2
+    assert lib.dfx_mlp_pack_bytes(_lib.F64, ctypes.byref(nb)) == -1
+    p = _mlp.MlpProblem(solver=_lib.TSIT5, precision=_lib.F32, d=64, hidden=256, n_ts=32,
+                        max_steps=4096, ckpt_every=64, batch=8192)
+    assert lib.dfx_mlp_n_ckpt(ctypes.byref(p)) == 64
+    io = _mlp.MlpIO()
+    z = ctypes.c_void_p(0)
+    for field, bad in (("d", 32), ("hidden", 128), ("precision", _lib.F64), ("solver", _lib.DOPRI5)):
+        q = _mlp.MlpProblem(solver=_lib.TSIT5, precision=_lib.F32, d=64, hidden=256, n_ts=32,
+                            max_steps=4096, ckpt_every=64, batch=8192)
+        setattr(q, field, bad)
+        assert lib.dfx_mlp_solve(ctypes.byref(q), z, z, z, z, ctypes.byref(io), None) == -1
+    adj = _mlp.MlpAdjIO()
+    # segment past the last checkpoint, segment longer than the checkpoint spacing
+    for seg, steps in ((64, 64), (0, 65)):
+        assert lib.dfx_mlp_adj_segment(ctypes.byref(p), z, z, z, z, ctypes.byref(adj), seg, steps,
+                                       None) == -1
+    zs, rows, part = (ctypes.c_size_t(0) for _ in range(3))
+    assert lib.dfx_mlp_adj_workspace(ctypes.byref(p), 4, ctypes.byref(zs), ctypes.byref(rows),
+                                     ctypes.byref(part)) == 0
+    assert zs.value == 8192 * 4 * 7 * 64 * 4       # f of every stage of the segment
+    chunks = -(-(4 * 7 * 8192) // 4096)
+    assert rows.value == chunks * 4096 * (256 + 64 + 256 + 256 + 64 + 256) * 4
