@@ -151,8 +151,9 @@ __device__ __forceinline__ float dsoftplus_from_value(float h) {
 // ---- one wave's share of a layer --------------------------------------------------------
 // acc[nt] (32x32 f32) = sum over k-steps [ks0, ks1) of A(lds rows, k) * B(fragment (nt0+nt, ks)),
 // where A = a_lds, or a_lds + a_lo (LO: the hi + lo halves of a split operand, two MFMAs per
-// B fragment).  Each B fragment is loaded from global memory (L2-resident) once, one k-step
-// ahead of its MFMAs.
+// B fragment).  16-byte fragments (bf16) are loaded one k-step ahead of their MFMAs; 4-byte
+// fragments (f32, 2 k per MFMA) are left to the compiler's unrolled schedule, where explicit
+// prefetch only adds instructions.
 template <class P, int NKS, int NT, bool LO = false>
 __device__ __forceinline__ void wave_gemm(const typename P::act_t* a_lds,
                                           const typename P::act_t* a_lo, int lda,
@@ -163,25 +164,38 @@ __device__ __forceinline__ void wave_gemm(const typename P::act_t* a_lds,
   const int kh = (lane >> 5) * P::KA;
 #pragma unroll
   for (int t = 0; t < NT; ++t) acc[t] = floatx16_t{};
-  frag_t bn[NT];
+  if constexpr (P::KA == 1) {
+    static_assert(!LO, "split operands are bfloat16");
+#pragma unroll 4
+    for (int ks = ks0; ks < ks1; ++ks) {
+      const frag_t a = P::load_a(a_lds + row * lda + ks * P::KSTEP + kh);
 #pragma unroll
-  for (int t = 0; t < NT; ++t) bn[t] = P::load_b(wp, ((nt0 + t) * NKS + ks0) * 64 + lane);
-#pragma unroll 2
-  for (int ks = ks0; ks < ks1; ++ks) {
-    frag_t b[NT];
-    const int kn = (ks + 1 < ks1) ? ks + 1 : ks;  // last step re-reads its own fragment
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      b[t] = bn[t];
-      bn[t] = P::load_b(wp, ((nt0 + t) * NKS + kn) * 64 + lane);
+      for (int t = 0; t < NT; ++t) {
+        const frag_t b = P::load_b(wp, ((nt0 + t) * NKS + ks) * 64 + lane);
+        acc[t] = P::mfma(a, b, acc[t]);
+      }
     }
-    const frag_t a = P::load_a(a_lds + row * lda + ks * P::KSTEP + kh);
+  } else {
+    frag_t bn[NT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = P::mfma(a, b[t], acc[t]);
-    if constexpr (LO) {
-      const frag_t al = P::load_a(a_lo + row * lda + ks * P::KSTEP + kh);
+    for (int t = 0; t < NT; ++t) bn[t] = P::load_b(wp, ((nt0 + t) * NKS + ks0) * 64 + lane);
+#pragma unroll 2
+    for (int ks = ks0; ks < ks1; ++ks) {
+      frag_t b[NT];
+      const int kn = (ks + 1 < ks1) ? ks + 1 : ks;  // last step re-reads its own fragment
 #pragma unroll
-      for (int t = 0; t < NT; ++t) acc[t] = P::mfma(al, b[t], acc[t]);
+      for (int t = 0; t < NT; ++t) {
+        b[t] = bn[t];
+        bn[t] = P::load_b(wp, ((nt0 + t) * NKS + kn) * 64 + lane);
+      }
+      const frag_t a = P::load_a(a_lds + row * lda + ks * P::KSTEP + kh);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) acc[t] = P::mfma(a, b[t], acc[t]);
+      if constexpr (LO) {
+        const frag_t al = P::load_a(a_lo + row * lda + ks * P::KSTEP + kh);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = P::mfma(al, b[t], acc[t]);
+      }
     }
   }
 }
