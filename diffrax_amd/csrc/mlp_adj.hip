@@ -308,13 +308,22 @@ __host__ __device__ inline DwLayout dw_layout() {
 // zero, so every chunk runs its full length.  Tiles in the first tile column of a matrix
 // also produce the chunk's column sums of A (the bias gradient partials, partial_db:
 // [chunk][db1 | db2 | db3]).
+// Launch order is XCD-aware: workgroups are dealt to the 8 XCDs round-robin by linear id,
+// so block b runs on XCD b % 8; XCD x takes chunks x, x + 8, ... and runs the 96 tiles of a
+// chunk back to back, which then read the chunk's rows through one L2 instead of eight.
+constexpr int kXcds = 8;
+
 template <class P>
 __global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R,
                                                          float* __restrict__ partial,
-                                                         float* __restrict__ partial_db) {
+                                                         float* __restrict__ partial_db,
+                                                         int nch) {
   const DwLayout lay = dw_layout();
-  const int tg = blockIdx.x;
-  const int chunk = blockIdx.y;
+  const int ntile = lay.first[3];
+  const int xcd = (int)(blockIdx.x % kXcds), slot = (int)(blockIdx.x / kXcds);
+  const int chunk = (slot / ntile) * kXcds + xcd;
+  const int tg = slot - (slot / ntile) * ntile;
+  if (chunk >= nch) return;
   int m = 0;
   while (tg >= lay.first[m + 1]) ++m;
   const int t = tg - lay.first[m];
@@ -450,8 +459,9 @@ int run_segment(const dfx_mlp_problem* p, const void* packed, const float* b1, c
   const DwLayout lay = dw_layout();
   float* partial = io->partial;
   float* partial_db = io->partial + (int64_t)nch * lay.first[3] * 1024;
-  hipLaunchKernelGGL(mlp_dw_gemm_kernel<P>, dim3((unsigned)lay.first[3], (unsigned)nch), dim3(64), 0,
-                     s, R, partial, partial_db);
+  const unsigned gemm_blocks = (unsigned)(kXcds * lay.first[3] * ((nch + kXcds - 1) / kXcds));
+  hipLaunchKernelGGL(mlp_dw_gemm_kernel<P>, dim3(gemm_blocks), dim3(64), 0, s, R, partial,
+                     partial_db, nch);
   const int nout = MLP_H * MLP_D + MLP_H * MLP_H + MLP_D * MLP_H + 2 * MLP_H + MLP_D;
   hipLaunchKernelGGL(mlp_dw_reduce_kernel, dim3((nout + 255) / 256), dim3(256), 0, s, partial,
                      partial_db, nch, io->dw);
