@@ -14,8 +14,8 @@ from ._solution import RESULTS, Solution, is_okay, is_successful
 from ._solver import (AbstractAdaptiveSolver, AbstractERK, AbstractSolver, ButcherTableau,
                       Dopri5, Heun, Tsit5)
 from ._step_size_controller import (AbstractAdaptiveStepSizeController,
-                                    AbstractStepSizeController, ConstantStepSize,
-                                    PIDController, rms_norm)
+                                    AbstractStepSizeController, ClipStepSizeController,
+                                    ConstantStepSize, PIDController, rms_norm)
 from ._term import (AbstractPath, AbstractTerm, ControlTerm, IncrementPath, MultiTerm,
                     ODETerm)
 from ._vector_fields import (DiagonalNoise, LinearField, LotkaVolterra, MLPField, OUDrift,

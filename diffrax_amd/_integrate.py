@@ -19,7 +19,7 @@ from ._adjoint import AbstractAdjoint, RecursiveCheckpointAdjoint
 from ._saveat import SaveAt, save_y
 from ._solution import RESULTS, Solution
 from ._solver import AbstractERK, AbstractSolver
-from ._step_size_controller import ConstantStepSize, PIDController
+from ._step_size_controller import ClipStepSizeController, ConstantStepSize, PIDController
 from ._term import ControlTerm, IncrementPath, MultiTerm, ODETerm
 from ._vector_fields import DiagonalNoise, MLPField, OUDrift, RegisteredVectorField
 
@@ -81,7 +81,8 @@ class ErkPlan:
         self.launch_cfg = launch_cfg
         self.direction_sign = direction_sign
         io = _lib.ErkIO()
-        for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid"):
+        for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid",
+                     "dense_ts", "dense_ys", "dense_k", "step_ts", "jump_ts"):
             v = tensors.get(name)
             setattr(io, name, v.data_ptr() if v is not None else None)
         self.io = io
@@ -167,13 +168,16 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
     else:
         dt0f = None
 
-    if saveat.dense:
-        raise NotImplementedError("SaveAt(dense=True) is not provided by diffrax_amd")
     if saveat.solver_state or saveat.controller_state or saveat.made_jump:
         raise NotImplementedError("saving solver/controller state is not provided by diffrax_amd")
     sub = saveat.subs
     if sub is None:
-        raise ValueError("SaveAt must save something")
+        if not saveat.dense:
+            raise ValueError("SaveAt must save something")
+        from ._saveat import SubSaveAt
+        sub = SubSaveAt.__new__(SubSaveAt)   # dense output only: no saved rows
+        sub.t0 = sub.t1 = False
+        sub.ts, sub.steps, sub.fn = None, 0, save_y
     if not hasattr(sub, "fn"):
         raise NotImplementedError("only a single SubSaveAt is supported")
     if sub.fn is not save_y:
@@ -181,6 +185,8 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
     if sub.steps not in (0, 1):
         raise NotImplementedError("SaveAt(steps=n) with n > 1 is not provided by diffrax_amd")
     if max_steps is None:
+        if saveat.dense:
+            raise ValueError("`max_steps=None` is incompatible with `saveat.dense=True`")
         raise NotImplementedError("max_steps=None is not supported; pass an integer")
 
     ts_t = None
@@ -197,6 +203,15 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
             raise ValueError(f"SaveAt(ts) must be [n] or [{B}, n]")
         _check_ts(ts_t.cpu().numpy(), t0f, t1_host, directions)
 
+    clip = None
+    if isinstance(stepsize_controller, ClipStepSizeController):
+        clip = stepsize_controller
+        if not isinstance(clip.controller, PIDController):
+            raise NotImplementedError(
+                "ClipStepSizeController is provided around a PIDController only")
+        if clip.store_rejected_steps is not None or clip.callback_on_reject is not None:
+            raise NotImplementedError("store_rejected_steps is not provided by diffrax_amd")
+        stepsize_controller = clip.controller
     if isinstance(stepsize_controller, PIDController):
         c = stepsize_controller
         controller = _lib.PID
@@ -229,6 +244,12 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         ts_stride=ts_stride, param_stride=param_stride, t0=t0f,
         dt0=_nan_if_none(dt0f), rtol=rtol, atol=atol, pcoeff=pcoeff, icoeff=icoeff,
         dcoeff=dcoeff, **extra)
+    clip_t = {}
+    for name in ("step_ts", "jump_ts"):
+        arr = None if clip is None else getattr(clip, name)
+        if arr is not None and arr.size > 0:
+            clip_t[name] = torch.tensor(arr, dtype=torch.float64, device=device)
+            setattr(prob, "n_" + name, int(arr.size))
     lib = _lib.load()
     n_out = lib.dfx_erk_n_out(ctypes.byref(prob))
     tensors = dict(
@@ -239,7 +260,14 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         result=torch.empty((B,), dtype=torch.int32, device=device),
         grid=(torch.empty((B, int(max_steps), 2), dtype=torch.float64, device=device)
               if record_grid else None),
+        step_ts=clip_t.get("step_ts"), jump_ts=clip_t.get("jump_ts"),
     )
+    if saveat.dense:
+        S = solver.tableau.num_stages
+        tensors.update(
+            dense_ts=torch.empty((B, int(max_steps) + 1), dtype=torch.float64, device=device),
+            dense_ys=torch.empty((B, int(max_steps) + 1, d), dtype=y0b.dtype, device=device),
+            dense_k=torch.empty((B, int(max_steps), S, d), dtype=y0b.dtype, device=device))
     return ErkPlan(prob, tensors, n_out, batched, directions, launch)
 
 
@@ -262,8 +290,69 @@ def _finish(plan, t0, t1, max_steps, throw):
             first = int(codes[bad][0].item())
             raise RuntimeError(RESULTS[first])
     res = result
-    sol = Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=res)
+    interp = DenseInterpolation(plan) if T.get("dense_ts") is not None else None
+    if plan.problem.n_ts == 0 and not (plan.problem.save_t0 or plan.problem.save_t1 or
+                                       plan.problem.save_steps):
+        ts = ys = None   # SaveAt(dense=True) alone saves no rows (as the reference)
+    sol = Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=res,
+                   interpolation=interp)
     return sol
+
+
+class DenseInterpolation:
+    """Dense output of a SaveAt(dense=True) solve (DenseInterpolation,
+    _global_interpolation.py:315-400), evaluated on the GPU by dfx_erk_dense_eval.
+
+    ``evaluate(t0, t1=None, left=True)`` and ``derivative(t, left=True)`` take a scalar
+    time, a 1-D tensor of times shared by the batch, or a [B, n] tensor of per-instance
+    times; they return [..., d] for every instance (the batch axis first when the solve was
+    batched, then the query axis when t was not a scalar)."""
+
+    def __init__(self, plan):
+        self.plan = plan
+        T = plan.t
+        self.t0 = plan.problem.t0
+        self.t1 = T["t1"]
+
+    def _query(self, t, derivative, left):
+        lib = _lib.load()
+        plan = self.plan
+        T = plan.t
+        dev = T["y0"].device
+        B, d = T["y0"].shape
+        tq = torch.as_tensor(t, dtype=torch.float64)
+        scalar = tq.ndim == 0
+        if tq.ndim == 0:
+            tq = tq.reshape(1)
+        if tq.ndim == 1:
+            n_q, stride = tq.shape[0], 0
+        elif tq.ndim == 2 and tq.shape[0] == B:
+            n_q, stride = tq.shape[1], tq.shape[1]
+        else:
+            raise ValueError(f"evaluation times must be a scalar, [n] or [{B}, n]")
+        tq = tq.to(dev).contiguous()
+        out = torch.empty((B, n_q, d), dtype=T["y0"].dtype, device=dev)
+        io = _lib.DenseIO(dense_ts=T["dense_ts"].data_ptr(), dense_ys=T["dense_ys"].data_ptr(),
+                          dense_k=T["dense_k"].data_ptr(), stats=T["stats"].data_ptr(),
+                          t1=T["t1"].data_ptr(), tq=tq.data_ptr(), n_q=n_q, tq_stride=stride,
+                          left=int(bool(left)), derivative=int(bool(derivative)),
+                          out=out.data_ptr())
+        _lib.check(lib.dfx_erk_dense_eval(ctypes.byref(plan.problem), ctypes.byref(io),
+                                          ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "dfx_erk_dense_eval")
+        if scalar:
+            out = out[:, 0]
+        if not plan.batched:
+            out = out[0]
+        return out
+
+    def evaluate(self, t0, t1=None, left: bool = True):
+        if t1 is not None:
+            return self.evaluate(t1, left=left) - self.evaluate(t0, left=left)
+        return self._query(t0, False, left)
+
+    def derivative(self, t, left: bool = True):
+        return self._query(t, True, left)
 
 
 def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None, *,

@@ -36,6 +36,7 @@ class ErkProblem(ctypes.Structure):
         ("dtmin", ctypes.c_double), ("dtmax", ctypes.c_double),
         ("factormin", ctypes.c_double), ("factormax", ctypes.c_double),
         ("safety", ctypes.c_double), ("error_order", ctypes.c_double),
+        ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32),
     ]
 
 
@@ -44,6 +45,17 @@ class ErkIO(ctypes.Structure):
         ("y0", ctypes.c_void_p), ("t1", ctypes.c_void_p), ("ts", ctypes.c_void_p),
         ("params", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("ts_out", ctypes.c_void_p),
         ("stats", ctypes.c_void_p), ("result", ctypes.c_void_p), ("grid", ctypes.c_void_p),
+        ("dense_ts", ctypes.c_void_p), ("dense_ys", ctypes.c_void_p), ("dense_k", ctypes.c_void_p),
+        ("step_ts", ctypes.c_void_p), ("jump_ts", ctypes.c_void_p),
+    ]
+
+
+class DenseIO(ctypes.Structure):
+    _fields_ = [
+        ("dense_ts", ctypes.c_void_p), ("dense_ys", ctypes.c_void_p), ("dense_k", ctypes.c_void_p),
+        ("stats", ctypes.c_void_p), ("t1", ctypes.c_void_p), ("tq", ctypes.c_void_p),
+        ("n_q", ctypes.c_int64), ("tq_stride", ctypes.c_int64), ("left", ctypes.c_int32),
+        ("derivative", ctypes.c_int32), ("out", ctypes.c_void_p),
     ]
 
 
@@ -63,6 +75,8 @@ _SIGS = {
                                 ctypes.c_int),
     "dfx_erk_solve": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(ErkIO), ctypes.c_void_p,
                        ctypes.c_size_t, ctypes.POINTER(Launch), ctypes.c_void_p], ctypes.c_int),
+    "dfx_erk_dense_eval": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(DenseIO), ctypes.c_void_p],
+                           ctypes.c_int),
 }
 
 _lib = None

@@ -34,22 +34,31 @@ class ConstantStepSize(AbstractStepSizeController):
         return "ConstantStepSize()"
 
 
-class PIDController(AbstractAdaptiveStepSizeController):
+class _MetaPID(type):
+    """PIDController(..., step_ts=..., jump_ts=...) builds the ClipStepSizeController
+    around a plain PIDController, as pid.py:91-97 does."""
+
+    def __call__(cls, *args, **kwargs):
+        step_ts = kwargs.pop("step_ts", None)
+        jump_ts = kwargs.pop("jump_ts", None)
+        if step_ts is not None or jump_ts is not None:
+            return ClipStepSizeController(cls(*args, **kwargs), step_ts, jump_ts)
+        return super().__call__(*args, **kwargs)
+
+
+class PIDController(AbstractAdaptiveStepSizeController, metaclass=_MetaPID):
     """PID step-size controller with the fields and defaults of pid.py:299-311."""
 
     def __init__(self, rtol, atol, norm: Callable = rms_norm, pcoeff=0, icoeff=1, dcoeff=0,
                  dtmin: Optional[float] = None, dtmax: Optional[float] = None,
                  force_dtmin: bool = True, factormin=0.2, factormax=10.0, safety=0.9,
-                 error_order: Optional[float] = None, step_ts=None, jump_ts=None):
+                 error_order: Optional[float] = None):
         if rtol is None or atol is None:
             raise ValueError(
                 "The default values for `rtol` and `atol` were removed in Diffrax "
                 "version 0.1.0. (As the choice of tolerance is nearly always "
                 "something that you, as an end user, should make an explicit choice "
                 "about.)")
-        if step_ts is not None or jump_ts is not None:
-            raise NotImplementedError(
-                "step_ts/jump_ts (ClipStepSizeController) are not provided by diffrax_amd")
         if norm is not rms_norm:
             raise NotImplementedError("diffrax_amd kernels implement rms_norm only")
         self.rtol = float(rtol)
@@ -68,3 +77,48 @@ class PIDController(AbstractAdaptiveStepSizeController):
 
     def __repr__(self):
         return f"PIDController(rtol={self.rtol}, atol={self.atol})"
+
+
+def _none_or_sorted(x):
+    if x is None:
+        return None
+    import numpy as np
+    return np.sort(np.asarray(x.detach().cpu() if hasattr(x, "detach") else x, dtype=np.float64))
+
+
+class ClipStepSizeController(AbstractAdaptiveStepSizeController):
+    """Clips the steps of an adaptive controller so that they land exactly on `step_ts`
+    and stop just before / restart just after `jump_ts` (clip.py:120-428).  The wrapped
+    controller must be a PIDController; `store_rejected_steps` (revisiting rejected steps
+    for SDEs with non-commutative noise) is not provided."""
+
+    def __init__(self, controller, step_ts=None, jump_ts=None, store_rejected_steps=None,
+                 _callback_on_reject=None):
+        if not isinstance(controller, AbstractAdaptiveStepSizeController):
+            raise ValueError(
+                "Can only apply `ClipStepSizeController` to adaptive step size "
+                f"controllers, but got {controller}.")
+        if (store_rejected_steps is not None) and (store_rejected_steps <= 0):
+            raise ValueError(
+                "`store_rejected_steps must either be `None` or a non-negative integer.")
+        self.controller = controller
+        self.step_ts = _none_or_sorted(step_ts)
+        self.jump_ts = _none_or_sorted(jump_ts)
+        self.store_rejected_steps = store_rejected_steps
+        self.callback_on_reject = _callback_on_reject
+
+    @property
+    def rtol(self):
+        return self.controller.rtol
+
+    @property
+    def atol(self):
+        return self.controller.atol
+
+    @property
+    def norm(self):
+        return self.controller.norm
+
+    def __repr__(self):
+        return (f"ClipStepSizeController({self.controller!r}, step_ts={self.step_ts}, "
+                f"jump_ts={self.jump_ts})")

@@ -7,6 +7,7 @@
 
 #include "../../include/diffrax_amd.h"
 #include "erk_small.hpp"
+#include "erk_dense.hpp"
 
 namespace dfx {
 int erk_small_solve_f64(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
@@ -146,6 +147,18 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
   a.stats = io->stats;
   a.result = io->result;
   a.grid = io->grid;
+  const int dense = (io->dense_ts != nullptr) + (io->dense_ys != nullptr) + (io->dense_k != nullptr);
+  if (dense != 0 && dense != 3) return DFX_E_ARG;
+  a.dense_ts = io->dense_ts;
+  a.dense_ys = io->dense_ys;
+  a.dense_k = io->dense_k;
+  if (p->n_step_ts < 0 || p->n_jump_ts < 0) return DFX_E_ARG;
+  if ((p->n_step_ts > 0 || p->n_jump_ts > 0) && p->controller != DFX_PID) return DFX_E_ARG;
+  if ((p->n_step_ts > 0 && !io->step_ts) || (p->n_jump_ts > 0 && !io->jump_ts)) return DFX_E_ARG;
+  a.step_ts = io->step_ts;
+  a.jump_ts = io->jump_ts;
+  a.n_step_ts = p->n_step_ts;
+  a.n_jump_ts = p->n_jump_ts;
   a.counter = reinterpret_cast<unsigned int*>(workspace);
   a.B = p->batch;
   a.t1_stride = p->t1_stride;
@@ -175,6 +188,40 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (p->precision == DFX_F64) return erk_small_solve_f64(p->solver, p->vf, p->d, a, block, lanes, s);
   return erk_small_solve_f32(p->solver, p->vf, p->d, a, block, lanes, s);
+}
+
+int dfx_erk_dense_eval(const dfx_erk_problem* p, const dfx_erk_dense_io* io, void* stream) {
+  if (!p || !io) return DFX_E_ARG;
+  if (p->solver < DFX_TSIT5 || p->solver > DFX_HEUN) return DFX_E_ARG;
+  if (p->precision != DFX_F64 && p->precision != DFX_F32S_F64T) return DFX_E_ARG;
+  if (!small_vf_supported(p->vf, p->d)) return DFX_E_ARG;
+  if (p->batch < 0 || p->max_steps < 0 || p->t1_stride < 0) return DFX_E_ARG;
+  if (io->n_q < 0 || io->tq_stride < 0) return DFX_E_ARG;
+  if (io->tq_stride != 0 && io->tq_stride < io->n_q) return DFX_E_ARG;
+  if (p->batch == 0 || io->n_q == 0) return DFX_OK;
+  if (!io->dense_ts || !io->dense_ys || !io->dense_k || !io->stats || !io->t1 || !io->tq ||
+      !io->out)
+    return DFX_E_ARG;
+  DenseKArgs a;
+  memset(&a, 0, sizeof(a));
+  a.dense_ts = io->dense_ts;
+  a.dense_ys = io->dense_ys;
+  a.dense_k = io->dense_k;
+  a.stats = io->stats;
+  a.t1 = io->t1;
+  a.tq = io->tq;
+  a.out = io->out;
+  a.B = p->batch;
+  a.n_q = io->n_q;
+  a.tq_stride = io->tq_stride;
+  a.t1_stride = p->t1_stride;
+  a.max_steps = p->max_steps;
+  a.left = io->left ? 1 : 0;
+  a.derivative = io->derivative ? 1 : 0;
+  a.t0 = p->t0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  return p->precision == DFX_F64 ? erk_dense_eval_f64(p->solver, p->d, a, s)
+                                 : erk_dense_eval_f32(p->solver, p->d, a, s);
 }
 
 }  // extern "C"

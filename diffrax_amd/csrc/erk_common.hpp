@@ -325,4 +325,32 @@ __device__ __forceinline__ void tsit5_weights(double th, double (&bb)[7]) {
   bb[6] = 2.5 * (th - 1.0) * (th - 0.6) * t2;
 }
 
+// d b_i / d theta of the weights above (product rule on the same factors).  The reference
+// differentiates the interpolant with jax.jvp (AbstractPath.derivative); this is the same
+// derivative in closed form.
+__device__ __forceinline__ void tsit5_dweights(double th, double (&db)[7]) {
+  const double t2 = th * th;
+  {
+    const double a = 1.3299890189751412, p = 1.4364028541716351, r = 0.7139816917074209;
+    const double q = t2 - p * th + r;
+    db[0] = -1.0530884977290216 * ((th - a) * q + th * q + th * (th - a) * (2.0 * th - p));
+  }
+  {
+    const double p = 2.1966568338249754, r = 1.2949852507374631;
+    db[1] = 0.1017 * (2.0 * th * (t2 - p * th + r) + t2 * (2.0 * th - p));
+  }
+  {
+    const double p = 2.38535645472061657, r = 1.57803468208092486;
+    db[2] = 2.490627285651252793 * (2.0 * th * (t2 - p * th + r) + t2 * (2.0 * th - p));
+  }
+  auto two_roots = [&](double c, double r1, double r2) {
+    // d/dth [c (th - r1)(th - r2) th^2]
+    return c * ((th - r2) * t2 + (th - r1) * t2 + 2.0 * th * (th - r1) * (th - r2));
+  };
+  db[3] = two_roots(-16.54810288924490272, 1.21712927295533244, 0.61620406037800089);
+  db[4] = two_roots(47.37952196281928122, 1.203071208372362603, 0.658047292653547382);
+  db[5] = two_roots(-34.87065786149660974, 1.2, 0.666666666666666667);
+  db[6] = two_roots(2.5, 1.0, 0.6);
+}
+
 }  // namespace dfx

@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
   for (int s = save_index; s < a.n_out; ++s) {
 #pragma unroll
     for (int j = 0; j < 8; ++j) ys_row[s * MLP_D + c0 + j] = __builtin_inff();
-    if ((tid & 7) == 0) tso_row[s] = (double)dir * __builtin_inf();
+    if ((tid & 7) == 0) tso_row[s] = __builtin_inf();  // (dir * inf) * dir: _integrate.py:1296, 1481
   }
   if ((tid & 7) == 0) {
     a.stats[traj * 3 + 0] = num_steps;

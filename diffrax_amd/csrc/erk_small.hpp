@@ -27,8 +27,14 @@ struct ErkKArgs {
   int32_t* stats;
   int32_t* result;
   double* grid;
+  double* dense_ts;   // SaveAt(dense=True): [B, max_steps + 1] accepted-step ends (t0 first)
+  void* dense_ys;     // [B, max_steps + 1, D] states at those times
+  void* dense_k;      // [B, max_steps, S, D] stage increments of each accepted step
+  const double* step_ts;  // ClipStepSizeController times, ascending (shared by the batch)
+  const double* jump_ts;
   unsigned int* counter;
   int64_t B, t1_stride, ts_stride, param_stride;
+  int32_t n_step_ts, n_jump_ts;
   int32_t n_ts, n_out, save_t0, save_t1, save_steps, max_steps, has_dt0, controller;
   double t0, dt0, error_order;
   PidParams pid;
@@ -103,12 +109,152 @@ __device__ __forceinline__ void static_for(F&& f) {
   }
 }
 
+// The solver's local interpolant on [ts0, ts1] (normalised time) at t: Tsit5's
+// (tsit5.py:103-156), Dopri5's quartic through y0, ymid, y1 (_local_interpolation.py:95-139,
+// dopri5.py:36-47) and Heun's cubic Hermite (_local_interpolation.py:50-92).
+template <class Tab, class T, int D>
+__device__ __forceinline__ void erk_interp(double ts0, double ts1, const T (&y0)[D],
+                                           const T (&y1)[D], const T (&k)[Tab::S][D], double t,
+                                           T (&out)[D]) {
+  constexpr int S = Tab::S;
+  const double th = linear_rescale(ts0, t, ts1);
+  if constexpr (Tab::id == SOLVER_TSIT5) {
+    // Tsit5
+    double bb[7];
+    tsit5_weights(th, bb);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      T acc = T(bb[0]) * k[0][j];
+#pragma unroll
+      for (int m = 1; m < 7; ++m) acc = acc + T(bb[m]) * k[m][j];
+      out[j] = y0[j] + acc;
+    }
+  } else if constexpr (Tab::id == SOLVER_DOPRI5) {
+    // Dopri5: quartic through y0, ymid, y1 with end slopes k0, k6.
+    const T tt = T(th);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      T kc[S];
+#pragma unroll
+      for (int m = 0; m < S; ++m) kc[m] = k[m][j];
+      const T ymid = y0[j] + comb<Tab, 3, 0, 0, S, false>(Dopri5Tab::cmid, kc, T(0));
+      const T f0 = k[0][j], f1 = k[6][j];
+      const T ca = T(2) * (f1 - f0) - T(8) * (y1[j] + y0[j]) + T(16) * ymid;
+      const T cb = T(5) * f0 - T(3) * f1 + T(18) * y0[j] + T(14) * y1[j] - T(32) * ymid;
+      const T cc = f1 - T(4) * f0 - T(11) * y0[j] - T(5) * y1[j] + T(16) * ymid;
+      out[j] = (((ca * tt + cb) * tt + cc) * tt + f0) * tt + y0[j];
+    }
+  } else {
+    // Heun: cubic Hermite from k0, k_last.
+    const T tt = T(th);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const T k0 = k[0][j], k1 = k[S - 1][j];
+      const T ca = k0 + k1 + T(2) * y0[j] - T(2) * y1[j];
+      const T cb = T(-2) * k0 - k1 - T(3) * y0[j] + T(3) * y1[j];
+      out[j] = ((ca * tt + cb) * tt + k0) * tt + y0[j];
+    }
+  }
+}
+
+// d/dt of erk_interp (in normalised time): the interpolating polynomial differentiated in
+// theta, divided by ts1 - ts0; zero on a zero-length interval (linear_rescale is constant
+// there).
+template <class Tab, class T, int D>
+__device__ __forceinline__ void erk_interp_deriv(double ts0, double ts1, const T (&y0)[D],
+                                                 const T (&y1)[D], const T (&k)[Tab::S][D],
+                                                 double t, T (&out)[D]) {
+  constexpr int S = Tab::S;
+  const double th = linear_rescale(ts0, t, ts1);
+  const double inv_h = (ts0 == ts1) ? 0.0 : 1.0 / (ts1 - ts0);
+  if constexpr (Tab::id == SOLVER_TSIT5) {
+    double db[7];
+    tsit5_dweights(th, db);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      T acc = T(db[0]) * k[0][j];
+#pragma unroll
+      for (int m = 1; m < 7; ++m) acc = acc + T(db[m]) * k[m][j];
+      out[j] = T((double)acc * inv_h);
+    }
+  } else if constexpr (Tab::id == SOLVER_DOPRI5) {
+    const T tt = T(th);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      T kc[S];
+#pragma unroll
+      for (int m = 0; m < S; ++m) kc[m] = k[m][j];
+      const T ymid = y0[j] + comb<Tab, 3, 0, 0, S, false>(Dopri5Tab::cmid, kc, T(0));
+      const T f0 = k[0][j], f1 = k[6][j];
+      const T ca = T(2) * (f1 - f0) - T(8) * (y1[j] + y0[j]) + T(16) * ymid;
+      const T cb = T(5) * f0 - T(3) * f1 + T(18) * y0[j] + T(14) * y1[j] - T(32) * ymid;
+      const T cc = f1 - T(4) * f0 - T(11) * y0[j] - T(5) * y1[j] + T(16) * ymid;
+      const T dp = ((T(4) * ca * tt + T(3) * cb) * tt + T(2) * cc) * tt + f0;
+      out[j] = T((double)dp * inv_h);
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const T tt = T(th);
+      const T k0 = k[0][j], k1 = k[S - 1][j];
+      const T ca = k0 + k1 + T(2) * y0[j] - T(2) * y1[j];
+      const T cb = T(-2) * k0 - k1 - T(3) * y0[j] + T(3) * y1[j];
+      const T dp = (T(3) * ca * tt + T(2) * cb) * tt + k0;
+      out[j] = T((double)dp * inv_h);
+    }
+  }
+}
+
+// ClipStepSizeController's view of step_ts / jump_ts (clip.py): the times scaled by the
+// direction and sorted (wrap, clip.py:231-241), i.e. ts[i] for dir > 0 and -ts[n-1-i] for
+// dir < 0.  Searches are binary (the reference's linear search from a hint finds the same
+// index).
+struct ClipTimes {
+  const double* ts;
+  int n, dir;
+  __device__ __forceinline__ double at(int i) const { return dir > 0 ? ts[i] : -ts[n - 1 - i]; }
+  // _get_t (clip.py:57-65)
+  __device__ __forceinline__ double get_t(int i) const {
+    return i >= n ? __builtin_inf() : at(i);
+  }
+  // _find_idx_with_hint (clip.py:104-118): first index with ts[i] > t
+  __device__ __forceinline__ int find(double t) const {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (at(mid) <= t) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+  }
+  __device__ __forceinline__ bool contains(double x) const {
+    const int i = find(x);
+    return i > 0 && at(i - 1) == x;
+  }
+  // _clip_t (clip.py:68-79)
+  __device__ __forceinline__ double clip(double t, int i, bool prev) const {
+    double u = get_t(i);
+    if (prev) u = prevbefore(u);
+    return jmin(u, t);
+  }
+  // _bump_next_t0 (clip.py:82-101)
+  __device__ __forceinline__ double bump(double t, bool& made_jump) const {
+    const double na = nextup(t);
+    const bool mj1 = contains(na), mj2 = contains(t);
+    double r = mj1 ? nextup(na) : t;
+    r = mj2 ? na : r;
+    made_jump = mj1 || mj2;
+    return r;
+  }
+};
+
 __device__ __forceinline__ unsigned int lane_rank(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned int)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
 }
 
-template <class Tab, class VF, class T, bool GenericPid>
+// Extras = the solve uses SaveAt(dense=True) or ClipStepSizeController; the plain
+// instantiation carries none of that code in its step loop.
+template <class Tab, class VF, class T, bool GenericPid, bool Extras>
 struct ErkLane {
   static constexpr int D = VF::D;
   static constexpr int P = VF::P;
@@ -123,6 +269,8 @@ struct ErkLane {
   T fsal_f[D];
   T q[P];
   bool first;
+  bool made_jump;            // the last accepted step ended at a jump (clip.py:383-384)
+  int step_index, jump_index;  // ClipStepSizeController hints
   PidState pid;
   int32_t cs_completed, cs_num;
   double cs_t0, cs_t1;
@@ -184,6 +332,8 @@ struct ErkLane {
     const double T0 = a.t0 * dir;
     T1 = t1r * dir;
     first = true;
+    made_jump = false;
+    step_index = jump_index = 0;
     pid.inv = 1.0; pid.pinv = 1.0; pid.at_dtmin = 0;
     if (a.controller == DFX_PID) {
       double dt0 = a.has_dt0 ? a.dt0 * dir : initial_step(a, T0);
@@ -191,6 +341,17 @@ struct ErkLane {
       if (a.pid.has_dtmin) dt0 = jmax(dt0, a.pid.dtmin);
       tnext = T0 + dt0;
       cs_completed = 0; cs_num = 0; cs_t0 = T0; cs_t1 = T1;
+      // ClipStepSizeController.init (clip.py:243-299)
+      if (Extras && a.n_step_ts > 0) {
+        const ClipTimes c{a.step_ts, a.n_step_ts, dir};
+        step_index = c.find(T0);
+        tnext = c.clip(tnext, step_index, false);
+      }
+      if (Extras && a.n_jump_ts > 0) {
+        const ClipTimes c{a.jump_ts, a.n_jump_ts, dir};
+        jump_index = c.find(T0);
+        tnext = c.clip(tnext, jump_index, true);
+      }
     } else {
       const double DT0 = a.dt0 * dir;
       const double qn = ceil((T1 - T0) / nextup(DT0));
@@ -214,6 +375,13 @@ struct ErkLane {
     if (a.save_t0) {
       write_save(a, T0, y);
     }
+    if (Extras && a.dense_ts) {
+      const int64_t row = traj * ((int64_t)a.max_steps + 1);
+      a.dense_ts[row] = T0;
+      T* dy = reinterpret_cast<T*>(a.dense_ys) + row * D;
+#pragma unroll
+      for (int j = 0; j < D; ++j) dy[j] = y[j];
+    }
   }
 
   __device__ __forceinline__ void write_save(const ErkKArgs& a, double t_norm, const T (&v)[D]) {
@@ -236,8 +404,9 @@ struct ErkLane {
     T k[S][D];
     T z[D], f[D];
     if constexpr (Tab::fsal) {
-      if (first) {
-        // First step: stage 0 is evaluated (z0 = y0 + 0, t = t0 + 0*dt).
+      if (first || (Extras && made_jump)) {
+        // First step, or the first step after a jump (runge_kutta.py:687): stage 0 is
+        // evaluated (z0 = y0 + 0, t = t0 + 0*dt).
 #pragma unroll
         for (int j = 0; j < D; ++j) z[j] = y[j] + T(0);
         eval_f((ts0 + 0.0 * dt) * dir, z, fsal_f);
@@ -303,6 +472,24 @@ struct ErkLane {
       ntprev = ts1;
       ntnext = tn;
     }
+    if (Extras && a.controller == DFX_PID && (a.n_step_ts > 0 || a.n_jump_ts > 0)) {
+      // ClipStepSizeController.adapt_step_size (clip.py:301-428)
+      bool mj = false;
+      if (a.n_step_ts > 0) {
+        const ClipTimes c{a.step_ts, a.n_step_ts, dir};
+        bool unused;
+        step_index = c.find(c.bump(ntprev, unused));
+        ntnext = c.clip(ntnext, step_index, false);
+      }
+      if (a.n_jump_ts > 0) {
+        const ClipTimes c{a.jump_ts, a.n_jump_ts, dir};
+        ntprev = c.bump(ntprev, mj);
+        ntnext = jmax(nextup(ntprev), ntnext);
+        jump_index = c.find(ntprev);
+        ntnext = c.clip(ntnext, jump_index, true);
+      }
+      if (keep) made_jump = mj;  // _integrate.py:425
+    }
     ntprev = jmin(ntprev, T1);
     if (ntnext > floor_t1) ntnext = keep ? T1 : ntprev + 0.5 * (T1 - ntprev);
 
@@ -315,6 +502,22 @@ struct ErkLane {
         double* g = a.grid + (traj * a.max_steps + (num_acc - 1)) * 2;
         g[0] = ts0;
         g[1] = ts1;
+      }
+      if (Extras && a.dense_ts) {
+        // SaveAt(dense=True): the step's end time, end state and stage increments
+        // (_integrate.py:529-540; the interpolant is rebuilt from them on evaluation).
+        const int64_t row = traj * ((int64_t)a.max_steps + 1) + num_acc;
+        a.dense_ts[row] = ntprev;
+        T* dy = reinterpret_cast<T*>(a.dense_ys) + row * D;
+        T* dk = reinterpret_cast<T*>(a.dense_k) +
+                ((traj * (int64_t)a.max_steps + (num_acc - 1)) * S) * D;
+#pragma unroll
+        for (int j = 0; j < D; ++j) dy[j] = y1[j];
+#pragma unroll
+        for (int m = 0; m < S; ++m) {
+#pragma unroll
+          for (int j = 0; j < D; ++j) dk[m * D + j] = k[m][j];
+        }
       }
       // SaveAt(ts): interpolate inside [ts0, ts1] (_integrate.py:456-491).
       while (next_ts <= ts1) {
@@ -341,44 +544,7 @@ struct ErkLane {
   __device__ __forceinline__ void interp(double ts0, double ts1, const T (&y0)[D],
                                          const T (&y1)[D], const T (&k)[S][D], double t,
                                          T (&out)[D]) const {
-    const double th = linear_rescale(ts0, t, ts1);
-    if constexpr (Tab::id == SOLVER_TSIT5) {
-      // Tsit5
-      double bb[7];
-      tsit5_weights(th, bb);
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        T acc = T(bb[0]) * k[0][j];
-#pragma unroll
-        for (int m = 1; m < 7; ++m) acc = acc + T(bb[m]) * k[m][j];
-        out[j] = y0[j] + acc;
-      }
-    } else if constexpr (Tab::id == SOLVER_DOPRI5) {
-      // Dopri5: quartic through y0, ymid, y1 with end slopes k0, k6.
-      const T tt = T(th);
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        T kc[S];
-#pragma unroll
-        for (int m = 0; m < S; ++m) kc[m] = k[m][j];
-        const T ymid = y0[j] + comb<Tab, 3, 0, 0, S, false>(Dopri5Tab::cmid, kc, T(0));
-        const T f0 = k[0][j], f1 = k[6][j];
-        const T ca = T(2) * (f1 - f0) - T(8) * (y1[j] + y0[j]) + T(16) * ymid;
-        const T cb = T(5) * f0 - T(3) * f1 + T(18) * y0[j] + T(14) * y1[j] - T(32) * ymid;
-        const T cc = f1 - T(4) * f0 - T(11) * y0[j] - T(5) * y1[j] + T(16) * ymid;
-        out[j] = (((ca * tt + cb) * tt + cc) * tt + f0) * tt + y0[j];
-      }
-    } else {
-      // Heun: cubic Hermite from k0, k_last.
-      const T tt = T(th);
-#pragma unroll
-      for (int j = 0; j < D; ++j) {
-        const T k0 = k[0][j], k1 = k[S - 1][j];
-        const T ca = k0 + k1 + T(2) * y0[j] - T(2) * y1[j];
-        const T cb = T(-2) * k0 - k1 - T(3) * y0[j] + T(3) * y1[j];
-        out[j] = ((ca * tt + cb) * tt + k0) * tt + y0[j];
-      }
-    }
+    erk_interp<Tab, T, D>(ts0, ts1, y0, y1, k, t, out);
   }
 
   // Loop exit bookkeeping (_integrate.py:823-883) and output of counters.
@@ -389,12 +555,14 @@ struct ErkLane {
     }
     if (a.save_t1 && !a.save_steps) write_save(a, tprev, y);
     if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
-    // Unfilled rows keep the reference's preset values (inf / direction*inf).
+    // Unfilled rows keep the reference's preset values: ys inf, ts direction*inf in
+    // normalised time, which the final rescaling by the direction makes +inf
+    // (_integrate.py:1296, 1481).
     T* rows = reinterpret_cast<T*>(a.ys) + traj * a.n_out * D;
     for (int s = save_index; s < a.n_out; ++s) {
 #pragma unroll
       for (int j = 0; j < D; ++j) rows[s * D + j] = T(__builtin_inf());
-      a.ts_out[traj * a.n_out + s] = (double)dir * __builtin_inf();
+      a.ts_out[traj * a.n_out + s] = __builtin_inf();
     }
     a.stats[traj * 3 + 0] = num_steps;
     a.stats[traj * 3 + 1] = num_acc;
@@ -406,11 +574,11 @@ struct ErkLane {
 // One trajectory per lane, no work queue: used when the launch has a lane for every
 // instance.  Lanes whose trajectory has finished idle (masked) until the wave's last lane
 // is done.
-template <class Tab, class VF, class T, bool GenericPid>
+template <class Tab, class VF, class T, bool GenericPid, bool Extras>
 __global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= a.B) return;
-  ErkLane<Tab, VF, T, GenericPid> L;
+  ErkLane<Tab, VF, T, GenericPid, Extras> L;
   L.start(a, id);
   while (L.running(a)) L.step(a);
   L.finish(a);
@@ -418,9 +586,9 @@ __global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
 
 // Persistent variant with refill: a lane whose trajectory finishes claims the next
 // instance id from the work counter (one atomic per wave per claim round).
-template <class Tab, class VF, class T, bool GenericPid>
+template <class Tab, class VF, class T, bool GenericPid, bool Extras>
 __global__ void __launch_bounds__(256) erk_small_queue_kernel(const ErkKArgs a) {
-  ErkLane<Tab, VF, T, GenericPid> L;
+  ErkLane<Tab, VF, T, GenericPid, Extras> L;
   bool active = false;
   bool drained = false;  // wave-uniform: the work queue is exhausted
   while (true) {

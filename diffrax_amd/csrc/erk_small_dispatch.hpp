@@ -5,24 +5,31 @@
 
 namespace dfx {
 
-template <class Tab, class VF, class T, bool G>
+template <class Tab, class VF, class T, bool G, bool X>
 static void launch_small_g(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
   const int64_t nblocks = (lanes + block - 1) / block;
   if (lanes >= a.B) {
-    hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T, G>), dim3((unsigned)nblocks), dim3(block), 0, s, a);
+    hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T, G, X>), dim3((unsigned)nblocks), dim3(block), 0, s,
+                       a);
   } else {
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
-    hipLaunchKernelGGL((erk_small_queue_kernel<Tab, VF, T, G>), dim3((unsigned)nblocks), dim3(block),
+    hipLaunchKernelGGL((erk_small_queue_kernel<Tab, VF, T, G, X>), dim3((unsigned)nblocks), dim3(block),
                        0, s, a);
   }
 }
 
+// Variants: generic PID coefficients (G) and dense output / step clipping (X).
 template <class Tab, class VF, class T>
 static int launch_small(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
-  if (a.controller == DFX_PID && !pid_is_integral(a.pid))
-    launch_small_g<Tab, VF, T, true>(a, block, lanes, s);
-  else
-    launch_small_g<Tab, VF, T, false>(a, block, lanes, s);
+  const bool g = a.controller == DFX_PID && !pid_is_integral(a.pid);
+  const bool x = a.dense_ts != nullptr || a.n_step_ts > 0 || a.n_jump_ts > 0;
+  if (g) {
+    if (x) launch_small_g<Tab, VF, T, true, true>(a, block, lanes, s);
+    else launch_small_g<Tab, VF, T, true, false>(a, block, lanes, s);
+  } else {
+    if (x) launch_small_g<Tab, VF, T, false, true>(a, block, lanes, s);
+    else launch_small_g<Tab, VF, T, false, false>(a, block, lanes, s);
+  }
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }
 

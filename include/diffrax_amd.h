@@ -62,6 +62,9 @@ typedef struct dfx_erk_problem {
   double t0, dt0;
   double rtol, atol, pcoeff, icoeff, dcoeff, dtmin, dtmax;
   double factormin, factormax, safety, error_order;
+  /* ClipStepSizeController(PIDController, step_ts, jump_ts) (clip.py:120-428): counts of
+   * the ascending time arrays in dfx_erk_io (0 = none; PID only). */
+  int32_t n_step_ts, n_jump_ts;
 } dfx_erk_problem;
 
 /* ---- device buffers of one solve --------------------------------------------------- */
@@ -71,11 +74,20 @@ typedef struct dfx_erk_io {
   const double* ts;     /* [B * ts_stride + n_ts] or NULL when n_ts == 0               */
   const void* params;   /* vector-field parameters, state dtype (MLP: see below)      */
   void* ys;             /* out [B, n_out, d] state dtype; unfilled entries = inf       */
-  double* ts_out;       /* out [B, n_out]; unfilled entries = direction * inf         */
+  double* ts_out;       /* out [B, n_out]; unfilled entries = +inf (as the reference's
+                           direction*inf preset after its rescaling by direction)    */
   int32_t* stats;       /* out [B, 3] = num_steps, num_accepted_steps, num_rejected   */
   int32_t* result;      /* out [B] DFX_SUCCESSFUL / DFX_MAX_STEPS_REACHED / ...      */
   double* grid;         /* optional out [B, max_steps, 2]: accepted (tprev, tnext) in
                            direction-normalised time; NULL to skip                  */
+  /* SaveAt(dense=True), all three or none (NULL):                                     */
+  double* dense_ts;     /* out [B, max_steps + 1]: t0 then each accepted step's end time,
+                           direction-normalised (entries past the accepted count unset) */
+  void* dense_ys;       /* out [B, max_steps + 1, d] state dtype: the states at dense_ts */
+  void* dense_k;        /* out [B, max_steps, stages, d]: each accepted step's stage
+                           increments k_i = dt f(z_i)                                 */
+  const double* step_ts; /* [n_step_ts] ascending, shared by the batch, or NULL           */
+  const double* jump_ts; /* [n_jump_ts] ascending, or NULL                                */
 } dfx_erk_io;
 
 /* Launch shape override (NULL = library default / tuned). */
@@ -114,6 +126,28 @@ int dfx_erk_workspace_bytes(const dfx_erk_problem* p, size_t* bytes);
  * _select_initial_step (_step_size_controller/pid.py:33-81). */
 int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspace,
                   size_t workspace_bytes, const dfx_launch* launch, void* stream);
+
+/* Query side of SaveAt(dense=True): buffers a dfx_erk_solve wrote, and query times. */
+typedef struct dfx_erk_dense_io {
+  const double* dense_ts;  /* [B, max_steps + 1] from dfx_erk_solve                       */
+  const void* dense_ys;    /* [B, max_steps + 1, d]                                       */
+  const void* dense_k;     /* [B, max_steps, stages, d]                                   */
+  const int32_t* stats;    /* [B, 3] from dfx_erk_solve (accepted-step counts)            */
+  const double* t1;        /* [B * t1_stride] as given to dfx_erk_solve                   */
+  const double* tq;        /* query times [B * tq_stride + n_q] (unnormalised)           */
+  int64_t n_q;             /* queries per instance                                        */
+  int64_t tq_stride;       /* 0: the same n_q times for every instance; n_q: per instance */
+  int32_t left;            /* 1: a time on a step boundary belongs to the earlier step    */
+  int32_t derivative;      /* 0: y(t); 1: dy/dt                                           */
+  void* out;               /* out [B, n_q, d] state dtype                                 */
+} dfx_erk_dense_io;
+
+/* Evaluate the dense solution (or its time derivative) of every instance at its query
+ * times.  p is the problem the buffers were solved with.
+ * Replaces: DenseInterpolation.evaluate / derivative (_global_interpolation.py:315-400)
+ * behind Solution.evaluate / Solution.derivative (_solution.py:103-132), with the
+ * solver's local interpolant (_solver/tsit5.py:103-156, _local_interpolation.py:50-139). */
+int dfx_erk_dense_eval(const dfx_erk_problem* p, const dfx_erk_dense_io* io, void* stream);
 
 /* ---- SDE: Heun with diagonal noise on the registered OU terms ----------------------- */
 /* dy = theta (mu - y) dt + sigma dW, theta/mu/sigma of length d shared by all paths. */

@@ -46,6 +46,8 @@ class Problem(ctypes.Structure):
         ("dtmin", ctypes.c_double), ("dtmax", ctypes.c_double),
         ("factormin", ctypes.c_double), ("factormax", ctypes.c_double),
         ("safety", ctypes.c_double), ("error_order", ctypes.c_double),
+        ("step_ts", ctypes.c_void_p), ("jump_ts", ctypes.c_void_p),
+        ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32),
     ]
 
 
@@ -99,9 +101,15 @@ def make_problem(solver="tsit5", vf="lv", d=2, *, t0=0.0, dt0=None, rtol=1e-6, a
                  controller="pid", n_ts=0, save_t0=False, save_t1=False, save_steps=False,
                  max_steps=4096, hidden=0, pcoeff=0.0, icoeff=1.0, dcoeff=0.0, dtmin=None,
                  dtmax=None, force_dtmin=True, factormin=0.2, factormax=10.0, safety=0.9,
-                 error_order=None):
+                 error_order=None, step_ts=None, jump_ts=None):
+    """The problem struct.  step_ts / jump_ts wrap the PID controller in
+    ClipStepSizeController (sorted here; the arrays stay referenced by the struct)."""
     nan = float("nan")
-    return Problem(
+    clip = {}
+    for name, v in (("step_ts", step_ts), ("jump_ts", jump_ts)):
+        arr = None if v is None else np.ascontiguousarray(np.sort(np.asarray(v, dtype=np.float64)))
+        clip[name] = arr
+    prob = Problem(
         solver=SOLVERS[solver], controller=CONTROLLERS[controller], vf=VFS[vf], d=d,
         hidden=hidden, n_ts=n_ts, save_t0=int(save_t0), save_t1=int(save_t1),
         save_steps=int(save_steps), max_steps=max_steps, force_dtmin=int(force_dtmin), pad_=0,
@@ -109,7 +117,13 @@ def make_problem(solver="tsit5", vf="lv", d=2, *, t0=0.0, dt0=None, rtol=1e-6, a
         atol=float(atol), pcoeff=float(pcoeff), icoeff=float(icoeff), dcoeff=float(dcoeff),
         dtmin=nan if dtmin is None else float(dtmin), dtmax=nan if dtmax is None else float(dtmax),
         factormin=float(factormin), factormax=float(factormax), safety=float(safety),
-        error_order=nan if error_order is None else float(error_order))
+        error_order=nan if error_order is None else float(error_order),
+        step_ts=None if clip["step_ts"] is None else clip["step_ts"].ctypes.data,
+        jump_ts=None if clip["jump_ts"] is None else clip["jump_ts"].ctypes.data,
+        n_step_ts=0 if clip["step_ts"] is None else clip["step_ts"].size,
+        n_jump_ts=0 if clip["jump_ts"] is None else clip["jump_ts"].size)
+    prob._keep = clip
+    return prob
 
 
 def solve(y0, t1, params, *, ts=None, dW=None, record_grid=False, nthreads=0, **kw):

@@ -181,6 +181,57 @@ static double prevbefore(double x) { return nextafter(x, -INFINITY); }
 /* eqxi.nextafter: next representable float64 towards +inf. */
 static double nextup(double x) { return nextafter(x, INFINITY); }
 
+/* ---- ClipStepSizeController (_step_size_controller/clip.py) -----------------------
+ * The controller works on the times scaled by the direction and sorted (wrap,
+ * clip.py:231-241): element i of that array is ts[i] for dir > 0 and -ts[n-1-i] for
+ * dir < 0 (ts ascending). */
+typedef struct { const double *ts; int n, dir; } clipts_t;
+static double clip_at(const clipts_t *c, int i) {
+    return c->dir > 0 ? c->ts[i] : -c->ts[c->n - 1 - i];
+}
+/* _get_t (clip.py:57-65): ts[i], or inf past the end. */
+static double clip_get_t(const clipts_t *c, int i) {
+    return (c->n == 0 || i >= c->n) ? INFINITY : clip_at(c, i);
+}
+/* _clip_t (clip.py:68-79): min(ts[i] (prevbefore it for jumps), t). */
+static double clip_t(double t, int i, const clipts_t *c, int prev) {
+    double u = clip_get_t(c, i);
+    if (prev) u = prevbefore(u);
+    return jmin(u, t);
+}
+static int clip_any_eq(const clipts_t *c, double x) {
+    for (int i = 0; i < c->n; ++i) if (clip_at(c, i) == x) return 1;
+    return 0;
+}
+/* _bump_next_t0 (clip.py:82-101): a step that ended prevbefore a listed time restarts
+ * two ULPs later, past the time; one that ended on it restarts one ULP later. */
+static double clip_bump(double t, const clipts_t *c, int *made_jump) {
+    const double na = nextup(t);
+    const int mj1 = clip_any_eq(c, na), mj2 = clip_any_eq(c, t);
+    double r = mj1 ? nextup(na) : t;
+    r = mj2 ? na : r;
+    *made_jump = mj1 | mj2;
+    return r;
+}
+/* _find_idx_with_hint (clip.py:104-118): first index with ts[i] > t. */
+static int clip_find(double t, const clipts_t *c, int hint);
+/* Test hooks: the two helpers above on an ascending array (dir = +1 / -1 as wrap). */
+int orc_clip_find(const double *ts, int n, int dir, double t, int hint) {
+    clipts_t c = {ts, n, dir};
+    return clip_find(t, &c, hint);
+}
+static double clip_bump(double t, const clipts_t *c, int *made_jump);
+double orc_clip_bump(const double *ts, int n, int dir, double t, int *made_jump) {
+    clipts_t c = {ts, n, dir};
+    return clip_bump(t, &c, made_jump);
+}
+static int clip_find(double t, const clipts_t *c, int hint) {
+    int i = hint;
+    while (i < c->n && clip_at(c, i) <= t) ++i;
+    while (i > 0 && clip_at(c, i - 1) > t) --i;
+    return i;
+}
+
 /* Controller powers, correctly rounded.  The PID factor is (1/scaled_error)^0.2 and the
  * automatic initial step raises to 1/error_order (pid.py:79, 511-517).  The oracle uses a
  * correctly rounded x^0.2 (fifth root by Newton, double-double residual, and the
@@ -533,6 +584,19 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
         cs_completed = 1; cs_t0 = T0;
         tnext = T0 + DT0;
     }
+    /* ClipStepSizeController.init (clip.py:243-299): the first step stops at the next
+     * step time, or prevbefore the next jump time. */
+    clipts_t cstep = {p->step_ts, p->step_ts ? p->n_step_ts : 0, dir};
+    clipts_t cjump = {p->jump_ts, p->jump_ts ? p->n_jump_ts : 0, dir};
+    int step_index = 0, jump_index = 0, made_jump = 0;
+    if (cstep.n > 0) {
+        step_index = clip_find(T0, &cstep, 0);
+        tnext = clip_t(tnext, step_index, &cstep, 0);
+    }
+    if (cjump.n > 0) {
+        jump_index = clip_find(T0, &cjump, 0);
+        tnext = clip_t(tnext, jump_index, &cjump, 1);
+    }
     tnext = jmin(tnext, T1);
 
     double t1_clip_floor = T1;
@@ -551,7 +615,9 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
     double ysave[ORC_MAXD];
 
     while (tprev < T1 && result == ORC_SUCCESSFUL && num_steps < p->max_steps) {
-        erk_step(tb, &cx, dir, tprev, tnext, y, fsal_first, fsal_f, num_steps, &so);
+        /* a step that starts after a jump evaluates its first stage afresh
+         * (runge_kutta.py:687, made_jump) */
+        erk_step(tb, &cx, dir, tprev, tnext, y, fsal_first || made_jump, fsal_f, num_steps, &so);
         for (int j = 0; j < d; ++j) if (isnan(so.yerr[j])) so.yerr[j] = INFINITY;
 
         int keep;
@@ -605,6 +671,23 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
             cs_completed = sc;
             ntprev = tnext;
             ntnext = tn;
+        }
+        if (p->controller == ORC_PID && (cstep.n > 0 || cjump.n > 0)) {
+            /* ClipStepSizeController.adapt_step_size (clip.py:301-428) */
+            int mj = 0;
+            if (cstep.n > 0) {
+                int unused;
+                double bumped = clip_bump(ntprev, &cstep, &unused);
+                step_index = clip_find(bumped, &cstep, step_index);
+                ntnext = clip_t(ntnext, step_index, &cstep, 0);
+            }
+            if (cjump.n > 0) {
+                ntprev = clip_bump(ntprev, &cjump, &mj);
+                ntnext = jmax(nextup(ntprev), ntnext);
+                jump_index = clip_find(ntprev, &cjump, jump_index);
+                ntnext = clip_t(ntnext, jump_index, &cjump, 1);
+            }
+            if (keep) made_jump = mj;   /* _integrate.py:425 */
         }
         ntprev = jmin(ntprev, T1);
         /* _clip_to_end */

@@ -22,6 +22,10 @@ typedef struct {
     double dtmin, dtmax;                  /* NaN means None */
     double factormin, factormax, safety;
     double error_order;                   /* NaN means the solver's own */
+    /* ClipStepSizeController(step_ts=..., jump_ts=...) around the PID controller:
+     * ascending times (unscaled), NULL / 0 when absent. */
+    const double *step_ts, *jump_ts;
+    int32_t n_step_ts, n_jump_ts;
 } orc_problem;
 
 int orc_n_out(const orc_problem *p);
@@ -36,6 +40,9 @@ int orc_solve(const orc_problem *p, long B, const double *y0, const double *t1,
               long param_stride, const double *dW, double *ys, double *ts_out,
               int32_t *stats, int32_t *result, double *grid, int nthreads);
 double orc_pow(double x, double c);
+/* ClipStepSizeController helpers (clip.py:82-118), exposed for their unit tests. */
+int orc_clip_find(const double *ts, int n, int dir, double t, int hint);
+double orc_clip_bump(const double *ts, int n, int dir, double t, int *made_jump);
 int orc_single_step(const orc_problem *p, const double *params, double ts0, double ts1,
                     const double *y0, double *y1, double *yerr, double *k);
 #endif
