@@ -100,17 +100,19 @@ def time_form(plan, steps, warmup):
                 kernel_ms_median=statistics.median(per), launch=list(cfg), tune_ms=tune_ms)
 
 
-def cpu_baseline(mu, y0, t1, ts, reps=3):
+def cpu_baseline(mu, y0, t1, ts, min_seconds=10.0, max_reps=200):
+    """The FP64 C oracle on the full workload, repeated until ~10 s of CPU time."""
     import numpy as np
     import oracle
     cores = min(len(os.sched_getaffinity(0)), 16)
     times = []
     out = None
-    for _ in range(reps):
+    while (sum(times) < min_seconds and len(times) < max_reps) or len(times) < 3:
         t = time.perf_counter()
         out = oracle.solve(y0, t1, mu[:, None], ts=ts, solver="tsit5", vf="vdp", dt0=0.1,
                            rtol=1e-6, atol=1e-9, max_steps=16384, nthreads=cores)
         times.append(time.perf_counter() - t)
+    reps = len(times)
     med = statistics.median(times)
     cpu_model = "unknown"
     try:
@@ -124,7 +126,8 @@ def cpu_baseline(mu, y0, t1, ts, reps=3):
     _ = np
     return dict(value=y0.shape[0] / med, unit="solves/s", cores=cores, kind="port",
                 sample=f"full workload ({y0.shape[0]} trajectories), FP64 C oracle, "
-                       f"OpenMP {cores} threads, median of {reps} runs ({med:.3f} s each); "
+                       f"OpenMP {cores} threads, median of {reps} runs ({med:.3f} s each, "
+                       f"{sum(times):.1f} s in total); "
                        f"host CPU: {cpu_model}; JAX-CPU diffrax: unavailable"), out
 
 
@@ -205,12 +208,18 @@ def bench_mlp(steps, warmup, with_cpu, check=256):
     q = oracle.mlp_pack(*W)
     y0c = y0[:check].astype(np.float32).astype(np.float64)
     cores = min(len(os.sched_getaffinity(0)), 16)
-    t = time.perf_counter()
     ref = oracle.solve(y0c, 1.0, q, ts=ts, solver="tsit5", vf="mlp", hidden=256, dt0=0.1,
                        rtol=1e-5, atol=1e-7, nthreads=cores)
-    cpu_s = time.perf_counter() - t
     true = oracle.solve(y0c, 1.0, q, ts=ts, solver="tsit5", vf="mlp", hidden=256, dt0=0.1,
                         rtol=1e-12, atol=1e-12, max_steps=100000, nthreads=cores)["ys"]
+    cpu_times = []
+    if with_cpu:   # the whole batch, repeated until ~10 s of CPU time
+        y0f = y0.astype(np.float32).astype(np.float64)
+        while (sum(cpu_times) < 10.0 and len(cpu_times) < 100) or len(cpu_times) < 2:
+            t = time.perf_counter()
+            oracle.solve(y0f, 1.0, q, ts=ts, solver="tsit5", vf="mlp", hidden=256, dt0=0.1,
+                         rtol=1e-5, atol=1e-7, nthreads=cores)
+            cpu_times.append(time.perf_counter() - t)
     fwd, grad = {}, {}
     for matmul in ("f32", "bf16"):
         vf = dfx.MLPField(*(torch.tensor(w, dtype=torch.float32, device=dev) for w in W),
@@ -251,16 +260,20 @@ def bench_mlp(steps, warmup, with_cpu, check=256):
                             checkpoint_every=int(gplan.prob.ckpt_every),
                             grad_finite=bool(all(torch.isfinite(x).all().item() for x in out["g"])))
         del plan, gplan, g
-    cpu = dict(value=round(check / cpu_s, 2), unit="solves/s", cores=cores, kind="port",
-               sample=f"{check} of the {B} trajectories, FP64 C oracle forward, OpenMP {cores} "
-                      f"threads, one run ({cpu_s:.3f} s); host CPU: {_cpu_model()}")
+    cpu = None
+    if cpu_times:
+        med = statistics.median(cpu_times)
+        cpu = dict(value=round(B / med, 2), unit="solves/s", cores=cores, kind="port",
+                   sample=f"the full batch ({B} trajectories), FP64 C oracle forward, OpenMP "
+                          f"{cores} threads, median of {len(cpu_times)} runs ({med:.3f} s each, "
+                          f"{sum(cpu_times):.1f} s in total); host CPU: {_cpu_model()}")
     return dict(neural_ode_forward=dict(batch=B, forms=fwd, cpu_baseline=cpu,
                                         error_subset=check),
                 neural_ode_grad=dict(batch=B, forms=grad,
                                      loss="mean((ys - targets)^2), targets ~ N(0, 0.25)"))
 
 
-def bench_sde(steps, warmup, with_cpu, P=262144, d=16, N=1000, cpu_paths=8192):
+def bench_sde(steps, warmup, with_cpu, P=262144, d=16, N=1000, cpu_paths=16384):
     """OU SDE, Heun, 1000 steps: forward and forward + adjoint (float32)."""
     import numpy as np
     import torch
@@ -306,15 +319,20 @@ def bench_sde(steps, warmup, with_cpu, P=262144, d=16, N=1000, cpu_paths=8192):
     if with_cpu:
         cores = min(len(os.sched_getaffinity(0)), 16)
         dWc = dW[:, :cpu_paths].double().cpu().numpy()
-        t = time.perf_counter()
-        oracle.solve(np.zeros((cpu_paths, d)), 1.0, np.concatenate([theta, mu, sigma]), dW=dWc,
-                     solver="heun", vf="ou", controller="constant", dt0=dt, save_t1=True,
-                     max_steps=100000, nthreads=cores)
-        cpu_s = time.perf_counter() - t
-        out["cpu_baseline"] = dict(value=round(cpu_paths / cpu_s, 1), unit="paths/s (forward)",
+        times = []
+        while (sum(times) < 10.0 and len(times) < 100) or len(times) < 2:
+            t = time.perf_counter()
+            oracle.solve(np.zeros((cpu_paths, d)), 1.0, np.concatenate([theta, mu, sigma]),
+                         dW=dWc, solver="heun", vf="ou", controller="constant", dt0=dt,
+                         save_t1=True, max_steps=100000, nthreads=cores)
+            times.append(time.perf_counter() - t)
+        med = statistics.median(times)
+        out["cpu_baseline"] = dict(value=round(cpu_paths / med, 1), unit="paths/s (forward)",
                                    cores=cores, kind="port",
-                                   sample=f"{cpu_paths} of the {P} paths, FP64 C oracle, OpenMP "
-                                          f"{cores} threads ({cpu_s:.3f} s)")
+                                   sample=f"{cpu_paths} of the {P} paths (all 1000 steps), FP64 C "
+                                          f"oracle, OpenMP {cores} threads, median of "
+                                          f"{len(times)} runs ({med:.3f} s each, "
+                                          f"{sum(times):.1f} s in total)")
     del dW, fwd, gs
     torch.cuda.empty_cache()
     return {"ou_sde_heun": out}

@@ -8,6 +8,7 @@ MLPField, OUDrift/DiagonalNoise); the integration loop runs in persistent HIP ke
 behind the C ABI of include/diffrax_amd.h.  There is no CPU fallback.
 """
 from ._adjoint import AbstractAdjoint, RecursiveCheckpointAdjoint
+from ._event import AffineCondition, Event, Newton
 from ._integrate import ErkPlan, diffeqsolve
 from ._saveat import SaveAt, SubSaveAt, save_y
 from ._solution import RESULTS, Solution, is_okay, is_successful

@@ -82,7 +82,8 @@ class ErkPlan:
         self.direction_sign = direction_sign
         io = _lib.ErkIO()
         for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid",
-                     "dense_ts", "dense_ys", "dense_k", "step_ts", "jump_ts"):
+                     "dense_ts", "dense_ys", "dense_k", "step_ts", "jump_ts",
+                     "event_coeffs", "event_flags", "event_mask"):
             v = tensors.get(name)
             setattr(io, name, v.data_ptr() if v is not None else None)
         self.io = io
@@ -122,7 +123,7 @@ def _check_ts(ts, t0, t1_vals, directions):
 
 
 def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
-                 max_steps, launch, record_grid=False):
+                 max_steps, launch, record_grid=False, event=None):
     # Host-side checks first (they need no GPU); device placement at the end.
     if not isinstance(terms, ODETerm) or not isinstance(terms.vector_field, RegisteredVectorField):
         raise NotImplementedError(
@@ -244,6 +245,14 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         ts_stride=ts_stride, param_stride=param_stride, t0=t0f,
         dt0=_nan_if_none(dt0f), rtol=rtol, atol=atol, pcoeff=pcoeff, icoeff=icoeff,
         dcoeff=dcoeff, **extra)
+    ev = {}
+    if event is not None:
+        prob.n_events = len(event.conds)
+        prob.event_root_find = int(event.root_finder is not None)
+        ev["event_coeffs"] = torch.tensor([c.coeffs(d) for c in event.conds],
+                                          dtype=torch.float64, device=device)
+        ev["event_flags"] = torch.tensor(event.flags(), dtype=torch.int32, device=device)
+        ev["event_mask"] = torch.zeros((B, len(event.conds)), dtype=torch.int32, device=device)
     clip_t = {}
     for name in ("step_ts", "jump_ts"):
         arr = None if clip is None else getattr(clip, name)
@@ -260,7 +269,7 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         result=torch.empty((B,), dtype=torch.int32, device=device),
         grid=(torch.empty((B, int(max_steps), 2), dtype=torch.float64, device=device)
               if record_grid else None),
-        step_ts=clip_t.get("step_ts"), jump_ts=clip_t.get("jump_ts"),
+        step_ts=clip_t.get("step_ts"), jump_ts=clip_t.get("jump_ts"), **ev,
     )
     if saveat.dense:
         S = solver.tableau.num_stages
@@ -285,7 +294,7 @@ def _finish(plan, t0, t1, max_steps, throw):
         stats = {k: (v[0] if isinstance(v, torch.Tensor) else v) for k, v in stats.items()}
     if throw:
         codes = T["result"]
-        bad = codes != 0
+        bad = (codes != 0) & (codes != int(RESULTS.event_occurred))   # is_okay
         if bool(bad.any()):
             first = int(codes[bad][0].item())
             raise RuntimeError(RESULTS[first])
@@ -294,8 +303,16 @@ def _finish(plan, t0, t1, max_steps, throw):
     if plan.problem.n_ts == 0 and not (plan.problem.save_t0 or plan.problem.save_t1 or
                                        plan.problem.save_steps):
         ts = ys = None   # SaveAt(dense=True) alone saves no rows (as the reference)
+    event_mask = None
+    if T.get("event_mask") is not None:
+        m = T["event_mask"].bool()
+        if not plan.batched:
+            m = m[0]
+        ev = getattr(plan, "event", None)
+        event_mask = [m[..., i] for i in range(m.shape[-1])] if ev is not None and ev.is_list \
+            else m[..., 0]
     sol = Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=res,
-                   interpolation=interp)
+                   interpolation=interp, event_mask=event_mask)
     return sol
 
 
@@ -365,12 +382,13 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
 
     Extra keyword ``launch=(block, lanes)`` overrides the persistent kernel's launch shape.
     Batched use: y0 of shape [B, d] solves B independent instances in one launch."""
-    if event is not None:
-        raise NotImplementedError("events are not provided by diffrax_amd")
     if solver_state is not None or controller_state is not None or made_jump is not None:
         raise NotImplementedError("resuming from solver/controller state is not provided")
     if progress_meter is not None and type(progress_meter).__name__ != "NoProgressMeter":
         raise NotImplementedError("progress meters are not provided by diffrax_amd")
+    if event is not None and (_is_ou_sde(terms) or (isinstance(terms, ODETerm) and
+                                                     isinstance(terms.vector_field, MLPField))):
+        raise NotImplementedError("events are provided for the small-system kernel only")
     if _is_ou_sde(terms):
         from ._sde import solve_sde
         device = _require_gpu()
@@ -391,7 +409,8 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
                                     stepsize_controller, adjoint, max_steps, throw,
                                     _require_gpu())
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
-                        max_steps, launch)
+                        max_steps, launch, event=event)
+    plan.event = event
     plan.launch()
     return _finish(plan, t0, t1, max_steps, throw)
 

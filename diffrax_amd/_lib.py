@@ -37,6 +37,7 @@ class ErkProblem(ctypes.Structure):
         ("factormin", ctypes.c_double), ("factormax", ctypes.c_double),
         ("safety", ctypes.c_double), ("error_order", ctypes.c_double),
         ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32),
+        ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
     ]
 
 
@@ -47,6 +48,8 @@ class ErkIO(ctypes.Structure):
         ("stats", ctypes.c_void_p), ("result", ctypes.c_void_p), ("grid", ctypes.c_void_p),
         ("dense_ts", ctypes.c_void_p), ("dense_ys", ctypes.c_void_p), ("dense_k", ctypes.c_void_p),
         ("step_ts", ctypes.c_void_p), ("jump_ts", ctypes.c_void_p),
+        ("event_coeffs", ctypes.c_void_p), ("event_flags", ctypes.c_void_p),
+        ("event_mask", ctypes.c_void_p),
     ]
 
 

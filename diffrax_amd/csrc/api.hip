@@ -159,6 +159,13 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
   a.jump_ts = io->jump_ts;
   a.n_step_ts = p->n_step_ts;
   a.n_jump_ts = p->n_jump_ts;
+  if (p->n_events < 0 || p->n_events > kMaxEvents) return DFX_E_ARG;
+  if (p->n_events > 0 && (!io->event_coeffs || !io->event_flags)) return DFX_E_ARG;
+  a.event_coeffs = io->event_coeffs;
+  a.event_flags = io->event_flags;
+  a.event_mask = io->event_mask;
+  a.n_events = p->n_events;
+  a.event_root_find = p->event_root_find ? 1 : 0;
   a.counter = reinterpret_cast<unsigned int*>(workspace);
   a.B = p->batch;
   a.t1_stride = p->t1_stride;

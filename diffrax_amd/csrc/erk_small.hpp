@@ -32,9 +32,12 @@ struct ErkKArgs {
   void* dense_k;      // [B, max_steps, S, D] stage increments of each accepted step
   const double* step_ts;  // ClipStepSizeController times, ascending (shared by the batch)
   const double* jump_ts;
+  const double* event_coeffs;  // Event: [n_events, D + 2] affine conditions (c_t, c_y, c_0)
+  const int32_t* event_flags;  // bit 0 boolean, bits 1-2 direction (0 any, 1 up, 2 down)
+  int32_t* event_mask;         // [B, n_events] out
   unsigned int* counter;
   int64_t B, t1_stride, ts_stride, param_stride;
-  int32_t n_step_ts, n_jump_ts;
+  int32_t n_step_ts, n_jump_ts, n_events, event_root_find;
   int32_t n_ts, n_out, save_t0, save_t1, save_steps, max_steps, has_dt0, controller;
   double t0, dt0, error_order;
   PidParams pid;
@@ -247,6 +250,55 @@ struct ClipTimes {
   }
 };
 
+// ---- Event (_event.py:13-118) with affine conditions c_t t + c_y . y + c_0 ----------------
+constexpr int kMaxEvents = 4;
+
+template <class T, int D>
+__device__ __forceinline__ double event_value(const ErkKArgs& a, int i, double t, const T (&y)[D]) {
+  const double* c = a.event_coeffs + i * (D + 2);
+  double v = c[0] * t;
+#pragma unroll
+  for (int j = 0; j < D; ++j) v = v + c[1 + j] * (double)y[j];
+  return v + c[1 + D];
+}
+// jnp.sign
+__device__ __forceinline__ double event_sign(double x) {
+  return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : (x == 0.0 ? 0.0 : x));
+}
+// The event test of one condition (_integrate.py:580-602).
+__device__ __forceinline__ bool event_triggered(int flags, double old_v, double new_v) {
+  if (flags & 1) return new_v > 0.0;
+  const int dirn = (flags >> 1) & 3;
+  const double so = event_sign(old_v), sn = event_sign(new_v);
+  if (dirn == 1) return (so <= 0.0) && (sn > 0.0);
+  if (dirn == 2) return (so > 0.0) && (sn <= 0.0);
+  return so != sn;
+}
+// Event time on the step [ta, tb] (normalised): the step end for a boolean condition, else
+// bisection of the condition along the step's interpolant down to adjacent floats, the end
+// with the smaller |value| (the reference runs optimistix.Newton on the same interpolant,
+// _integrate.py:697-776).
+template <class Tab, class T, int D>
+__device__ double event_root(const ErkKArgs& a, int i, double ta, double tb, const T (&y0)[D],
+                             const T (&y1)[D], const T (&k)[Tab::S][D]) {
+  if (a.event_flags[i] & 1) return tb;
+  T ya[D], yb[D], ym[D];
+  erk_interp<Tab, T, D>(ta, tb, y0, y1, k, ta, ya);
+  erk_interp<Tab, T, D>(ta, tb, y0, y1, k, tb, yb);
+  double ga = event_value<T, D>(a, i, ta, ya), gb = event_value<T, D>(a, i, tb, yb);
+  if (!(event_sign(ga) != event_sign(gb)) || ga != ga || gb != gb) return tb;
+  double lo = ta, hi = tb;
+  for (int it = 0; it < 2100; ++it) {
+    const double m = lo + 0.5 * (hi - lo);
+    if (!(m > lo && m < hi)) break;
+    erk_interp<Tab, T, D>(ta, tb, y0, y1, k, m, ym);
+    const double gm = event_value<T, D>(a, i, m, ym);
+    if (gm == 0.0) return m;
+    if (event_sign(gm) == event_sign(ga)) { lo = m; ga = gm; } else { hi = m; gb = gm; }
+  }
+  return (fabs(ga) <= fabs(gb)) ? lo : hi;
+}
+
 __device__ __forceinline__ unsigned int lane_rank(unsigned long long mask) {
   return __builtin_amdgcn_mbcnt_hi((unsigned int)(mask >> 32),
                                    __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
@@ -270,6 +322,10 @@ struct ErkLane {
   T q[P];
   bool first;
   bool made_jump;            // the last accepted step ended at a jump (clip.py:383-384)
+  double ev_old[kMaxEvents];   // condition values at the current state
+  int ev_mask;                 // bit i: condition i triggered (first one wins)
+  double tfinal;               // event time and state (the solve's end after an event)
+  T yfinal[D];
   int step_index, jump_index;  // ClipStepSizeController hints
   PidState pid;
   int32_t cs_completed, cs_num;
@@ -374,6 +430,23 @@ struct ErkLane {
     load_next_ts(a);
     if (a.save_t0) {
       write_save(a, T0, y);
+    }
+    ev_mask = 0;
+    if (Extras && a.n_events > 0) {
+      // values at (t0, y0); a boolean condition that holds ends the solve here
+      // (_integrate.py:1333-1426)
+      for (int i = 0; i < a.n_events; ++i) {
+        const double v = event_value<T, D>(a, i, T0, y);
+        const bool boolean = a.event_flags[i] & 1;
+        ev_old[i] = boolean ? (double)(v > 0.0) : v;
+        if (boolean && v > 0.0 && ev_mask == 0) ev_mask = 1 << i;
+      }
+      if (ev_mask) {
+        result = DFX_EVENT_OCCURRED;
+        tfinal = T0;
+#pragma unroll
+        for (int j = 0; j < D; ++j) yfinal[j] = y[j];
+      }
     }
     if (Extras && a.dense_ts) {
       const int64_t row = traj * ((int64_t)a.max_steps + 1);
@@ -497,6 +570,37 @@ struct ErkLane {
     if (keep) ++num_acc; else ++num_rej;
     if (result == DFX_SUCCESSFUL) result = code;
 
+    bool event_now = false;
+    if (Extras && a.n_events > 0) {
+      // Event check at the new state (_integrate.py:542-633); with a root finder the time
+      // is located on this step's interpolant (686-779), and saves after it are dropped
+      // (781-806).
+      int mask = 0, fired = -1;
+      for (int i = 0; i < a.n_events; ++i) {
+        T yn[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) yn[j] = keep ? y1[j] : y[j];
+        const double v = event_value<T, D>(a, i, ntprev, yn);
+        const int flags = a.event_flags[i];
+        const bool trig = event_triggered(flags, ev_old[i], v);
+        if (trig && mask == 0) { mask = 1 << i; fired = i; }
+        ev_old[i] = (flags & 1) ? (double)(v > 0.0) : v;
+      }
+      ev_mask = mask;
+      if (mask) {
+        event_now = true;
+        result = DFX_EVENT_OCCURRED;
+        if (a.event_root_find) {
+          tfinal = event_root<Tab, T, D>(a, fired, ts0, ts1, y, y1, k);
+          erk_interp<Tab, T, D>(ts0, ts1, y, y1, k, tfinal, yfinal);
+        } else {
+          tfinal = ntprev;
+#pragma unroll
+          for (int j = 0; j < D; ++j) yfinal[j] = keep ? y1[j] : y[j];
+        }
+      }
+    }
+
     if (keep) {
       if (a.grid) {
         double* g = a.grid + (traj * a.max_steps + (num_acc - 1)) * 2;
@@ -520,7 +624,7 @@ struct ErkLane {
         }
       }
       // SaveAt(ts): interpolate inside [ts0, ts1] (_integrate.py:456-491).
-      while (next_ts <= ts1) {
+      while (next_ts <= ts1 && !(Extras && event_now && next_ts > tfinal)) {
         T v[D];
         interp(ts0, ts1, y, y1, k, next_ts, v);
         write_save(a, next_ts, v);
@@ -535,7 +639,7 @@ struct ErkLane {
 #pragma unroll
         for (int j = 0; j < D; ++j) fsal_f[j] = f[j];
       }
-      if (a.save_steps) write_save(a, ntprev, y);
+      if (a.save_steps && !(Extras && event_now && ntprev > tfinal)) write_save(a, ntprev, y);
     }
     tprev = ntprev;
     tnext = ntnext;
@@ -553,8 +657,16 @@ struct ErkLane {
     if (T0 == T1) {
       for (int r = 0; r < a.n_ts; ++r) write_save(a, T0, y);
     }
+    if (Extras && result == DFX_EVENT_OCCURRED) {  // the solve ends at the event
+      tprev = tfinal;
+#pragma unroll
+      for (int j = 0; j < D; ++j) y[j] = yfinal[j];
+    }
     if (a.save_t1 && !a.save_steps) write_save(a, tprev, y);
     if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
+    if (Extras && a.n_events > 0 && a.event_mask) {
+      for (int i = 0; i < a.n_events; ++i) a.event_mask[traj * a.n_events + i] = (ev_mask >> i) & 1;
+    }
     // Unfilled rows keep the reference's preset values: ys inf, ts direction*inf in
     // normalised time, which the final rescaling by the direction makes +inf
     // (_integrate.py:1296, 1481).

@@ -18,11 +18,11 @@ static void launch_small_g(const ErkKArgs& a, int block, int64_t lanes, hipStrea
   }
 }
 
-// Variants: generic PID coefficients (G) and dense output / step clipping (X).
+// Variants: generic PID coefficients (G) and dense output / step clipping / events (X).
 template <class Tab, class VF, class T>
 static int launch_small(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
   const bool g = a.controller == DFX_PID && !pid_is_integral(a.pid);
-  const bool x = a.dense_ts != nullptr || a.n_step_ts > 0 || a.n_jump_ts > 0;
+  const bool x = a.dense_ts != nullptr || a.n_step_ts > 0 || a.n_jump_ts > 0 || a.n_events > 0;
   if (g) {
     if (x) launch_small_g<Tab, VF, T, true, true>(a, block, lanes, s);
     else launch_small_g<Tab, VF, T, true, false>(a, block, lanes, s);

@@ -37,7 +37,8 @@ enum { DFX_F64 = 0,          /* state, stages, time and controller in float64   
        DFX_F32S_F64T = 1,    /* float32 state and stages; float64 time and controller    */
        DFX_F32 = 2,          /* MLP: float32 matmuls (MFMA f32)                          */
        DFX_BF16_F32 = 3 };   /* MLP: bf16 MFMA matmuls, f32 accumulate, f32 RK algebra   */
-enum { DFX_SUCCESSFUL = 0, DFX_MAX_STEPS_REACHED = 1, DFX_DT_MIN_REACHED = 2 };
+enum { DFX_SUCCESSFUL = 0, DFX_MAX_STEPS_REACHED = 1, DFX_DT_MIN_REACHED = 2,
+       DFX_EVENT_OCCURRED = 3 };
 
 #define DFX_OK 0
 #define DFX_E_ARG (-1)        /* invalid argument / unsupported combination */
@@ -65,6 +66,10 @@ typedef struct dfx_erk_problem {
   /* ClipStepSizeController(PIDController, step_ts, jump_ts) (clip.py:120-428): counts of
    * the ascending time arrays in dfx_erk_io (0 = none; PID only). */
   int32_t n_step_ts, n_jump_ts;
+  /* Event (_event.py:13-118): n_events (<= 4) affine conditions c_t t + c_y . y + c_0 in
+   * direction-normalised time, as the reference evaluates cond_fn; event_root_find = 1
+   * locates the event time on the triggering step's interpolant. */
+  int32_t n_events, event_root_find;
 } dfx_erk_problem;
 
 /* ---- device buffers of one solve --------------------------------------------------- */
@@ -88,6 +93,10 @@ typedef struct dfx_erk_io {
                            increments k_i = dt f(z_i)                                 */
   const double* step_ts; /* [n_step_ts] ascending, shared by the batch, or NULL           */
   const double* jump_ts; /* [n_jump_ts] ascending, or NULL                                */
+  const double* event_coeffs; /* [n_events, d + 2]: (c_t, c_y[d], c_0) per condition       */
+  const int32_t* event_flags; /* [n_events]: bit 0 boolean (value > 0), bits 1-2 direction
+                                 (0 either sign change, 1 upcrossing, 2 downcrossing)      */
+  int32_t* event_mask;        /* out [B, n_events]: the condition that ended the solve    */
 } dfx_erk_io;
 
 /* Launch shape override (NULL = library default / tuned). */

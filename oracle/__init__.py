@@ -48,6 +48,9 @@ class Problem(ctypes.Structure):
         ("safety", ctypes.c_double), ("error_order", ctypes.c_double),
         ("step_ts", ctypes.c_void_p), ("jump_ts", ctypes.c_void_p),
         ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32),
+        ("event_coeffs", ctypes.c_void_p), ("event_flags", ctypes.c_void_p),
+        ("event_mask", ctypes.c_void_p),
+        ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
     ]
 
 
@@ -101,9 +104,12 @@ def make_problem(solver="tsit5", vf="lv", d=2, *, t0=0.0, dt0=None, rtol=1e-6, a
                  controller="pid", n_ts=0, save_t0=False, save_t1=False, save_steps=False,
                  max_steps=4096, hidden=0, pcoeff=0.0, icoeff=1.0, dcoeff=0.0, dtmin=None,
                  dtmax=None, force_dtmin=True, factormin=0.2, factormax=10.0, safety=0.9,
-                 error_order=None, step_ts=None, jump_ts=None):
+                 error_order=None, step_ts=None, jump_ts=None, events=None,
+                 event_root_find=False):
     """The problem struct.  step_ts / jump_ts wrap the PID controller in
-    ClipStepSizeController (sorted here; the arrays stay referenced by the struct)."""
+    ClipStepSizeController (sorted here; the arrays stay referenced by the struct).
+    events: list of (c_t, c_y [d], c_0, boolean, direction) affine event conditions,
+    direction None / True (up) / False (down)."""
     nan = float("nan")
     clip = {}
     for name, v in (("step_ts", step_ts), ("jump_ts", jump_ts)):
@@ -122,6 +128,17 @@ def make_problem(solver="tsit5", vf="lv", d=2, *, t0=0.0, dt0=None, rtol=1e-6, a
         jump_ts=None if clip["jump_ts"] is None else clip["jump_ts"].ctypes.data,
         n_step_ts=0 if clip["step_ts"] is None else clip["step_ts"].size,
         n_jump_ts=0 if clip["jump_ts"] is None else clip["jump_ts"].size)
+    if events:
+        co = np.ascontiguousarray([[float(e[0])] + [float(v) for v in e[1]] + [float(e[2])]
+                                   for e in events], dtype=np.float64)
+        fl = np.ascontiguousarray(
+            [int(bool(e[3])) | ((0 if e[4] is None else (1 if e[4] else 2)) << 1) for e in events],
+            dtype=np.int32)
+        prob.event_coeffs = co.ctypes.data
+        prob.event_flags = fl.ctypes.data
+        prob.n_events = len(events)
+        prob.event_root_find = int(bool(event_root_find))
+        clip["events"] = (co, fl)
     prob._keep = clip
     return prob
 
@@ -159,12 +176,18 @@ def solve(y0, t1, params, *, ts=None, dW=None, record_grid=False, nthreads=0, **
     grid = np.full((B, p.max_steps, 2), np.nan) if record_grid else None
     if dW is not None:
         dW = np.ascontiguousarray(np.asarray(dW, dtype=np.float64))
+    ev_mask = None
+    if p.n_events > 0:
+        ev_mask = np.zeros((B, p.n_events), dtype=np.int32)
+        p.event_mask = ev_mask.ctypes.data
     rc = L.orc_solve(ctypes.byref(p), B, _dp(y0), _dp(t1), t1_stride, _dp(ts), ts_stride,
                      _dp(params), param_stride, _dp(dW), _dp(ys), _dp(tso), _ip(stats),
                      _ip(res), _dp(grid), int(nthreads))
     if rc != 0:
         raise ValueError(f"oracle rejected the problem (code {rc})")
     out = dict(ys=ys, ts=tso, stats=stats, result=res)
+    if ev_mask is not None:
+        out["event_mask"] = ev_mask.astype(bool)
     if record_grid:
         out["grid"] = grid
     return out

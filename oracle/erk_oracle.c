@@ -508,6 +508,48 @@ static void interp_eval(int solver, const tableau_t *tb, int d, double ts0, doub
     }
 }
 
+/* ---- Event (_event.py:13-118; loop checks _integrate.py:542-633, root find and unsave
+ * 686-821) with affine conditions, see erk_oracle.h ---- */
+static double ev_value(const orc_problem *p, int i, double t, const double *y) {
+    const double *c = p->event_coeffs + (long)i * (p->d + 2);
+    double v = c[0] * t;
+    for (int j = 0; j < p->d; ++j) v = v + c[1 + j] * y[j];
+    return v + c[1 + p->d];
+}
+/* jnp.sign */
+static double ev_sign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : (x == 0.0 ? 0.0 : x)); }
+/* The event test of one condition between its previous value and the new one. */
+static int ev_triggered(int flags, double old_v, double new_v) {
+    if (flags & 1) return new_v > 0.0;
+    const int dirn = (flags >> 1) & 3;
+    const double so = ev_sign(old_v), sn = ev_sign(new_v);
+    if (dirn == 1) return (so <= 0.0) && (sn > 0.0);
+    if (dirn == 2) return (so > 0.0) && (sn <= 0.0);
+    return so != sn;
+}
+/* Time of the event on [a, b] (normalised): the end of the step for a boolean condition,
+ * else bisection of the condition along the step's interpolant down to adjacent floats,
+ * returning the end point with the smaller |value|. */
+static double ev_root(const orc_problem *p, const tableau_t *tb, int i, double a, double b,
+                      const double *y0, const double *y1, const stepout_t *o) {
+    if (p->event_flags[i] & 1) return b;
+    double ya[ORC_MAXD], yb[ORC_MAXD], ym[ORC_MAXD];
+    interp_eval(p->solver, tb, p->d, a, b, y0, y1, o, a, ya);
+    interp_eval(p->solver, tb, p->d, a, b, y0, y1, o, b, yb);
+    double ga = ev_value(p, i, a, ya), gb = ev_value(p, i, b, yb);
+    if (!(ev_sign(ga) != ev_sign(gb)) || isnan(ga) || isnan(gb)) return b;
+    const double lo0 = a, hi0 = b;
+    for (int it = 0; it < 2100; ++it) {
+        const double m = a + 0.5 * (b - a);
+        if (!(m > a && m < b)) break;
+        interp_eval(p->solver, tb, p->d, lo0, hi0, y0, y1, o, m, ym);
+        const double gm = ev_value(p, i, m, ym);
+        if (gm == 0.0) return m;
+        if (ev_sign(gm) == ev_sign(ga)) { a = m; ga = gm; } else { b = m; gb = gm; }
+    }
+    return (fabs(ga) <= fabs(gb)) ? a : b;
+}
+
 /* ------------------------------------------------------------------------- */
 /* One trajectory                                                              */
 /* ------------------------------------------------------------------------- */
@@ -614,6 +656,25 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
     stepout_t so;
     double ysave[ORC_MAXD];
 
+    /* Event set-up (_integrate.py:1333-1426): values at (t0, y0); a boolean condition
+     * that already holds ends the solve before the first step. */
+    const int ne = (p->n_events > 0 && p->event_coeffs && p->event_flags) ? p->n_events : 0;
+    double ev_old[8];
+    int ev_mask[8] = {0};
+    int ev_hit = 0;
+    double tfinal = NAN, yfinal[ORC_MAXD];
+    for (int i = 0; i < ne; ++i) {
+        const double v = ev_value(p, i, T0, y);
+        ev_old[i] = (p->event_flags[i] & 1) ? (double)(v > 0.0) : v;
+        ev_mask[i] = (p->event_flags[i] & 1) ? (v > 0.0) && !ev_hit : 0;
+        ev_hit |= ev_mask[i];
+    }
+    if (ev_hit) {
+        result = ORC_EVENT_OCCURRED;
+        tfinal = T0;
+        for (int j = 0; j < d; ++j) yfinal[j] = y[j];
+    }
+
     while (tprev < T1 && result == ORC_SUCCESSFUL && num_steps < p->max_steps) {
         /* a step that starts after a jump evaluates its first stage afresh
          * (runge_kutta.py:687, made_jump) */
@@ -697,13 +758,41 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
         if (keep) num_acc += 1; else num_rej += 1;
         if (result == ORC_SUCCESSFUL) result = step_result;
 
+        /* Event check at the new state (_integrate.py:542-633); with a root finder the
+         * event time is located on this step's interpolant (686-779) and the saves after
+         * it are dropped (781-806). */
+        if (ne > 0) {
+            const double *ynew = keep ? so.y1 : y;
+            int had = 0, fired = -1;
+            for (int i = 0; i < ne; ++i) {
+                const double v = ev_value(p, i, ntprev, ynew);
+                const double nv = (p->event_flags[i] & 1) ? (double)(v > 0.0) : v;
+                const int trig = ev_triggered(p->event_flags[i], ev_old[i], v);
+                ev_mask[i] = trig && !had;
+                if (ev_mask[i]) fired = i;
+                had |= trig;
+                ev_old[i] = nv;
+            }
+            if (had) {
+                result = ORC_EVENT_OCCURRED;
+                if (p->event_root_find) {
+                    tfinal = ev_root(p, tb, fired, tprev, tnext, y, so.y1, &so);
+                    interp_eval(p->solver, tb, d, tprev, tnext, y, so.y1, &so, tfinal, yfinal);
+                } else {
+                    tfinal = ntprev;
+                    for (int j = 0; j < d; ++j) yfinal[j] = ynew[j];
+                }
+            }
+        }
+
         if (keep) {
             if (grid) {
                 grid[(long)(num_acc - 1) * 2 + 0] = tprev;
                 grid[(long)(num_acc - 1) * 2 + 1] = tnext;
             }
             /* SaveAt(ts): the interpolant spans the step just taken [tprev, tnext]. */
-            while (ts_index < n_ts && ts_in[ts_index] * dir <= tnext) {
+            while (ts_index < n_ts && ts_in[ts_index] * dir <= tnext &&
+                   !(result == ORC_EVENT_OCCURRED && ts_in[ts_index] * dir > tfinal)) {
                 double tsv = ts_in[ts_index] * dir;
                 interp_eval(p->solver, tb, d, tprev, tnext, y, so.y1, &so, tsv, ysave);
                 tso[save_index] = tsv;
@@ -712,7 +801,7 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
             }
             for (int j = 0; j < d; ++j) y[j] = so.y1[j];
             if (tb->fsal) { fsal_first = 0; for (int j = 0; j < d; ++j) fsal_f[j] = so.flast[j]; }
-            if (p->save_steps) {
+            if (p->save_steps && !(result == ORC_EVENT_OCCURRED && ntprev > tfinal)) {
                 tso[save_index] = ntprev;
                 for (int j = 0; j < d; ++j) ys[(long)save_index * d + j] = y[j];
                 save_index++;
@@ -730,11 +819,17 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
         }
         save_index += n_ts;
     }
+    if (result == ORC_EVENT_OCCURRED) {   /* the solve ends at the event */
+        tprev = tfinal;
+        for (int j = 0; j < d; ++j) y[j] = yfinal[j];
+    }
     if (p->save_t1 && !p->save_steps) {
         tso[save_index] = tprev;
         for (int j = 0; j < d; ++j) ys[(long)save_index * d + j] = y[j];
         save_index++;
     }
+    if (ne > 0 && p->event_mask)
+        for (int i = 0; i < ne; ++i) p->event_mask[bi * ne + i] = ev_mask[i];
     if (tprev < T1 && result == ORC_SUCCESSFUL) result = ORC_MAX_STEPS_REACHED;
     for (int s = 0; s < n_out; ++s) tso[s] = tso[s] * dir;
     stats[0] = num_steps; stats[1] = num_acc; stats[2] = num_rej;

@@ -12,7 +12,8 @@ enum { ORC_TSIT5 = 0, ORC_DOPRI5 = 1, ORC_HEUN = 2 };
 enum { ORC_PID = 0, ORC_CONSTANT = 1 };
 enum { ORC_VF_LV = 0, ORC_VF_VDP = 1, ORC_VF_LINEAR = 2, ORC_VF_MLP = 3, ORC_VF_OU = 4 };
 /* Result codes: names follow diffrax RESULTS (_solution.py:13-31); successful == 0. */
-enum { ORC_SUCCESSFUL = 0, ORC_MAX_STEPS_REACHED = 1, ORC_DT_MIN_REACHED = 2 };
+enum { ORC_SUCCESSFUL = 0, ORC_MAX_STEPS_REACHED = 1, ORC_DT_MIN_REACHED = 2,
+       ORC_EVENT_OCCURRED = 3 };
 
 typedef struct {
     int32_t solver, controller, vf, d, hidden, n_ts;
@@ -26,6 +27,15 @@ typedef struct {
      * ascending times (unscaled), NULL / 0 when absent. */
     const double *step_ts, *jump_ts;
     int32_t n_step_ts, n_jump_ts;
+    /* Event (_event.py:13-118) with affine conditions c_t t + c_y . y + c_0 in normalised
+     * time: event_coeffs [n_events, d + 2] = (c_t, c_y[d], c_0); event_flags[i] bit 0 =
+     * boolean (value > 0), bits 1-2 = direction (0 either, 1 up, 2 down);
+     * event_root_find = 1 locates the time by bisection on the step's interpolant;
+     * event_mask [B, n_events] out (NULL to skip). */
+    const double *event_coeffs;
+    const int32_t *event_flags;
+    int32_t *event_mask;
+    int32_t n_events, event_root_find;
 } orc_problem;
 
 int orc_n_out(const orc_problem *p);
