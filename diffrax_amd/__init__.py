@@ -2,12 +2,13 @@
 
 Drop-in objects with the reference names and defaults (diffrax 0.7.2):
 diffeqsolve, ODETerm, ControlTerm, MultiTerm, Tsit5, Dopri5, Heun, PIDController,
-ConstantStepSize, SaveAt, SubSaveAt, RecursiveCheckpointAdjoint, RESULTS, Solution.
+ConstantStepSize, SaveAt, SubSaveAt, RecursiveCheckpointAdjoint, BacksolveAdjoint, RESULTS,
+Solution.
 Vector fields are registered device functions (LotkaVolterra, VanDerPol, LinearField,
 MLPField, OUDrift/DiagonalNoise); the integration loop runs in persistent HIP kernels
 behind the C ABI of include/diffrax_amd.h.  There is no CPU fallback.
 """
-from ._adjoint import AbstractAdjoint, RecursiveCheckpointAdjoint
+from ._adjoint import AbstractAdjoint, BacksolveAdjoint, RecursiveCheckpointAdjoint
 from ._event import AffineCondition, Event, Newton
 from ._integrate import ErkPlan, diffeqsolve
 from ._saveat import SaveAt, SubSaveAt, save_y

@@ -15,7 +15,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._adjoint import AbstractAdjoint, RecursiveCheckpointAdjoint
+from ._adjoint import AbstractAdjoint, BacksolveAdjoint, RecursiveCheckpointAdjoint
 from ._saveat import SaveAt, save_y
 from ._solution import RESULTS, Solution
 from ._solver import AbstractERK, AbstractSolver
@@ -122,6 +122,25 @@ def _check_ts(ts, t0, t1_vals, directions):
         raise RuntimeError("saveat.ts must lie between t0 and t1.")
 
 
+def _controller_fields(stepsize_controller, dt0f):
+    """The dfx_erk_problem fields of a PIDController or ConstantStepSize."""
+    if isinstance(stepsize_controller, PIDController):
+        c = stepsize_controller
+        return dict(controller=_lib.PID, pcoeff=float(c.pcoeff), icoeff=float(c.icoeff),
+                    dcoeff=float(c.dcoeff), rtol=c.rtol, atol=c.atol,
+                    dtmin=_nan_if_none(c.dtmin), dtmax=_nan_if_none(c.dtmax),
+                    factormin=c.factormin, factormax=c.factormax, safety=c.safety,
+                    error_order=_nan_if_none(c.error_order), force_dtmin=int(c.force_dtmin))
+    if isinstance(stepsize_controller, ConstantStepSize):
+        if dt0f is None:
+            raise ValueError("Constant step size solvers cannot select step size "
+                             "automatically; please pass a value for `dt0`.")
+        return dict(controller=_lib.CONSTANT, pcoeff=0.0, icoeff=1.0, dcoeff=0.0, rtol=0.0,
+                    atol=0.0, dtmin=float("nan"), dtmax=float("nan"), factormin=0.2,
+                    factormax=10.0, safety=0.9, error_order=float("nan"), force_dtmin=1)
+    raise NotImplementedError(f"stepsize controller {stepsize_controller!r}")
+
+
 def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                  max_steps, launch, record_grid=False, event=None):
     # Host-side checks first (they need no GPU); device placement at the end.
@@ -213,24 +232,7 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         if clip.store_rejected_steps is not None or clip.callback_on_reject is not None:
             raise NotImplementedError("store_rejected_steps is not provided by diffrax_amd")
         stepsize_controller = clip.controller
-    if isinstance(stepsize_controller, PIDController):
-        c = stepsize_controller
-        controller = _lib.PID
-        pcoeff, icoeff, dcoeff = float(c.pcoeff), float(c.icoeff), float(c.dcoeff)
-        rtol, atol = c.rtol, c.atol
-        extra = dict(dtmin=_nan_if_none(c.dtmin), dtmax=_nan_if_none(c.dtmax),
-                     factormin=c.factormin, factormax=c.factormax, safety=c.safety,
-                     error_order=_nan_if_none(c.error_order), force_dtmin=int(c.force_dtmin))
-    elif isinstance(stepsize_controller, ConstantStepSize):
-        controller = _lib.CONSTANT
-        if dt0f is None:
-            raise ValueError("Constant step size solvers cannot select step size "
-                             "automatically; please pass a value for `dt0`.")
-        pcoeff, icoeff, dcoeff, rtol, atol = 0.0, 1.0, 0.0, 0.0, 0.0
-        extra = dict(dtmin=float("nan"), dtmax=float("nan"), factormin=0.2, factormax=10.0,
-                     safety=0.9, error_order=float("nan"), force_dtmin=1)
-    else:
-        raise NotImplementedError(f"stepsize controller {stepsize_controller!r}")
+    ctrl = _controller_fields(stepsize_controller, dt0f)
 
     device = _require_gpu()
     y0b = y0b.to(device).contiguous()
@@ -239,12 +241,11 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         ts_t = ts_t.to(device).contiguous()
     params, param_stride = vfo.pack(args, B, d, y0b.dtype, device)
     prob = _lib.ErkProblem(
-        solver=solver.solver_id, controller=controller, vf=vfo.vf_id, precision=precision,
+        solver=solver.solver_id, vf=vfo.vf_id, precision=precision,
         d=d, hidden=0, n_ts=n_ts, save_t0=int(sub.t0), save_t1=int(sub.t1),
         save_steps=int(sub.steps), max_steps=int(max_steps), batch=B, t1_stride=t1_stride,
         ts_stride=ts_stride, param_stride=param_stride, t0=t0f,
-        dt0=_nan_if_none(dt0f), rtol=rtol, atol=atol, pcoeff=pcoeff, icoeff=icoeff,
-        dcoeff=dcoeff, **extra)
+        dt0=_nan_if_none(dt0f), **ctrl)
     ev = {}
     if event is not None:
         prob.n_events = len(event.conds)
@@ -280,7 +281,7 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
     return ErkPlan(prob, tensors, n_out, batched, directions, launch)
 
 
-def _finish(plan, t0, t1, max_steps, throw):
+def _finish(plan, t0, t1, max_steps, throw, ys=None):
     T = plan.t
     stats = {
         "num_steps": T["stats"][:, 0],
@@ -288,7 +289,8 @@ def _finish(plan, t0, t1, max_steps, throw):
         "num_rejected_steps": T["stats"][:, 2],
         "max_steps": max_steps,
     }
-    ys, ts, result = T["ys"], T["ts_out"], T["result"]
+    ys = T["ys"] if ys is None else ys
+    ts, result = T["ts_out"], T["result"]
     if not plan.batched:
         ys, ts, result = ys[0], ts[0], result[0]
         stats = {k: (v[0] if isinstance(v, torch.Tensor) else v) for k, v in stats.items()}
@@ -408,11 +410,112 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
         return _mlp.diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat,
                                     stepsize_controller, adjoint, max_steps, throw,
                                     _require_gpu())
+    if isinstance(adjoint, BacksolveAdjoint):
+        _check_backsolve(saveat, event, adjoint, stepsize_controller)
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                         max_steps, launch, event=event)
     plan.event = event
-    plan.launch()
-    return _finish(plan, t0, t1, max_steps, throw)
+    T = plan.t
+    wants_grad = torch.is_grad_enabled() and (T["y0"].requires_grad or
+                                              T["params"].requires_grad)
+    if not wants_grad:
+        plan.launch()
+        return _finish(plan, t0, t1, max_steps, throw)
+    if not isinstance(adjoint, BacksolveAdjoint):
+        raise NotImplementedError(
+            "gradients of the small-system solve are provided by adjoint=BacksolveAdjoint(); "
+            "RecursiveCheckpointAdjoint gradients are provided for the MLP neural ODE and the "
+            "SDE kernels")
+    back = _backsolve_problem(plan.problem, adjoint, stepsize_controller)
+    ys = _ErkBacksolve.apply(plan, back, bool(adjoint.kwargs.get("throw", throw)),
+                             T["y0"], T["params"])
+    return _finish(plan, t0, t1, max_steps, throw, ys=ys)
+
+
+def _check_backsolve(saveat, event, adjoint, stepsize_controller):
+    """The guards of BacksolveAdjoint.loop (_adjoint.py:798-858), and the backward
+    controller this build provides, checked before any GPU work."""
+    if saveat.subs is not None and not hasattr(saveat.subs, "fn"):
+        raise NotImplementedError(
+            "Cannot use `adjoint=BacksolveAdjoint()` with `SaveAt(subs=...)`.")
+    if saveat.dense or (saveat.subs is not None and saveat.subs.steps):
+        raise NotImplementedError(
+            "Cannot use `adjoint=BacksolveAdjoint()` with `saveat=SaveAt(steps=True)` or "
+            "saveat=SaveAt(dense=True)`.")
+    if saveat.subs is not None and saveat.subs.fn is not save_y:
+        raise NotImplementedError(
+            "Cannot use `adjoint=BacksolveAdjoint()` with `saveat=SaveAt(fn=...)`.")
+    if event is not None:
+        raise NotImplementedError("`diffrax.BacksolveAdjoint` is not compatible with events.")
+    ctrl = adjoint.kwargs.get("stepsize_controller", stepsize_controller)
+    if isinstance(ctrl, ClipStepSizeController):
+        raise NotImplementedError("BacksolveAdjoint with ClipStepSizeController is not provided")
+    if isinstance(ctrl, PIDController) and (float(ctrl.pcoeff) != 0.0 or
+                                            float(ctrl.dcoeff) != 0.0 or ctrl.dtmin is not None):
+        raise NotImplementedError(
+            "BacksolveAdjoint is provided with integral step control (pcoeff = dcoeff = 0) "
+            "and no dtmin: the backward solves then carry no controller state between save "
+            "times")
+    if "solver" in adjoint.kwargs and not isinstance(adjoint.kwargs["solver"], AbstractERK):
+        raise NotImplementedError(
+            f"backward solver {adjoint.kwargs['solver']!r} is not an explicit Runge-Kutta method")
+    if adjoint.kwargs.get("max_steps", 0) is None:
+        raise NotImplementedError("max_steps=None is not supported; pass an integer")
+
+
+def _backsolve_problem(prob, adjoint, stepsize_controller):
+    """The problem the backward solves run: the forward's, with BacksolveAdjoint's kwargs
+    applied as the reference applies them (_adjoint.py:623-632).  dfx_erk_backsolve solves
+    with -problem.dt0, so a dt0 given to BacksolveAdjoint is stored negated."""
+    kw = adjoint.kwargs
+    back = _lib.ErkProblem()
+    ctypes.memmove(ctypes.byref(back), ctypes.byref(prob), ctypes.sizeof(prob))
+    if "solver" in kw:
+        back.solver = kw["solver"].solver_id
+    if "dt0" in kw:
+        d = kw["dt0"]
+        back.dt0 = float("nan") if d is None else -float(d)
+    dt0b = None if math.isnan(back.dt0) else -back.dt0
+    ctrl = kw.get("stepsize_controller", stepsize_controller)
+    for k, v in _controller_fields(ctrl, dt0b).items():
+        setattr(back, k, v)
+    if "max_steps" in kw:
+        back.max_steps = int(kw["max_steps"])
+    return back
+
+
+class _ErkBacksolve(torch.autograd.Function):
+    """ys of a small-system solve, with the continuous adjoint as its backward."""
+
+    @staticmethod
+    def forward(ctx, plan, back, throw, y0, params):
+        plan.launch()
+        ctx.plan, ctx.back, ctx.throw = plan, back, throw
+        return plan.t["ys"].clone()
+
+    @staticmethod
+    def backward(ctx, gys):
+        plan, back = ctx.plan, ctx.back
+        T = plan.t
+        B, d = T["y0"].shape
+        ys = T["ys"]
+        gys = torch.zeros_like(ys) if gys is None else gys.to(ys.dtype).contiguous()
+        P = T["params"].shape[-1]
+        y0_bar = torch.empty((B, d), dtype=ys.dtype, device=ys.device)
+        params_bar = torch.empty((B, P), dtype=ys.dtype, device=ys.device)
+        result = torch.empty((B,), dtype=torch.int32, device=ys.device)
+        io = _lib.BacksolveIO(params=T["params"].data_ptr(), ys=ys.data_ptr(),
+                              ts_out=T["ts_out"].data_ptr(), ys_bar=gys.data_ptr(),
+                              y0_bar=y0_bar.data_ptr(), params_bar=params_bar.data_ptr(),
+                              result=result.data_ptr())
+        lib = _lib.load()
+        _lib.check(lib.dfx_erk_backsolve(ctypes.byref(back), ctypes.byref(io),
+                                         ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "dfx_erk_backsolve")
+        if ctx.throw and bool((result != 0).any()):
+            raise RuntimeError(RESULTS[int(result[result != 0][0].item())])
+        gparams = params_bar.sum(0) if plan.problem.param_stride == 0 else params_bar
+        return None, None, None, y0_bar, gparams.reshape(T["params"].shape)
 
 
 def _is_ou_sde(terms):

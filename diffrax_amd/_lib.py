@@ -62,6 +62,14 @@ class DenseIO(ctypes.Structure):
     ]
 
 
+class BacksolveIO(ctypes.Structure):
+    _fields_ = [
+        ("params", ctypes.c_void_p), ("ys", ctypes.c_void_p), ("ts_out", ctypes.c_void_p),
+        ("ys_bar", ctypes.c_void_p), ("y0_bar", ctypes.c_void_p),
+        ("params_bar", ctypes.c_void_p), ("result", ctypes.c_void_p),
+    ]
+
+
 class Launch(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("lanes", ctypes.c_int32)]
 
@@ -80,6 +88,8 @@ _SIGS = {
                        ctypes.c_size_t, ctypes.POINTER(Launch), ctypes.c_void_p], ctypes.c_int),
     "dfx_erk_dense_eval": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(DenseIO), ctypes.c_void_p],
                            ctypes.c_int),
+    "dfx_erk_backsolve": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(BacksolveIO),
+                           ctypes.c_void_p], ctypes.c_int),
 }
 
 _lib = None

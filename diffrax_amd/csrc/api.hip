@@ -14,6 +14,10 @@ int erk_small_solve_f64(int solver, int vf, int d, const ErkKArgs& a, int block,
                         hipStream_t s);
 int erk_small_solve_f32(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
                         hipStream_t s);
+int erk_backsolve_f64(int solver, int vf, int d, const ErkKArgs& a, const BacksolveArgs& b,
+                      hipStream_t s);
+int erk_backsolve_f32(int solver, int vf, int d, const ErkKArgs& a, const BacksolveArgs& b,
+                      hipStream_t s);
 }  // namespace dfx
 
 using namespace dfx;
@@ -195,6 +199,51 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (p->precision == DFX_F64) return erk_small_solve_f64(p->solver, p->vf, p->d, a, block, lanes, s);
   return erk_small_solve_f32(p->solver, p->vf, p->d, a, block, lanes, s);
+}
+
+int dfx_erk_backsolve(const dfx_erk_problem* p, const dfx_erk_backsolve_io* io, void* stream) {
+  if (!p || !io) return DFX_E_ARG;
+  if (p->solver < DFX_TSIT5 || p->solver > DFX_HEUN) return DFX_E_ARG;
+  if (p->precision != DFX_F64 && p->precision != DFX_F32S_F64T) return DFX_E_ARG;
+  const bool vf_ok = (p->vf == DFX_VF_LV || p->vf == DFX_VF_VDP) ? p->d == 2
+                     : (p->vf == DFX_VF_LINEAR && p->d >= 1 && p->d <= 3);
+  if (!vf_ok) return DFX_E_ARG;
+  if (p->controller != DFX_PID && p->controller != DFX_CONSTANT) return DFX_E_ARG;
+  if (p->controller == DFX_CONSTANT && isnan(p->dt0)) return DFX_E_ARG;
+  if (p->save_steps || p->n_step_ts > 0 || p->n_jump_ts > 0 || p->n_events > 0) return DFX_E_ARG;
+  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0 || p->param_stride < 0) return DFX_E_ARG;
+  if (p->batch == 0) return DFX_OK;
+  if (!io->params || !io->ys_bar || !io->y0_bar || !io->params_bar || !io->result) return DFX_E_ARG;
+  const int n_out = dfx_erk_n_out(p);
+  if (n_out > 0 && (!io->ys || !io->ts_out)) return DFX_E_ARG;
+  const double error_order = isnan(p->error_order) ? (double)solver_order(p->solver) : p->error_order;
+  ErkKArgs a;
+  memset(&a, 0, sizeof(a));
+  a.params = io->params;
+  a.B = p->batch;
+  a.param_stride = p->param_stride;
+  a.max_steps = p->max_steps;
+  a.has_dt0 = !isnan(p->dt0);
+  a.dt0 = -p->dt0;                  // the backward solves take -dt0 (_adjoint.py:629)
+  a.controller = p->controller;
+  a.error_order = error_order;
+  a.pid = make_pid(p, error_order);
+  // the adjoint kernels carry no controller state across intervals: integral PID only
+  if (a.controller == DFX_PID && (!pid_is_integral(a.pid) || a.pid.has_dtmin)) return DFX_E_ARG;
+  BacksolveArgs b;
+  memset(&b, 0, sizeof(b));
+  b.ys = io->ys;
+  b.ts_out = io->ts_out;
+  b.ys_bar = io->ys_bar;
+  b.y0_bar = io->y0_bar;
+  b.q_bar = io->params_bar;
+  b.result = io->result;
+  b.n_out = n_out;
+  b.save_t0 = p->save_t0;
+  b.t0 = p->t0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (p->precision == DFX_F64) return erk_backsolve_f64(p->solver, p->vf, p->d, a, b, s);
+  return erk_backsolve_f32(p->solver, p->vf, p->d, a, b, s);
 }
 
 int dfx_erk_dense_eval(const dfx_erk_problem* p, const dfx_erk_dense_io* io, void* stream) {

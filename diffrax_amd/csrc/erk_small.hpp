@@ -374,18 +374,20 @@ struct ErkLane {
 
   // diffeqsolve set-up for one instance (_integrate.py:1156-1305, pid.py:316-392,
   // constant.py:30-54).
-  __device__ void start(const ErkKArgs& a, int64_t id) {
-    tr.load();
-    traj = id;
-    const T* y0p = reinterpret_cast<const T*>(a.y0) + id * D;
-#pragma unroll
-    for (int j = 0; j < D; ++j) { y[j] = y0p[j]; fsal_f[j] = T(0); }
+  __device__ __forceinline__ void load_params(const ErkKArgs& a, int64_t id) {
     const T* qp = reinterpret_cast<const T*>(a.params) + id * a.param_stride;
 #pragma unroll
     for (int j = 0; j < P; ++j) q[j] = qp[j];
-    const double t1r = a.t1[id * a.t1_stride];
-    dir = (a.t0 < t1r) ? 1 : -1;
-    const double T0 = a.t0 * dir;
+  }
+
+  // Time normalisation and controller initialisation of a solve from t0r to t1r for the
+  // state in y (_integrate.py:1156-1265, pid.py:316-392, constant.py:30-54, clip.py:243-299);
+  // returns the normalised t0.
+  __device__ double begin_times(const ErkKArgs& a, double t0r, double t1r) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) fsal_f[j] = T(0);
+    dir = (t0r < t1r) ? 1 : -1;
+    const double T0 = t0r * dir;
     T1 = t1r * dir;
     first = true;
     made_jump = false;
@@ -397,7 +399,6 @@ struct ErkLane {
       if (a.pid.has_dtmin) dt0 = jmax(dt0, a.pid.dtmin);
       tnext = T0 + dt0;
       cs_completed = 0; cs_num = 0; cs_t0 = T0; cs_t1 = T1;
-      // ClipStepSizeController.init (clip.py:243-299)
       if (Extras && a.n_step_ts > 0) {
         const ClipTimes c{a.step_ts, a.n_step_ts, dir};
         step_index = c.find(T0);
@@ -424,6 +425,31 @@ struct ErkLane {
     tprev = T0;
     num_steps = num_acc = num_rej = 0;
     result = DFX_SUCCESSFUL;
+    return T0;
+  }
+
+  // One backward solve between two save times of the continuous adjoint: the state in y
+  // is integrated from t0r to t1r with no saves (BacksolveAdjoint._scan_fun,
+  // _adjoint.py:645-674).
+  __device__ void solve_segment(const ErkKArgs& a, double t0r, double t1r) {
+    begin_times(a, t0r, t1r);
+    save_index = 0;
+    ts_index = 0;
+    ts_row = nullptr;
+    next_ts = __builtin_inf();
+    ev_mask = 0;
+    while (running(a)) step(a);
+    if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
+  }
+
+  __device__ void start(const ErkKArgs& a, int64_t id) {
+    tr.load();
+    traj = id;
+    const T* y0p = reinterpret_cast<const T*>(a.y0) + id * D;
+#pragma unroll
+    for (int j = 0; j < D; ++j) y[j] = y0p[j];
+    load_params(a, id);
+    const double T0 = begin_times(a, a.t0, a.t1[id * a.t1_stride]);
     save_index = 0;
     ts_index = 0;
     ts_row = a.ts ? a.ts + id * a.ts_stride : nullptr;
@@ -682,6 +708,74 @@ struct ErkLane {
     a.result[traj] = result;
   }
 };
+
+// ---- continuous adjoint (BacksolveAdjoint, _adjoint.py:539-873) -------------------------
+// Input: the forward solve's saved rows (SaveAt t0 / ts / t1, no steps) and their
+// cotangents.  Between consecutive save times, from the last one back to t0, the adjoint
+// state a = (a_y, a_q) is integrated backwards together with y (restarted from the saved
+// forward state) under AdjointVF, after adding that save's cotangent to a_y.  Every
+// backward solve uses the forward's solver and step-size controller (dt0 negated), as
+// the reference's per-interval diffeqsolve does; with the integral PID controller the
+// carried controller state has no effect, so each interval is a fresh solve whose first
+// stage is evaluated (made_jump).
+struct BacksolveArgs {
+  const void* ys;        // [B, n_out, DB] forward saved states
+  const double* ts_out;  // [B, n_out] their times
+  const void* ys_bar;    // [B, n_out, DB] cotangents
+  void* y0_bar;          // out [B, DB]
+  void* q_bar;           // out [B, P] (per instance; summed by the caller when shared)
+  int32_t* result;       // out [B]
+  int32_t n_out, save_t0;
+  double t0;             // forward t0
+};
+
+template <class Tab, class VF, class T>
+__global__ void __launch_bounds__(256) erk_backsolve_kernel(const ErkKArgs a, const BacksolveArgs b) {
+  using AVF = AdjointVF<VF, T>;
+  constexpr int DB = VF::D, P = VF::P;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= a.B) return;
+  ErkLane<Tab, AVF, T, false, false> L;
+  L.tr.load();
+  L.traj = id;
+  L.load_params(a, id);
+  T ay[DB], aq[P];
+#pragma unroll
+  for (int j = 0; j < DB; ++j) ay[j] = T(0);
+#pragma unroll
+  for (int m = 0; m < P; ++m) aq[m] = T(0);
+  const T* ys = reinterpret_cast<const T*>(b.ys) + id * b.n_out * DB;
+  const T* g = reinterpret_cast<const T*>(b.ys_bar) + id * b.n_out * DB;
+  const double* tso = b.ts_out + id * b.n_out;
+  int32_t res = DFX_SUCCESSFUL;
+  const int s_end = b.save_t0 ? 1 : 0;
+  for (int s = b.n_out - 1; s >= s_end; --s) {
+#pragma unroll
+    for (int j = 0; j < DB; ++j) {
+      L.y[j] = ys[s * DB + j];
+      L.y[DB + j] = ay[j] + g[s * DB + j];
+    }
+#pragma unroll
+    for (int m = 0; m < P; ++m) L.y[2 * DB + m] = aq[m];
+    L.solve_segment(a, tso[s], s > 0 ? tso[s - 1] : b.t0);
+    if (res == DFX_SUCCESSFUL) res = L.result;
+#pragma unroll
+    for (int j = 0; j < DB; ++j) ay[j] = L.y[DB + j];
+#pragma unroll
+    for (int m = 0; m < P; ++m) aq[m] = L.y[2 * DB + m];
+  }
+  if (b.save_t0 && b.n_out > 0) {
+#pragma unroll
+    for (int j = 0; j < DB; ++j) ay[j] = ay[j] + g[j];
+  }
+  T* yb = reinterpret_cast<T*>(b.y0_bar) + id * DB;
+  T* qb = reinterpret_cast<T*>(b.q_bar) + id * P;
+#pragma unroll
+  for (int j = 0; j < DB; ++j) yb[j] = ay[j];
+#pragma unroll
+  for (int m = 0; m < P; ++m) qb[m] = aq[m];
+  b.result[id] = res;
+}
 
 // One trajectory per lane, no work queue: used when the launch has a lane for every
 // instance.  Lanes whose trajectory has finished idle (masked) until the wave's last lane

@@ -57,6 +57,45 @@ static int dispatch_vf(int vf, int d, const ErkKArgs& a, int block, int64_t lane
   return DFX_E_ARG;
 }
 
+// Continuous adjoint (BacksolveAdjoint) kernels: the same (solver, vector field, d) map.
+template <class Tab, class VF, class T>
+static int launch_backsolve(const ErkKArgs& a, const BacksolveArgs& b, hipStream_t s) {
+  const int64_t nb = (a.B + 63) / 64;
+  hipLaunchKernelGGL((erk_backsolve_kernel<Tab, VF, T>), dim3((unsigned)nb), dim3(64), 0, s, a, b);
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+template <class Tab, class T>
+static int dispatch_backsolve_vf(int vf, int d, const ErkKArgs& a, const BacksolveArgs& b,
+                                 hipStream_t s) {
+  switch (vf) {
+    case VF_LV:
+      if (d == 2) return launch_backsolve<Tab, LotkaVolterraVF<T>, T>(a, b, s);
+      break;
+    case VF_VDP:
+      if (d == 2) return launch_backsolve<Tab, VanDerPolVF<T>, T>(a, b, s);
+      break;
+    case VF_LINEAR:
+      if (d == 1) return launch_backsolve<Tab, LinearVF<T, 1>, T>(a, b, s);
+      if (d == 2) return launch_backsolve<Tab, LinearVF<T, 2>, T>(a, b, s);
+      if (d == 3) return launch_backsolve<Tab, LinearVF<T, 3>, T>(a, b, s);
+      break;
+    default: break;
+  }
+  return DFX_E_ARG;
+}
+
+template <class T>
+static int dispatch_backsolve(int solver, int vf, int d, const ErkKArgs& a, const BacksolveArgs& b,
+                             hipStream_t s) {
+  switch (solver) {
+    case SOLVER_TSIT5: return dispatch_backsolve_vf<Tsit5Tab, T>(vf, d, a, b, s);
+    case SOLVER_DOPRI5: return dispatch_backsolve_vf<Dopri5Tab, T>(vf, d, a, b, s);
+    case SOLVER_HEUN: return dispatch_backsolve_vf<HeunTab, T>(vf, d, a, b, s);
+    default: return DFX_E_ARG;
+  }
+}
+
 template <class T>
 static int dispatch_small(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
                           hipStream_t s) {

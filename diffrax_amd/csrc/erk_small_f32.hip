@@ -8,4 +8,8 @@ int erk_small_solve_f32(int solver, int vf, int d, const ErkKArgs& a, int block,
                         hipStream_t s) {
   return dispatch_small<float>(solver, vf, d, a, block, lanes, s);
 }
+int erk_backsolve_f32(int solver, int vf, int d, const ErkKArgs& a, const BacksolveArgs& b,
+                      hipStream_t s) {
+  return dispatch_backsolve<float>(solver, vf, d, a, b, s);
+}
 }  // namespace dfx

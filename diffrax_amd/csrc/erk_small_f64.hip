@@ -8,4 +8,8 @@ int erk_small_solve_f64(int solver, int vf, int d, const ErkKArgs& a, int block,
                         hipStream_t s) {
   return dispatch_small<double>(solver, vf, d, a, block, lanes, s);
 }
+int erk_backsolve_f64(int solver, int vf, int d, const ErkKArgs& a, const BacksolveArgs& b,
+                      hipStream_t s) {
+  return dispatch_backsolve<double>(solver, vf, d, a, b, s);
+}
 }  // namespace dfx

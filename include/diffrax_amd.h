@@ -136,6 +136,27 @@ int dfx_erk_workspace_bytes(const dfx_erk_problem* p, size_t* bytes);
 int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspace,
                   size_t workspace_bytes, const dfx_launch* launch, void* stream);
 
+/* Continuous adjoint (BacksolveAdjoint): buffers of a finished dfx_erk_solve with
+ * SaveAt(t0 / ts / t1) and the cotangents of its saved states. */
+typedef struct dfx_erk_backsolve_io {
+  const void* params;      /* as given to dfx_erk_solve (state dtype)                        */
+  const void* ys;          /* [B, n_out, d] saved states of the forward solve                */
+  const double* ts_out;    /* [B, n_out] their times                                         */
+  const void* ys_bar;      /* [B, n_out, d] cotangents of ys                                 */
+  void* y0_bar;            /* out [B, d]: dL/dy0                                             */
+  void* params_bar;        /* out [B, P]: dL/dparams per instance (sum them if shared)       */
+  int32_t* result;         /* out [B]: result codes of the backward solves                   */
+} dfx_erk_backsolve_io;
+
+/* Gradients by the continuous adjoint: from the last save back to t0, the augmented
+ * system (y, a_y, a_params) with vector field (f, -a_y^T df/dy, -a_y^T df/dparams) is
+ * solved backwards between consecutive save times with the forward's solver and
+ * controller, y restarting from the saved state and a_y receiving each save's cotangent.
+ * p is the forward problem (LV, VdP, linear with d <= 3; PID with integral control and no
+ * dtmin, or constant steps; no steps / clip / events).
+ * Replaces: BacksolveAdjoint (_adjoint.py:539-873) with AdjointTerm (_term.py:767-917). */
+int dfx_erk_backsolve(const dfx_erk_problem* p, const dfx_erk_backsolve_io* io, void* stream);
+
 /* Query side of SaveAt(dense=True): buffers a dfx_erk_solve wrote, and query times. */
 typedef struct dfx_erk_dense_io {
   const double* dense_ts;  /* [B, max_steps + 1] from dfx_erk_solve                       */

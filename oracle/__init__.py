@@ -51,6 +51,7 @@ class Problem(ctypes.Structure):
         ("event_coeffs", ctypes.c_void_p), ("event_flags", ctypes.c_void_p),
         ("event_mask", ctypes.c_void_p),
         ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
+        ("adj_base_d", ctypes.c_int32), ("pad2_", ctypes.c_int32),
     ]
 
 
@@ -80,6 +81,11 @@ def lib():
         L.orc_tableau.argtypes = [ctypes.c_int, dp, dp, dp, dp, ip]
         L.orc_single_step.argtypes = [P(Problem), dp, ctypes.c_double, ctypes.c_double,
                                       dp, dp, dp, dp]
+        L.orc_param_count.argtypes = [P(Problem)]
+        L.orc_param_count.restype = ctypes.c_long
+        L.orc_backsolve.argtypes = [P(Problem), ctypes.c_long, dp, ctypes.c_long, dp, dp, dp,
+                                    dp, dp, ip, ctypes.c_int]
+        L.orc_backsolve.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -208,6 +214,32 @@ def mlp_pack(W1, b1, W2, b2, W3, b3):
     """Flatten MLP weights in the oracle's parameter order."""
     return np.concatenate([np.asarray(x, dtype=np.float64).ravel()
                            for x in (W1, b1, W2, b2, W3, b3)])
+
+
+def backsolve(y0, t1, params, ys, ts_out, ys_bar, *, ts=None, nthreads=0, **kw):
+    """Gradients of sum(ys * ys_bar) by the continuous adjoint (BacksolveAdjoint) for the
+    forward problem described by y0 / t1 / params / ts / **kw (as for solve), whose saved
+    states and times are ys / ts_out.  Returns (y0_bar [B, d], params_bar [B, P], result)."""
+    y0 = np.ascontiguousarray(np.atleast_2d(np.asarray(y0, dtype=np.float64)))
+    B, d = y0.shape
+    params = np.asarray(params, dtype=np.float64)
+    param_stride = params.shape[-1] if params.ndim == 2 else 0
+    params = np.ascontiguousarray(params)
+    n_ts = 0 if ts is None else np.asarray(ts).shape[-1]
+    p = make_problem(d=d, n_ts=n_ts, **kw)
+    L = lib()
+    P = int(L.orc_param_count(ctypes.byref(p)))
+    y0b = np.zeros((B, d))
+    qb = np.zeros((B, P))
+    res = np.zeros(B, dtype=np.int32)
+    ys = np.ascontiguousarray(ys, dtype=np.float64)
+    ts_out = np.ascontiguousarray(ts_out, dtype=np.float64)
+    ys_bar = np.ascontiguousarray(ys_bar, dtype=np.float64)
+    rc = L.orc_backsolve(ctypes.byref(p), B, _dp(params), param_stride, _dp(ys), _dp(ts_out),
+                         _dp(ys_bar), _dp(y0b), _dp(qb), _ip(res), int(nthreads))
+    if rc != 0:
+        raise ValueError(f"oracle rejected the backsolve problem (code {rc})")
+    return y0b, qb, res
 
 
 def max_rel_err(y, y_ref, atol):

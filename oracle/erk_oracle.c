@@ -22,6 +22,12 @@
  *   - _select_initial_step (dt0 = None) ........................ _step_size_controller/pid.py:33-81
  *   - ConstantStepSize.init / adapt_step_size ................... _step_size_controller/constant.py:30-104
  *   - SDE increment k = f*dt + g.dW (MultiTerm/ControlTerm) .... _term.py:711-722, 420-427
+ *   - ClipStepSizeController (step_ts, jump_ts) ................ _step_size_controller/clip.py:120-428
+ *   - affine Event conditions, root find on the interpolant .... _event.py:13-118, _integrate.py:542-633, 686-821
+ *   - BacksolveAdjoint with AdjointTerm ........................ _adjoint.py:539-873, _term.py:767-917
+ *
+ * Each part is pinned by tests/test_oracle_*.py against closed forms, exact rational
+ * arithmetic, scipy, finite differences or the reference's own test recipes.
  *
  * Parity notes.  Operation order follows the reference where it is visible in the
  * Python source (k = control*f first, then y0 + (sum_j a_ij k_j) summed in stage order;
@@ -309,10 +315,56 @@ static inline double softplus(double x) {
 }
 
 /* f(t, y) in the caller's time (after the WrapTerm direction multiply). */
+static void vf_eval(const vfctx_t *cx, double t, const double *y, double *out);
+/* w^T df/dy and w^T df/dq of the registered small-system fields (the same products, in
+ * the same order, as the device's VF::vjp). */
+static void vf_vjp(int vf, int d, const double *q, const double *y, const double *w,
+                   double *gy, double *gq) {
+    if (vf == ORC_VF_LV) {
+        const double x = y[0], v = y[1];
+        gy[0] = w[0] * (q[0] - q[1] * v) + w[1] * (q[3] * v);
+        gy[1] = w[0] * (-(q[1] * x)) + w[1] * (q[3] * x - q[2]);
+        gq[0] = w[0] * x;
+        gq[1] = -(w[0] * (x * v));
+        gq[2] = -(w[1] * v);
+        gq[3] = w[1] * (x * v);
+    } else if (vf == ORC_VF_VDP) {
+        const double x = y[0], v = y[1];
+        gy[0] = w[1] * (-2.0 * q[0] * x * v - 1.0);
+        gy[1] = w[0] + w[1] * (q[0] * (1.0 - x * x));
+        gq[0] = w[1] * ((1.0 - x * x) * v);
+    } else {   /* linear */
+        for (int j = 0; j < d; ++j) {
+            double s = 0.0;
+            for (int i = 0; i < d; ++i) s = s + q[i * d + j] * w[i];
+            gy[j] = s;
+        }
+        for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) gq[i * d + j] = w[i] * y[j];
+    }
+}
+
 static void vf_eval(const vfctx_t *cx, double t, const double *y, double *out) {
     const orc_problem *p = cx->p;
     const double *q = cx->prm;
     int d = p->d;
+    if (p->adj_base_d > 0) {
+        /* AdjointTerm.vf_prod (_term.py:893-917): (f(y), -a^T df/dy, -a^T df/dq) */
+        const int db = p->adj_base_d;
+        orc_problem base = *p;
+        base.d = db;
+        base.adj_base_d = 0;
+        vfctx_t bcx = *cx;
+        bcx.p = &base;
+        double w[ORC_MAXD], gy[ORC_MAXD], gq[ORC_MAXD];
+        for (int j = 0; j < db; ++j) w[j] = -y[db + j];
+        vf_eval(&bcx, t, y, out);
+        vf_vjp(p->vf, db, q, y, w, gy, gq);
+        const long P = d - 2 * db;
+        for (int j = 0; j < db; ++j) out[db + j] = gy[j];
+        for (long m = 0; m < P; ++m) out[2 * db + m] = gq[m];
+        return;
+    }
     switch (p->vf) {
     case ORC_VF_LV: {
         /* dx = alpha x - beta x y ; dy = delta x y - gamma y ; q = [alpha, beta, gamma, delta] */
@@ -847,6 +899,12 @@ int orc_n_out(const orc_problem *p) {
 }
 
 long orc_param_count(const orc_problem *p) {
+    if (p->adj_base_d > 0) {
+        orc_problem base = *p;
+        base.d = p->adj_base_d;
+        base.adj_base_d = 0;
+        return orc_param_count(&base);
+    }
     switch (p->vf) {
     case ORC_VF_LV: return 4;
     case ORC_VF_VDP: return 1;
@@ -905,5 +963,72 @@ int orc_single_step(const orc_problem *p, const double *params, double ts0, doub
     erk_step(tb, &cx, 1, ts0, ts1, y0, 1, zero, 0, &so);
     for (int j = 0; j < p->d; ++j) { y1[j] = so.y1[j]; yerr[j] = so.yerr[j]; }
     if (k) for (int m = 0; m < tb->S; ++m) for (int j = 0; j < p->d; ++j) k[m * p->d + j] = so.k[m][j];
+    return 0;
+}
+
+/* BacksolveAdjoint (_adjoint.py:539-873): for each instance, from the last save back to
+ * t0, add the save's cotangent to a_y and solve the augmented system backwards over the
+ * interval with the forward's solver and controller (dt0 negated, _adjoint.py:629),
+ * y restarting from the saved forward state; a fresh solve per interval reproduces the
+ * reference's (its carried controller state is inert for the integral PID controller and
+ * the first stage after the jump is re-evaluated). */
+int orc_backsolve(const orc_problem *p, long B, const double *params, long param_stride,
+                  const double *ys, const double *ts_out, const double *ys_bar,
+                  double *y0_bar, double *params_bar, int32_t *result, int nthreads) {
+    init_tableaux();
+    const int db = p->d;
+    const long P = orc_param_count(p);
+    const int D = 2 * db + (int)P;
+    if (D > ORC_MAXD || p->save_steps) return -1;
+    const int n_out = orc_n_out(p);
+    orc_problem ap = *p;
+    ap.d = D;
+    ap.adj_base_d = db;
+    ap.n_ts = 0;
+    ap.save_t0 = 0;
+    ap.save_t1 = 1;
+    ap.save_steps = 0;
+    ap.dt0 = -p->dt0;
+    ap.n_events = 0;
+    ap.event_coeffs = NULL;
+    ap.event_flags = NULL;
+    ap.event_mask = NULL;
+    ap.step_ts = ap.jump_ts = NULL;
+    ap.n_step_ts = ap.n_jump_ts = 0;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long b = 0; b < B; ++b) {
+        const double *prm = params + b * param_stride;
+        const double *yb = ys + b * (long)n_out * db;
+        const double *gb = ys_bar + b * (long)n_out * db;
+        const double *tb = ts_out + b * (long)n_out;
+        double ay[ORC_MAXD], aq[ORC_MAXD], aug[ORC_MAXD], out[ORC_MAXD], tso1;
+        int32_t st[3], res_seg, res = ORC_SUCCESSFUL;
+        for (int j = 0; j < db; ++j) ay[j] = 0.0;
+        for (long m = 0; m < P; ++m) aq[m] = 0.0;
+        for (int s = n_out - 1; s >= (p->save_t0 ? 1 : 0); --s) {
+            for (int j = 0; j < db; ++j) {
+                aug[j] = yb[s * db + j];
+                aug[db + j] = ay[j] + gb[s * db + j];
+            }
+            for (long m = 0; m < P; ++m) aug[2 * db + m] = aq[m];
+            orc_problem seg = ap;
+            seg.t0 = tb[s];
+            const double t_end = s > 0 ? tb[s - 1] : p->t0;
+            solve_one(&seg, b, aug, t_end, NULL, prm, NULL, 0, out, &tso1, st, &res_seg, NULL, NULL);
+            if (res == ORC_SUCCESSFUL) res = res_seg;
+            for (int j = 0; j < db; ++j) ay[j] = out[db + j];
+            for (long m = 0; m < P; ++m) aq[m] = out[2 * db + m];
+        }
+        if (p->save_t0 && n_out > 0)
+            for (int j = 0; j < db; ++j) ay[j] = ay[j] + gb[j];
+        for (int j = 0; j < db; ++j) y0_bar[b * db + j] = ay[j];
+        for (long m = 0; m < P; ++m) params_bar[b * P + m] = aq[m];
+        result[b] = res;
+    }
     return 0;
 }

@@ -36,6 +36,9 @@ typedef struct {
     const int32_t *event_flags;
     int32_t *event_mask;
     int32_t n_events, event_root_find;
+    /* > 0: the vector field is the continuous-adjoint augmentation (AdjointTerm,
+     * _term.py:767-917) of vf with state dimension adj_base_d; d = 2 adj_base_d + P. */
+    int32_t adj_base_d, pad2_;
 } orc_problem;
 
 int orc_n_out(const orc_problem *p);
@@ -50,6 +53,12 @@ int orc_solve(const orc_problem *p, long B, const double *y0, const double *t1,
               long param_stride, const double *dW, double *ys, double *ts_out,
               int32_t *stats, int32_t *result, double *grid, int nthreads);
 double orc_pow(double x, double c);
+/* BacksolveAdjoint (_adjoint.py:539-873) for the forward problem p (SaveAt t0 / ts / t1):
+ * ys, ts_out [B, n_out, d] / [B, n_out] from orc_solve, ys_bar their cotangents; writes
+ * y0_bar [B, d], params_bar [B, P], result [B]. */
+int orc_backsolve(const orc_problem *p, long B, const double *params, long param_stride,
+                  const double *ys, const double *ts_out, const double *ys_bar,
+                  double *y0_bar, double *params_bar, int32_t *result, int nthreads);
 /* ClipStepSizeController helpers (clip.py:82-118), exposed for their unit tests. */
 int orc_clip_find(const double *ts, int n, int dir, double t, int hint);
 double orc_clip_bump(const double *ts, int n, int dir, double t, int *made_jump);

@@ -103,6 +103,51 @@ def test_host_validation_errors():
     assert isinstance(dfx.RESULTS[dfx.RESULTS.max_steps_reached], str)
 
 
+def test_backsolve_rejects_bad_arguments_without_launching():
+    lib = _lib.load()
+    z = ctypes.c_void_p(8)    # never dereferenced: every case fails validation first
+    io = _lib.BacksolveIO(params=z, ys=z, ts_out=z, ys_bar=z, y0_bar=z, params_bar=z, result=z)
+
+    def prob(**kw):
+        base = dict(solver=_lib.TSIT5, controller=_lib.PID, vf=_lib.VF_LV, precision=_lib.F64,
+                    d=2, batch=4, max_steps=10, save_t1=1, dt0=0.1, rtol=1e-6, atol=1e-8,
+                    pcoeff=0.0, icoeff=1.0, dcoeff=0.0, dtmin=float("nan"),
+                    dtmax=float("nan"), error_order=float("nan"), factormin=0.2,
+                    factormax=10.0, safety=0.9)
+        base.update(kw)
+        return _lib.ErkProblem(**base)
+
+    call = lambda p, i=io: lib.dfx_erk_backsolve(ctypes.byref(p), ctypes.byref(i), None)
+    for bad in (dict(pcoeff=0.3), dict(dcoeff=0.1), dict(dtmin=1e-3), dict(save_steps=1),
+                dict(vf=_lib.VF_LINEAR, d=4), dict(vf=_lib.VF_MLP, d=64), dict(n_events=1),
+                dict(n_step_ts=1), dict(controller=_lib.CONSTANT, dt0=float("nan")),
+                dict(precision=_lib.BF16_F32), dict(solver=9)):
+        assert call(prob(**bad)) == -1, bad
+    assert call(prob(), _lib.BacksolveIO()) == -1          # missing buffers
+    assert call(prob(batch=0), _lib.BacksolveIO()) == 0    # empty batch: nothing to do
+
+
+def test_backsolve_host_guards():
+    """BacksolveAdjoint's guards (_adjoint.py:781-858) raise before any GPU work."""
+    y0 = torch.ones(2, dtype=torch.float64, requires_grad=True)
+    with pytest.raises(ValueError, match="not valid"):
+        dfx.BacksolveAdjoint(rtol=1e-3)
+    for saveat in (dfx.SaveAt(steps=True), dfx.SaveAt(dense=True),
+                   dfx.SaveAt(subs=[dfx.SubSaveAt(t1=True), dfx.SubSaveAt(t0=True)])):
+        with pytest.raises(NotImplementedError):
+            _solve(y0=y0, saveat=saveat, adjoint=dfx.BacksolveAdjoint())
+    with pytest.raises(NotImplementedError, match="events"):
+        _solve(y0=y0, adjoint=dfx.BacksolveAdjoint(),
+               event=dfx.Event(dfx.AffineCondition(y=[1.0, 0.0], offset=-5.0)))
+    with pytest.raises(NotImplementedError, match="integral"):
+        _solve(y0=y0, adjoint=dfx.BacksolveAdjoint(),
+               stepsize_controller=dfx.PIDController(rtol=1e-6, atol=1e-8, pcoeff=0.3,
+                                                     icoeff=0.3))
+    with pytest.raises(NotImplementedError, match="integral"):
+        _solve(y0=y0, adjoint=dfx.BacksolveAdjoint(
+            stepsize_controller=dfx.PIDController(rtol=1e-6, atol=1e-8, dtmin=1e-4)))
+
+
 def test_no_cpu_fallback():
     """Without a GPU a well-formed solve raises instead of computing on the CPU."""
     if torch.cuda.is_available():
