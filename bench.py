@@ -23,6 +23,9 @@ headline; each with its own timing, error and CPU-baseline fields):
   neural_ode_grad: the same solve plus the checkpointed discrete adjoint of
       loss = mean((ys - targets)^2), targets ~ N(0, 0.5^2), gradients for all 98,880 MLP
       parameters and y0.
+  vdp_forward_grad: the headline solve plus the discrete-adjoint gradient (the default
+      RecursiveCheckpointAdjoint) of loss = mean(ys^2) for y0 and mu; float64 (bit-identical
+      to the FP64 oracle's adjoint on a 512-instance subset) and float32-state forms.
   ou_sde_heun: 262,144 paths of a 16-dim Ornstein-Uhlenbeck SDE, Heun, 1,000 steps on
       [0, 1], float32 Brownian increments [1000, 262144, 16] (16.8 GB) resident in HBM;
       forward, and forward + adjoint of loss = mean y(1)^2 for theta, mu, sigma.
@@ -338,6 +341,121 @@ def bench_sde(steps, warmup, with_cpu, P=262144, d=16, N=1000, cpu_paths=16384):
     return {"ou_sde_heun": out}
 
 
+# Discrete adjoint of one accepted Tsit5 step on Van der Pol (d = 2), algorithmic flop:
+# stage recompute 2*(42 + 7) + 7*5 = 133; y1 transpose 2*(12 + 1) = 26; per stage
+# w = control*k_bar (2), vjp (14), accumulations (2 + 1) over 7 stages = 133; a-transpose
+# 2*2*21 = 84: 376 per accepted step.  Per save: Tsit5 weights (35) + 2*(1 + 14) = 65.
+ADJ_FLOP_PER_STEP = 376
+ADJ_FLOP_PER_SAVE = 65
+
+
+def bench_vdp_grad(mu, y0, t1, ts, steps, warmup, with_cpu, forms=("f64", "f32"), check=512):
+    """The headline solve plus its gradient: the discrete adjoint (RecursiveCheckpointAdjoint,
+    the default adjoint) of loss = mean(ys^2) with respect to y0 and mu, for all 65,536
+    instances.  One step = forward with the accepted steps recorded + the cotangent + the
+    reverse sweep, replayed from one HIP graph."""
+    import ctypes
+    import numpy as np
+    import torch
+    from diffrax_amd import _lib
+    import oracle
+    lib = _lib.load()
+    dev = torch.device("cuda", 0)
+    B = y0.shape[0]
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    out = {}
+    ref = None
+    sub = slice(0, check)
+    kw = dict(solver="tsit5", vf="vdp", dt0=0.1, rtol=1e-6, atol=1e-9, max_steps=16384)
+    # FP64 oracle on a subset: forward with recorded steps, then its discrete adjoint
+    rf = oracle.solve(y0[sub], t1[sub], mu[sub, None], ts=ts[sub], record_grid=True,
+                      nthreads=cores, **kw)
+    n_el = float(B * ts.shape[1] * 2)
+    ryb, rqb = oracle.adjoint(t1[sub], mu[sub, None], rf, rf["ys"] * (2.0 / n_el), ts=ts[sub],
+                              nthreads=cores, **kw)
+    for form in forms:
+        dtype = torch.float64 if form == "f64" else torch.float32
+        plan = prepare(dtype, mu, y0, t1, ts)
+        plan.record_steps()
+        T = plan.t
+        ys = T["ys"]
+        gys = torch.empty_like(ys)
+        y0_bar = torch.empty((B, 2), dtype=dtype, device=dev)
+        mu_bar = torch.empty((B, 1), dtype=dtype, device=dev)
+        io = _lib.AdjointIO(params=T["params"].data_ptr(), t1=T["t1"].data_ptr(),
+                            ts=T["ts"].data_ptr(), grid=T["grid"].data_ptr(),
+                            grid_y=T["grid_y"].data_ptr(), stats=T["stats"].data_ptr(),
+                            ys_bar=gys.data_ptr(), y0_bar=y0_bar.data_ptr(),
+                            params_bar=mu_bar.data_ptr())
+
+        def adjoint():
+            _lib.check(lib.dfx_erk_adjoint(
+                ctypes.byref(plan.problem), ctypes.byref(io),
+                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "dfx_erk_adjoint")
+
+        def fwd_grad():
+            plan.launch()
+            torch.mul(ys, 2.0 / n_el, out=gys)      # d mean(ys^2) / d ys
+            adjoint()
+
+        g_all = _graphed(fwd_grad)
+        k_mean, k_med, wall = _event_time(g_all.replay, steps, warmup)
+        g_fwd = _graphed(plan.launch)
+        f_mean, _, _ = _event_time(g_fwd.replay, steps, warmup)
+        g_adj = _graphed(adjoint)
+        a_mean, a_med, _ = _event_time(g_adj.replay, steps, warmup)
+        g_all.replay()
+        torch.cuda.synchronize()
+        st = T["stats"].cpu().numpy()
+        acc = int(st[:, 1].sum())
+        saves = int(B * ts.shape[1])
+        flop = acc * ADJ_FLOP_PER_STEP + saves * ADJ_FLOP_PER_SAVE
+        yb = y0_bar[sub].double().cpu().numpy()
+        qb = mu_bar[sub].double().cpu().numpy()
+        o = dict(solves_per_s=round(B / (wall * 1e-3), 1), ms_per_step=round(wall, 4),
+                 event_ms_mean=round(k_mean, 4), event_ms_median=round(k_med, 4),
+                 forward_recording_ms=round(f_mean, 4), adjoint_kernel_ms=round(a_mean, 4),
+                 accepted_steps=acc, adjoint_algorithmic_flop=flop,
+                 adjoint_achieved_tflops=round(flop / (a_mean * 1e-3) / 1e12, 4),
+                 adjoint_frac_of_peak=round(flop / (a_mean * 1e-3) / 1e12 /
+                                            (FP64_PEAK_TFLOPS if form == "f64"
+                                             else FP32_PEAK_TFLOPS), 5),
+                 grad_normwise_err_vs_fp64_oracle_subset=max(
+                     float(np.max(np.abs(yb - ryb)) / np.max(np.abs(ryb))),
+                     float(np.max(np.abs(qb - rqb)) / np.max(np.abs(rqb)))))
+        if form == "f64":
+            o["grad_bit_identical_to_fp64_oracle_subset"] = bool(
+                np.array_equal(yb, ryb) and np.array_equal(qb, rqb))
+        out[form] = o
+        T.clear()
+        del g_all, g_fwd, g_adj, gys, y0_bar, mu_bar
+        torch.cuda.empty_cache()
+    res = dict(workload="65,536 uneven Van der Pol, Tsit5 + PID(1e-6, 1e-9), 64 saves, "
+                        "forward + discrete-adjoint gradient of mean(ys^2) for y0 and mu",
+               check_subset=check, forms=out)
+    if with_cpu:
+        n_cpu = 4096
+        csub = slice(0, n_cpu)
+        ckw = dict(kw, max_steps=1024)   # no instance takes more than 869 steps
+        times = []
+        while (sum(times) < 10.0 and len(times) < 500) or len(times) < 3:
+            t = time.perf_counter()
+            f = oracle.solve(y0[csub], t1[csub], mu[csub, None], ts=ts[csub], record_grid=True,
+                             nthreads=cores, **ckw)
+            oracle.adjoint(t1[csub], mu[csub, None], f, f["ys"] * (2.0 / n_el), ts=ts[csub],
+                           nthreads=cores, **ckw)
+            times.append(time.perf_counter() - t)
+        med = statistics.median(times)
+        res["cpu_baseline"] = dict(
+            value=round(n_cpu / med, 1), unit="solves/s (forward + gradient)", cores=cores,
+            kind="port",
+            sample=f"first {n_cpu} of the {B} instances, FP64 C oracle forward with recorded "
+                   f"steps + its discrete adjoint, OpenMP {cores} threads, max_steps 1024 (no "
+                   f"instance needs more than 869), median of {len(times)} runs "
+                   f"({med:.3f} s each, {sum(times):.1f} s in total)")
+    return {"vdp_forward_grad": res}
+
+
 def read_traffic(kernel_tag):
     """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
     path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.json")
@@ -354,8 +472,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--forms", default="f64,f32", help="device forms to time")
-    ap.add_argument("--workloads", default="mlp,sde",
-                    help="extra workloads to time besides the headline (mlp, sde; '' for none)")
+    ap.add_argument("--workloads", default="grad,mlp,sde",
+                    help="extra workloads to time besides the headline (grad, mlp, sde; '' for "
+                         "none)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter scale)")
     args = ap.parse_args()
@@ -467,6 +586,8 @@ def main():
     torch.cuda.empty_cache()
     extra = {}
     wl = [w for w in args.workloads.split(",") if w]
+    if "grad" in wl:
+        extra.update(bench_vdp_grad(mu, y0, t1, ts, args.steps, args.warmup, not args.no_cpu))
     if "mlp" in wl:
         extra.update(bench_mlp(args.steps, args.warmup, not args.no_cpu))
     if "sde" in wl:

@@ -22,7 +22,11 @@ class RecursiveCheckpointAdjoint(AbstractAdjoint):
 
     checkpoints: number of stored states per trajectory; None picks ceil(sqrt(N)) for N
     accepted steps (the reference picks ~log(max_steps) for its online binomial scheme,
-    _adjoint.py:315-329; both bound memory, the device kernels use uniform sqrt(N))."""
+    _adjoint.py:315-329; both bound memory, the device kernels use uniform sqrt(N)).
+    The small-system solves (LotkaVolterra, VanDerPol, LinearField) keep every accepted
+    step's start state instead: d values per step, no more than the step's own time
+    record, so their backward pass recomputes nothing but each step's stages and
+    ``checkpoints`` does not apply to them."""
 
     def __init__(self, checkpoints: Optional[int] = None):
         self.checkpoints = checkpoints

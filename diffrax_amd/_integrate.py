@@ -81,12 +81,8 @@ class ErkPlan:
         self.launch_cfg = launch_cfg
         self.direction_sign = direction_sign
         io = _lib.ErkIO()
-        for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid",
-                     "dense_ts", "dense_ys", "dense_k", "step_ts", "jump_ts",
-                     "event_coeffs", "event_flags", "event_mask"):
-            v = tensors.get(name)
-            setattr(io, name, v.data_ptr() if v is not None else None)
         self.io = io
+        self._bind()
         nbytes = ctypes.c_size_t(0)
         lib = _lib.load()
         _lib.check(lib.dfx_erk_workspace_bytes(ctypes.byref(problem), ctypes.byref(nbytes)),
@@ -94,6 +90,24 @@ class ErkPlan:
         self.ws_bytes = int(nbytes.value)
         self.ws = torch.empty(max(self.ws_bytes, 256), dtype=torch.uint8,
                               device=tensors["y0"].device)
+
+    def _bind(self):
+        for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid",
+                     "dense_ts", "dense_ys", "dense_k", "step_ts", "jump_ts",
+                     "event_coeffs", "event_flags", "event_mask", "grid_y"):
+            v = self.t.get(name)
+            setattr(self.io, name, v.data_ptr() if v is not None else None)
+
+    def record_steps(self):
+        """Record every accepted step's (t_start, t_end) and start state, the input of the
+        discrete adjoint."""
+        B, d = self.t["y0"].shape
+        M = int(self.problem.max_steps)
+        dev = self.t["y0"].device
+        if self.t.get("grid") is None:
+            self.t["grid"] = torch.empty((B, M, 2), dtype=torch.float64, device=dev)
+        self.t["grid_y"] = torch.empty((B, M, d), dtype=self.t["y0"].dtype, device=dev)
+        self._bind()
 
     def launch(self, stream=None):
         lib = _lib.load()
@@ -421,15 +435,32 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
     if not wants_grad:
         plan.launch()
         return _finish(plan, t0, t1, max_steps, throw)
-    if not isinstance(adjoint, BacksolveAdjoint):
-        raise NotImplementedError(
-            "gradients of the small-system solve are provided by adjoint=BacksolveAdjoint(); "
-            "RecursiveCheckpointAdjoint gradients are provided for the MLP neural ODE and the "
-            "SDE kernels")
-    back = _backsolve_problem(plan.problem, adjoint, stepsize_controller)
-    ys = _ErkBacksolve.apply(plan, back, bool(adjoint.kwargs.get("throw", throw)),
-                             T["y0"], T["params"])
+    if isinstance(adjoint, BacksolveAdjoint):
+        back = _backsolve_problem(plan.problem, adjoint, stepsize_controller)
+        ys = _ErkBacksolve.apply(plan, back, bool(adjoint.kwargs.get("throw", throw)),
+                                 T["y0"], T["params"])
+        return _finish(plan, t0, t1, max_steps, throw, ys=ys)
+    if not isinstance(adjoint, RecursiveCheckpointAdjoint):
+        raise NotImplementedError(f"adjoint {adjoint!r} is not provided by diffrax_amd")
+    _check_discrete_adjoint(plan, event, saveat)
+    plan.record_steps()
+    ys = _ErkDiscreteAdjoint.apply(plan, T["y0"], T["params"])
     return _finish(plan, t0, t1, max_steps, throw, ys=ys)
+
+
+def _check_discrete_adjoint(plan, event, saveat):
+    p = plan.problem
+    if event is not None:
+        raise NotImplementedError("gradients through a solve with events are not provided")
+    if saveat.dense:
+        raise NotImplementedError(
+            "gradients through SaveAt(dense=True) are not provided: Solution.evaluate is not "
+            "differentiable here")
+    ok = (p.vf in (_lib.VF_LV, _lib.VF_VDP) and p.d == 2) or (p.vf == _lib.VF_LINEAR and
+                                                               1 <= p.d <= 4)
+    if not ok:
+        raise NotImplementedError("gradients of the small-system solve are provided for "
+                                  "LotkaVolterra, VanDerPol and LinearField with d <= 4")
 
 
 def _check_backsolve(saveat, event, adjoint, stepsize_controller):
@@ -484,6 +515,38 @@ def _backsolve_problem(prob, adjoint, stepsize_controller):
     return back
 
 
+class _ErkDiscreteAdjoint(torch.autograd.Function):
+    """ys of a small-system solve, with reverse mode through the accepted steps
+    (dfx_erk_adjoint) as its backward."""
+
+    @staticmethod
+    def forward(ctx, plan, y0, params):
+        plan.launch()
+        ctx.plan = plan
+        return plan.t["ys"]
+
+    @staticmethod
+    def backward(ctx, gys):
+        plan = ctx.plan
+        T = plan.t
+        B, d = T["y0"].shape
+        ys = T["ys"]
+        gys = torch.zeros_like(ys) if gys is None else gys.to(ys.dtype).contiguous()
+        P = T["params"].shape[-1]
+        y0_bar = torch.empty((B, d), dtype=ys.dtype, device=ys.device)
+        params_bar = torch.empty((B, P), dtype=ys.dtype, device=ys.device)
+        ptr = lambda v: v.data_ptr() if v is not None else None
+        io = _lib.AdjointIO(params=ptr(T["params"]), t1=ptr(T["t1"]), ts=ptr(T.get("ts")),
+                            grid=ptr(T["grid"]), grid_y=ptr(T["grid_y"]), stats=ptr(T["stats"]),
+                            ys_bar=ptr(gys), y0_bar=ptr(y0_bar), params_bar=ptr(params_bar))
+        lib = _lib.load()
+        _lib.check(lib.dfx_erk_adjoint(ctypes.byref(plan.problem), ctypes.byref(io),
+                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
+                   "dfx_erk_adjoint")
+        gparams = params_bar.sum(0) if plan.problem.param_stride == 0 else params_bar
+        return None, y0_bar, gparams.reshape(T["params"].shape)
+
+
 class _ErkBacksolve(torch.autograd.Function):
     """ys of a small-system solve, with the continuous adjoint as its backward."""
 
@@ -491,7 +554,7 @@ class _ErkBacksolve(torch.autograd.Function):
     def forward(ctx, plan, back, throw, y0, params):
         plan.launch()
         ctx.plan, ctx.back, ctx.throw = plan, back, throw
-        return plan.t["ys"].clone()
+        return plan.t["ys"]
 
     @staticmethod
     def backward(ctx, gys):

@@ -49,7 +49,7 @@ class ErkIO(ctypes.Structure):
         ("dense_ts", ctypes.c_void_p), ("dense_ys", ctypes.c_void_p), ("dense_k", ctypes.c_void_p),
         ("step_ts", ctypes.c_void_p), ("jump_ts", ctypes.c_void_p),
         ("event_coeffs", ctypes.c_void_p), ("event_flags", ctypes.c_void_p),
-        ("event_mask", ctypes.c_void_p),
+        ("event_mask", ctypes.c_void_p), ("grid_y", ctypes.c_void_p),
     ]
 
 
@@ -59,6 +59,15 @@ class DenseIO(ctypes.Structure):
         ("stats", ctypes.c_void_p), ("t1", ctypes.c_void_p), ("tq", ctypes.c_void_p),
         ("n_q", ctypes.c_int64), ("tq_stride", ctypes.c_int64), ("left", ctypes.c_int32),
         ("derivative", ctypes.c_int32), ("out", ctypes.c_void_p),
+    ]
+
+
+class AdjointIO(ctypes.Structure):
+    _fields_ = [
+        ("params", ctypes.c_void_p), ("t1", ctypes.c_void_p), ("ts", ctypes.c_void_p),
+        ("grid", ctypes.c_void_p), ("grid_y", ctypes.c_void_p), ("stats", ctypes.c_void_p),
+        ("ys_bar", ctypes.c_void_p), ("y0_bar", ctypes.c_void_p),
+        ("params_bar", ctypes.c_void_p),
     ]
 
 
@@ -90,6 +99,8 @@ _SIGS = {
                            ctypes.c_int),
     "dfx_erk_backsolve": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(BacksolveIO),
                            ctypes.c_void_p], ctypes.c_int),
+    "dfx_erk_adjoint": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(AdjointIO),
+                         ctypes.c_void_p], ctypes.c_int),
 }
 
 _lib = None

@@ -8,6 +8,7 @@
 #include "../../include/diffrax_amd.h"
 #include "erk_small.hpp"
 #include "erk_dense.hpp"
+#include "erk_adjoint.hpp"
 
 namespace dfx {
 int erk_small_solve_f64(int solver, int vf, int d, const ErkKArgs& a, int block, int64_t lanes,
@@ -151,6 +152,8 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
   a.stats = io->stats;
   a.result = io->result;
   a.grid = io->grid;
+  if (io->grid_y && !io->grid) return DFX_E_ARG;
+  a.grid_y = io->grid_y;
   const int dense = (io->dense_ts != nullptr) + (io->dense_ys != nullptr) + (io->dense_k != nullptr);
   if (dense != 0 && dense != 3) return DFX_E_ARG;
   a.dense_ts = io->dense_ts;
@@ -244,6 +247,49 @@ int dfx_erk_backsolve(const dfx_erk_problem* p, const dfx_erk_backsolve_io* io, 
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (p->precision == DFX_F64) return erk_backsolve_f64(p->solver, p->vf, p->d, a, b, s);
   return erk_backsolve_f32(p->solver, p->vf, p->d, a, b, s);
+}
+
+int dfx_erk_adjoint(const dfx_erk_problem* p, const dfx_erk_adjoint_io* io, void* stream) {
+  if (!p || !io) return DFX_E_ARG;
+  if (p->solver < DFX_TSIT5 || p->solver > DFX_HEUN) return DFX_E_ARG;
+  if (p->precision != DFX_F64 && p->precision != DFX_F32S_F64T) return DFX_E_ARG;
+  const bool vf_ok = (p->vf == DFX_VF_LV || p->vf == DFX_VF_VDP) ? p->d == 2
+                     : (p->vf == DFX_VF_LINEAR && p->d >= 1 && p->d <= 4);
+  if (!vf_ok) return DFX_E_ARG;
+  if (p->n_events > 0) return DFX_E_ARG;
+  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0) return DFX_E_ARG;
+  if (p->t1_stride < 0 || p->ts_stride < 0 || p->param_stride < 0) return DFX_E_ARG;
+  if (p->batch == 0) return DFX_OK;
+  if (!io->params || !io->t1 || !io->stats || !io->ys_bar || !io->y0_bar || !io->params_bar)
+    return DFX_E_ARG;
+  if (p->max_steps > 0 && (!io->grid || !io->grid_y)) return DFX_E_ARG;
+  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
+  ErkKArgs a;
+  memset(&a, 0, sizeof(a));
+  a.t1 = io->t1;
+  a.ts = io->ts;
+  a.params = io->params;
+  a.B = p->batch;
+  a.t1_stride = p->t1_stride;
+  a.ts_stride = p->ts_stride;
+  a.param_stride = p->param_stride;
+  a.n_ts = p->n_ts;
+  a.n_out = dfx_erk_n_out(p);
+  a.save_t0 = p->save_t0;
+  a.save_t1 = p->save_t1;
+  a.save_steps = p->save_steps;
+  a.max_steps = p->max_steps;
+  a.t0 = p->t0;
+  DiscreteAdjArgs b;
+  b.grid = io->grid;
+  b.grid_y = io->grid_y;
+  b.stats = io->stats;
+  b.ys_bar = io->ys_bar;
+  b.y0_bar = io->y0_bar;
+  b.q_bar = io->params_bar;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if (p->precision == DFX_F64) return erk_adjoint_f64(p->solver, p->vf, p->d, a, b, s);
+  return erk_adjoint_f32(p->solver, p->vf, p->d, a, b, s);
 }
 
 int dfx_erk_dense_eval(const dfx_erk_problem* p, const dfx_erk_dense_io* io, void* stream) {

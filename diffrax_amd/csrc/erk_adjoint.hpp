@@ -1,0 +1,268 @@
+// erk_adjoint.hpp -- discrete adjoint of the small-system solve: reverse mode through the
+// accepted steps, the gradient RecursiveCheckpointAdjoint takes (_adjoint.py:264-312).
+//
+// The forward solve records, for every accepted step, its normalised (t_start, t_end)
+// (ErkKArgs::grid) and its start state (ErkKArgs::grid_y).  A state is d values, as small
+// as the step's time record, so the whole accepted sequence is kept and the backward pass
+// is one reverse sweep with no forward recomputation beyond each step's own stages.  The
+// step-size controller's factors are constants of the gradient (pid.py:374, 498-530) and
+// rejected steps contribute nothing, so only the accepted steps enter.
+//
+// Per accepted step, newest first (one lane per instance):
+//   - its stages are recomputed from the recorded start state, exactly as the forward
+//     computes them (z_i = y0 + sum_l a_il k_l, k_i = control * f(z_i));
+//   - the cotangents of the rows the step saved are pulled back through the interpolant
+//     (erk_interp_vjp) and, for SaveAt(steps), onto the end state;
+//   - y1 = y0 + sum_i b_i k_i and the stages are transposed: for i from the last stage
+//     down, w = control * k_bar_i, (z_bar_i, q_bar_i) = w^T (df/dy, df/dq) at z_i, and
+//     z_bar_i flows to y0 and to k_bar_l with weight a_il.
+// FSAL reuse of the previous step's last stage is f at the same point as y1 (a_S = b), so
+// its derivative is that of f(y0) of the next step.
+#pragma once
+#include "erk_small.hpp"
+
+namespace dfx {
+
+struct DiscreteAdjArgs {
+  const double* grid;     // [B, max_steps, 2] normalised (t_start, t_end) of accepted steps
+  const void* grid_y;     // [B, max_steps, D] start state of every accepted step
+  const int32_t* stats;   // [B, 3] (column 1: accepted steps)
+  const void* ys_bar;     // [B, n_out, D] cotangents of the saved rows
+  void* y0_bar;           // out [B, D]
+  void* q_bar;            // out [B, P] per instance
+};
+
+// Transpose of erk_interp: for out = interp(y0, y1, k, t) and cotangent g, adds the
+// cotangents of y0, y1 and k.  The interpolants are linear in (y0, y1, k) at fixed t.
+template <class Tab, class T, int D>
+__device__ __forceinline__ void erk_interp_vjp(double ts0, double ts1, double t, const T (&g)[D],
+                                               T (&gy0)[D], T (&gy1)[D], T (&gk)[Tab::S][D]) {
+  constexpr int S = Tab::S;
+  const double th = linear_rescale(ts0, t, ts1);
+  if constexpr (Tab::id == SOLVER_TSIT5) {
+    double bb[7];
+    tsit5_weights(th, bb);
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      gy0[j] = gy0[j] + g[j];
+#pragma unroll
+      for (int m = 0; m < 7; ++m) gk[m][j] = gk[m][j] + T(bb[m]) * g[j];
+    }
+  } else if constexpr (Tab::id == SOLVER_DOPRI5) {
+    // out = ca t^4 + cb t^3 + cc t^2 + k0 t + y0, ymid = y0 + sum cmid_m k_m
+    const T tt = T(th);
+    const T t2 = tt * tt, t3 = t2 * tt, t4 = t3 * tt;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const T gca = t4 * g[j], gcb = t3 * g[j], gcc = t2 * g[j];
+      const T gmid = T(16) * gca - T(32) * gcb + T(16) * gcc;
+      gy0[j] = gy0[j] + ((g[j] + (T(-8) * gca + T(18) * gcb - T(11) * gcc)) + gmid);
+      gy1[j] = gy1[j] + (T(-8) * gca + T(14) * gcb - T(5) * gcc);
+      gk[0][j] = gk[0][j] + (tt * g[j] + (T(-2) * gca + T(5) * gcb - T(4) * gcc));
+      gk[6][j] = gk[6][j] + (T(2) * gca - T(3) * gcb + gcc);
+      static_for<0, S>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        if constexpr (Dopri5Tab::cmid[m] != 0.0) gk[m][j] = gk[m][j] + T(Dopri5Tab::cmid[m]) * gmid;
+      });
+    }
+  } else {
+    // Heun: out = ca t^3 + cb t^2 + k0 t + y0
+    const T tt = T(th);
+    const T t2 = tt * tt, t3 = t2 * tt;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      const T gca = t3 * g[j], gcb = t2 * g[j];
+      gy0[j] = gy0[j] + (g[j] + (T(2) * gca - T(3) * gcb));
+      gy1[j] = gy1[j] + (T(-2) * gca + T(3) * gcb);
+      gk[0][j] = gk[0][j] + (tt * g[j] + (gca - T(2) * gcb));
+      gk[S - 1][j] = gk[S - 1][j] + (gca - gcb);
+    }
+  }
+}
+
+template <class Tab, class VF, class T>
+__global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const DiscreteAdjArgs b) {
+  constexpr int S = Tab::S, D = VF::D, P = VF::P;
+  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= a.B) return;
+  TabRegs<Tab> tr;
+  tr.load();
+  T q[P], gq[P];
+  const T* qp = reinterpret_cast<const T*>(a.params) + id * a.param_stride;
+#pragma unroll
+  for (int m = 0; m < P; ++m) { q[m] = qp[m]; gq[m] = T(0); }
+  const double t1r = a.t1[id * a.t1_stride];
+  const int dir = (a.t0 < t1r) ? 1 : -1;
+  const double T0 = a.t0 * dir, T1 = t1r * dir;
+  const int n = b.stats[id * 3 + 1];
+  const double* grid = b.grid + id * (int64_t)a.max_steps * 2;
+  const T* gyp = reinterpret_cast<const T*>(b.grid_y) + id * (int64_t)a.max_steps * D;
+  const T* g = reinterpret_cast<const T*>(b.ys_bar) + id * (int64_t)a.n_out * D;
+  const double* tsr = a.ts ? a.ts + id * a.ts_stride : nullptr;
+
+  // SaveAt(ts) rows the forward wrote: all of them for a zero-length solve, else those at
+  // or before the end of the last accepted step (a prefix: ts is sorted).
+  int n_saved = 0;
+  if (T0 == T1) {
+    n_saved = a.n_ts;
+  } else if (n > 0) {
+    const double tend = grid[(int64_t)(n - 1) * 2 + 1];
+    while (n_saved < a.n_ts && tsr[n_saved] * dir <= tend) ++n_saved;
+  }
+  T ab[D];  // cotangent of the state at the end of the step being processed
+#pragma unroll
+  for (int j = 0; j < D; ++j) ab[j] = T(0);
+  int r = a.save_t0 + n_saved + (a.save_steps ? n : 0);  // the SaveAt(t1) row
+  if (a.save_t1 && !a.save_steps) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[(int64_t)r * D + j];
+  }
+  r -= 1;               // the last row a step wrote
+  int ti = n_saved - 1;  // the last ts index saved
+  if (T0 == T1) ti = -1;
+
+  for (int s = n - 1; s >= 0; --s) {
+    const double ts0 = grid[(int64_t)s * 2], ts1 = grid[(int64_t)s * 2 + 1];
+    const double prev_end = s > 0 ? grid[(int64_t)(s - 1) * 2 + 1] : -__builtin_inf();
+    const double dt = ts1 - ts0;
+    const T control = T((double)dir * dt);
+    T y0[D], zs[S][D], k[S][D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) y0[j] = gyp[(int64_t)s * D + j];
+    // the step's stages, as the forward computes them
+    static_for<0, S>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        T kc[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : T(0);
+        zs[i][j] = y0[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, T(0));
+      }
+      constexpr double ci = Tab::c[i];
+      const double tsi = (ci == 1.0) ? ts1 : ts0 + ci * dt;
+      T f[D];
+      VF::eval(tsi * dir, zs[i], q, f);
+#pragma unroll
+      for (int j = 0; j < D; ++j) k[i][j] = control * f[j];
+    });
+    T gy1[D], gy0[D], gk[S][D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      gy1[j] = ab[j];
+      gy0[j] = T(0);
+#pragma unroll
+      for (int m = 0; m < S; ++m) gk[m][j] = T(0);
+    }
+    if (a.save_steps) {  // the row of the step's end state
+#pragma unroll
+      for (int j = 0; j < D; ++j) gy1[j] = gy1[j] + g[(int64_t)r * D + j];
+      --r;
+    }
+    // SaveAt(ts) rows inside this step: t in (end of the previous step, ts1]
+    while (ti >= 0 && tsr[ti] * dir > prev_end) {
+      T gs[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) gs[j] = g[(int64_t)r * D + j];
+      erk_interp_vjp<Tab, T, D>(ts0, ts1, tsr[ti] * dir, gs, gy0, gy1, gk);
+      --ti;
+      --r;
+    }
+    // y1 = y0 + sum_m b_m k_m
+#pragma unroll
+    for (int j = 0; j < D; ++j) gy0[j] = gy0[j] + gy1[j];
+    static_for<0, S>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      if constexpr (Tab::b[m] != 0.0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) gk[m][j] = gk[m][j] + T(Tab::b[m]) * gy1[j];
+      }
+    });
+    // stages, last first
+    static_for<0, S>([&](auto ic) {
+      constexpr int i = S - 1 - decltype(ic)::value;
+      T w[D], gz[D], gqi[P];
+#pragma unroll
+      for (int j = 0; j < D; ++j) w[j] = control * gk[i][j];
+      VF::vjp(zs[i], q, w, gz, gqi);
+#pragma unroll
+      for (int m = 0; m < P; ++m) gq[m] = gq[m] + gqi[m];
+#pragma unroll
+      for (int j = 0; j < D; ++j) gy0[j] = gy0[j] + gz[j];
+      static_for<0, i>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        if constexpr (Tab::a[i][l] != 0.0) {
+#pragma unroll
+          for (int j = 0; j < D; ++j) gk[l][j] = gk[l][j] + T(Tab::a[i][l]) * gz[j];
+        }
+      });
+    });
+#pragma unroll
+    for (int j = 0; j < D; ++j) ab[j] = gy0[j];
+  }
+  if (T0 == T1) {  // a zero-length solve saves y0 in every ts row
+    for (int rr = a.save_t0; rr < a.save_t0 + a.n_ts; ++rr) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[(int64_t)rr * D + j];
+    }
+  }
+  if (a.save_t0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[j];
+  }
+  T* yb = reinterpret_cast<T*>(b.y0_bar) + id * D;
+  T* qb = reinterpret_cast<T*>(b.q_bar) + id * P;
+#pragma unroll
+  for (int j = 0; j < D; ++j) yb[j] = ab[j];
+#pragma unroll
+  for (int m = 0; m < P; ++m) qb[m] = gq[m];
+}
+
+template <class Tab, class VF, class T>
+static int launch_adjoint(const ErkKArgs& a, const DiscreteAdjArgs& b, hipStream_t s) {
+  const int64_t nb = (a.B + 63) / 64;
+  hipLaunchKernelGGL((erk_adjoint_kernel<Tab, VF, T>), dim3((unsigned)nb), dim3(64), 0, s, a, b);
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+template <class Tab, class T>
+static int dispatch_adjoint_vf(int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
+                               hipStream_t s) {
+  switch (vf) {
+    case VF_LV:
+      if (d == 2) return launch_adjoint<Tab, LotkaVolterraVF<T>, T>(a, b, s);
+      break;
+    case VF_VDP:
+      if (d == 2) return launch_adjoint<Tab, VanDerPolVF<T>, T>(a, b, s);
+      break;
+    case VF_LINEAR:
+      switch (d) {
+        case 1: return launch_adjoint<Tab, LinearVF<T, 1>, T>(a, b, s);
+        case 2: return launch_adjoint<Tab, LinearVF<T, 2>, T>(a, b, s);
+        case 3: return launch_adjoint<Tab, LinearVF<T, 3>, T>(a, b, s);
+        case 4: return launch_adjoint<Tab, LinearVF<T, 4>, T>(a, b, s);
+        default: break;
+      }
+      break;
+    default: break;
+  }
+  return DFX_E_ARG;
+}
+
+template <class T>
+static int dispatch_adjoint(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
+                            hipStream_t s) {
+  switch (solver) {
+    case SOLVER_TSIT5: return dispatch_adjoint_vf<Tsit5Tab, T>(vf, d, a, b, s);
+    case SOLVER_DOPRI5: return dispatch_adjoint_vf<Dopri5Tab, T>(vf, d, a, b, s);
+    case SOLVER_HEUN: return dispatch_adjoint_vf<HeunTab, T>(vf, d, a, b, s);
+    default: return DFX_E_ARG;
+  }
+}
+
+int erk_adjoint_f64(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
+                    hipStream_t s);
+int erk_adjoint_f32(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
+                    hipStream_t s);
+
+}  // namespace dfx

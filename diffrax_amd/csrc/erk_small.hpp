@@ -26,7 +26,8 @@ struct ErkKArgs {
   double* ts_out;
   int32_t* stats;
   int32_t* result;
-  double* grid;
+  double* grid;       // [B, max_steps, 2] normalised (t_start, t_end) of accepted steps
+  void* grid_y;       // [B, max_steps, D] start state of each accepted step (with grid)
   double* dense_ts;   // SaveAt(dense=True): [B, max_steps + 1] accepted-step ends (t0 first)
   void* dense_ys;     // [B, max_steps + 1, D] states at those times
   void* dense_k;      // [B, max_steps, S, D] stage increments of each accepted step
@@ -632,6 +633,11 @@ struct ErkLane {
         double* g = a.grid + (traj * a.max_steps + (num_acc - 1)) * 2;
         g[0] = ts0;
         g[1] = ts1;
+        if (a.grid_y) {  // the accepted step's start state, for the discrete adjoint
+          T* gy = reinterpret_cast<T*>(a.grid_y) + (traj * a.max_steps + (num_acc - 1)) * D;
+#pragma unroll
+          for (int j = 0; j < D; ++j) gy[j] = y[j];
+        }
       }
       if (Extras && a.dense_ts) {
         // SaveAt(dense=True): the step's end time, end state and stage increments

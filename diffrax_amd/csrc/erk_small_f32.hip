@@ -1,6 +1,7 @@
 // erk_small_f32.hip -- float32-state / float64-time build of the small-system kernel.
 // Stage algebra runs in float32 (contraction into FMA allowed); time, step size, the
 // error norm and the controller stay in float64.
+#include "erk_adjoint.hpp"
 #include "erk_small_dispatch.hpp"
 
 namespace dfx {
@@ -11,5 +12,9 @@ int erk_small_solve_f32(int solver, int vf, int d, const ErkKArgs& a, int block,
 int erk_backsolve_f32(int solver, int vf, int d, const ErkKArgs& a, const BacksolveArgs& b,
                       hipStream_t s) {
   return dispatch_backsolve<float>(solver, vf, d, a, b, s);
+}
+int erk_adjoint_f32(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
+                    hipStream_t s) {
+  return dispatch_adjoint<float>(solver, vf, d, a, b, s);
 }
 }  // namespace dfx

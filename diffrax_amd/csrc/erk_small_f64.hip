@@ -1,6 +1,7 @@
 // erk_small_f64.hip -- float64 parity build of the small-system kernel.  Compiled with
 // -ffp-contract=off so every a*b+c is the rounded product then the rounded sum, as the
 // reference formulas are written.
+#include "erk_adjoint.hpp"
 #include "erk_small_dispatch.hpp"
 
 namespace dfx {
@@ -11,5 +12,9 @@ int erk_small_solve_f64(int solver, int vf, int d, const ErkKArgs& a, int block,
 int erk_backsolve_f64(int solver, int vf, int d, const ErkKArgs& a, const BacksolveArgs& b,
                       hipStream_t s) {
   return dispatch_backsolve<double>(solver, vf, d, a, b, s);
+}
+int erk_adjoint_f64(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
+                    hipStream_t s) {
+  return dispatch_adjoint<double>(solver, vf, d, a, b, s);
 }
 }  // namespace dfx

@@ -97,6 +97,8 @@ typedef struct dfx_erk_io {
   const int32_t* event_flags; /* [n_events]: bit 0 boolean (value > 0), bits 1-2 direction
                                  (0 either sign change, 1 upcrossing, 2 downcrossing)      */
   int32_t* event_mask;        /* out [B, n_events]: the condition that ended the solve    */
+  void* grid_y;         /* optional out [B, max_steps, d] state dtype: each accepted step's
+                           start state (needs grid; input of dfx_erk_adjoint)          */
 } dfx_erk_io;
 
 /* Launch shape override (NULL = library default / tuned). */
@@ -156,6 +158,32 @@ typedef struct dfx_erk_backsolve_io {
  * dtmin, or constant steps; no steps / clip / events).
  * Replaces: BacksolveAdjoint (_adjoint.py:539-873) with AdjointTerm (_term.py:767-917). */
 int dfx_erk_backsolve(const dfx_erk_problem* p, const dfx_erk_backsolve_io* io, void* stream);
+
+/* Discrete adjoint (RecursiveCheckpointAdjoint): buffers of a finished dfx_erk_solve run
+ * with grid and grid_y, and the cotangents of its saved rows. */
+typedef struct dfx_erk_adjoint_io {
+  const void* params;      /* as given to dfx_erk_solve (state dtype)                        */
+  const double* t1;        /* as given to dfx_erk_solve                                      */
+  const double* ts;        /* as given to dfx_erk_solve (NULL when n_ts == 0)                */
+  const double* grid;      /* [B, max_steps, 2] written by dfx_erk_solve                     */
+  const void* grid_y;      /* [B, max_steps, d] written by dfx_erk_solve                     */
+  const int32_t* stats;    /* [B, 3] written by dfx_erk_solve                                */
+  const void* ys_bar;      /* [B, n_out, d] cotangents of the saved rows                     */
+  void* y0_bar;            /* out [B, d]: dL/dy0                                             */
+  void* params_bar;        /* out [B, P]: dL/dparams per instance (sum them if shared)       */
+} dfx_erk_adjoint_io;
+
+/* Gradients of a loss of the saved rows with respect to y0 and the vector-field
+ * parameters, by reverse mode through the forward's accepted steps: per accepted step,
+ * newest first, the stages are recomputed from the recorded start state and transposed,
+ * together with the interpolation of the rows saved inside the step.  Step sizes are
+ * constants of the gradient and rejected steps contribute nothing, as in the reference.
+ * p is the forward problem (LV, VdP, linear with d <= 4; no events).
+ * Replaces: RecursiveCheckpointAdjoint.loop (_adjoint.py:264-312) differentiating the
+ * accepted steps of _integrate.py:302-662 (AbstractRungeKutta.step,
+ * _solver/runge_kutta.py:446-1203, and the save interpolation, _integrate.py:456-491),
+ * with the stop-gradient sites of pid.py:374, 498-530. */
+int dfx_erk_adjoint(const dfx_erk_problem* p, const dfx_erk_adjoint_io* io, void* stream);
 
 /* Query side of SaveAt(dense=True): buffers a dfx_erk_solve wrote, and query times. */
 typedef struct dfx_erk_dense_io {

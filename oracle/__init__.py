@@ -52,6 +52,7 @@ class Problem(ctypes.Structure):
         ("event_mask", ctypes.c_void_p),
         ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
         ("adj_base_d", ctypes.c_int32), ("pad2_", ctypes.c_int32),
+        ("grid_y", ctypes.c_void_p),
     ]
 
 
@@ -86,6 +87,10 @@ def lib():
         L.orc_backsolve.argtypes = [P(Problem), ctypes.c_long, dp, ctypes.c_long, dp, dp, dp,
                                     dp, dp, ip, ctypes.c_int]
         L.orc_backsolve.restype = ctypes.c_int
+        L.orc_adjoint.argtypes = [P(Problem), ctypes.c_long, dp, ctypes.c_long, dp,
+                                  ctypes.c_long, dp, ctypes.c_long, dp, dp, ip, dp, dp, dp,
+                                  ctypes.c_int]
+        L.orc_adjoint.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -179,7 +184,11 @@ def solve(y0, t1, params, *, ts=None, dW=None, record_grid=False, nthreads=0, **
     tso = np.empty((B, n_out))
     stats = np.empty((B, 3), dtype=np.int32)
     res = np.empty(B, dtype=np.int32)
-    grid = np.full((B, p.max_steps, 2), np.nan) if record_grid else None
+    # entries past each instance's accepted-step count are left unset (lazily allocated)
+    grid = np.empty((B, p.max_steps, 2)) if record_grid else None
+    grid_y = np.empty((B, p.max_steps, d)) if record_grid else None
+    if record_grid:
+        p.grid_y = grid_y.ctypes.data
     if dW is not None:
         dW = np.ascontiguousarray(np.asarray(dW, dtype=np.float64))
     ev_mask = None
@@ -196,7 +205,43 @@ def solve(y0, t1, params, *, ts=None, dW=None, record_grid=False, nthreads=0, **
         out["event_mask"] = ev_mask.astype(bool)
     if record_grid:
         out["grid"] = grid
+        out["grid_y"] = grid_y
     return out
+
+
+def adjoint(t1, params, fwd, ys_bar, *, ts=None, nthreads=0, **kw):
+    """Gradients of sum(ys * ys_bar) by the discrete adjoint (reverse mode through the
+    accepted steps, RecursiveCheckpointAdjoint) for the forward problem described by
+    t1 / params / ts / **kw (as for solve), whose solve(..., record_grid=True) result is
+    fwd.  Returns (y0_bar [B, d], params_bar [B, P])."""
+    grid, grid_y, stats = fwd["grid"], fwd["grid_y"], fwd["stats"]
+    B, _, d = grid_y.shape
+    t1 = np.asarray(t1, dtype=np.float64)
+    t1_stride = 1 if t1.ndim == 1 else 0
+    t1 = np.ascontiguousarray(np.atleast_1d(t1))
+    params = np.asarray(params, dtype=np.float64)
+    param_stride = params.shape[-1] if params.ndim == 2 else 0
+    params = np.ascontiguousarray(params)
+    if ts is not None:
+        ts = np.ascontiguousarray(np.asarray(ts, dtype=np.float64))
+        n_ts, ts_stride = ts.shape[-1], (ts.shape[-1] if ts.ndim == 2 else 0)
+    else:
+        n_ts, ts_stride = 0, 0
+    p = make_problem(d=d, n_ts=n_ts, **kw)
+    L = lib()
+    P = int(L.orc_param_count(ctypes.byref(p)))
+    y0b = np.zeros((B, d))
+    qb = np.zeros((B, P))
+    grid = np.ascontiguousarray(grid, dtype=np.float64)
+    grid_y = np.ascontiguousarray(grid_y, dtype=np.float64)
+    stats = np.ascontiguousarray(stats, dtype=np.int32)
+    ys_bar = np.ascontiguousarray(ys_bar, dtype=np.float64)
+    rc = L.orc_adjoint(ctypes.byref(p), B, _dp(params), param_stride, _dp(t1), t1_stride,
+                       _dp(ts), ts_stride, _dp(grid), _dp(grid_y), _ip(stats), _dp(ys_bar),
+                       _dp(y0b), _dp(qb), int(nthreads))
+    if rc != 0:
+        raise ValueError(f"oracle rejected the adjoint problem (code {rc})")
+    return y0b, qb
 
 
 def single_step(solver, vf, y0, ts0, ts1, params, hidden=0):

@@ -512,22 +512,27 @@ static double linear_rescale(double t0, double t, double t1) {
     return num / den;
 }
 
+/* Tsit5 dense-output weights b_i(theta) (tsit5.py:116-156), in its factorised form. */
+static void tsit5_bb(double th, double *bb) {
+    double t2 = th * th;
+    bb[0] = -1.0530884977290216 * th * (th - 1.3299890189751412) *
+            (t2 - 1.4364028541716351 * th + 0.7139816917074209);
+    bb[1] = 0.1017 * t2 * (t2 - 2.1966568338249754 * th + 1.2949852507374631);
+    bb[2] = 2.490627285651252793 * t2 * (t2 - 2.38535645472061657 * th + 1.57803468208092486);
+    bb[3] = -16.54810288924490272 * (th - 1.21712927295533244) * (th - 0.61620406037800089) * t2;
+    bb[4] = 47.37952196281928122 * (th - 1.203071208372362603) * (th - 0.658047292653547382) * t2;
+    bb[5] = -34.87065786149660974 * (th - 1.2) * (th - 0.666666666666666667) * t2;
+    bb[6] = 2.5 * (th - 1.0) * (th - 0.6) * t2;
+}
+
 static void interp_eval(int solver, const tableau_t *tb, int d, double ts0, double ts1,
                         const double *y0, const double *y1, const stepout_t *o, double t,
                         double *out) {
     double th = linear_rescale(ts0, t, ts1);
     if (solver == ORC_TSIT5) {
         /* _Tsit5Interpolation.evaluate (tsit5.py:116-156) */
-        double t2 = th * th;
         double bb[7];
-        bb[0] = -1.0530884977290216 * th * (th - 1.3299890189751412) *
-                (t2 - 1.4364028541716351 * th + 0.7139816917074209);
-        bb[1] = 0.1017 * t2 * (t2 - 2.1966568338249754 * th + 1.2949852507374631);
-        bb[2] = 2.490627285651252793 * t2 * (t2 - 2.38535645472061657 * th + 1.57803468208092486);
-        bb[3] = -16.54810288924490272 * (th - 1.21712927295533244) * (th - 0.61620406037800089) * t2;
-        bb[4] = 47.37952196281928122 * (th - 1.203071208372362603) * (th - 0.658047292653547382) * t2;
-        bb[5] = -34.87065786149660974 * (th - 1.2) * (th - 0.666666666666666667) * t2;
-        bb[6] = 2.5 * (th - 1.0) * (th - 0.6) * t2;
+        tsit5_bb(th, bb);
         for (int j = 0; j < d; ++j) {
             double acc = bb[0] * o->k[0][j];
             for (int m = 1; m < 7; ++m) acc = acc + bb[m] * o->k[m][j];
@@ -841,6 +846,10 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
             if (grid) {
                 grid[(long)(num_acc - 1) * 2 + 0] = tprev;
                 grid[(long)(num_acc - 1) * 2 + 1] = tnext;
+                if (p->grid_y) {
+                    double *gy = p->grid_y + ((long)bi * p->max_steps + (num_acc - 1)) * d;
+                    for (int j = 0; j < d; ++j) gy[j] = y[j];
+                }
             }
             /* SaveAt(ts): the interpolant spans the step just taken [tprev, tnext]. */
             while (ts_index < n_ts && ts_in[ts_index] * dir <= tnext &&
@@ -1029,6 +1038,167 @@ int orc_backsolve(const orc_problem *p, long B, const double *params, long param
         for (int j = 0; j < db; ++j) y0_bar[b * db + j] = ay[j];
         for (long m = 0; m < P; ++m) params_bar[b * P + m] = aq[m];
         result[b] = res;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Discrete adjoint (RecursiveCheckpointAdjoint)                               */
+/* ------------------------------------------------------------------------- */
+
+/* Transpose of interp_eval at time t: adds the cotangents of y0, y1 and k for the
+ * cotangent g of the interpolated value (the interpolants are linear in y0, y1, k). */
+static void interp_vjp(int solver, const tableau_t *tb, int d, double ts0, double ts1,
+                       double t, const double *g, double *gy0, double *gy1,
+                       double gk[MAXS][ORC_MAXD]) {
+    double th = linear_rescale(ts0, t, ts1);
+    if (solver == ORC_TSIT5) {
+        double bb[7];
+        tsit5_bb(th, bb);
+        for (int j = 0; j < d; ++j) {
+            gy0[j] = gy0[j] + g[j];
+            for (int m = 0; m < 7; ++m) gk[m][j] = gk[m][j] + bb[m] * g[j];
+        }
+    } else if (solver == ORC_DOPRI5) {
+        /* out = a th^4 + b th^3 + c th^2 + k0 th + y0 with ymid = y0 + cmid . k */
+        double t2 = th * th, t3 = t2 * th, t4 = t3 * th;
+        for (int j = 0; j < d; ++j) {
+            double ga = t4 * g[j], gb = t3 * g[j], gc = t2 * g[j];
+            double gmid = 16.0 * ga - 32.0 * gb + 16.0 * gc;
+            gy0[j] = gy0[j] + ((g[j] + (-8.0 * ga + 18.0 * gb - 11.0 * gc)) + gmid);
+            gy1[j] = gy1[j] + (-8.0 * ga + 14.0 * gb - 5.0 * gc);
+            gk[0][j] = gk[0][j] + (th * g[j] + (-2.0 * ga + 5.0 * gb - 4.0 * gc));
+            gk[6][j] = gk[6][j] + (2.0 * ga - 3.0 * gb + gc);
+            for (int m = 0; m < 7; ++m)
+                if (tb->cmid[m] != 0.0) gk[m][j] = gk[m][j] + tb->cmid[m] * gmid;
+        }
+    } else {
+        /* Hermite: out = a th^3 + b th^2 + k0 th + y0 */
+        double t2 = th * th, t3 = t2 * th;
+        int L = tb->S - 1;
+        for (int j = 0; j < d; ++j) {
+            double ga = t3 * g[j], gb = t2 * g[j];
+            gy0[j] = gy0[j] + (g[j] + (2.0 * ga - 3.0 * gb));
+            gy1[j] = gy1[j] + (-2.0 * ga + 3.0 * gb);
+            gk[0][j] = gk[0][j] + (th * g[j] + (ga - 2.0 * gb));
+            gk[L][j] = gk[L][j] + (ga - gb);
+        }
+    }
+}
+
+/* One instance: reverse sweep over the accepted steps recorded by the forward solve.
+ * Per step, newest first: the stages are recomputed from the recorded start state (as
+ * erk_step computes them, every stage evaluated), the cotangents of the rows saved in the
+ * step are pulled back through the interpolant, then y1 = y0 + b . k and the stages are
+ * transposed, last stage first (runge_kutta.py:847-1052 read backwards). */
+static void adjoint_one(const orc_problem *p, const tableau_t *tb, const double *prm,
+                        double t1r, const double *tsb, const double *grid, const double *gy,
+                        int n, const double *g, double *y0b, double *qb) {
+    const int d = p->d, S = tb->S;
+    const long P = orc_param_count(p);
+    vfctx_t cx = {p, prm, NULL, 0, NULL};
+    const int dir = (p->t0 < t1r) ? 1 : -1;
+    const double T0 = p->t0 * dir, T1 = t1r * dir;
+    int n_saved = 0;
+    if (T0 == T1) {
+        n_saved = p->n_ts;
+    } else if (n > 0) {
+        const double tend = grid[(long)(n - 1) * 2 + 1];
+        while (n_saved < p->n_ts && tsb[n_saved] * dir <= tend) ++n_saved;
+    }
+    double ab[ORC_MAXD], gq[ORC_MAXD * ORC_MAXD], gqi[ORC_MAXD * ORC_MAXD];
+    for (int j = 0; j < d; ++j) ab[j] = 0.0;
+    for (long m = 0; m < P; ++m) gq[m] = 0.0;
+    int r = p->save_t0 + n_saved + (p->save_steps ? n : 0);
+    if (p->save_t1 && !p->save_steps)
+        for (int j = 0; j < d; ++j) ab[j] = ab[j] + g[(long)r * d + j];
+    r -= 1;
+    int ti = (T0 == T1) ? -1 : n_saved - 1;
+    for (int s = n - 1; s >= 0; --s) {
+        const double ts0 = grid[(long)s * 2], ts1 = grid[(long)s * 2 + 1];
+        const double prev_end = s > 0 ? grid[(long)(s - 1) * 2 + 1] : -INFINITY;
+        const double dt = ts1 - ts0;
+        const double control = (double)dir * dt;
+        double y0[ORC_MAXD], z[MAXS][ORC_MAXD], k[MAXS][ORC_MAXD], f[ORC_MAXD];
+        for (int j = 0; j < d; ++j) y0[j] = gy[(long)s * d + j];
+        for (int i = 0; i < S; ++i) {
+            for (int j = 0; j < d; ++j) {
+                double acc = 0.0; int first = 1;
+                for (int m = 0; m < i; ++m) {
+                    if (tb->a[i][m] == 0.0) continue;
+                    double term = tb->a[i][m] * k[m][j];
+                    if (first) { acc = term; first = 0; } else acc = acc + term;
+                }
+                z[i][j] = y0[j] + acc;
+            }
+            double tsi = (tb->c[i] == 1.0) ? ts1 : ts0 + tb->c[i] * dt;
+            vf_eval(&cx, tsi * (double)dir, z[i], f);
+            for (int j = 0; j < d; ++j) k[i][j] = control * f[j];
+        }
+        double gy1[ORC_MAXD], gy0[ORC_MAXD], gk[MAXS][ORC_MAXD];
+        for (int j = 0; j < d; ++j) {
+            gy1[j] = ab[j];
+            gy0[j] = 0.0;
+            for (int m = 0; m < S; ++m) gk[m][j] = 0.0;
+        }
+        if (p->save_steps) {
+            for (int j = 0; j < d; ++j) gy1[j] = gy1[j] + g[(long)r * d + j];
+            --r;
+        }
+        while (ti >= 0 && tsb[ti] * dir > prev_end) {
+            interp_vjp(p->solver, tb, d, ts0, ts1, tsb[ti] * dir, g + (long)r * d, gy0, gy1, gk);
+            --ti;
+            --r;
+        }
+        for (int j = 0; j < d; ++j) gy0[j] = gy0[j] + gy1[j];
+        for (int m = 0; m < S; ++m) {
+            if (tb->b[m] == 0.0) continue;
+            for (int j = 0; j < d; ++j) gk[m][j] = gk[m][j] + tb->b[m] * gy1[j];
+        }
+        for (int i = S - 1; i >= 0; --i) {
+            double w[ORC_MAXD], gz[ORC_MAXD];
+            for (int j = 0; j < d; ++j) w[j] = control * gk[i][j];
+            vf_vjp(p->vf, d, prm, z[i], w, gz, gqi);
+            for (long m = 0; m < P; ++m) gq[m] = gq[m] + gqi[m];
+            for (int j = 0; j < d; ++j) gy0[j] = gy0[j] + gz[j];
+            for (int l = 0; l < i; ++l) {
+                if (tb->a[i][l] == 0.0) continue;
+                for (int j = 0; j < d; ++j) gk[l][j] = gk[l][j] + tb->a[i][l] * gz[j];
+            }
+        }
+        for (int j = 0; j < d; ++j) ab[j] = gy0[j];
+    }
+    if (T0 == T1)
+        for (int rr = p->save_t0; rr < p->save_t0 + p->n_ts; ++rr)
+            for (int j = 0; j < d; ++j) ab[j] = ab[j] + g[(long)rr * d + j];
+    if (p->save_t0)
+        for (int j = 0; j < d; ++j) ab[j] = ab[j] + g[j];
+    for (int j = 0; j < d; ++j) y0b[j] = ab[j];
+    for (long m = 0; m < P; ++m) qb[m] = gq[m];
+}
+
+int orc_adjoint(const orc_problem *p, long B, const double *params, long param_stride,
+                const double *t1, long t1_stride, const double *ts, long ts_stride,
+                const double *grid, const double *grid_y, const int32_t *stats,
+                const double *ys_bar, double *y0_bar, double *params_bar, int nthreads) {
+    init_tableaux();
+    if (p->vf != ORC_VF_LV && p->vf != ORC_VF_VDP && p->vf != ORC_VF_LINEAR) return -1;
+    if (p->vf == ORC_VF_LINEAR && (p->d < 1 || p->d > 4)) return -1;
+    if (p->n_events > 0) return -1;
+    const tableau_t *tb = &TAB[p->solver];
+    const long P = orc_param_count(p);
+    const int n_out = orc_n_out(p);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long b = 0; b < B; ++b) {
+        adjoint_one(p, tb, params + b * param_stride, t1[b * t1_stride],
+                    ts ? ts + b * ts_stride : NULL, grid + b * (long)p->max_steps * 2,
+                    grid_y + b * (long)p->max_steps * p->d, stats[b * 3 + 1],
+                    ys_bar + b * (long)n_out * p->d, y0_bar + b * p->d, params_bar + b * P);
     }
     return 0;
 }

@@ -39,6 +39,8 @@ typedef struct {
     /* > 0: the vector field is the continuous-adjoint augmentation (AdjointTerm,
      * _term.py:767-917) of vf with state dimension adj_base_d; d = 2 adj_base_d + P. */
     int32_t adj_base_d, pad2_;
+    /* With a grid: [B, max_steps, d] start state of every accepted step (or NULL). */
+    double *grid_y;
 } orc_problem;
 
 int orc_n_out(const orc_problem *p);
@@ -59,6 +61,14 @@ double orc_pow(double x, double c);
 int orc_backsolve(const orc_problem *p, long B, const double *params, long param_stride,
                   const double *ys, const double *ts_out, const double *ys_bar,
                   double *y0_bar, double *params_bar, int32_t *result, int nthreads);
+/* Discrete adjoint (RecursiveCheckpointAdjoint, _adjoint.py:264-312) of the forward
+ * problem p: grid / grid_y [B, max_steps, 2] / [B, max_steps, d] and stats [B, 3] recorded
+ * by orc_solve, ys_bar [B, n_out, d] the cotangents of its saved rows; writes y0_bar
+ * [B, d] and params_bar [B, P]. */
+int orc_adjoint(const orc_problem *p, long B, const double *params, long param_stride,
+                const double *t1, long t1_stride, const double *ts, long ts_stride,
+                const double *grid, const double *grid_y, const int32_t *stats,
+                const double *ys_bar, double *y0_bar, double *params_bar, int nthreads);
 /* ClipStepSizeController helpers (clip.py:82-118), exposed for their unit tests. */
 int orc_clip_find(const double *ts, int n, int dir, double t, int hint);
 double orc_clip_bump(const double *ts, int n, int dir, double t, int *made_jump);

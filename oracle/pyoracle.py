@@ -362,9 +362,10 @@ def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller
                 direction=direction)
 
 
-def replay(f, solver, grid, y0, ts, direction=1, g_dw=None, save_t1=False):
+def replay(f, solver, grid, y0, ts, direction=1, g_dw=None, save_t1=False, save_steps=False):
     """Re-run a fixed accepted-step grid (normalised (tprev, tnext) pairs) and return the
-    list of saved states at ts (then y(t1) if save_t1).  Only arithmetic touches y, so y0
+    list of saved states at ts, in the forward's row order (a step's ts rows, then its end
+    state if save_steps), then y(t1) if save_t1.  Only arithmetic touches y, so y0
     may be a torch tensor with requires_grad (autograd reference) or complex (complex
     step).  The step index passed to g_dw is the accepted-step index, which equals the
     attempted-step index for ConstantStepSize."""
@@ -388,6 +389,8 @@ def replay(f, solver, grid, y0, ts, direction=1, g_dw=None, save_t1=False):
             out.append(interpolate(tab, a, b, y, y1, ks, ts_n[ts_index]))
             ts_index += 1
         y = y1
+        if save_steps:
+            out.append(y)
         if tab.fsal:
             fsal = flast
     if save_t1:

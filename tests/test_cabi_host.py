@@ -127,6 +127,27 @@ def test_backsolve_rejects_bad_arguments_without_launching():
     assert call(prob(batch=0), _lib.BacksolveIO()) == 0    # empty batch: nothing to do
 
 
+def test_adjoint_rejects_bad_arguments_without_launching():
+    lib = _lib.load()
+    z = ctypes.c_void_p(8)    # never dereferenced: every case fails validation first
+    io = _lib.AdjointIO(params=z, t1=z, ts=z, grid=z, grid_y=z, stats=z, ys_bar=z, y0_bar=z,
+                        params_bar=z)
+
+    def prob(**kw):
+        base = dict(solver=_lib.TSIT5, controller=_lib.PID, vf=_lib.VF_LV, precision=_lib.F64,
+                    d=2, batch=4, max_steps=10, save_t1=1)
+        base.update(kw)
+        return _lib.ErkProblem(**base)
+
+    call = lambda p, i=io: lib.dfx_erk_adjoint(ctypes.byref(p), ctypes.byref(i), None)
+    for bad in (dict(vf=_lib.VF_LINEAR, d=10), dict(vf=_lib.VF_MLP, d=64), dict(vf=_lib.VF_VDP, d=3),
+                dict(n_events=1), dict(precision=_lib.BF16_F32), dict(solver=9), dict(batch=-1)):
+        assert call(prob(**bad)) == -1, bad
+    missing = _lib.AdjointIO(params=z, t1=z, stats=z, ys_bar=z, y0_bar=z, params_bar=z)
+    assert call(prob(), missing) == -1                     # no recorded grid
+    assert call(prob(batch=0), _lib.AdjointIO()) == 0      # empty batch: nothing to do
+
+
 def test_backsolve_host_guards():
     """BacksolveAdjoint's guards (_adjoint.py:781-858) raise before any GPU work."""
     y0 = torch.ones(2, dtype=torch.float64, requires_grad=True)
