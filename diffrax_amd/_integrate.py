@@ -16,7 +16,7 @@ import torch
 
 from . import _lib
 from ._adjoint import AbstractAdjoint, BacksolveAdjoint, RecursiveCheckpointAdjoint
-from ._saveat import SaveAt, save_y
+from ._saveat import SaveAt, SubSaveAt, save_y
 from ._solution import RESULTS, Solution
 from ._solver import AbstractERK, AbstractSolver
 from ._step_size_controller import ClipStepSizeController, ConstantStepSize, PIDController
@@ -208,16 +208,13 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
     if sub is None:
         if not saveat.dense:
             raise ValueError("SaveAt must save something")
-        from ._saveat import SubSaveAt
         sub = SubSaveAt.__new__(SubSaveAt)   # dense output only: no saved rows
         sub.t0 = sub.t1 = False
         sub.ts, sub.steps, sub.fn = None, 0, save_y
     if not hasattr(sub, "fn"):
         raise NotImplementedError("only a single SubSaveAt is supported")
-    if sub.fn is not save_y:
-        raise NotImplementedError("SaveAt(fn=...) is not provided by diffrax_amd")
-    if sub.steps not in (0, 1):
-        raise NotImplementedError("SaveAt(steps=n) with n > 1 is not provided by diffrax_amd")
+    if sub.steps < 0:
+        raise ValueError("SaveAt(steps=n) needs n >= 0")
     if max_steps is None:
         if saveat.dense:
             raise ValueError("`max_steps=None` is incompatible with `saveat.dense=True`")
@@ -397,7 +394,75 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
     """Solve an initial value problem on the GPU (signature of _integrate.py:890-911).
 
     Extra keyword ``launch=(block, lanes)`` overrides the persistent kernel's launch shape.
-    Batched use: y0 of shape [B, d] solves B independent instances in one launch."""
+    Batched use: y0 of shape [B, d] solves B independent instances in one launch.
+
+    SaveAt(fn=...) is applied to the saved (t, y) rows after the solve, batched with
+    torch.func.vmap (unfilled rows stay inf, as the reference presets them,
+    _integrate.py:1272-1305); SaveAt(subs=[...]) with several SubSaveAts runs one solve per
+    SubSaveAt and returns ts / ys in the same container (the solves are deterministic, so
+    each equals what one combined solve would save)."""
+    kw = dict(stepsize_controller=stepsize_controller, adjoint=adjoint, event=event,
+              max_steps=max_steps, throw=throw, progress_meter=progress_meter,
+              solver_state=solver_state, controller_state=controller_state,
+              made_jump=made_jump, launch=launch)
+    if isinstance(adjoint, BacksolveAdjoint):
+        _check_backsolve(saveat, event, adjoint, stepsize_controller)
+    subs = saveat.subs
+    if subs is not None and not isinstance(subs, SubSaveAt):
+        leaves, spec = torch.utils._pytree.tree_flatten(subs)
+        if not leaves or not all(isinstance(x, SubSaveAt) for x in leaves):
+            raise ValueError("SaveAt(subs=...) must be a SubSaveAt or a container of them")
+        sols = [diffeqsolve(terms, solver, t0, t1, dt0, y0, args,
+                            saveat=SaveAt(subs=leaf, dense=saveat.dense and i == 0), **kw)
+                for i, leaf in enumerate(leaves)]
+        first = sols[0]
+        first.ts = torch.utils._pytree.tree_unflatten([x.ts for x in sols], spec)
+        first.ys = torch.utils._pytree.tree_unflatten([x.ys for x in sols], spec)
+        return first
+    fn = save_y if subs is None else subs.fn
+    if fn is not save_y:
+        plain = SubSaveAt.__new__(SubSaveAt)
+        plain.__dict__.update(subs.__dict__)
+        plain.fn = save_y
+        inner = SaveAt.__new__(SaveAt)
+        inner.__dict__.update(saveat.__dict__)
+        inner.subs = plain
+        sol = _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, saveat=inner, **kw)
+        sol.ys = _apply_save_fn(fn, sol.ts, sol.ys, args, torch.as_tensor(y0).ndim == 2)
+        return sol
+    return _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, saveat=saveat, **kw)
+
+
+def _apply_save_fn(fn, ts, ys, args, batched):
+    """ys rows replaced by fn(t, y, args) (_integrate.py:244-275, 456-527 save fn(t, y,
+    args) in the caller's time); rows the solve did not fill stay inf."""
+    from torch.func import vmap
+    from torch.utils import _pytree as pytree
+    per_row = vmap(fn, in_dims=(0, 0, None))
+    args = pytree.tree_map(
+        lambda a: a.to(ys.device) if isinstance(a, torch.Tensor) else a, args)
+    if batched:
+        B = ys.shape[0]
+        if ts.ndim == 1:
+            ts = ts.expand(B, ts.shape[0])
+        dims = pytree.tree_map(
+            lambda a: 0 if isinstance(a, torch.Tensor) and a.ndim >= 1 and a.shape[0] == B
+            else None, args)
+        out = vmap(per_row, in_dims=(0, 0, dims))(ts, ys, args)
+    else:
+        out = per_row(ts, ys, args)
+    filled = torch.isfinite(ts)
+
+    def mask(o):
+        m = filled.reshape(filled.shape + (1,) * (o.ndim - filled.ndim))
+        return torch.where(m.to(o.device), o, torch.full_like(o, float("inf")))
+
+    return pytree.tree_map(mask, out)
+
+
+def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_controller,
+                     adjoint, event, max_steps, throw, progress_meter, solver_state,
+                     controller_state, made_jump, launch):
     if solver_state is not None or controller_state is not None or made_jump is not None:
         raise NotImplementedError("resuming from solver/controller state is not provided")
     if progress_meter is not None and type(progress_meter).__name__ != "NoProgressMeter":
@@ -424,8 +489,6 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
         return _mlp.diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat,
                                     stepsize_controller, adjoint, max_steps, throw,
                                     _require_gpu())
-    if isinstance(adjoint, BacksolveAdjoint):
-        _check_backsolve(saveat, event, adjoint, stepsize_controller)
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                         max_steps, launch, event=event)
     plan.event = event

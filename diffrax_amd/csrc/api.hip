@@ -75,8 +75,10 @@ int dfx_erk_n_out(const dfx_erk_problem* p) {
   int n = 0;
   if (p->save_t0) n += 1;
   n += p->n_ts;
-  if (p->save_steps) n += p->max_steps;
-  if (p->save_t1 && !p->save_steps) n += 1;
+  // SaveAt(steps=k): every k-th accepted step; the t1 row is kept unless max_steps is a
+  // multiple of k (_integrate.py:1272-1305)
+  if (p->save_steps > 0) n += p->max_steps / p->save_steps;
+  if (p->save_t1 && (p->save_steps <= 0 || p->max_steps % p->save_steps != 0)) n += 1;
   return n;
 }
 
@@ -130,7 +132,7 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
   if (p->controller != DFX_PID && p->controller != DFX_CONSTANT) return DFX_E_ARG;
   if (p->precision != DFX_F64 && p->precision != DFX_F32S_F64T) return DFX_E_ARG;
   if (!small_vf_supported(p->vf, p->d)) return DFX_E_ARG;
-  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0) return DFX_E_ARG;
+  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0 || p->save_steps < 0) return DFX_E_ARG;
   if (p->t1_stride < 0 || p->ts_stride < 0 || p->param_stride < 0) return DFX_E_ARG;
   if (p->controller == DFX_CONSTANT && isnan(p->dt0)) return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;

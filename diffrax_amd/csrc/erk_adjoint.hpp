@@ -112,8 +112,9 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
   T ab[D];  // cotangent of the state at the end of the step being processed
 #pragma unroll
   for (int j = 0; j < D; ++j) ab[j] = T(0);
-  int r = a.save_t0 + n_saved + (a.save_steps ? n : 0);  // the SaveAt(t1) row
-  if (a.save_t1 && !a.save_steps) {
+  const int ksave = a.save_steps;
+  int r = a.save_t0 + n_saved + (ksave > 0 ? n / ksave : 0);  // the SaveAt(t1) row
+  if (a.save_t1 && !(ksave > 0 && n % ksave == 0)) {
 #pragma unroll
     for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[(int64_t)r * D + j];
   }
@@ -154,7 +155,7 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
 #pragma unroll
       for (int m = 0; m < S; ++m) gk[m][j] = T(0);
     }
-    if (a.save_steps) {  // the row of the step's end state
+    if (ksave > 0 && (s + 1) % ksave == 0) {  // the step's end state (SaveAt(steps=k))
 #pragma unroll
       for (int j = 0; j < D; ++j) gy1[j] = gy1[j] + g[(int64_t)r * D + j];
       --r;

@@ -671,7 +671,9 @@ struct ErkLane {
 #pragma unroll
         for (int j = 0; j < D; ++j) fsal_f[j] = f[j];
       }
-      if (a.save_steps && !(Extras && event_now && ntprev > tfinal)) write_save(a, ntprev, y);
+      // SaveAt(steps=k): every k-th accepted step (_integrate.py:493-527)
+      if (a.save_steps && num_acc % a.save_steps == 0 && !(Extras && event_now && ntprev > tfinal))
+        write_save(a, ntprev, y);
     }
     tprev = ntprev;
     tnext = ntnext;
@@ -694,7 +696,13 @@ struct ErkLane {
 #pragma unroll
       for (int j = 0; j < D; ++j) y[j] = yfinal[j];
     }
-    if (a.save_t1 && !a.save_steps) write_save(a, tprev, y);
+    // _save_t1 (_integrate.py:847-877): the t1 row unless the last step was saved via
+    // steps; with an event root finder, also when it was (its row after the event time
+    // was removed).
+    const bool via_steps = a.save_steps && num_acc % a.save_steps == 0;
+    const bool root = Extras && a.n_events > 0 && a.event_root_find;
+    const bool t1_pred = root ? (a.save_t1 || via_steps) : (a.save_t1 && !via_steps);
+    if (t1_pred && save_index < a.n_out) write_save(a, tprev, y);
     if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
     if (Extras && a.n_events > 0 && a.event_mask) {
       for (int i = 0; i < a.n_events; ++i) a.event_mask[traj * a.n_events + i] = (ev_mask >> i) & 1;

@@ -54,7 +54,8 @@ typedef struct dfx_erk_problem {
   int32_t d;            /* state dimension                                          */
   int32_t hidden;       /* MLP width (vf == DFX_VF_MLP), else 0                       */
   int32_t n_ts;         /* SaveAt(ts=...) length                                      */
-  int32_t save_t0, save_t1, save_steps;
+  int32_t save_t0, save_t1;
+  int32_t save_steps;   /* SaveAt(steps=k): k > 0 saves every k-th accepted step, 0 none */
   int32_t max_steps, force_dtmin;
   int64_t batch;        /* B instances                                                */
   int64_t t1_stride;    /* 0: one t1 for all; 1: t1[B]                                */

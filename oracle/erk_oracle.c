@@ -862,7 +862,9 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
             }
             for (int j = 0; j < d; ++j) y[j] = so.y1[j];
             if (tb->fsal) { fsal_first = 0; for (int j = 0; j < d; ++j) fsal_f[j] = so.flast[j]; }
-            if (p->save_steps && !(result == ORC_EVENT_OCCURRED && ntprev > tfinal)) {
+            /* SaveAt(steps=k): every k-th accepted step (_integrate.py:493-527) */
+            if (p->save_steps > 0 && num_acc % p->save_steps == 0 &&
+                !(result == ORC_EVENT_OCCURRED && ntprev > tfinal)) {
                 tso[save_index] = ntprev;
                 for (int j = 0; j < d; ++j) ys[(long)save_index * d + j] = y[j];
                 save_index++;
@@ -884,7 +886,13 @@ static void solve_one(const orc_problem *p, long bi, const double *y0_in, double
         tprev = tfinal;
         for (int j = 0; j < d; ++j) y[j] = yfinal[j];
     }
-    if (p->save_t1 && !p->save_steps) {
+    /* _save_t1 (_integrate.py:847-877): the t1 row unless the last step was saved via
+     * steps; with an event root finder, also when it was (its row after the event time
+     * was removed). */
+    const int via_steps = p->save_steps > 0 && num_acc % p->save_steps == 0;
+    const int root = ne > 0 && p->event_root_find;
+    const int t1_pred = root ? (p->save_t1 || via_steps) : (p->save_t1 && !via_steps);
+    if (t1_pred && save_index < n_out) {
         tso[save_index] = tprev;
         for (int j = 0; j < d; ++j) ys[(long)save_index * d + j] = y[j];
         save_index++;
@@ -902,8 +910,9 @@ int orc_n_out(const orc_problem *p) {
     int n = 0;
     if (p->save_t0) n += 1;
     n += p->n_ts;
-    if (p->save_steps) n += p->max_steps;
-    if (p->save_t1 && !p->save_steps) n += 1;
+    /* SaveAt(steps=k) (_integrate.py:1272-1305) */
+    if (p->save_steps > 0) n += p->max_steps / p->save_steps;
+    if (p->save_t1 && (p->save_steps <= 0 || p->max_steps % p->save_steps != 0)) n += 1;
     return n;
 }
 
@@ -1109,8 +1118,9 @@ static void adjoint_one(const orc_problem *p, const tableau_t *tb, const double 
     double ab[ORC_MAXD], gq[ORC_MAXD * ORC_MAXD], gqi[ORC_MAXD * ORC_MAXD];
     for (int j = 0; j < d; ++j) ab[j] = 0.0;
     for (long m = 0; m < P; ++m) gq[m] = 0.0;
-    int r = p->save_t0 + n_saved + (p->save_steps ? n : 0);
-    if (p->save_t1 && !p->save_steps)
+    const int ksave = p->save_steps;
+    int r = p->save_t0 + n_saved + (ksave > 0 ? n / ksave : 0);
+    if (p->save_t1 && !(ksave > 0 && n % ksave == 0))
         for (int j = 0; j < d; ++j) ab[j] = ab[j] + g[(long)r * d + j];
     r -= 1;
     int ti = (T0 == T1) ? -1 : n_saved - 1;
@@ -1141,7 +1151,7 @@ static void adjoint_one(const orc_problem *p, const tableau_t *tb, const double 
             gy0[j] = 0.0;
             for (int m = 0; m < S; ++m) gk[m][j] = 0.0;
         }
-        if (p->save_steps) {
+        if (ksave > 0 && (s + 1) % ksave == 0) {
             for (int j = 0; j < d; ++j) gy1[j] = gy1[j] + g[(long)r * d + j];
             --r;
         }

@@ -389,7 +389,7 @@ def replay(f, solver, grid, y0, ts, direction=1, g_dw=None, save_t1=False, save_
             out.append(interpolate(tab, a, b, y, y1, ks, ts_n[ts_index]))
             ts_index += 1
         y = y1
-        if save_steps:
+        if save_steps and (n + 1) % int(save_steps) == 0:
             out.append(y)
         if tab.fsal:
             fsal = flast

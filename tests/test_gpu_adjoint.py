@@ -65,13 +65,14 @@ def _run(field, solver, y0, q, w_fn, *, t0=0.0, t1=3.0, dt0=0.05, saveat, ctrl,
 
 @pytest.mark.parametrize("solver", sorted(SOLVERS))
 @pytest.mark.parametrize("field", sorted(FIELDS))
-@pytest.mark.parametrize("save", ["t0+ts+t1", "steps"])
+@pytest.mark.parametrize("save", ["t0+ts+t1", "steps", "steps3+ts+t1"])
 def test_bitwise_against_oracle(cuda, field, solver, save):
     rng = np.random.default_rng([sorted(FIELDS).index(field), len(solver), len(save)])
     B = 96
     _, d, y0, q = _case(field, B, rng)
     ts = np.array([0.0, 0.5, 1.0, 2.25, 3.0]) if save != "steps" else None
-    flags = dict(save_t0=save != "steps", save_t1=save != "steps", save_steps=save == "steps")
+    flags = dict(save_t0=save == "t0+ts+t1", save_t1=save != "steps",
+                 save_steps={"steps": 1, "steps3+ts+t1": 3}.get(save, 0))
     saveat = dfx.SaveAt(t0=flags["save_t0"], t1=flags["save_t1"], ts=ts,
                         steps=flags["save_steps"])
     rtol = 1e-4 if solver == "heun" else 1e-6

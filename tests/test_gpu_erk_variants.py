@@ -109,6 +109,17 @@ def test_save_steps(cuda):
     _assert_exact(got, ref)
 
 
+@pytest.mark.parametrize("k,max_steps", [(3, 256), (4, 256), (7, 300)])
+def test_save_every_kth_step(cuda, k, max_steps):
+    """SaveAt(steps=k) with t0, ts and t1: every k-th accepted step, the t1 row kept when
+    the last step was not a multiple of k and max_steps is not (_integrate.py:493-527,
+    847-877, 1272-1305)."""
+    got, ref = _run("dopri5", "lv", LV_Y0[:64], LV_Q, 0.0, 4.0, 0.1, np.linspace(0, 4, 5),
+                    save_t0=True, save_t1=True, save_steps=k, max_steps=max_steps)
+    _assert_exact(got, ref)
+    assert got["ys"].shape[1] == 1 + 5 + max_steps // k + (max_steps % k != 0)
+
+
 def test_max_steps_reached_and_per_instance_t1(cuda):
     t1 = np.where(np.arange(256) % 2 == 0, 2.0, 40.0)
     got, ref = _run("tsit5", "vdp", VDP_Y0, VDP_MU, 0.0, t1, 0.1, None, save_t1=True,

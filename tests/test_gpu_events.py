@@ -99,3 +99,19 @@ def test_event_before_first_step_and_boolean_without_root_finder(cuda):
     _assert_equal(sol, ref)
     st = sol.stats["num_steps"].cpu().numpy()
     assert st[0] == 0 and st[1] == 10          # instance 0 stops before stepping
+
+
+@pytest.mark.parametrize("k,root", [(1, True), (1, False), (3, True)])
+def test_event_with_saveat_steps(cuda, k, root):
+    """SaveAt(steps=k) with an event: the step row after the event time is dropped and,
+    with a root finder, the final (t_event, y_event) row is written even when the last
+    step was saved via steps (_integrate.py:797-822, 847-877)."""
+    y0 = (np.arange(8.0) + 2.0)[:, None]
+    sol, ref = _both(y0, [[-1.0]], "linear", math.inf, [(0.0, [1.0], -1.0, False)], [None], root,
+                     dict(steps=k), dict(save_steps=k, rtol=1e-3, atol=1e-6),
+                     dfx.PIDController(rtol=1e-3, atol=1e-6), 0.1)
+    _assert_equal(sol, ref)
+    if root and k == 1:   # the event state closes the rows (k > 1: only if n % k == 0)
+        ys = sol.ys.cpu().numpy()[..., 0]
+        last = np.array([row[np.isfinite(row)][-1] for row in ys])
+        assert np.all(np.abs(last - 1.0) <= 1e-5)

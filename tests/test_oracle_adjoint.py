@@ -57,7 +57,7 @@ CASES = [("lv", 4), ("vdp", 1), ("linear", 9)]
 
 @pytest.mark.parametrize("solver", ["tsit5", "dopri5", "heun"])
 @pytest.mark.parametrize("vf,P", CASES)
-@pytest.mark.parametrize("save", ["t0+ts+t1", "steps", "ts"])
+@pytest.mark.parametrize("save", ["t0+ts+t1", "steps", "ts", "steps3+ts+t1"])
 def test_against_autograd_replay(solver, vf, P, save):
     rng = np.random.default_rng([CASES.index((vf, P)), len(solver), len(save)])
     B = 3
@@ -70,8 +70,8 @@ def test_against_autograd_replay(solver, vf, P, save):
     else:
         q = (rng.normal(size=(B, 3, 3)) * 0.5 - np.eye(3)).reshape(B, 9)
     ts = None if save == "steps" else np.sort(rng.uniform(0.0, 3.0, size=7))
-    flags = dict(save_t0=save == "t0+ts+t1", save_t1=save == "t0+ts+t1",
-                 save_steps=save == "steps")
+    flags = dict(save_t0=save == "t0+ts+t1", save_t1=save in ("t0+ts+t1", "steps3+ts+t1"),
+                 save_steps={"steps": 1, "steps3+ts+t1": 3}.get(save, 0))
     kw = dict(solver=solver, vf=vf, dt0=0.05, rtol=1e-3 if solver == "heun" else 1e-6,
               atol=1e-8, max_steps=2000, **flags)
     fwd = oracle.solve(y0, 3.0, q, ts=ts, record_grid=True, **kw)
@@ -81,10 +81,12 @@ def test_against_autograd_replay(solver, vf, P, save):
     yb, qb = oracle.adjoint(3.0, q, fwd, w, ts=ts, **kw)
     for b in range(B):
         n = int(fwd["stats"][b, 1])
-        nrows = (flags["save_t0"] + (0 if ts is None else ts.size) + (n if flags["save_steps"]
-                 else 0) + flags["save_t1"])
+        k = flags["save_steps"]
+        nrows = (flags["save_t0"] + (0 if ts is None else ts.size) + (n // k if k else 0)
+                 + (flags["save_t1"] and not (k and n % k == 0)))
+        t1_row = bool(flags["save_t1"] and not (k and n % k == 0))
         gy, gq = _replay_grads(vf, solver, fwd, b, y0[b], q[b], [] if ts is None else ts,
-                               w[b, :nrows], 1, **flags)
+                               w[b, :nrows], 1, flags["save_t0"], t1_row, k)
         assert _rel(yb[b], gy) <= 1e-12
         assert _rel(qb[b], gq) <= 1e-12
 

@@ -89,3 +89,20 @@ def test_direction_and_saves_after_the_event_are_dropped():
                         events=[(0.0, [1.0, 0.0], -2.5, False, False)], event_root_find=True, **kw)
     n_dn = int(np.isfinite(down["ts"][0]).sum())
     assert down["ts"][0, n_dn - 1] > te    # the first downcrossing comes later
+
+
+def test_steps_rows_with_root_finder():
+    """SaveAt(steps=1) + root finder: rows are the accepted step ends before the event,
+    then (t_event, y_event) (_integrate.py:797-822 unsave, 847-877 _save_t1 with
+    pred = t1 | saved_via_steps)."""
+    r = _solve([2.0], [-1.0], np.inf, [(0.0, [1.0], -1.0, False, None)], save_t1=False,
+               save_steps=1, controller="pid", rtol=1e-3, atol=1e-6, dt0=0.1)
+    ts = r["ts"][0][np.isfinite(r["ts"][0])]
+    ys = r["ys"][0, :ts.size, 0]
+    n = r["stats"][0, 1]
+    assert ts.size == n                          # n - 1 step rows + the event row
+    assert np.all(np.diff(ts) > 0) and np.isclose(ts[-1], np.log(2.0), rtol=5e-3)
+    assert abs(ys[-1] - 1.0) <= 1e-5 and np.all(ys[:-1] > 1.0)
+    plain = _solve([2.0], [-1.0], np.inf, [(0.0, [1.0], -1.0, False, None)], root=False,
+                   save_t1=False, save_steps=1, controller="pid", rtol=1e-3, atol=1e-6, dt0=0.1)
+    assert np.isfinite(plain["ts"][0]).sum() == n  # no root finder: the n step rows only
