@@ -122,6 +122,15 @@ __host__ __device__ inline double jmin(double a, double b) {
   return (a != a || b != b) ? __builtin_nan("") : (a < b ? a : b);
 }
 __host__ __device__ inline double jclip(double x, double lo, double hi) { return jmin(jmax(x, lo), hi); }
+// The same results through the hardware max / min where the sign of a zero cannot differ
+// (jmax of two values >= 0; jclip of a value >= 0 between bounds > 0): the NaN case is
+// the only one these instructions treat differently, and it is taken first.
+__device__ __forceinline__ double jmax_nonneg(double a, double b) {
+  return (a != a || b != b) ? __builtin_nan("") : __builtin_fmax(a, b);
+}
+__device__ __forceinline__ double jclip_pos(double x, double lo, double hi) {
+  return (x != x) ? __builtin_nan("") : __builtin_fmin(__builtin_fmax(x, lo), hi);
+}
 
 // Next float64 towards -inf (equinox.internal.prevbefore), by integer stepping of the
 // IEEE bit pattern.
@@ -266,7 +275,7 @@ __device__ __forceinline__ bool pid_adapt(const PidParams& p, double scaled, dou
   }
   const double fmin = keep ? 1.0 : p.factormin;
   const double fmax = keep ? p.factormax : p.safety;
-  const double factor = jclip(p.safety * f1 * f2 * f3, fmin, fmax);
+  const double factor = jclip_pos(p.safety * f1 * f2 * f3, fmin, fmax);
   double dt = prev_dt * factor;
   if (inv == 0.0 || __builtin_isinf(inv)) inv = 1.0;
   code = 0;

@@ -329,6 +329,7 @@ struct ErkLane {
   T yfinal[D];
   int step_index, jump_index;  // ClipStepSizeController hints
   PidState pid;
+  PidParams pv;                // a.pid with the hot float64 fields in VGPRs
   int32_t cs_completed, cs_num;
   double cs_t0, cs_t1;
   int32_t num_steps, num_acc, num_rej, result;
@@ -379,6 +380,14 @@ struct ErkLane {
     const T* qp = reinterpret_cast<const T*>(a.params) + id * a.param_stride;
 #pragma unroll
     for (int j = 0; j < P; ++j) q[j] = qp[j];
+    // The controller's float64 parameters, per lane in VGPRs (see in_vgpr): kept in SGPRs
+    // they overflow the scalar file and are reloaded through v_readlane in every step.
+    pv = a.pid;
+    pv.rtol = in_vgpr(a.pid.rtol);
+    pv.atol = in_vgpr(a.pid.atol);
+    pv.safety = in_vgpr(a.pid.safety);
+    pv.factormin = in_vgpr(a.pid.factormin);
+    pv.factormax = in_vgpr(a.pid.factormax);
   }
 
   // Time normalisation and controller initialisation of a solve from t0r to t1r for the
@@ -426,6 +435,16 @@ struct ErkLane {
     tprev = T0;
     num_steps = num_acc = num_rej = 0;
     result = DFX_SUCCESSFUL;
+    if constexpr (Tab::fsal && !Extras) {
+      // The first step's stage 0 (runge_kutta.py:687: z0 = y0 + 0 at t0 + 0*dt), evaluated
+      // here once instead of behind a per-lane branch in every step: later steps reuse the
+      // last stage (FSAL), and a rejected first step retries from the same y.
+      T z[D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) z[j] = y[j] + T(0);
+      eval_f((T0 + 0.0 * (tnext - T0)) * dir, z, fsal_f);
+      first = false;
+    }
     return T0;
   }
 
@@ -503,14 +522,16 @@ struct ErkLane {
     const T control = T((double)dir * dt);
     T k[S][D];
     T z[D], f[D];
-    if constexpr (Tab::fsal) {
-      if (first || (Extras && made_jump)) {
+    if constexpr (Tab::fsal && Extras) {
+      if (first || made_jump) {
         // First step, or the first step after a jump (runge_kutta.py:687): stage 0 is
         // evaluated (z0 = y0 + 0, t = t0 + 0*dt).
 #pragma unroll
         for (int j = 0; j < D; ++j) z[j] = y[j] + T(0);
         eval_f((ts0 + 0.0 * dt) * dir, z, fsal_f);
       }
+    }
+    if constexpr (Tab::fsal) {
 #pragma unroll
       for (int j = 0; j < D; ++j) k[0][j] = control * fsal_f[j];
     }
@@ -557,11 +578,11 @@ struct ErkLane {
 #pragma unroll
       for (int j = 0; j < D; ++j) {
         const double y1c = anynan ? (double)y[j] : (double)y1[j];
-        const double yy = jmax(fabs((double)y[j]), fabs(y1c));
-        e[j] = (double)yerr[j] / (a.pid.atol + yy * a.pid.rtol);
+        const double yy = jmax_nonneg(fabs((double)y[j]), fabs(y1c));
+        e[j] = (double)yerr[j] / (pv.atol + yy * pv.rtol);
       }
       const double scaled = rms_norm<D>(e);
-      keep = pid_adapt<GenericPid>(a.pid, scaled, ts0, ts1, pid, ntprev, ntnext, code);
+      keep = pid_adapt<GenericPid>(pv, scaled, ts0, ts1, pid, ntprev, ntnext, code);
     } else {
       keep = true;
       const int32_t sc = cs_completed + 1;
