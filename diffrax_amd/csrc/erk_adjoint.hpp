@@ -122,14 +122,39 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
   int ti = n_saved - 1;  // the last ts index saved
   if (T0 == T1) ti = -1;
 
+  // The reads of every step are issued one step (and one save row) ahead: a lane walks its
+  // own records serially, so a load issued where it is used would expose its full latency
+  // once per step.
+  double nts0 = 0.0, nts1 = 0.0;
+  T ny0[D], grow[D];
+#pragma unroll
+  for (int j = 0; j < D; ++j) { ny0[j] = T(0); grow[j] = T(0); }
+  if (n > 0) {
+    nts0 = grid[(int64_t)(n - 1) * 2];
+    nts1 = grid[(int64_t)(n - 1) * 2 + 1];
+#pragma unroll
+    for (int j = 0; j < D; ++j) ny0[j] = gyp[(int64_t)(n - 1) * D + j];
+  }
+  if (r >= 0) {
+#pragma unroll
+    for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
+  }
+  double tsv = (ti >= 0) ? tsr[ti] * dir : -__builtin_inf();
+
   for (int s = n - 1; s >= 0; --s) {
-    const double ts0 = grid[(int64_t)s * 2], ts1 = grid[(int64_t)s * 2 + 1];
-    const double prev_end = s > 0 ? grid[(int64_t)(s - 1) * 2 + 1] : -__builtin_inf();
-    const double dt = ts1 - ts0;
-    const T control = T((double)dir * dt);
+    const double ts0 = nts0, ts1 = nts1;
     T y0[D], zs[S][D], k[S][D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) y0[j] = gyp[(int64_t)s * D + j];
+    for (int j = 0; j < D; ++j) y0[j] = ny0[j];
+    if (s > 0) {
+      nts0 = grid[(int64_t)(s - 1) * 2];
+      nts1 = grid[(int64_t)(s - 1) * 2 + 1];
+#pragma unroll
+      for (int j = 0; j < D; ++j) ny0[j] = gyp[(int64_t)(s - 1) * D + j];
+    }
+    const double prev_end = s > 0 ? nts1 : -__builtin_inf();
+    const double dt = ts1 - ts0;
+    const T control = T((double)dir * dt);
     // the step's stages, as the forward computes them
     static_for<0, S>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
@@ -157,17 +182,23 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
     }
     if (ksave > 0 && (s + 1) % ksave == 0) {  // the step's end state (SaveAt(steps=k))
 #pragma unroll
-      for (int j = 0; j < D; ++j) gy1[j] = gy1[j] + g[(int64_t)r * D + j];
+      for (int j = 0; j < D; ++j) gy1[j] = gy1[j] + grow[j];
       --r;
+      if (r >= 0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
+      }
     }
     // SaveAt(ts) rows inside this step: t in (end of the previous step, ts1]
-    while (ti >= 0 && tsr[ti] * dir > prev_end) {
-      T gs[D];
-#pragma unroll
-      for (int j = 0; j < D; ++j) gs[j] = g[(int64_t)r * D + j];
-      erk_interp_vjp<Tab, T, D>(ts0, ts1, tsr[ti] * dir, gs, gy0, gy1, gk);
+    while (ti >= 0 && tsv > prev_end) {
+      erk_interp_vjp<Tab, T, D>(ts0, ts1, tsv, grow, gy0, gy1, gk);
       --ti;
       --r;
+      tsv = (ti >= 0) ? tsr[ti] * dir : -__builtin_inf();
+      if (r >= 0) {
+#pragma unroll
+        for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
+      }
     }
     // y1 = y0 + sum_m b_m k_m
 #pragma unroll
