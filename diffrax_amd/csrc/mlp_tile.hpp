@@ -59,7 +59,10 @@ struct PolicyF32 {
   static constexpr int KSTEP = 2;
   static constexpr int KA = 1;
   static constexpr int PAD = 1;     // odd row stride: conflict-free column reads
-  static constexpr bool fast_math = false;
+  // Softplus / tanh through the hardware exp2, log2 and reciprocal (about 1 ulp each): the
+  // vector field stays within 8.4e-7 of a float64 evaluation (the libm forms gave 8e-7)
+  // and the forward runs 2x faster, its VALU work having been the larger part of a step.
+  static constexpr bool fast_math = true;
   static constexpr bool split_input = false;
   __device__ __forceinline__ static frag_t load_a(const act_t* p) { return *p; }
   __device__ __forceinline__ static frag_t load_b(const act_t* packed, int frag) {
