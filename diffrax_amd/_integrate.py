@@ -94,7 +94,7 @@ class ErkPlan:
     def _bind(self):
         for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid",
                      "dense_ts", "dense_ys", "dense_k", "step_ts", "jump_ts",
-                     "event_coeffs", "event_flags", "event_mask", "grid_y"):
+                     "event_coeffs", "event_flags", "event_mask", "grid_y", "rtol", "atol"):
             v = self.t.get(name)
             setattr(self.io, name, v.data_ptr() if v is not None else None)
 
@@ -140,8 +140,10 @@ def _controller_fields(stepsize_controller, dt0f):
     """The dfx_erk_problem fields of a PIDController or ConstantStepSize."""
     if isinstance(stepsize_controller, PIDController):
         c = stepsize_controller
+        per_instance = np.ndim(c.rtol) > 0 or np.ndim(c.atol) > 0
         return dict(controller=_lib.PID, pcoeff=float(c.pcoeff), icoeff=float(c.icoeff),
-                    dcoeff=float(c.dcoeff), rtol=c.rtol, atol=c.atol,
+                    dcoeff=float(c.dcoeff), rtol=0.0 if per_instance else c.rtol,
+                    atol=0.0 if per_instance else c.atol,
                     dtmin=_nan_if_none(c.dtmin), dtmax=_nan_if_none(c.dtmax),
                     factormin=c.factormin, factormax=c.factormax, safety=c.safety,
                     error_order=_nan_if_none(c.error_order), force_dtmin=int(c.force_dtmin))
@@ -271,6 +273,15 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         if arr is not None and arr.size > 0:
             clip_t[name] = torch.tensor(arr, dtype=torch.float64, device=device)
             setattr(prob, "n_" + name, int(arr.size))
+    tol_t = {}
+    if isinstance(stepsize_controller, PIDController) and (
+            np.ndim(stepsize_controller.rtol) > 0 or np.ndim(stepsize_controller.atol) > 0):
+        for name in ("rtol", "atol"):
+            v = np.asarray(getattr(stepsize_controller, name), dtype=np.float64)
+            if v.ndim > 0 and v.shape[0] != B:
+                raise ValueError(f"per-instance {name} must have shape [{B}]")
+            tol_t[name] = torch.tensor(np.broadcast_to(v, (B,)).copy(), dtype=torch.float64,
+                                       device=device)
     lib = _lib.load()
     n_out = lib.dfx_erk_n_out(ctypes.byref(prob))
     tensors = dict(
@@ -281,7 +292,7 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
         result=torch.empty((B,), dtype=torch.int32, device=device),
         grid=(torch.empty((B, int(max_steps), 2), dtype=torch.float64, device=device)
               if record_grid else None),
-        step_ts=clip_t.get("step_ts"), jump_ts=clip_t.get("jump_ts"), **ev,
+        step_ts=clip_t.get("step_ts"), jump_ts=clip_t.get("jump_ts"), **ev, **tol_t,
     )
     if saveat.dense:
         S = solver.tableau.num_stages
@@ -544,6 +555,8 @@ def _check_backsolve(saveat, event, adjoint, stepsize_controller):
     ctrl = adjoint.kwargs.get("stepsize_controller", stepsize_controller)
     if isinstance(ctrl, ClipStepSizeController):
         raise NotImplementedError("BacksolveAdjoint with ClipStepSizeController is not provided")
+    if isinstance(ctrl, PIDController) and (np.ndim(ctrl.rtol) > 0 or np.ndim(ctrl.atol) > 0):
+        raise NotImplementedError("BacksolveAdjoint with per-instance tolerances is not provided")
     if isinstance(ctrl, PIDController) and (float(ctrl.pcoeff) != 0.0 or
                                             float(ctrl.dcoeff) != 0.0 or ctrl.dtmin is not None):
         raise NotImplementedError(

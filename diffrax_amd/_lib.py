@@ -50,6 +50,7 @@ class ErkIO(ctypes.Structure):
         ("step_ts", ctypes.c_void_p), ("jump_ts", ctypes.c_void_p),
         ("event_coeffs", ctypes.c_void_p), ("event_flags", ctypes.c_void_p),
         ("event_mask", ctypes.c_void_p), ("grid_y", ctypes.c_void_p),
+        ("rtol", ctypes.c_void_p), ("atol", ctypes.c_void_p),
     ]
 
 

@@ -11,6 +11,12 @@ import torch
 from . import _lib
 
 
+
+def _scalar_tol(v):
+    if np.ndim(v) > 0:
+        raise NotImplementedError("the neural-ODE kernel takes scalar rtol / atol")
+    return float(v)
+
 class MlpProblem(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("d", ctypes.c_int32), ("hidden", ctypes.c_int32), ("n_ts", ctypes.c_int32),
@@ -93,7 +99,8 @@ class MlpPlan:
             solver=_lib.TSIT5, precision=PRECISION[vfo.matmul], d=64, hidden=256, n_ts=n_ts,
             save_t0=int(sub.t0), save_t1=int(sub.t1), max_steps=int(max_steps),
             ckpt_every=int(ckpt_every), force_dtmin=int(ctrl.force_dtmin), batch=B,
-            t0=float(t0), t1=float(t1), dt0=float(dt0), rtol=ctrl.rtol, atol=ctrl.atol,
+            t0=float(t0), t1=float(t1), dt0=float(dt0), rtol=_scalar_tol(ctrl.rtol),
+            atol=_scalar_tol(ctrl.atol),
             pcoeff=float(ctrl.pcoeff), icoeff=float(ctrl.icoeff), dcoeff=float(ctrl.dcoeff),
             dtmin=nan if ctrl.dtmin is None else float(ctrl.dtmin),
             dtmax=nan if ctrl.dtmax is None else float(ctrl.dtmax),

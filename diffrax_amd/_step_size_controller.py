@@ -7,6 +7,21 @@ from __future__ import annotations
 import math
 from typing import Callable, Optional
 
+import numpy as np
+
+
+def _tolerance(v, name):
+    """A scalar tolerance, or a float64 [B] array of per-instance tolerances (what a
+    jax.vmap over the controller's rtol / atol gives the reference)."""
+    if hasattr(v, "detach"):
+        v = v.detach().cpu().numpy()
+    if np.ndim(v) == 0:
+        return float(v)
+    a = np.asarray(v, dtype=np.float64)
+    if a.ndim != 1:
+        raise ValueError(f"{name} must be a scalar or have shape [B]")
+    return a
+
 
 def rms_norm(x):
     """optimistix.rms_norm: sqrt(mean(x^2)) (the only norm the kernels implement)."""
@@ -61,8 +76,8 @@ class PIDController(AbstractAdaptiveStepSizeController, metaclass=_MetaPID):
                 "about.)")
         if norm is not rms_norm:
             raise NotImplementedError("diffrax_amd kernels implement rms_norm only")
-        self.rtol = float(rtol)
-        self.atol = float(atol)
+        self.rtol = _tolerance(rtol, "rtol")
+        self.atol = _tolerance(atol, "atol")
         self.norm = norm
         self.pcoeff = pcoeff
         self.icoeff = icoeff

@@ -156,6 +156,8 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
   a.grid = io->grid;
   if (io->grid_y && !io->grid) return DFX_E_ARG;
   a.grid_y = io->grid_y;
+  a.rtol_b = io->rtol;
+  a.atol_b = io->atol;
   const int dense = (io->dense_ts != nullptr) + (io->dense_ys != nullptr) + (io->dense_k != nullptr);
   if (dense != 0 && dense != 3) return DFX_E_ARG;
   a.dense_ts = io->dense_ts;

@@ -28,6 +28,8 @@ struct ErkKArgs {
   int32_t* result;
   double* grid;       // [B, max_steps, 2] normalised (t_start, t_end) of accepted steps
   void* grid_y;       // [B, max_steps, D] start state of each accepted step (with grid)
+  const double* rtol_b;  // per-instance PIDController tolerances [B], or null for pid.rtol
+  const double* atol_b;
   double* dense_ts;   // SaveAt(dense=True): [B, max_steps + 1] accepted-step ends (t0 first)
   void* dense_ys;     // [B, max_steps + 1, D] states at those times
   void* dense_k;      // [B, max_steps, S, D] stage increments of each accepted step
@@ -347,7 +349,7 @@ struct ErkLane {
 
   // _select_initial_step (pid.py:33-81) in normalised time.
   __device__ double initial_step(const ErkKArgs& a, double T0) const {
-    const double rtol = a.pid.rtol, atol = a.pid.atol;
+    const double rtol = pv.rtol, atol = pv.atol;
     double scale[D], e[D];
     T f0[D], f1[D], y1[D];
 #pragma unroll
@@ -383,8 +385,8 @@ struct ErkLane {
     // The controller's float64 parameters, per lane in VGPRs (see in_vgpr): kept in SGPRs
     // they overflow the scalar file and are reloaded through v_readlane in every step.
     pv = a.pid;
-    pv.rtol = in_vgpr(a.pid.rtol);
-    pv.atol = in_vgpr(a.pid.atol);
+    pv.rtol = a.rtol_b ? a.rtol_b[id] : in_vgpr(a.pid.rtol);
+    pv.atol = a.atol_b ? a.atol_b[id] : in_vgpr(a.pid.atol);
     pv.safety = in_vgpr(a.pid.safety);
     pv.factormin = in_vgpr(a.pid.factormin);
     pv.factormax = in_vgpr(a.pid.factormax);

@@ -100,6 +100,9 @@ typedef struct dfx_erk_io {
   int32_t* event_mask;        /* out [B, n_events]: the condition that ended the solve    */
   void* grid_y;         /* optional out [B, max_steps, d] state dtype: each accepted step's
                            start state (needs grid; input of dfx_erk_adjoint)          */
+  const double* rtol;   /* optional [B]: per-instance PIDController rtol (a vmapped
+                           tolerance); NULL uses dfx_erk_problem.rtol                  */
+  const double* atol;   /* optional [B]: per-instance atol; NULL uses the problem's     */
 } dfx_erk_io;
 
 /* Launch shape override (NULL = library default / tuned). */

@@ -53,6 +53,7 @@ class Problem(ctypes.Structure):
         ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
         ("adj_base_d", ctypes.c_int32), ("pad2_", ctypes.c_int32),
         ("grid_y", ctypes.c_void_p),
+        ("rtol_b", ctypes.c_void_p), ("atol_b", ctypes.c_void_p),
     ]
 
 
@@ -117,7 +118,8 @@ def make_problem(solver="tsit5", vf="lv", d=2, *, t0=0.0, dt0=None, rtol=1e-6, a
                  dtmax=None, force_dtmin=True, factormin=0.2, factormax=10.0, safety=0.9,
                  error_order=None, step_ts=None, jump_ts=None, events=None,
                  event_root_find=False):
-    """The problem struct.  step_ts / jump_ts wrap the PID controller in
+    """The problem struct.  rtol / atol may be [B] arrays (per-instance tolerances, as a
+    jax.vmap over the PIDController's tolerances).  step_ts / jump_ts wrap the PID controller in
     ClipStepSizeController (sorted here; the arrays stay referenced by the struct).
     events: list of (c_t, c_y [d], c_0, boolean, direction) affine event conditions,
     direction None / True (up) / False (down)."""
@@ -130,8 +132,10 @@ def make_problem(solver="tsit5", vf="lv", d=2, *, t0=0.0, dt0=None, rtol=1e-6, a
         solver=SOLVERS[solver], controller=CONTROLLERS[controller], vf=VFS[vf], d=d,
         hidden=hidden, n_ts=n_ts, save_t0=int(save_t0), save_t1=int(save_t1),
         save_steps=int(save_steps), max_steps=max_steps, force_dtmin=int(force_dtmin), pad_=0,
-        t0=float(t0), dt0=nan if dt0 is None else float(dt0), rtol=float(rtol),
-        atol=float(atol), pcoeff=float(pcoeff), icoeff=float(icoeff), dcoeff=float(dcoeff),
+        t0=float(t0), dt0=nan if dt0 is None else float(dt0),
+        rtol=float(rtol) if np.ndim(rtol) == 0 else 0.0,
+        atol=float(atol) if np.ndim(atol) == 0 else 0.0,
+        pcoeff=float(pcoeff), icoeff=float(icoeff), dcoeff=float(dcoeff),
         dtmin=nan if dtmin is None else float(dtmin), dtmax=nan if dtmax is None else float(dtmax),
         factormin=float(factormin), factormax=float(factormax), safety=float(safety),
         error_order=nan if error_order is None else float(error_order),
@@ -139,6 +143,11 @@ def make_problem(solver="tsit5", vf="lv", d=2, *, t0=0.0, dt0=None, rtol=1e-6, a
         jump_ts=None if clip["jump_ts"] is None else clip["jump_ts"].ctypes.data,
         n_step_ts=0 if clip["step_ts"] is None else clip["step_ts"].size,
         n_jump_ts=0 if clip["jump_ts"] is None else clip["jump_ts"].size)
+    for name, v in (("rtol_b", rtol), ("atol_b", atol)):   # per-instance tolerances [B]
+        if np.ndim(v) > 0:
+            arr = np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(-1))
+            setattr(prob, name, arr.ctypes.data)
+            clip[name] = arr
     if events:
         co = np.ascontiguousarray([[float(e[0])] + [float(v) for v in e[1]] + [float(e[2])]
                                    for e in events], dtype=np.float64)

@@ -639,11 +639,16 @@ static double select_initial_step(const vfctx_t *cx, int dir, double t0, const d
     return jmin(100.0 * h0, h1);
 }
 
-static void solve_one(const orc_problem *p, long bi, const double *y0_in, double t1_in,
+static void solve_one(const orc_problem *p_in, long bi, const double *y0_in, double t1_in,
                       const double *ts_in, const double *prm, const double *dW,
                       long dW_step_stride, double *ys, double *tso, int32_t *stats,
                       int32_t *result_out, double *grid, double *scratch) {
     init_tableaux();
+    /* this instance's tolerances (jax.vmap over PIDController(rtol, atol)) */
+    orc_problem pl = *p_in;
+    if (p_in->rtol_b) pl.rtol = p_in->rtol_b[bi];
+    if (p_in->atol_b) pl.atol = p_in->atol_b[bi];
+    const orc_problem *p = &pl;
     const tableau_t *tb = &TAB[p->solver];
     int d = p->d;
     int n_ts = p->n_ts;

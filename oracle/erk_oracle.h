@@ -41,6 +41,9 @@ typedef struct {
     int32_t adj_base_d, pad2_;
     /* With a grid: [B, max_steps, d] start state of every accepted step (or NULL). */
     double *grid_y;
+    /* Per-instance PIDController tolerances [B] (a vmapped rtol / atol), or NULL for the
+     * scalars above. */
+    const double *rtol_b, *atol_b;
 } orc_problem;
 
 int orc_n_out(const orc_problem *p);

@@ -160,3 +160,27 @@ def test_mixed_precision_form_error_bound(cuda):
     err = oracle.max_rel_err(got["ys"], ref["ys"], 1e-8)
     print("mixed-precision LV max rel err", err)
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("dt0", [0.1, None])
+def test_per_instance_tolerances(cuda, dt0):
+    """rtol / atol per instance (the reference under jax.vmap over the controller's
+    tolerances): bit-identical to the oracle, which equals B separate scalar-tolerance
+    solves (tests/test_oracle_saveat.py pins that)."""
+    rng = np.random.default_rng(21)
+    B = 256
+    rt = 10 ** rng.uniform(-9, -4, size=B)
+    at = 10 ** rng.uniform(-10, -6, size=B)
+    ts = np.linspace(0.0, 10.0, 11)
+    sol = dfx.diffeqsolve(dfx.ODETerm(dfx.VanDerPol()), dfx.Tsit5(), 0.0, 10.0, dt0,
+                          torch.tensor(VDP_Y0), torch.tensor(VDP_MU[:, 0]),
+                          saveat=dfx.SaveAt(ts=ts),
+                          stepsize_controller=dfx.PIDController(rtol=rt, atol=at),
+                          max_steps=100000)
+    ref = oracle.solve(VDP_Y0, 10.0, VDP_MU, ts=ts, solver="tsit5", vf="vdp", dt0=dt0,
+                       rtol=rt, atol=at, max_steps=100000)
+    assert np.array_equal(sol.ys.cpu().numpy(), ref["ys"])
+    st = np.stack([sol.stats[k].cpu().numpy() for k in
+                   ("num_steps", "num_accepted_steps", "num_rejected_steps")], -1)
+    assert np.array_equal(st, ref["stats"])
+    assert len(np.unique(st[:, 0])) > 20          # the tolerances really differ per instance

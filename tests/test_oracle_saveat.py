@@ -32,3 +32,21 @@ def test_every_kth_step(k):
             assert got["ts"][b, m] == 10.0
             rows += 1
         assert np.isinf(got["ys"][b, rows:]).all() and np.isinf(got["ts"][b, rows:]).all()
+
+
+def test_per_instance_tolerances_equal_separate_solves():
+    """rtol / atol arrays [B] (a vmapped PIDController): each instance equals its own
+    scalar-tolerance solve, with dt0 given or selected automatically."""
+    rng = np.random.default_rng(3)
+    B = 12
+    y0 = rng.uniform(0.5, 2.0, size=(B, 2))
+    q = np.array([1.5, 1.0, 3.0, 1.0])
+    rt, at = 10 ** rng.uniform(-9, -3, B), 10 ** rng.uniform(-10, -6, B)
+    ts = np.linspace(0.0, 10.0, 21)
+    for dt0 in (0.1, None):
+        kw = dict(solver="tsit5", vf="lv", dt0=dt0, max_steps=100000)
+        both = oracle.solve(y0, 10.0, q, ts=ts, rtol=rt, atol=at, **kw)
+        for b in range(B):
+            one = oracle.solve(y0[b:b + 1], 10.0, q, ts=ts, rtol=rt[b], atol=at[b], **kw)
+            assert np.array_equal(one["ys"][0], both["ys"][b])
+            assert np.array_equal(one["stats"][0], both["stats"][b])
