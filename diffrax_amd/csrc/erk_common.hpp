@@ -171,15 +171,15 @@ __host__ __device__ inline unsigned long long double_to_bits(double x) {
   return b;
 }
 
-__host__ __device__ inline double cr_pow_fifth(double x) {
+__host__ __device__ inline double cr_pow_fifth(double x_in) {
 #pragma clang fp contract(off)
-  if (!(x > 0.0)) return (x == 0.0) ? 0.0 : __builtin_nan("");
-  if (x == __builtin_inf()) return x;
-  int adj = 0;
-  if (x < 2.2250738585072014e-308) {  // subnormal: scale by 2^100
-    x = x * 1.2676506002282294e30;
-    adj = -20;
-  }
+  // Zero, negative, NaN and infinite inputs are resolved by one select at the end and the
+  // arithmetic runs on a stand-in for them, so the device code has no branches.
+  const bool finite_pos = (x_in > 0.0) && (x_in < __builtin_inf());
+  const bool sub = x_in < 2.2250738585072014e-308;  // subnormal: scale by 2^100
+  double x = finite_pos ? x_in : 1.0;
+  x = sub ? x * 1.2676506002282294e30 : x;
+  const int adj = sub ? -20 : 0;
   const unsigned long long bx = double_to_bits(x);
   const int E = (int)((bx >> 52) & 0x7ff) - 1023;
   const int k = (E >= 0) ? E / 5 : -((4 - E) / 5);
@@ -220,7 +220,8 @@ __host__ __device__ inline double cr_pow_fifth(double x) {
   const double corr = r * (eps * lnx);
   const double out = r + (delta + corr);
   const double scale = bits_to_double((unsigned long long)(1023 + k + adj) << 52);
-  return out * scale;
+  const double special = (x_in == 0.0) ? 0.0 : ((x_in > 0.0) ? x_in : __builtin_nan(""));
+  return finite_pos ? out * scale : special;
 }
 
 // Exponents other than 0.2 / 0.5 / 1 (non-default PID coefficients or error orders) use
@@ -255,9 +256,16 @@ struct PidState {
 
 // PIDController.adapt_step_size after the scaled error is known.  Returns keep and
 // writes next (t0, t1), the new state and an error code (2 = dt_min_reached).
-// Generic = false is the integral-only controller (pcoeff = dcoeff = 0, the default) with
-// exponent 0.2, 0.5 or 1: factor2 = factor3 = 1 and no library pow is compiled in.
-template <bool Generic>
+// The controller kinds:
+//   PID_GENERIC  any coefficients (library pow for exponents other than 0.2 / 0.5 / 1);
+//   PID_INTEGRAL the integral-only controller (pcoeff = dcoeff = 0, the default) with
+//                exponent 0.2, 0.5 or 1 chosen at run time: factor2 = factor3 = 1;
+//   PID_FIFTH    the integral-only controller with exponent 0.2 (the default for the
+//                fifth-order embedded pairs): the correctly rounded fifth root inline,
+//                with no run-time dispatch and no branches.
+enum : int { PID_INTEGRAL = 0, PID_GENERIC = 1, PID_FIFTH = 2 };
+
+template <int Ctl>
 __device__ __forceinline__ bool pid_adapt(const PidParams& p, double scaled, double tprev,
                                           double tnext, PidState& st, double& ntprev,
                                           double& ntnext, int& code) {
@@ -266,10 +274,12 @@ __device__ __forceinline__ bool pid_adapt(const PidParams& p, double scaled, dou
   if (p.has_dtmin) keep = keep || st.at_dtmin;
   double inv = 1.0 / scaled;
   double f1, f2 = 1.0, f3 = 1.0;
-  if constexpr (Generic) {
+  if constexpr (Ctl == PID_GENERIC) {
     f1 = (p.coeff1 == 0.0) ? 1.0 : pow_ctl<true>(inv, p.coeff1);
     f2 = (p.coeff2 == 0.0) ? 1.0 : pow_ctl<true>(st.inv, p.coeff2);
     f3 = (p.coeff3 == 0.0) ? 1.0 : pow_ctl<true>(st.pinv, p.coeff3);
+  } else if constexpr (Ctl == PID_FIFTH) {
+    f1 = cr_pow_fifth(inv);
   } else {
     f1 = pow_ctl<false>(inv, p.coeff1);
   }
@@ -279,7 +289,11 @@ __device__ __forceinline__ bool pid_adapt(const PidParams& p, double scaled, dou
   double dt = prev_dt * factor;
   if (inv == 0.0 || __builtin_isinf(inv)) inv = 1.0;
   code = 0;
-  if (p.has_dtmax) dt = jmin(dt, p.dtmax);
+  if constexpr (Ctl == PID_FIFTH) {
+    dt = jmin(dt, p.has_dtmax ? p.dtmax : __builtin_inf());  // jmin(dt, inf) is dt
+  } else {
+    if (p.has_dtmax) dt = jmin(dt, p.dtmax);
+  }
   if (p.has_dtmin) {
     if (!p.force_dtmin && dt < p.dtmin) code = 2;
     if (st.at_dtmin && factor == 1.0) dt = p.dtmin;
@@ -295,10 +309,14 @@ __device__ __forceinline__ bool pid_adapt(const PidParams& p, double scaled, dou
   return keep;
 }
 
-// Whether the fast (non-generic) controller path reproduces these coefficients.
+// Whether the PID_INTEGRAL controller path reproduces these coefficients.
 inline bool pid_is_integral(const PidParams& p) {
   return p.coeff2 == 0.0 && p.coeff3 == 0.0 &&
          (p.coeff1 == 0.2 || p.coeff1 == 0.5 || p.coeff1 == 1.0);
+}
+// Whether the PID_FIFTH controller path reproduces these coefficients.
+inline bool pid_is_fifth(const PidParams& p) {
+  return p.coeff2 == 0.0 && p.coeff3 == 0.0 && p.coeff1 == 0.2;
 }
 
 // optimistix.rms_norm restated: sqrt(sum x^2)/sqrt(n), |x| when n == 1.

@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
     double ntprev, ntnext;
     int code = 0;
     PidState npid = pid;
-    bool keep = pid_adapt<false>(a.pid, scaled, ts0, ts1, npid, ntprev, ntnext, code);
+    bool keep = pid_adapt<PID_INTEGRAL>(a.pid, scaled, ts0, ts1, npid, ntprev, ntnext, code);
     ntprev = jmin(ntprev, T1);
     if (ntnext > floor_t1) ntnext = keep ? T1 : ntprev + 0.5 * (T1 - ntprev);
     if (run) {

@@ -307,9 +307,11 @@ __device__ __forceinline__ unsigned int lane_rank(unsigned long long mask) {
                                    __builtin_amdgcn_mbcnt_lo((unsigned int)mask, 0u));
 }
 
-// Extras = the solve uses SaveAt(dense=True) or ClipStepSizeController; the plain
-// instantiation carries none of that code in its step loop.
-template <class Tab, class VF, class T, bool GenericPid, bool Extras>
+// Ctl = the PID controller kind (PID_FIFTH, PID_INTEGRAL or PID_GENERIC; ignored under
+// ConstantStepSize).  Extras = the solve uses SaveAt(dense=True) or
+// ClipStepSizeController; the plain instantiation carries none of that code in its step
+// loop.
+template <class Tab, class VF, class T, int Ctl, bool Extras>
 struct ErkLane {
   static constexpr int D = VF::D;
   static constexpr int P = VF::P;
@@ -336,11 +338,27 @@ struct ErkLane {
   double cs_t0, cs_t1;
   int32_t num_steps, num_acc, num_rej, result;
   int32_t save_index, ts_index;
-  double next_ts;  // normalised next save time, +inf when none left
+  double next_ts;    // normalised save time ts[ts_index], +inf when none left
+  double after_raw;  // ts[min(ts_index + 1, n_ts - 1)] as stored, loaded one save ahead
+                     // and used only when the next save is reached, so a save never
+                     // waits on memory
   const double* ts_row;
 
+  // Unconditional (no branch for the compiler to sink the previous value's use past): the
+  // index is clamped into the row, and without save times ts_row points at t1, which is
+  // always readable.
+  __device__ __forceinline__ void prefetch_ts(const ErkKArgs& a) {
+    after_raw = ts_row[max(min(ts_index + 1, a.n_ts - 1), 0)];
+  }
+  __device__ __forceinline__ void first_ts(const ErkKArgs& a) {
+    next_ts = (a.ts != nullptr && a.n_ts > 0) ? ts_row[0] * (double)dir : __builtin_inf();
+    after_raw = 0.0;
+    prefetch_ts(a);
+  }
+  // After ++ts_index: the prefetched time becomes the next one (+inf past the last).
   __device__ __forceinline__ void load_next_ts(const ErkKArgs& a) {
-    next_ts = (ts_index < a.n_ts) ? ts_row[ts_index] * (double)dir : __builtin_inf();
+    next_ts = (ts_index < a.n_ts) ? after_raw * (double)dir : __builtin_inf();
+    prefetch_ts(a);
   }
 
   __device__ __forceinline__ void eval_f(double t, const T (&z)[D], T (&f)[D]) const {
@@ -457,8 +475,8 @@ struct ErkLane {
     begin_times(a, t0r, t1r);
     save_index = 0;
     ts_index = 0;
-    ts_row = nullptr;
-    next_ts = __builtin_inf();
+    ts_row = a.t1;  // no save times (next_ts = +inf keeps the save loop from running)
+    next_ts = after_raw = __builtin_inf();
     ev_mask = 0;
     while (running(a)) step(a);
     if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
@@ -474,8 +492,8 @@ struct ErkLane {
     const double T0 = begin_times(a, a.t0, a.t1[id * a.t1_stride]);
     save_index = 0;
     ts_index = 0;
-    ts_row = a.ts ? a.ts + id * a.ts_stride : nullptr;
-    load_next_ts(a);
+    ts_row = a.ts ? a.ts + id * a.ts_stride : a.t1;
+    first_ts(a);
     if (a.save_t0) {
       write_save(a, T0, y);
     }
@@ -584,7 +602,7 @@ struct ErkLane {
         e[j] = (double)yerr[j] / (pv.atol + yy * pv.rtol);
       }
       const double scaled = rms_norm<D>(e);
-      keep = pid_adapt<GenericPid>(pv, scaled, ts0, ts1, pid, ntprev, ntnext, code);
+      keep = pid_adapt<Ctl>(pv, scaled, ts0, ts1, pid, ntprev, ntnext, code);
     } else {
       keep = true;
       const int32_t sc = cs_completed + 1;
@@ -772,7 +790,7 @@ __global__ void __launch_bounds__(256) erk_backsolve_kernel(const ErkKArgs a, co
   constexpr int DB = VF::D, P = VF::P;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= a.B) return;
-  ErkLane<Tab, AVF, T, false, false> L;
+  ErkLane<Tab, AVF, T, PID_INTEGRAL, false> L;
   L.tr.load();
   L.traj = id;
   L.load_params(a, id);
@@ -817,11 +835,11 @@ __global__ void __launch_bounds__(256) erk_backsolve_kernel(const ErkKArgs a, co
 // One trajectory per lane, no work queue: used when the launch has a lane for every
 // instance.  Lanes whose trajectory has finished idle (masked) until the wave's last lane
 // is done.
-template <class Tab, class VF, class T, bool GenericPid, bool Extras>
+template <class Tab, class VF, class T, int Ctl, bool Extras>
 __global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= a.B) return;
-  ErkLane<Tab, VF, T, GenericPid, Extras> L;
+  ErkLane<Tab, VF, T, Ctl, Extras> L;
   L.start(a, id);
   while (L.running(a)) L.step(a);
   L.finish(a);
@@ -829,9 +847,9 @@ __global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
 
 // Persistent variant with refill: a lane whose trajectory finishes claims the next
 // instance id from the work counter (one atomic per wave per claim round).
-template <class Tab, class VF, class T, bool GenericPid, bool Extras>
+template <class Tab, class VF, class T, int Ctl, bool Extras>
 __global__ void __launch_bounds__(256) erk_small_queue_kernel(const ErkKArgs a) {
-  ErkLane<Tab, VF, T, GenericPid, Extras> L;
+  ErkLane<Tab, VF, T, Ctl, Extras> L;
   bool active = false;
   bool drained = false;  // wave-uniform: the work queue is exhausted
   while (true) {

@@ -5,7 +5,7 @@
 
 namespace dfx {
 
-template <class Tab, class VF, class T, bool G, bool X>
+template <class Tab, class VF, class T, int G, bool X>
 static void launch_small_g(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
   const int64_t nblocks = (lanes + block - 1) / block;
   if (lanes >= a.B) {
@@ -18,17 +18,19 @@ static void launch_small_g(const ErkKArgs& a, int block, int64_t lanes, hipStrea
   }
 }
 
-// Variants: generic PID coefficients (G) and dense output / step clipping / events (X).
+// Variants: the controller kind (the default integral controller with exponent 0.2 on
+// PID_FIFTH, every other PID on PID_GENERIC; ConstantStepSize takes PID_FIFTH, whose
+// controller code it never runs) and dense output / step clipping / events (X).
 template <class Tab, class VF, class T>
 static int launch_small(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
-  const bool g = a.controller == DFX_PID && !pid_is_integral(a.pid);
+  const bool g = a.controller == DFX_PID && !pid_is_fifth(a.pid);
   const bool x = a.dense_ts != nullptr || a.n_step_ts > 0 || a.n_jump_ts > 0 || a.n_events > 0;
   if (g) {
-    if (x) launch_small_g<Tab, VF, T, true, true>(a, block, lanes, s);
-    else launch_small_g<Tab, VF, T, true, false>(a, block, lanes, s);
+    if (x) launch_small_g<Tab, VF, T, PID_GENERIC, true>(a, block, lanes, s);
+    else launch_small_g<Tab, VF, T, PID_GENERIC, false>(a, block, lanes, s);
   } else {
-    if (x) launch_small_g<Tab, VF, T, false, true>(a, block, lanes, s);
-    else launch_small_g<Tab, VF, T, false, false>(a, block, lanes, s);
+    if (x) launch_small_g<Tab, VF, T, PID_FIFTH, true>(a, block, lanes, s);
+    else launch_small_g<Tab, VF, T, PID_FIFTH, false>(a, block, lanes, s);
   }
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }

@@ -184,3 +184,23 @@ def test_per_instance_tolerances(cuda, dt0):
                    ("num_steps", "num_accepted_steps", "num_rejected_steps")], -1)
     assert np.array_equal(st, ref["stats"])
     assert len(np.unique(st[:, 0])) > 20          # the tolerances really differ per instance
+
+
+@pytest.mark.parametrize("dtmax", [None, 0.7])
+def test_zero_and_nan_error_norms(cuda, dtmax):
+    """The default controller's fifth root at its special inputs: instances whose field is
+    zero (scaled error 0, so 1/error = inf and the factor clips to factormax) and instances
+    started from NaN (error NaN -> inf, 1/error = 0), next to ordinary ones, with and
+    without dtmax: identical to the oracle, NaN rows included."""
+    rng = np.random.default_rng(33)
+    B = 192
+    A = (rng.normal(size=(B, 2, 2)) * 0.5 - np.eye(2)).reshape(B, 4)
+    A[::3] = 0.0                         # zero field
+    y0 = rng.uniform(-1.0, 1.0, size=(B, 2))
+    y0[1::6, 0] = np.nan                 # NaN state
+    ts = np.linspace(0.0, 6.0, 13)
+    kw = {} if dtmax is None else {"dtmax": dtmax}
+    got, ref = _run("tsit5", "linear", y0, A, 0.0, 6.0, 0.05, ts, max_steps=256, **kw)
+    _assert_exact(got, ref)
+    assert (got["stats"][::3, 2] == 0).all()          # zero field: nothing rejected
+    assert (got["result"][1::6] != 0).all()           # NaN: the solve does not succeed
