@@ -465,6 +465,34 @@ def read_traffic(kernel_tag):
         return json.load(f).get("hbm_bytes_per_launch")
 
 
+def sweep_batch(steps, warmup):
+    """The headline solve at 1, 2 and 4 waves per SIMD (1,024 SIMDs x 64 lanes = 65,536).
+    With one wave per SIMD nothing hides the latency of a step's dependent FP64 chain; the
+    time for 2x and 4x the instances shows how much issue capacity one wave leaves idle."""
+    import torch
+    from diffrax_amd import _autotune
+    base = None
+    for B in (65536, 131072, 262144):
+        mu, y0, t1, ts = make_inputs(B=B)
+        for block in (64, 256):
+            plan = prepare(torch.float64, mu, y0, t1, ts)
+            tune = _autotune.tune
+            _autotune.tune = lambda p, verbose=False, c=(block, B): (setattr(p, "launch_cfg", c) or (c, {}))
+            try:
+                r = time_form(plan, steps, warmup)
+            finally:
+                _autotune.tune = tune
+            st = plan.t["stats"].cpu().numpy()
+            if base is None:
+                base = r["kernel_ms_mean"]
+            print(json.dumps({"batch": B, "waves_per_simd": B // 65536, "block": block,
+                              "kernel_ms_mean": round(r["kernel_ms_mean"], 4),
+                              "solves_per_s": round(B / (r["kernel_ms_mean"] * 1e-3), 1),
+                              "time_vs_one_wave": round(r["kernel_ms_mean"] / base, 3),
+                              "max_steps_one_trajectory": int(st[:, 0].max()),
+                              "mean_steps": round(float(st[:, 0].mean()), 2)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -477,7 +505,13 @@ def main():
                          "none)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter scale)")
+    ap.add_argument("--sweep-batch", action="store_true",
+                    help="only time the FP64 Van der Pol solve at 1, 2 and 4 waves per SIMD "
+                         "(65,536 / 131,072 / 262,144 instances), one JSON line per batch")
     args = ap.parse_args()
+    if args.sweep_batch:
+        sweep_batch(args.steps, args.warmup)
+        return
     if args.calibrate_copy:
         import torch
         n = args.calibrate_copy * (1 << 20) // 4
