@@ -340,14 +340,16 @@ __global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R,
   floatx16_t acc = floatx16_t{};
   float csum = 0.0f;
   const int ca = ti * 32 + (lane & 31), cb = tj * 32 + (lane & 31);
-  for (int64_t r0 = r_begin; r0 < r_begin + ADJ_CHUNK; r0 += KS * U) {
-    typename P::frag_t fa[U], fb[U];
+  using frag_t = typename P::frag_t;
+  auto load_group = [&](int64_t r0, frag_t (&fa)[U], frag_t (&fb)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t rb = (r0 + u * KS) / KA + (lane >> 5);
       fa[u] = P::load_a(A + (rb * WA + ca) * KA);
       fb[u] = P::load_a(Bm + (rb * WB + cb) * KA);
     }
+  };
+  auto mma_group = [&](const frag_t (&fa)[U], const frag_t (&fb)[U]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       acc = P::mfma(fa[u], fb[u], acc);
@@ -360,6 +362,18 @@ __global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R,
         }
       }
     }
+  };
+  // Two register buffers: group g + 1 is in flight while group g feeds the MFMAs (the last
+  // pass re-loads the final group instead of branching).
+  constexpr int G = ADJ_CHUNK / (KS * U);
+  static_assert(G % 2 == 0, "groups are processed in pairs");
+  frag_t a0[U], b0[U], a1[U], b1[U];
+  load_group(r_begin, a0, b0);
+  for (int g = 0; g < G; g += 2) {
+    load_group(r_begin + (int64_t)(g + 1) * KS * U, a1, b1);
+    mma_group(a0, b0);
+    load_group(r_begin + (int64_t)min(g + 2, G - 1) * KS * U, a0, b0);
+    mma_group(a1, b1);
   }
   float* o = partial + ((int64_t)chunk * lay.first[3] + tg) * 1024 + lane * 16;
 #pragma unroll
