@@ -303,8 +303,8 @@ __host__ __device__ inline DwLayout dw_layout() {
 }
 
 // One 32x32 output tile of one chunk of ADJ_CHUNK rows: partial[chunk][tile] = A^T B over
-// the chunk (A = deltas, B = layer inputs), 8 k-steps of fragments loaded ahead of their
-// MFMAs.  The row buffers are allocated in whole chunks and rows past the segment's last are
+// the chunk (A = deltas, B = layer inputs), in groups of 8 k-steps of fragments held in two
+// register buffers (the next group loads while the current one feeds the MFMAs).  The row buffers are allocated in whole chunks and rows past the segment's last are
 // zero, so every chunk runs its full length.  Tiles in the first tile column of a matrix
 // also produce the chunk's column sums of A (the bias gradient partials, partial_db:
 // [chunk][db1 | db2 | db3]).
