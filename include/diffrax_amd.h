@@ -105,9 +105,10 @@ typedef struct dfx_erk_io {
   const double* atol;   /* optional [B]: per-instance atol; NULL uses the problem's     */
 } dfx_erk_io;
 
-/* Launch shape override (NULL = library default / tuned). */
+/* Launch shape (NULL: block 64, one lane per instance).  Fewer lanes than instances
+ * select the refill kernel; the Python package picks the shape by timing candidates. */
 typedef struct dfx_launch {
-  int32_t block;        /* threads per workgroup (multiple of 64)                     */
+  int32_t block;        /* threads per workgroup: a multiple of 64, at most 256       */
   int32_t lanes;        /* total persistent lanes (trajectory slots); 0 = batch      */
 } dfx_launch;
 
