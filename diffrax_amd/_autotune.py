@@ -15,9 +15,17 @@ _CACHE: dict = {}
 
 
 def _key(plan):
+    """Everything that selects a different kernel or changes its memory traffic: the
+    (solver, field, precision, d) kernel, the controller kind (PID coefficients), the
+    dense / clipping / event variant, step recording, and the batch and device."""
     p = plan.problem
+    io = getattr(plan, "io", None)
     name = torch.cuda.get_device_name(torch.cuda.current_device())
-    return f"erk:{p.solver}:{p.vf}:{p.precision}:{p.d}:{p.controller}:{p.batch}:{name}"
+    pid = f"{p.pcoeff}/{p.icoeff}/{p.dcoeff}/{p.error_order}"
+    extras = (f"{p.n_step_ts}/{p.n_jump_ts}/{p.n_events}/"
+              f"{int(bool(io is not None and io.dense_ts))}/{int(bool(io is not None and io.grid))}")
+    return (f"erk:{p.solver}:{p.vf}:{p.precision}:{p.d}:{p.controller}:{pid}:{extras}:"
+            f"{p.batch}:{name}")
 
 
 def _load_file():
