@@ -204,3 +204,26 @@ def test_zero_and_nan_error_norms(cuda, dtmax):
     _assert_exact(got, ref)
     assert (got["stats"][::3, 2] == 0).all()          # zero field: nothing rejected
     assert (got["result"][1::6] != 0).all()           # NaN: the solve does not succeed
+
+
+@pytest.mark.parametrize("ts_kind", ["one", "two_equal", "dense", "per_instance_to_t1"])
+def test_save_time_prefetch_edges(cuda, ts_kind):
+    """The next save time is read one save ahead: a single save time, two equal ones
+    (two rows from one step), 1,001 save times (many saves inside one step) and per-instance
+    rows that end exactly at t1: identical to the oracle."""
+    rng = np.random.default_rng(44)
+    B = 128
+    y0 = VDP_Y0[:B]
+    mu = VDP_MU[:B]
+    if ts_kind == "one":
+        ts = np.array([3.7])
+    elif ts_kind == "two_equal":
+        ts = np.array([2.5, 2.5])
+    elif ts_kind == "dense":
+        ts = np.linspace(0.0, 8.0, 1001)
+    else:
+        ts = np.sort(rng.uniform(0.0, 8.0, size=(B, 9)), axis=1)
+        ts[:, -1] = 8.0
+    got, ref = _run("tsit5", "vdp", y0, mu, 0.0, 8.0, 0.05, ts, save_t1=True)
+    _assert_exact(got, ref)
+    assert np.isfinite(got["ys"]).all()
