@@ -170,7 +170,7 @@ __device__ __forceinline__ void wave_gemm(const typename P::act_t* a_lds,
   for (int t = 0; t < NT; ++t) acc[t] = floatx16_t{};
   if constexpr (P::KA == 1 && PF == 0) {
     static_assert(!LO, "split operands are bfloat16");
-#pragma unroll 4
+#pragma unroll 8
     for (int ks = ks0; ks < ks1; ++ks) {
       const frag_t a = P::load_a(a_lds + row * lda + ks * P::KSTEP + kh);
 #pragma unroll
