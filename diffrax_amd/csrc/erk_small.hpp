@@ -83,6 +83,10 @@ constexpr double tab_coef() {
   else return Dopri5Tab::cmid[M];
 }
 
+// a * b + c with one rounding, in the precision of the state.
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Sum over m < END of coef[m] * k[m] for the coefficients that are nonzero in the tableau,
 // in ascending m, starting from the first nonzero product (the order the reference
 // tensordot is restated with).  Zero entries, and the choice of the first term, are
@@ -94,11 +98,11 @@ __device__ __forceinline__ T comb(const double* vals, const T (&k)[S], T acc) {
   } else {
     constexpr double c = tab_coef<Tab, WHICH, ROW, M>();
     if constexpr (c != 0.0) {
-      const T term = T(vals[M]) * k[M];
       if constexpr (STARTED) {
-        return comb<Tab, WHICH, ROW, M + 1, END, true>(vals, k, acc + term);
+        return comb<Tab, WHICH, ROW, M + 1, END, true>(vals, k,
+                                                       fma_t(T(vals[M]), k[M], acc));
       } else {
-        return comb<Tab, WHICH, ROW, M + 1, END, true>(vals, k, term);
+        return comb<Tab, WHICH, ROW, M + 1, END, true>(vals, k, T(vals[M]) * k[M]);
       }
     } else {
       return comb<Tab, WHICH, ROW, M + 1, END, STARTED>(vals, k, acc);

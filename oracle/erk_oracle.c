@@ -35,8 +35,12 @@
  * delegates to an absent third-party routine the restatement states its choice:
  *   - optimistix.rms_norm is restated as sqrt(sum x_i^2) / sqrt(n) (|x| when n == 1);
  *     its exact rounding is not pinned by any reference test (parity unpinned).
- *   - jnp.tensordot summation order (XLA) is restated as ascending stage order.
- * Build with -ffp-contract=off so no a*b+c is fused.
+ *   - jnp.tensordot (XLA, precision=HIGHEST) is restated as a fused multiply-add chain in
+ *     ascending stage order: the first nonzero product, then acc = fma(c_m, k_m, acc)
+ *     (one rounding per term, as a fused dot product evaluates it); y0 is added after.
+ *     Stage sums, the b and b_err sums and Dopri5's midpoint weights all use it.  The
+ *     exact rounding of XLA's dot is not pinned by any reference test (parity unpinned).
+ * Build with -ffp-contract=off, so no other a*b+c is fused.
  */
 #include <math.h>
 #include <stdint.h>
@@ -459,8 +463,8 @@ static void erk_step(const tableau_t *tb, const vfctx_t *cx, int dir, double ts0
             int first = 1;
             for (int m = 0; m < i; ++m) {
                 if (tb->a[i][m] == 0.0) continue;
-                double term = tb->a[i][m] * o->k[m][j];
-                if (first) { acc = term; first = 0; } else acc = acc + term;
+                if (first) { acc = tb->a[i][m] * o->k[m][j]; first = 0; }
+                else acc = fma(tb->a[i][m], o->k[m][j], acc);
             }
             z[j] = y0[j] + acc;
         }
@@ -484,8 +488,8 @@ static void erk_step(const tableau_t *tb, const vfctx_t *cx, int dir, double ts0
             double acc = 0.0; int first = 1;
             for (int m = 0; m < S; ++m) {
                 if (tb->b[m] == 0.0) continue;
-                double term = tb->b[m] * o->k[m][j];
-                if (first) { acc = term; first = 0; } else acc = acc + term;
+                if (first) { acc = tb->b[m] * o->k[m][j]; first = 0; }
+                else acc = fma(tb->b[m], o->k[m][j], acc);
             }
             o->y1[j] = y0[j] + acc;
         }
@@ -494,8 +498,8 @@ static void erk_step(const tableau_t *tb, const vfctx_t *cx, int dir, double ts0
         double acc = 0.0; int first = 1;
         for (int m = 0; m < S; ++m) {
             if (tb->berr[m] == 0.0) continue;
-            double term = tb->berr[m] * o->k[m][j];
-            if (first) { acc = term; first = 0; } else acc = acc + term;
+            if (first) { acc = tb->berr[m] * o->k[m][j]; first = 0; }
+            else acc = fma(tb->berr[m], o->k[m][j], acc);
         }
         o->yerr[j] = acc;
     }
@@ -544,8 +548,8 @@ static void interp_eval(int solver, const tableau_t *tb, int d, double ts0, doub
             double acc = 0.0; int first = 1;
             for (int m = 0; m < 7; ++m) {
                 if (tb->cmid[m] == 0.0) continue;
-                double term = tb->cmid[m] * o->k[m][j];
-                if (first) { acc = term; first = 0; } else acc = acc + term;
+                if (first) { acc = tb->cmid[m] * o->k[m][j]; first = 0; }
+                else acc = fma(tb->cmid[m], o->k[m][j], acc);
             }
             double ymid = y0[j] + acc;
             double f0 = o->k[0][j], f1 = o->k[6][j];
@@ -1141,8 +1145,8 @@ static void adjoint_one(const orc_problem *p, const tableau_t *tb, const double 
                 double acc = 0.0; int first = 1;
                 for (int m = 0; m < i; ++m) {
                     if (tb->a[i][m] == 0.0) continue;
-                    double term = tb->a[i][m] * k[m][j];
-                    if (first) { acc = term; first = 0; } else acc = acc + term;
+                    if (first) { acc = tb->a[i][m] * k[m][j]; first = 0; }
+                    else acc = fma(tb->a[i][m], k[m][j], acc);
                 }
                 z[i][j] = y0[j] + acc;
             }

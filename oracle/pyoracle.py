@@ -19,6 +19,8 @@ from __future__ import annotations
 
 import math
 
+from fractions import Fraction
+
 import numpy as np
 
 TSIT5_A = [
@@ -104,13 +106,43 @@ TABLEAUX = {
 }
 
 
+def _fma1(a, b, c):
+    """a*b + c rounded once to float64 (C fma, v_fma_f64), by exact rational arithmetic."""
+    if not (math.isfinite(a) and math.isfinite(b) and math.isfinite(c)):
+        return a * b + c          # inf / NaN operands: the single and double rounding agree
+    exact = Fraction(a) * Fraction(b) + Fraction(c)
+    if exact == 0:
+        # IEEE sign of an exact zero sum: -0 only when both addends are -0
+        prod_neg = (math.copysign(1.0, a) * math.copysign(1.0, b)) < 0
+        prod_zero = a == 0 or b == 0
+        return -0.0 if (prod_zero and prod_neg and math.copysign(1.0, c) < 0) else 0.0
+    try:
+        return float(exact)
+    except OverflowError:
+        return math.inf if exact > 0 else -math.inf
+
+
+def _fma(a, k, acc):
+    """acc + a*k with one rounding for float64 values (the restatement of the fused dot
+    product); torch tensors and complex values (the autograd / complex-step replays, which
+    are compared at 1e-12) use a*k + acc."""
+    if isinstance(k, np.ndarray) and k.dtype == np.float64:
+        out = np.empty(np.broadcast(k, acc).shape)
+        kk, aa = np.broadcast_arrays(k, acc)
+        for i in np.ndindex(out.shape):
+            out[i] = _fma1(float(a), float(kk[i]), float(aa[i]))
+        return out
+    if isinstance(k, (float, np.floating)) and isinstance(acc, (float, np.floating)):
+        return _fma1(float(a), float(k), float(acc))
+    return a * k + acc
+
+
 def _lincomb(coeffs, ks):
     acc = None
     for a, k in zip(coeffs, ks):
         if a == 0:
             continue
-        term = a * k
-        acc = term if acc is None else acc + term
+        acc = a * k if acc is None else _fma(a, k, acc)
     return acc
 
 

@@ -274,7 +274,7 @@ def test_pid_second_step_size(dt0):
     g = r["grid"][0]
     if keep:
         assert g[0, 0] == 0.0 and g[0, 1] == dt0
-        assert g[1, 1] - g[1, 0] == dt0 * factor
+        assert g[1, 1] == g[1, 0] + dt0 * factor   # next t1 = t0 + prev_dt * factor (pid.py:531, 557-558)
     else:
         assert r["stats"][0, 2] >= 1
         # after the rejection the retry starts at 0 with dt0 * factor
