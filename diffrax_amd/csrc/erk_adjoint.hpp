@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
       constexpr int m = decltype(mc)::value;
       if constexpr (Tab::b[m] != 0.0) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) gk[m][j] = gk[m][j] + T(Tab::b[m]) * gy1[j];
+        for (int j = 0; j < D; ++j) gk[m][j] = fma_t(T(Tab::b[m]), gy1[j], gk[m][j]);
       }
     });
     // stages, last first
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
         constexpr int l = decltype(lc)::value;
         if constexpr (Tab::a[i][l] != 0.0) {
 #pragma unroll
-          for (int j = 0; j < D; ++j) gk[l][j] = gk[l][j] + T(Tab::a[i][l]) * gz[j];
+          for (int j = 0; j < D; ++j) gk[l][j] = fma_t(T(Tab::a[i][l]), gz[j], gk[l][j]);
         }
       });
     });

@@ -38,7 +38,9 @@
  *   - jnp.tensordot (XLA, precision=HIGHEST) is restated as a fused multiply-add chain in
  *     ascending stage order: the first nonzero product, then acc = fma(c_m, k_m, acc)
  *     (one rounding per term, as a fused dot product evaluates it); y0 is added after.
- *     Stage sums, the b and b_err sums and Dopri5's midpoint weights all use it.  The
+ *     Stage sums, the b and b_err sums and Dopri5's midpoint weights all use it, and the
+ *     discrete adjoint accumulates its transposes (k_bar += b_m y1_bar, += a_il z_bar) the
+ *     same way.  The
  *     exact rounding of XLA's dot is not pinned by any reference test (parity unpinned).
  * Build with -ffp-contract=off, so no other a*b+c is fused.
  */
@@ -1172,7 +1174,7 @@ static void adjoint_one(const orc_problem *p, const tableau_t *tb, const double 
         for (int j = 0; j < d; ++j) gy0[j] = gy0[j] + gy1[j];
         for (int m = 0; m < S; ++m) {
             if (tb->b[m] == 0.0) continue;
-            for (int j = 0; j < d; ++j) gk[m][j] = gk[m][j] + tb->b[m] * gy1[j];
+            for (int j = 0; j < d; ++j) gk[m][j] = fma(tb->b[m], gy1[j], gk[m][j]);
         }
         for (int i = S - 1; i >= 0; --i) {
             double w[ORC_MAXD], gz[ORC_MAXD];
@@ -1182,7 +1184,7 @@ static void adjoint_one(const orc_problem *p, const tableau_t *tb, const double 
             for (int j = 0; j < d; ++j) gy0[j] = gy0[j] + gz[j];
             for (int l = 0; l < i; ++l) {
                 if (tb->a[i][l] == 0.0) continue;
-                for (int j = 0; j < d; ++j) gk[l][j] = gk[l][j] + tb->a[i][l] * gz[j];
+                for (int j = 0; j < d; ++j) gk[l][j] = fma(tb->a[i][l], gz[j], gk[l][j]);
             }
         }
         for (int j = 0; j < d; ++j) ab[j] = gy0[j];
