@@ -46,7 +46,7 @@ __device__ __forceinline__ void erk_interp_vjp(double ts0, double ts1, double t,
     for (int j = 0; j < D; ++j) {
       gy0[j] = gy0[j] + g[j];
 #pragma unroll
-      for (int m = 0; m < 7; ++m) gk[m][j] = gk[m][j] + T(bb[m]) * g[j];
+      for (int m = 0; m < 7; ++m) gk[m][j] = fma_t(T(bb[m]), g[j], gk[m][j]);
     }
   } else if constexpr (Tab::id == SOLVER_DOPRI5) {
     // out = ca t^4 + cb t^3 + cc t^2 + k0 t + y0, ymid = y0 + sum cmid_m k_m

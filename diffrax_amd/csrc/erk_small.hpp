@@ -136,7 +136,7 @@ __device__ __forceinline__ void erk_interp(double ts0, double ts1, const T (&y0)
     for (int j = 0; j < D; ++j) {
       T acc = T(bb[0]) * k[0][j];
 #pragma unroll
-      for (int m = 1; m < 7; ++m) acc = acc + T(bb[m]) * k[m][j];
+      for (int m = 1; m < 7; ++m) acc = fma_t(T(bb[m]), k[m][j], acc);
       out[j] = y0[j] + acc;
     }
   } else if constexpr (Tab::id == SOLVER_DOPRI5) {

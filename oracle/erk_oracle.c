@@ -38,7 +38,8 @@
  *   - jnp.tensordot (XLA, precision=HIGHEST) is restated as a fused multiply-add chain in
  *     ascending stage order: the first nonzero product, then acc = fma(c_m, k_m, acc)
  *     (one rounding per term, as a fused dot product evaluates it); y0 is added after.
- *     Stage sums, the b and b_err sums and Dopri5's midpoint weights all use it, and the
+ *     Stage sums, the b and b_err sums, Dopri5's midpoint weights and the Tsit5
+ *     interpolant's b(theta) . k (tsit5.py:153-156) all use it, and the
  *     discrete adjoint accumulates its transposes (k_bar += b_m y1_bar, += a_il z_bar) the
  *     same way.  The
  *     exact rounding of XLA's dot is not pinned by any reference test (parity unpinned).
@@ -541,7 +542,7 @@ static void interp_eval(int solver, const tableau_t *tb, int d, double ts0, doub
         tsit5_bb(th, bb);
         for (int j = 0; j < d; ++j) {
             double acc = bb[0] * o->k[0][j];
-            for (int m = 1; m < 7; ++m) acc = acc + bb[m] * o->k[m][j];
+            for (int m = 1; m < 7; ++m) acc = fma(bb[m], o->k[m][j], acc);
             out[j] = y0[j] + acc;
         }
     } else if (solver == ORC_DOPRI5) {
@@ -1077,7 +1078,7 @@ static void interp_vjp(int solver, const tableau_t *tb, int d, double ts0, doubl
         tsit5_bb(th, bb);
         for (int j = 0; j < d; ++j) {
             gy0[j] = gy0[j] + g[j];
-            for (int m = 0; m < 7; ++m) gk[m][j] = gk[m][j] + bb[m] * g[j];
+            for (int m = 0; m < 7; ++m) gk[m][j] = fma(bb[m], g[j], gk[m][j]);
         }
     } else if (solver == ORC_DOPRI5) {
         /* out = a th^4 + b th^3 + c th^2 + k0 th + y0 with ymid = y0 + cmid . k */

@@ -196,7 +196,7 @@ def interpolate(tab, ts0, ts1, y0, y1, ks, t):
         ]
         acc = bb[0] * ks[0]
         for m in range(1, 7):
-            acc = acc + bb[m] * ks[m]
+            acc = _fma(bb[m], ks[m], acc)
         return y0 + acc
     if tab.name == "dopri5":
         ymid = y0 + _lincomb(tab.cmid, ks)
