@@ -88,9 +88,10 @@ __device__ __forceinline__ float fma_t(float a, float b, float c) { return __bui
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
 // Sum over m < END of coef[m] * k[m] for the coefficients that are nonzero in the tableau,
-// in ascending m, starting from the first nonzero product (the order the reference
-// tensordot is restated with).  Zero entries, and the choice of the first term, are
-// resolved at compile time.  `vals` holds the coefficient values (in VGPRs).
+// in ascending m: the first nonzero product, then acc = fma(coef[m], k[m], acc) (the fused
+// dot product the reference's tensordot, _solver/base.py:37-41, is restated as).  Zero
+// entries, and the choice of the first term, are resolved at compile time.  `vals` holds
+// the coefficient values (in VGPRs).
 template <class Tab, int WHICH, int ROW, int M, int END, bool STARTED, class T, int S>
 __device__ __forceinline__ T comb(const double* vals, const T (&k)[S], T acc) {
   if constexpr (M >= END) {

@@ -14,6 +14,11 @@ complex) or torch tensors, so the same code serves DETEST problems, complex-step
 derivatives and torch autograd replays of a fixed accepted-step grid (the reference
 stops gradients through the step-size factor, pid.py:498-530, so its gradient is the
 derivative of exactly such a replay).
+
+Every jnp.tensordot of the reference (the stage, b, b_err, Dopri5-midpoint and Tsit5
+interpolant sums, _solver/base.py:37-41) is restated as a fused multiply-add chain in
+index order: the first nonzero product, then acc = fma(c_m, k_m, acc), each fma rounded
+once (exactly, through fractions.Fraction); y0 is added afterwards.
 """
 from __future__ import annotations
 
