@@ -113,8 +113,10 @@ TABLEAUX = {
 
 def _fma1(a, b, c):
     """a*b + c rounded once to float64 (C fma, v_fma_f64), by exact rational arithmetic."""
-    if not (math.isfinite(a) and math.isfinite(b) and math.isfinite(c)):
-        return a * b + c          # inf / NaN operands: the single and double rounding agree
+    if not (math.isfinite(a) and math.isfinite(b)):
+        return a * b + c          # an infinite / NaN factor: IEEE a*b + c is the fma result
+    if not math.isfinite(c):
+        return c                  # the exact product is finite, so inf / NaN c decides
     exact = Fraction(a) * Fraction(b) + Fraction(c)
     if exact == 0:
         # IEEE sign of an exact zero sum: -0 only when both addends are -0

@@ -346,3 +346,32 @@ def test_sde_step_grid_matches_constant_controller():
                      max_steps=2000, record_grid=True)
     grid = r["grid"][0, :1000]
     assert np.array_equal(grid[:, 0], g[:-1]) and np.array_equal(grid[:, 1], g[1:])
+
+
+def test_python_fma_matches_libm():
+    """pyoracle's exact fma (fractions.Fraction, one rounding) equals the C library's fma --
+    the operation of the C oracle's and the kernels' fused dot products -- on random
+    magnitudes, exact cancellations, signed zeros, overflow and non-finite operands."""
+    import ctypes.util
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.fma.argtypes = [ctypes.c_double] * 3
+    libm.fma.restype = ctypes.c_double
+    rng = np.random.default_rng(12)
+    n = 20000
+    a = rng.normal(size=n) * 10.0 ** rng.integers(-300, 300, size=n)
+    b = rng.normal(size=n) * 10.0 ** rng.integers(-20, 20, size=n)
+    c = rng.normal(size=n) * 10.0 ** rng.integers(-300, 300, size=n)
+    with np.errstate(over="ignore"):                  # some products overflow: c = -inf
+        c[::7] = -(a[::7] * b[::7])                   # near-cancellation
+    cases = list(zip(a, b, c)) + [
+        (0.0, 1.0, -0.0), (-0.0, 1.0, -0.0), (-0.0, -1.0, -0.0), (1.0, -0.0, 0.0),
+        (1e308, 10.0, 1.0), (-1e308, 10.0, 1.0), (3.0, 5.0, -15.0), (-3.0, 5.0, 15.0),
+        (math.inf, 1.0, 1.0), (math.nan, 1.0, 0.0), (1.0, 1.0, -math.inf),
+        (5e-324, 0.5, 0.0), (2.0 ** -1074, 2.0 ** -1, 2.0 ** -1074)]
+    for x, y, z in cases:
+        got = pyoracle._fma1(float(x), float(y), float(z))
+        ref = libm.fma(float(x), float(y), float(z))
+        if math.isnan(ref):
+            assert math.isnan(got), (x, y, z)
+        else:
+            assert got == ref and math.copysign(1.0, got) == math.copysign(1.0, ref), (x, y, z, got, ref)
