@@ -251,6 +251,30 @@ __host__ __device__ inline double pow_ctl(double x, double c) {
   return __builtin_nan("");
 }
 
+// rms_norm for D >= 2 with the division by c = sqrt(D) (rounded) written out: the float64
+// division a / c is d = div_scale(c), n = div_scale(a), r = rcp(d) refined by two steps
+// r += r (1 - d r), q = n r, q + r (n - d q) (div_fmas) and div_fixup; for a = sqrt(s) in
+// {0} or [2^-537, 2^512) the scalings leave their operands unchanged and the fixup returns
+// q unchanged, so the same steps with r = rc (computed once from c) give the same bits.
+// sqrt(s) = +inf is selected; NaN propagates.
+__device__ __forceinline__ double refined_rcp(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+  r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
+  return r;
+}
+template <int D>
+__device__ __forceinline__ double rms_norm_rc(const double (&e)[D], double c, double rc) {
+  static_assert(D >= 2, "D = 1 is |e| (no division)");
+  double s = e[0] * e[0];
+#pragma unroll
+  for (int j = 1; j < D; ++j) s = s + e[j] * e[j];
+  const double a = sqrt(s);
+  const double q = a * rc;
+  const double out = __builtin_fma(__builtin_fma(-c, q, a), rc, q);
+  return (a == __builtin_inf()) ? a : out;
+}
+
 // ----------------------------------------------------------------------------------
 // Controller parameters (PIDController fields, pid.py:299-311) and state.
 // ----------------------------------------------------------------------------------

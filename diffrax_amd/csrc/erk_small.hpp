@@ -339,6 +339,7 @@ struct ErkLane {
   int step_index, jump_index;  // ClipStepSizeController hints
   PidState pid;
   PidParams pv;                // a.pid with the hot float64 fields in VGPRs
+  double norm_c, norm_rc;      // sqrt(D) (rounded) and its refined reciprocal (rms_norm_rc)
   int32_t cs_completed, cs_num;
   double cs_t0, cs_t1;
   int32_t num_steps, num_acc, num_rej, result;
@@ -413,6 +414,8 @@ struct ErkLane {
     pv.safety = in_vgpr(a.pid.safety);
     pv.factormin = in_vgpr(a.pid.factormin);
     pv.factormax = in_vgpr(a.pid.factormax);
+    norm_c = in_vgpr(sqrt((double)D));
+    norm_rc = refined_rcp(norm_c);
   }
 
   // Time normalisation and controller initialisation of a solve from t0r to t1r for the
@@ -606,7 +609,9 @@ struct ErkLane {
         const double yy = jmax_nonneg(fabs((double)y[j]), fabs(y1c));
         e[j] = (double)yerr[j] / (pv.atol + yy * pv.rtol);
       }
-      const double scaled = rms_norm<D>(e);
+      double scaled;
+      if constexpr (D >= 2) scaled = rms_norm_rc<D>(e, norm_c, norm_rc);
+      else scaled = rms_norm<D>(e);
       keep = pid_adapt<Ctl>(pv, scaled, ts0, ts1, pid, ntprev, ntnext, code);
     } else {
       keep = true;
