@@ -251,6 +251,7 @@ __host__ __device__ inline double pow_ctl(double x, double c) {
   return __builtin_nan("");
 }
 
+#if defined(__HIPCC__)  // device builtins: HIP compilation only (not a host C++ build)
 // rms_norm for D >= 2 with the division by c = sqrt(D) (rounded) written out: the float64
 // division a / c is d = div_scale(c), n = div_scale(a), r = rcp(d) refined by two steps
 // r += r (1 - d r), q = n r, q + r (n - d q) (div_fmas) and div_fixup; for a = sqrt(s) in
@@ -274,6 +275,7 @@ __device__ __forceinline__ double rms_norm_rc(const double (&e)[D], double c, do
   const double out = __builtin_fma(__builtin_fma(-c, q, a), rc, q);
   return (a == __builtin_inf()) ? a : out;
 }
+#endif
 
 // ----------------------------------------------------------------------------------
 // Controller parameters (PIDController fields, pid.py:299-311) and state.
