@@ -345,6 +345,8 @@ def bench_sde(steps, warmup, with_cpu, P=262144, d=16, N=1000, cpu_paths=16384):
 # stage recompute 2*(42 + 7) + 7*5 = 133; y1 transpose 2*(12 + 1) = 26; per stage
 # w = control*k_bar (2), vjp (14), accumulations (2 + 1) over 7 stages = 133; a-transpose
 # 2*2*21 = 84: 376 per accepted step.  Per save: Tsit5 weights (35) + 2*(1 + 14) = 65.
+# The checkpoint replay repeats the forward's attempted steps (FLOP_PER_STEP each, no
+# saves) before the sweep.
 ADJ_FLOP_PER_STEP = 376
 ADJ_FLOP_PER_SAVE = 65
 
@@ -352,8 +354,9 @@ ADJ_FLOP_PER_SAVE = 65
 def bench_vdp_grad(mu, y0, t1, ts, steps, warmup, with_cpu, forms=("f64", "f32"), check=512):
     """The headline solve plus its gradient: the discrete adjoint (RecursiveCheckpointAdjoint,
     the default adjoint) of loss = mean(ys^2) with respect to y0 and mu, for all 65,536
-    instances.  One step = forward with the accepted steps recorded + the cotangent + the
-    reverse sweep, replayed from one HIP graph."""
+    instances.  One step = forward with sqrt(max_steps)-spaced checkpoints + the cotangent
+    + the adjoint (segment replays from the checkpoints and reverse sweeps), replayed from
+    one HIP graph."""
     import ctypes
     import numpy as np
     import torch
@@ -376,22 +379,17 @@ def bench_vdp_grad(mu, y0, t1, ts, steps, warmup, with_cpu, forms=("f64", "f32")
     for form in forms:
         dtype = torch.float64 if form == "f64" else torch.float32
         plan = prepare(dtype, mu, y0, t1, ts)
-        plan.record_steps()
+        plan.record_checkpoints()           # sqrt(max_steps) checkpoints, the default
         T = plan.t
         ys = T["ys"]
         gys = torch.empty_like(ys)
         y0_bar = torch.empty((B, 2), dtype=dtype, device=dev)
         mu_bar = torch.empty((B, 1), dtype=dtype, device=dev)
-        io = _lib.AdjointIO(params=T["params"].data_ptr(), t1=T["t1"].data_ptr(),
-                            ts=T["ts"].data_ptr(), grid=T["grid"].data_ptr(),
-                            grid_y=T["grid_y"].data_ptr(), stats=T["stats"].data_ptr(),
-                            ys_bar=gys.data_ptr(), y0_bar=y0_bar.data_ptr(),
-                            params_bar=mu_bar.data_ptr())
+        ws = torch.empty(plan.adjoint_workspace_bytes(), dtype=torch.uint8, device=dev)
+        extra_bytes = T["ckpt"].numel() * 8 + ws.numel()
 
         def adjoint():
-            _lib.check(lib.dfx_erk_adjoint(
-                ctypes.byref(plan.problem), ctypes.byref(io),
-                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "dfx_erk_adjoint")
+            plan.adjoint(gys, y0_bar, mu_bar, workspace=ws)
 
         def fwd_grad():
             plan.launch()
@@ -409,12 +407,15 @@ def bench_vdp_grad(mu, y0, t1, ts, steps, warmup, with_cpu, forms=("f64", "f32")
         st = T["stats"].cpu().numpy()
         acc = int(st[:, 1].sum())
         saves = int(B * ts.shape[1])
-        flop = acc * ADJ_FLOP_PER_STEP + saves * ADJ_FLOP_PER_SAVE
+        attempted = int(st[:, 0].sum())
+        flop = attempted * FLOP_PER_STEP + acc * ADJ_FLOP_PER_STEP + saves * ADJ_FLOP_PER_SAVE
         yb = y0_bar[sub].double().cpu().numpy()
         qb = mu_bar[sub].double().cpu().numpy()
         o = dict(solves_per_s=round(B / (wall * 1e-3), 1), ms_per_step=round(wall, 4),
                  event_ms_mean=round(k_mean, 4), event_ms_median=round(k_med, 4),
                  forward_recording_ms=round(f_mean, 4), adjoint_kernel_ms=round(a_mean, 4),
+                 checkpoint_every=int(plan.problem.ckpt_every),
+                 gradient_memory_bytes=int(extra_bytes),
                  accepted_steps=acc, adjoint_algorithmic_flop=flop,
                  adjoint_achieved_tflops=round(flop / (a_mean * 1e-3) / 1e12, 4),
                  adjoint_frac_of_peak=round(flop / (a_mean * 1e-3) / 1e12 /
@@ -428,7 +429,7 @@ def bench_vdp_grad(mu, y0, t1, ts, steps, warmup, with_cpu, forms=("f64", "f32")
                 np.array_equal(yb, ryb) and np.array_equal(qb, rqb))
         out[form] = o
         T.clear()
-        del g_all, g_fwd, g_adj, gys, y0_bar, mu_bar
+        del g_all, g_fwd, g_adj, gys, y0_bar, mu_bar, ws
         torch.cuda.empty_cache()
     res = dict(workload="65,536 uneven Van der Pol, Tsit5 + PID(1e-6, 1e-9), 64 saves, "
                         "forward + discrete-adjoint gradient of mean(ys^2) for y0 and mu",

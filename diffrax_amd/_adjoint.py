@@ -20,13 +20,14 @@ class AbstractAdjoint:
 class RecursiveCheckpointAdjoint(AbstractAdjoint):
     """Backpropagate through the solver with checkpointed recomputation.
 
-    checkpoints: number of stored states per trajectory; None picks ceil(sqrt(N)) for N
-    accepted steps (the reference picks ~log(max_steps) for its online binomial scheme,
-    _adjoint.py:315-329; both bound memory, the device kernels use uniform sqrt(N)).
-    The small-system solves (LotkaVolterra, VanDerPol, LinearField) keep every accepted
-    step's start state instead: d values per step, no more than the step's own time
-    record, so their backward pass recomputes nothing but each step's stages and
-    ``checkpoints`` does not apply to them."""
+    checkpoints: number of stored states per trajectory.  The small-system solves
+    (LotkaVolterra, VanDerPol, LinearField) store the full step state every
+    K = ceil(max_steps / checkpoints) accepted steps, checkpoints = ceil(sqrt(max_steps))
+    when None; the backward pass replays each K-step segment from its checkpoint and
+    sweeps it in reverse, so memory is O(checkpoints + K) per trajectory.  The neural-ODE
+    solve stores a state every K accepted steps by the same rule.  The reference picks ~log(max_steps) checkpoints for its
+    online binomial scheme (_adjoint.py:315-329); both bound memory, the device kernels
+    use uniform spacing."""
 
     def __init__(self, checkpoints: Optional[int] = None):
         self.checkpoints = checkpoints

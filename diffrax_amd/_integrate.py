@@ -94,13 +94,14 @@ class ErkPlan:
     def _bind(self):
         for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid",
                      "dense_ts", "dense_ys", "dense_k", "step_ts", "jump_ts",
-                     "event_coeffs", "event_flags", "event_mask", "grid_y", "rtol", "atol"):
+                     "event_coeffs", "event_flags", "event_mask", "grid_y", "rtol", "atol",
+                     "ckpt"):
             v = self.t.get(name)
             setattr(self.io, name, v.data_ptr() if v is not None else None)
 
-    def record_steps(self):
-        """Record every accepted step's (t_start, t_end) and start state, the input of the
-        discrete adjoint."""
+    def record_grid(self):
+        """Record every accepted step's (t_start, t_end) and start state ([B, max_steps, 2]
+        and [B, max_steps, d]; a diagnostic output, not needed by the adjoint)."""
         B, d = self.t["y0"].shape
         M = int(self.problem.max_steps)
         dev = self.t["y0"].device
@@ -108,6 +109,52 @@ class ErkPlan:
             self.t["grid"] = torch.empty((B, M, 2), dtype=torch.float64, device=dev)
         self.t["grid_y"] = torch.empty((B, M, d), dtype=self.t["y0"].dtype, device=dev)
         self._bind()
+
+    def record_checkpoints(self, checkpoints=None):
+        """Store the lane state every K accepted steps, the restart points of the discrete
+        adjoint: K = ceil(max_steps / checkpoints), checkpoints = ceil(sqrt(max_steps)) when
+        None (RecursiveCheckpointAdjoint.checkpoints, _adjoint.py:264-329).  Memory is
+        [B, ceil(max_steps / K), d + 8] float64 here and [B, K] records of (2 + d) values in
+        the adjoint's workspace."""
+        M = max(int(self.problem.max_steps), 1)
+        c = int(math.ceil(math.sqrt(M))) if checkpoints is None else int(checkpoints)
+        if c < 1:
+            raise ValueError("RecursiveCheckpointAdjoint(checkpoints=...) must be at least 1")
+        self.problem.ckpt_every = -(-M // min(c, M))
+        n_ck, words = ctypes.c_int32(0), ctypes.c_int32(0)
+        lib = _lib.load()
+        _lib.check(lib.dfx_erk_ckpt_layout(ctypes.byref(self.problem), ctypes.byref(n_ck),
+                                           ctypes.byref(words)), "dfx_erk_ckpt_layout")
+        B = self.t["y0"].shape[0]
+        self.t["ckpt"] = torch.empty((B, n_ck.value, words.value), dtype=torch.float64,
+                                     device=self.t["y0"].device)
+        self._bind()
+
+    def adjoint(self, ys_bar, y0_bar, params_bar, workspace=None, stream=None):
+        """dfx_erk_adjoint over this plan's checkpoints (after launch()): cotangents of
+        the saved rows in, per-instance y0 and parameter cotangents out.  ``workspace``
+        may be a uint8 tensor of at least ``adjoint_workspace_bytes()``."""
+        lib = _lib.load()
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        if workspace is None:
+            workspace = torch.empty(self.adjoint_workspace_bytes(), dtype=torch.uint8,
+                                    device=self.t["y0"].device)
+        T = self.t
+        io = _lib.AdjointIO(ckpt=T["ckpt"].data_ptr(), stats=T["stats"].data_ptr(),
+                            ys_bar=ys_bar.data_ptr(), y0_bar=y0_bar.data_ptr(),
+                            params_bar=params_bar.data_ptr())
+        _lib.check(lib.dfx_erk_adjoint(ctypes.byref(self.problem), ctypes.byref(self.io),
+                                       ctypes.byref(io), ctypes.c_void_p(workspace.data_ptr()),
+                                       ctypes.c_size_t(workspace.numel()),
+                                       ctypes.c_void_p(stream)), "dfx_erk_adjoint")
+
+    def adjoint_workspace_bytes(self):
+        nbytes = ctypes.c_size_t(0)
+        _lib.check(_lib.load().dfx_erk_adjoint_workspace_bytes(ctypes.byref(self.problem),
+                                                                ctypes.byref(nbytes)),
+                   "dfx_erk_adjoint_workspace_bytes")
+        return int(nbytes.value)
 
     def launch(self, stream=None):
         lib = _lib.load()
@@ -517,7 +564,7 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
     if not isinstance(adjoint, RecursiveCheckpointAdjoint):
         raise NotImplementedError(f"adjoint {adjoint!r} is not provided by diffrax_amd")
     _check_discrete_adjoint(plan, event, saveat)
-    plan.record_steps()
+    plan.record_checkpoints(adjoint.checkpoints)
     ys = _ErkDiscreteAdjoint.apply(plan, T["y0"], T["params"])
     return _finish(plan, t0, t1, max_steps, throw, ys=ys)
 
@@ -593,7 +640,8 @@ def _backsolve_problem(prob, adjoint, stepsize_controller):
 
 class _ErkDiscreteAdjoint(torch.autograd.Function):
     """ys of a small-system solve, with reverse mode through the accepted steps
-    (dfx_erk_adjoint) as its backward."""
+    (dfx_erk_adjoint, replaying each segment from the forward's checkpoints) as its
+    backward."""
 
     @staticmethod
     def forward(ctx, plan, y0, params):
@@ -611,33 +659,30 @@ class _ErkDiscreteAdjoint(torch.autograd.Function):
         P = T["params"].shape[-1]
         y0_bar = torch.empty((B, d), dtype=ys.dtype, device=ys.device)
         params_bar = torch.empty((B, P), dtype=ys.dtype, device=ys.device)
-        ptr = lambda v: v.data_ptr() if v is not None else None
-        io = _lib.AdjointIO(params=ptr(T["params"]), t1=ptr(T["t1"]), ts=ptr(T.get("ts")),
-                            grid=ptr(T["grid"]), grid_y=ptr(T["grid_y"]), stats=ptr(T["stats"]),
-                            ys_bar=ptr(gys), y0_bar=ptr(y0_bar), params_bar=ptr(params_bar))
-        lib = _lib.load()
-        _lib.check(lib.dfx_erk_adjoint(ctypes.byref(plan.problem), ctypes.byref(io),
-                                       ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)),
-                   "dfx_erk_adjoint")
+        plan.adjoint(gys, y0_bar, params_bar)
         gparams = params_bar.sum(0) if plan.problem.param_stride == 0 else params_bar
         return None, y0_bar, gparams.reshape(T["params"].shape)
 
 
 class _ErkBacksolve(torch.autograd.Function):
-    """ys of a small-system solve, with the continuous adjoint as its backward."""
+    """ys of a small-system solve, with the continuous adjoint as its backward.  The saved
+    rows are the backward solves' restart states, so they are kept through
+    save_for_backward: an in-place change of the returned ys makes backward raise."""
 
     @staticmethod
     def forward(ctx, plan, back, throw, y0, params):
         plan.launch()
         ctx.plan, ctx.back, ctx.throw = plan, back, throw
-        return plan.t["ys"]
+        ys = plan.t["ys"]
+        ctx.save_for_backward(ys)
+        return ys
 
     @staticmethod
     def backward(ctx, gys):
         plan, back = ctx.plan, ctx.back
         T = plan.t
         B, d = T["y0"].shape
-        ys = T["ys"]
+        ys, = ctx.saved_tensors
         gys = torch.zeros_like(ys) if gys is None else gys.to(ys.dtype).contiguous()
         P = T["params"].shape[-1]
         y0_bar = torch.empty((B, d), dtype=ys.dtype, device=ys.device)

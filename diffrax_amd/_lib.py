@@ -38,6 +38,7 @@ class ErkProblem(ctypes.Structure):
         ("safety", ctypes.c_double), ("error_order", ctypes.c_double),
         ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32),
         ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
+        ("ckpt_every", ctypes.c_int32), ("reserved_", ctypes.c_int32),
     ]
 
 
@@ -50,7 +51,7 @@ class ErkIO(ctypes.Structure):
         ("step_ts", ctypes.c_void_p), ("jump_ts", ctypes.c_void_p),
         ("event_coeffs", ctypes.c_void_p), ("event_flags", ctypes.c_void_p),
         ("event_mask", ctypes.c_void_p), ("grid_y", ctypes.c_void_p),
-        ("rtol", ctypes.c_void_p), ("atol", ctypes.c_void_p),
+        ("rtol", ctypes.c_void_p), ("atol", ctypes.c_void_p), ("ckpt", ctypes.c_void_p),
     ]
 
 
@@ -65,10 +66,8 @@ class DenseIO(ctypes.Structure):
 
 class AdjointIO(ctypes.Structure):
     _fields_ = [
-        ("params", ctypes.c_void_p), ("t1", ctypes.c_void_p), ("ts", ctypes.c_void_p),
-        ("grid", ctypes.c_void_p), ("grid_y", ctypes.c_void_p), ("stats", ctypes.c_void_p),
-        ("ys_bar", ctypes.c_void_p), ("y0_bar", ctypes.c_void_p),
-        ("params_bar", ctypes.c_void_p),
+        ("ckpt", ctypes.c_void_p), ("stats", ctypes.c_void_p), ("ys_bar", ctypes.c_void_p),
+        ("y0_bar", ctypes.c_void_p), ("params_bar", ctypes.c_void_p),
     ]
 
 
@@ -100,7 +99,12 @@ _SIGS = {
                            ctypes.c_int),
     "dfx_erk_backsolve": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(BacksolveIO),
                            ctypes.c_void_p], ctypes.c_int),
-    "dfx_erk_adjoint": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(AdjointIO),
+    "dfx_erk_ckpt_layout": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(ctypes.c_int32),
+                             ctypes.POINTER(ctypes.c_int32)], ctypes.c_int),
+    "dfx_erk_adjoint_workspace_bytes": ([ctypes.POINTER(ErkProblem),
+                                         ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "dfx_erk_adjoint": ([ctypes.POINTER(ErkProblem), ctypes.POINTER(ErkIO),
+                         ctypes.POINTER(AdjointIO), ctypes.c_void_p, ctypes.c_size_t,
                          ctypes.c_void_p], ctypes.c_int),
 }
 

@@ -54,6 +54,66 @@ PidParams make_pid(const dfx_erk_problem* p, double error_order) {
   return q;
 }
 
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int32_t ckpt_count(const dfx_erk_problem* p) {
+  return (int32_t)(((int64_t)p->max_steps + p->ckpt_every - 1) / p->ckpt_every);
+}
+
+// Checks a small-system problem and the input buffers of its dfx_erk_io, and fills the
+// kernel arguments the forward solve and its discrete adjoint share (no outputs).
+int fill_erk_inputs(const dfx_erk_problem* p, const dfx_erk_io* io, ErkKArgs& a) {
+  if (!p || !io) return DFX_E_ARG;
+  if (p->solver < DFX_TSIT5 || p->solver > DFX_HEUN) return DFX_E_ARG;
+  if (p->controller != DFX_PID && p->controller != DFX_CONSTANT) return DFX_E_ARG;
+  if (p->precision != DFX_F64 && p->precision != DFX_F32S_F64T) return DFX_E_ARG;
+  if (!small_vf_supported(p->vf, p->d)) return DFX_E_ARG;
+  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0 || p->save_steps < 0) return DFX_E_ARG;
+  if (p->t1_stride < 0 || p->ts_stride < 0 || p->param_stride < 0) return DFX_E_ARG;
+  if (p->controller == DFX_CONSTANT && isnan(p->dt0)) return DFX_E_ARG;
+  if (p->ckpt_every < 0) return DFX_E_ARG;
+  if (p->n_step_ts < 0 || p->n_jump_ts < 0) return DFX_E_ARG;
+  if ((p->n_step_ts > 0 || p->n_jump_ts > 0) && p->controller != DFX_PID) return DFX_E_ARG;
+  if (p->n_events < 0 || p->n_events > kMaxEvents) return DFX_E_ARG;
+  memset(&a, 0, sizeof(a));
+  if (p->batch == 0) return DFX_OK;
+  if (!io->t1 || !io->params) return DFX_E_ARG;
+  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
+  if ((p->n_step_ts > 0 && !io->step_ts) || (p->n_jump_ts > 0 && !io->jump_ts)) return DFX_E_ARG;
+  if (p->n_events > 0 && (!io->event_coeffs || !io->event_flags)) return DFX_E_ARG;
+  const double error_order = isnan(p->error_order) ? (double)solver_order(p->solver) : p->error_order;
+  a.t1 = io->t1;
+  a.ts = p->n_ts > 0 ? io->ts : nullptr;  // no save times: the kernels read t1 instead
+  a.params = io->params;
+  a.rtol_b = io->rtol;
+  a.atol_b = io->atol;
+  a.step_ts = io->step_ts;
+  a.jump_ts = io->jump_ts;
+  a.n_step_ts = p->n_step_ts;
+  a.n_jump_ts = p->n_jump_ts;
+  a.event_coeffs = io->event_coeffs;
+  a.event_flags = io->event_flags;
+  a.n_events = p->n_events;
+  a.event_root_find = p->event_root_find ? 1 : 0;
+  a.B = p->batch;
+  a.t1_stride = p->t1_stride;
+  a.ts_stride = p->ts_stride;
+  a.param_stride = p->param_stride;
+  a.n_ts = p->n_ts;
+  a.n_out = dfx_erk_n_out(p);
+  a.save_t0 = p->save_t0;
+  a.save_t1 = p->save_t1;
+  a.save_steps = p->save_steps;
+  a.max_steps = p->max_steps;
+  a.has_dt0 = !isnan(p->dt0);
+  a.controller = p->controller;
+  a.t0 = p->t0;
+  a.dt0 = p->dt0;
+  a.error_order = error_order;
+  a.pid = make_pid(p, error_order);
+  return DFX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -125,30 +185,33 @@ int dfx_erk_workspace_bytes(const dfx_erk_problem* p, size_t* bytes) {
   return DFX_OK;
 }
 
+int dfx_erk_ckpt_layout(const dfx_erk_problem* p, int32_t* n_ckpt, int32_t* words) {
+  if (!p || !n_ckpt || !words || p->ckpt_every <= 0 || p->max_steps < 0 || p->d < 1)
+    return DFX_E_ARG;
+  *n_ckpt = ckpt_count(p);
+  *words = p->d + 8;
+  return DFX_OK;
+}
+
+int dfx_erk_adjoint_workspace_bytes(const dfx_erk_problem* p, size_t* bytes) {
+  if (!p || !bytes || p->ckpt_every <= 0 || p->batch < 0 || p->d < 1) return DFX_E_ARG;
+  const size_t recs = (size_t)p->batch * (size_t)p->ckpt_every;
+  const size_t elt = p->precision == DFX_F64 ? 8 : 4;
+  *bytes = align256(recs * 2 * sizeof(double)) + align256(recs * (size_t)p->d * elt);
+  if (*bytes < 256) *bytes = 256;
+  return DFX_OK;
+}
+
 int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspace,
                   size_t workspace_bytes, const dfx_launch* launch, void* stream) {
-  if (!p || !io) return DFX_E_ARG;
-  if (p->solver < DFX_TSIT5 || p->solver > DFX_HEUN) return DFX_E_ARG;
-  if (p->controller != DFX_PID && p->controller != DFX_CONSTANT) return DFX_E_ARG;
-  if (p->precision != DFX_F64 && p->precision != DFX_F32S_F64T) return DFX_E_ARG;
-  if (!small_vf_supported(p->vf, p->d)) return DFX_E_ARG;
-  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0 || p->save_steps < 0) return DFX_E_ARG;
-  if (p->t1_stride < 0 || p->ts_stride < 0 || p->param_stride < 0) return DFX_E_ARG;
-  if (p->controller == DFX_CONSTANT && isnan(p->dt0)) return DFX_E_ARG;
-  if (p->batch == 0) return DFX_OK;
-  if (!io->y0 || !io->t1 || !io->params || !io->stats || !io->result) return DFX_E_ARG;
-  const int n_out = dfx_erk_n_out(p);
-  if (n_out > 0 && (!io->ys || !io->ts_out)) return DFX_E_ARG;
-  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
-  if (!workspace || workspace_bytes < 256) return DFX_E_WORKSPACE;
-
-  const double error_order = isnan(p->error_order) ? (double)solver_order(p->solver) : p->error_order;
   ErkKArgs a;
-  memset(&a, 0, sizeof(a));
+  const int rc = fill_erk_inputs(p, io, a);
+  if (rc != DFX_OK) return rc;
+  if (p->batch == 0) return DFX_OK;
+  if (!io->y0 || !io->stats || !io->result) return DFX_E_ARG;
+  if (a.n_out > 0 && (!io->ys || !io->ts_out)) return DFX_E_ARG;
+  if (!workspace || workspace_bytes < 256) return DFX_E_WORKSPACE;
   a.y0 = io->y0;
-  a.t1 = io->t1;
-  a.ts = io->ts;
-  a.params = io->params;
   a.ys = io->ys;
   a.ts_out = io->ts_out;
   a.stats = io->stats;
@@ -156,44 +219,20 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
   a.grid = io->grid;
   if (io->grid_y && !io->grid) return DFX_E_ARG;
   a.grid_y = io->grid_y;
-  a.rtol_b = io->rtol;
-  a.atol_b = io->atol;
+  a.grid_stride = p->max_steps;
   const int dense = (io->dense_ts != nullptr) + (io->dense_ys != nullptr) + (io->dense_k != nullptr);
   if (dense != 0 && dense != 3) return DFX_E_ARG;
   a.dense_ts = io->dense_ts;
   a.dense_ys = io->dense_ys;
   a.dense_k = io->dense_k;
-  if (p->n_step_ts < 0 || p->n_jump_ts < 0) return DFX_E_ARG;
-  if ((p->n_step_ts > 0 || p->n_jump_ts > 0) && p->controller != DFX_PID) return DFX_E_ARG;
-  if ((p->n_step_ts > 0 && !io->step_ts) || (p->n_jump_ts > 0 && !io->jump_ts)) return DFX_E_ARG;
-  a.step_ts = io->step_ts;
-  a.jump_ts = io->jump_ts;
-  a.n_step_ts = p->n_step_ts;
-  a.n_jump_ts = p->n_jump_ts;
-  if (p->n_events < 0 || p->n_events > kMaxEvents) return DFX_E_ARG;
-  if (p->n_events > 0 && (!io->event_coeffs || !io->event_flags)) return DFX_E_ARG;
-  a.event_coeffs = io->event_coeffs;
-  a.event_flags = io->event_flags;
   a.event_mask = io->event_mask;
-  a.n_events = p->n_events;
-  a.event_root_find = p->event_root_find ? 1 : 0;
+  if (p->ckpt_every > 0) {
+    if (!io->ckpt) return DFX_E_ARG;
+    a.ckpt = io->ckpt;
+    a.ckpt_every = p->ckpt_every;
+    a.n_ckpt = ckpt_count(p);
+  }
   a.counter = reinterpret_cast<unsigned int*>(workspace);
-  a.B = p->batch;
-  a.t1_stride = p->t1_stride;
-  a.ts_stride = p->ts_stride;
-  a.param_stride = p->param_stride;
-  a.n_ts = p->n_ts;
-  a.n_out = n_out;
-  a.save_t0 = p->save_t0;
-  a.save_t1 = p->save_t1;
-  a.save_steps = p->save_steps;
-  a.max_steps = p->max_steps;
-  a.has_dt0 = !isnan(p->dt0);
-  a.controller = p->controller;
-  a.t0 = p->t0;
-  a.dt0 = p->dt0;
-  a.error_order = error_order;
-  a.pid = make_pid(p, error_order);
 
   int block = 64;
   int64_t lanes = p->batch;
@@ -253,47 +292,45 @@ int dfx_erk_backsolve(const dfx_erk_problem* p, const dfx_erk_backsolve_io* io, 
   return erk_backsolve_f32(p->solver, p->vf, p->d, a, b, s);
 }
 
-int dfx_erk_adjoint(const dfx_erk_problem* p, const dfx_erk_adjoint_io* io, void* stream) {
-  if (!p || !io) return DFX_E_ARG;
-  if (p->solver < DFX_TSIT5 || p->solver > DFX_HEUN) return DFX_E_ARG;
-  if (p->precision != DFX_F64 && p->precision != DFX_F32S_F64T) return DFX_E_ARG;
+int dfx_erk_adjoint(const dfx_erk_problem* p, const dfx_erk_io* fwd,
+                    const dfx_erk_adjoint_io* io, void* workspace, size_t workspace_bytes,
+                    void* stream) {
+  if (!p || !fwd || !io) return DFX_E_ARG;
   const bool vf_ok = (p->vf == DFX_VF_LV || p->vf == DFX_VF_VDP) ? p->d == 2
                      : (p->vf == DFX_VF_LINEAR && p->d >= 1 && p->d <= 4);
   if (!vf_ok) return DFX_E_ARG;
-  if (p->n_events > 0) return DFX_E_ARG;
-  if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0) return DFX_E_ARG;
-  if (p->t1_stride < 0 || p->ts_stride < 0 || p->param_stride < 0) return DFX_E_ARG;
-  if (p->batch == 0) return DFX_OK;
-  if (!io->params || !io->t1 || !io->stats || !io->ys_bar || !io->y0_bar || !io->params_bar)
-    return DFX_E_ARG;
-  if (p->max_steps > 0 && (!io->grid || !io->grid_y)) return DFX_E_ARG;
-  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
+  if (p->n_events > 0 || p->ckpt_every <= 0) return DFX_E_ARG;
   ErkKArgs a;
-  memset(&a, 0, sizeof(a));
-  a.t1 = io->t1;
-  a.ts = io->ts;
-  a.params = io->params;
-  a.B = p->batch;
-  a.t1_stride = p->t1_stride;
-  a.ts_stride = p->ts_stride;
-  a.param_stride = p->param_stride;
-  a.n_ts = p->n_ts;
-  a.n_out = dfx_erk_n_out(p);
-  a.save_t0 = p->save_t0;
-  a.save_t1 = p->save_t1;
-  a.save_steps = p->save_steps;
-  a.max_steps = p->max_steps;
-  a.t0 = p->t0;
+  const int rc = fill_erk_inputs(p, fwd, a);
+  if (rc != DFX_OK) return rc;
+  if (p->batch == 0) return DFX_OK;
+  if (!io->ckpt || !io->stats || !io->ys_bar || !io->y0_bar || !io->params_bar) return DFX_E_ARG;
+  size_t need = 0;
+  dfx_erk_adjoint_workspace_bytes(p, &need);
+  if (!workspace || workspace_bytes < need) return DFX_E_WORKSPACE;
+  a.ckpt_every = p->ckpt_every;
+  a.n_ckpt = ckpt_count(p);
+  // The replay's arguments: no saves, no checkpoint writes; accepted steps recorded into
+  // the per-instance segment scratch of ckpt_every records.
+  ErkKArgs ra = a;
+  ra.ts = nullptr;
+  ra.n_ts = 0;
+  ra.n_out = 0;
+  ra.save_t0 = ra.save_t1 = ra.save_steps = 0;
+  ra.ckpt = nullptr;
+  const size_t recs = (size_t)p->batch * (size_t)p->ckpt_every;
+  ra.grid = reinterpret_cast<double*>(workspace);
+  ra.grid_y = reinterpret_cast<char*>(workspace) + align256(recs * 2 * sizeof(double));
+  ra.grid_stride = p->ckpt_every;
   DiscreteAdjArgs b;
-  b.grid = io->grid;
-  b.grid_y = io->grid_y;
+  b.ckpt = io->ckpt;
   b.stats = io->stats;
   b.ys_bar = io->ys_bar;
   b.y0_bar = io->y0_bar;
   b.q_bar = io->params_bar;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (p->precision == DFX_F64) return erk_adjoint_f64(p->solver, p->vf, p->d, a, b, s);
-  return erk_adjoint_f32(p->solver, p->vf, p->d, a, b, s);
+  if (p->precision == DFX_F64) return erk_adjoint_f64(p->solver, p->vf, p->d, a, ra, b, s);
+  return erk_adjoint_f32(p->solver, p->vf, p->d, a, ra, b, s);
 }
 
 int dfx_erk_dense_eval(const dfx_erk_problem* p, const dfx_erk_dense_io* io, void* stream) {

@@ -1,12 +1,16 @@
 // erk_adjoint.hpp -- discrete adjoint of the small-system solve: reverse mode through the
 // accepted steps, the gradient RecursiveCheckpointAdjoint takes (_adjoint.py:264-312).
 //
-// The forward solve records, for every accepted step, its normalised (t_start, t_end)
-// (ErkKArgs::grid) and its start state (ErkKArgs::grid_y).  A state is d values, as small
-// as the step's time record, so the whole accepted sequence is kept and the backward pass
-// is one reverse sweep with no forward recomputation beyond each step's own stages.  The
-// step-size controller's factors are constants of the gradient (pid.py:374, 498-530) and
-// rejected steps contribute nothing, so only the accepted steps enter.
+// The forward solve stores the whole lane state every K accepted steps (ErkKArgs::ckpt,
+// ErkLane::write_ckpt; K = ceil(max_steps / checkpoints), sqrt(max_steps) by default), so
+// memory is O(B (max_steps / K + K)) instead of a record per step.  The backward pass
+// takes the segments newest first: it restarts the forward step loop from the segment's
+// checkpoint (ErkLane::resume; same code and arithmetic, so the accepted steps repeat
+// bit for bit, rejected ones included), writes each accepted step's (t_start, t_end) and
+// start state to a K-record per-lane scratch (ErkKArgs::grid / grid_y of the replay
+// arguments) and sweeps those records in reverse.  The step-size controller's factors are
+// constants of the gradient (pid.py:374, 498-530) and rejected steps contribute nothing,
+// so only the accepted steps enter the sweep.
 //
 // Per accepted step, newest first (one lane per instance):
 //   - its stages are recomputed from the recorded start state, exactly as the forward
@@ -24,8 +28,7 @@
 namespace dfx {
 
 struct DiscreteAdjArgs {
-  const double* grid;     // [B, max_steps, 2] normalised (t_start, t_end) of accepted steps
-  const void* grid_y;     // [B, max_steps, D] start state of every accepted step
+  const double* ckpt;     // [B, n_ckpt, D + 8] the forward's checkpoints
   const int32_t* stats;   // [B, 3] (column 1: accepted steps)
   const void* ys_bar;     // [B, n_out, D] cotangents of the saved rows
   void* y0_bar;           // out [B, D]
@@ -80,157 +83,174 @@ __device__ __forceinline__ void erk_interp_vjp(double ts0, double ts1, double t,
   }
 }
 
-template <class Tab, class VF, class T>
-__global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const DiscreteAdjArgs b) {
+// a: the forward's arguments (save layout, ts, t1, params); ra: the same with saves,
+// checkpoint writes and events off, and grid / grid_y pointing at the per-lane segment
+// scratch (grid_stride = ckpt_every records per lane).
+template <class Tab, class VF, class T, int Ctl, bool Extras>
+__global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const ErkKArgs ra,
+                                                         const DiscreteAdjArgs b) {
   constexpr int S = Tab::S, D = VF::D, P = VF::P;
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= a.B) return;
-  TabRegs<Tab> tr;
-  tr.load();
-  T q[P], gq[P];
-  const T* qp = reinterpret_cast<const T*>(a.params) + id * a.param_stride;
+  ErkLane<Tab, VF, T, Ctl, Extras> L;
+  L.replay_setup(ra, id);
+  T gq[P];
 #pragma unroll
-  for (int m = 0; m < P; ++m) { q[m] = qp[m]; gq[m] = T(0); }
-  const double t1r = a.t1[id * a.t1_stride];
-  const int dir = (a.t0 < t1r) ? 1 : -1;
-  const double T0 = a.t0 * dir, T1 = t1r * dir;
-  const int n = b.stats[id * 3 + 1];
-  const double* grid = b.grid + id * (int64_t)a.max_steps * 2;
-  const T* gyp = reinterpret_cast<const T*>(b.grid_y) + id * (int64_t)a.max_steps * D;
+  for (int m = 0; m < P; ++m) gq[m] = T(0);
+  const int dir = L.dir;
+  const double T0 = a.t0 * dir, T1 = L.T1;
+  const int n = min(max(b.stats[id * 3 + 1], 0), a.max_steps);
+  const int K = a.ckpt_every;
+  const double* sg = ra.grid + id * ra.grid_stride * 2;
+  const T* sy = reinterpret_cast<const T*>(ra.grid_y) + id * ra.grid_stride * D;
   const T* g = reinterpret_cast<const T*>(b.ys_bar) + id * (int64_t)a.n_out * D;
   const double* tsr = a.ts ? a.ts + id * a.ts_stride : nullptr;
-
-  // SaveAt(ts) rows the forward wrote: all of them for a zero-length solve, else those at
-  // or before the end of the last accepted step (a prefix: ts is sorted).
-  int n_saved = 0;
-  if (T0 == T1) {
-    n_saved = a.n_ts;
-  } else if (n > 0) {
-    const double tend = grid[(int64_t)(n - 1) * 2 + 1];
-    while (n_saved < a.n_ts && tsr[n_saved] * dir <= tend) ++n_saved;
-  }
-  T ab[D];  // cotangent of the state at the end of the step being processed
-#pragma unroll
-  for (int j = 0; j < D; ++j) ab[j] = T(0);
   const int ksave = a.save_steps;
-  int r = a.save_t0 + n_saved + (ksave > 0 ? n / ksave : 0);  // the SaveAt(t1) row
-  if (a.save_t1 && !(ksave > 0 && n % ksave == 0)) {
-#pragma unroll
-    for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[(int64_t)r * D + j];
-  }
-  r -= 1;               // the last row a step wrote
-  int ti = n_saved - 1;  // the last ts index saved
-  if (T0 == T1) ti = -1;
 
-  // The reads of every step are issued one step (and one save row) ahead: a lane walks its
-  // own records serially, so a load issued where it is used would expose its full latency
-  // once per step.
-  double nts0 = 0.0, nts1 = 0.0;
-  T ny0[D], grow[D];
+  T ab[D];    // cotangent of the state at the end of the step being processed
+  T grow[D];  // cotangent of save row r, loaded one row ahead
 #pragma unroll
-  for (int j = 0; j < D; ++j) { ny0[j] = T(0); grow[j] = T(0); }
-  if (n > 0) {
-    nts0 = grid[(int64_t)(n - 1) * 2];
-    nts1 = grid[(int64_t)(n - 1) * 2 + 1];
-#pragma unroll
-    for (int j = 0; j < D; ++j) ny0[j] = gyp[(int64_t)(n - 1) * D + j];
-  }
-  if (r >= 0) {
-#pragma unroll
-    for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
-  }
-  double tsv = (ti >= 0) ? tsr[ti] * dir : -__builtin_inf();
-
-  for (int s = n - 1; s >= 0; --s) {
-    const double ts0 = nts0, ts1 = nts1;
-    T y0[D], zs[S][D], k[S][D];
-#pragma unroll
-    for (int j = 0; j < D; ++j) y0[j] = ny0[j];
-    if (s > 0) {
-      nts0 = grid[(int64_t)(s - 1) * 2];
-      nts1 = grid[(int64_t)(s - 1) * 2 + 1];
-#pragma unroll
-      for (int j = 0; j < D; ++j) ny0[j] = gyp[(int64_t)(s - 1) * D + j];
+  for (int j = 0; j < D; ++j) { ab[j] = T(0); grow[j] = T(0); }
+  int r = -1, ti = -1;
+  double tsv = -__builtin_inf();
+  // SaveAt(ts) rows the forward wrote: all of them for a zero-length solve, else those at
+  // or before the end of the last accepted step (a prefix: ts is sorted); then the t1 row.
+  auto setup_rows = [&](double tend) {
+    int n_saved = 0;
+    if (T0 == T1) {
+      n_saved = a.n_ts;
+    } else if (n > 0) {
+      while (n_saved < a.n_ts && tsr[n_saved] * dir <= tend) ++n_saved;
     }
-    const double prev_end = s > 0 ? nts1 : -__builtin_inf();
-    const double dt = ts1 - ts0;
-    const T control = T((double)dir * dt);
-    // the step's stages, as the forward computes them
-    static_for<0, S>([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
+    r = a.save_t0 + n_saved + (ksave > 0 ? n / ksave : 0);  // the SaveAt(t1) row
+    if (a.save_t1 && !(ksave > 0 && n % ksave == 0)) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[(int64_t)r * D + j];
+    }
+    r -= 1;           // the last row a step wrote
+    ti = (T0 == T1) ? -1 : n_saved - 1;  // the last ts index saved
+    if (r >= 0) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
+    }
+    tsv = (ti >= 0) ? tsr[ti] * dir : -__builtin_inf();
+  };
+  if (n == 0) setup_rows(-__builtin_inf());
+
+  const int c_last = n > 0 ? (n - 1) / K : -1;
+  for (int c = c_last; c >= 0; --c) {
+    // replay the segment's accepted steps [lo, target) from its checkpoint
+    const double tl = L.resume(ra, b.ckpt, c);
+    const int lo = c * K;
+    const int target = min(lo + K, n);
+    while (L.num_acc < target && L.running(ra)) L.step(ra);
+    const int hi = L.num_acc;  // == target: the replay repeats the forward
+    if (c == c_last) setup_rows(hi > lo ? sg[(int64_t)(hi - 1 - lo) * 2 + 1] : -__builtin_inf());
+
+    // The records are read one step ahead: a lane walks them serially, so a load issued
+    // where it is used would expose its full latency once per step.
+    double nts0 = 0.0, nts1 = 0.0;
+    T ny0[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) ny0[j] = T(0);
+    if (hi > lo) {
+      nts0 = sg[(int64_t)(hi - 1 - lo) * 2];
+      nts1 = sg[(int64_t)(hi - 1 - lo) * 2 + 1];
+#pragma unroll
+      for (int j = 0; j < D; ++j) ny0[j] = sy[(int64_t)(hi - 1 - lo) * D + j];
+    }
+    for (int s = hi - 1; s >= lo; --s) {
+      const int i = s - lo;
+      const double ts0 = nts0, ts1 = nts1;
+      T y0[D], zs[S][D], k[S][D];
+#pragma unroll
+      for (int j = 0; j < D; ++j) y0[j] = ny0[j];
+      if (i > 0) {
+        nts0 = sg[(int64_t)(i - 1) * 2];
+        nts1 = sg[(int64_t)(i - 1) * 2 + 1];
+#pragma unroll
+        for (int j = 0; j < D; ++j) ny0[j] = sy[(int64_t)(i - 1) * D + j];
+      }
+      const double prev_end = i > 0 ? nts1 : tl;  // end of the previous accepted step
+      const double dt = ts1 - ts0;
+      const T control = T((double)dir * dt);
+      // the step's stages, as the forward computes them
+      static_for<0, S>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+          T kc[S];
+#pragma unroll
+          for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : T(0);
+          zs[i][j] = y0[j] + comb<Tab, 0, i, 0, i, false>(L.tr.a[i], kc, T(0));
+        }
+        constexpr double ci = Tab::c[i];
+        const double tsi = (ci == 1.0) ? ts1 : ts0 + ci * dt;
+        T f[D];
+        VF::eval(tsi * dir, zs[i], L.q, f);
+#pragma unroll
+        for (int j = 0; j < D; ++j) k[i][j] = control * f[j];
+      });
+      T gy1[D], gy0[D], gk[S][D];
 #pragma unroll
       for (int j = 0; j < D; ++j) {
-        T kc[S];
+        gy1[j] = ab[j];
+        gy0[j] = T(0);
 #pragma unroll
-        for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : T(0);
-        zs[i][j] = y0[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, T(0));
+        for (int m = 0; m < S; ++m) gk[m][j] = T(0);
       }
-      constexpr double ci = Tab::c[i];
-      const double tsi = (ci == 1.0) ? ts1 : ts0 + ci * dt;
-      T f[D];
-      VF::eval(tsi * dir, zs[i], q, f);
+      if (ksave > 0 && (s + 1) % ksave == 0) {  // the step's end state (SaveAt(steps=k))
 #pragma unroll
-      for (int j = 0; j < D; ++j) k[i][j] = control * f[j];
-    });
-    T gy1[D], gy0[D], gk[S][D];
+        for (int j = 0; j < D; ++j) gy1[j] = gy1[j] + grow[j];
+        --r;
+        if (r >= 0) {
 #pragma unroll
-    for (int j = 0; j < D; ++j) {
-      gy1[j] = ab[j];
-      gy0[j] = T(0);
-#pragma unroll
-      for (int m = 0; m < S; ++m) gk[m][j] = T(0);
-    }
-    if (ksave > 0 && (s + 1) % ksave == 0) {  // the step's end state (SaveAt(steps=k))
-#pragma unroll
-      for (int j = 0; j < D; ++j) gy1[j] = gy1[j] + grow[j];
-      --r;
-      if (r >= 0) {
-#pragma unroll
-        for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
+          for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
+        }
       }
-    }
-    // SaveAt(ts) rows inside this step: t in (end of the previous step, ts1]
-    while (ti >= 0 && tsv > prev_end) {
-      erk_interp_vjp<Tab, T, D>(ts0, ts1, tsv, grow, gy0, gy1, gk);
-      --ti;
-      --r;
-      tsv = (ti >= 0) ? tsr[ti] * dir : -__builtin_inf();
-      if (r >= 0) {
+      // SaveAt(ts) rows inside this step: t in (end of the previous step, ts1]
+      while (ti >= 0 && tsv > prev_end) {
+        erk_interp_vjp<Tab, T, D>(ts0, ts1, tsv, grow, gy0, gy1, gk);
+        --ti;
+        --r;
+        tsv = (ti >= 0) ? tsr[ti] * dir : -__builtin_inf();
+        if (r >= 0) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
+          for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
+        }
       }
-    }
-    // y1 = y0 + sum_m b_m k_m
+      // y1 = y0 + sum_m b_m k_m
 #pragma unroll
-    for (int j = 0; j < D; ++j) gy0[j] = gy0[j] + gy1[j];
-    static_for<0, S>([&](auto mc) {
-      constexpr int m = decltype(mc)::value;
-      if constexpr (Tab::b[m] != 0.0) {
+      for (int j = 0; j < D; ++j) gy0[j] = gy0[j] + gy1[j];
+      static_for<0, S>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        if constexpr (Tab::b[m] != 0.0) {
 #pragma unroll
-        for (int j = 0; j < D; ++j) gk[m][j] = fma_t(T(Tab::b[m]), gy1[j], gk[m][j]);
-      }
-    });
-    // stages, last first
-    static_for<0, S>([&](auto ic) {
-      constexpr int i = S - 1 - decltype(ic)::value;
-      T w[D], gz[D], gqi[P];
-#pragma unroll
-      for (int j = 0; j < D; ++j) w[j] = control * gk[i][j];
-      VF::vjp(zs[i], q, w, gz, gqi);
-#pragma unroll
-      for (int m = 0; m < P; ++m) gq[m] = gq[m] + gqi[m];
-#pragma unroll
-      for (int j = 0; j < D; ++j) gy0[j] = gy0[j] + gz[j];
-      static_for<0, i>([&](auto lc) {
-        constexpr int l = decltype(lc)::value;
-        if constexpr (Tab::a[i][l] != 0.0) {
-#pragma unroll
-          for (int j = 0; j < D; ++j) gk[l][j] = fma_t(T(Tab::a[i][l]), gz[j], gk[l][j]);
+          for (int j = 0; j < D; ++j) gk[m][j] = fma_t(T(Tab::b[m]), gy1[j], gk[m][j]);
         }
       });
-    });
+      // stages, last first
+      static_for<0, S>([&](auto ic) {
+        constexpr int i = S - 1 - decltype(ic)::value;
+        T w[D], gz[D], gqi[P];
 #pragma unroll
-    for (int j = 0; j < D; ++j) ab[j] = gy0[j];
+        for (int j = 0; j < D; ++j) w[j] = control * gk[i][j];
+        VF::vjp(zs[i], L.q, w, gz, gqi);
+#pragma unroll
+        for (int m = 0; m < P; ++m) gq[m] = gq[m] + gqi[m];
+#pragma unroll
+        for (int j = 0; j < D; ++j) gy0[j] = gy0[j] + gz[j];
+        static_for<0, i>([&](auto lc) {
+          constexpr int l = decltype(lc)::value;
+          if constexpr (Tab::a[i][l] != 0.0) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) gk[l][j] = fma_t(T(Tab::a[i][l]), gz[j], gk[l][j]);
+          }
+        });
+      });
+#pragma unroll
+      for (int j = 0; j < D; ++j) ab[j] = gy0[j];
+    }
   }
   if (T0 == T1) {  // a zero-length solve saves y0 in every ts row
     for (int rr = a.save_t0; rr < a.save_t0 + a.n_ts; ++rr) {
@@ -250,29 +270,46 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
   for (int m = 0; m < P; ++m) qb[m] = gq[m];
 }
 
-template <class Tab, class VF, class T>
-static int launch_adjoint(const ErkKArgs& a, const DiscreteAdjArgs& b, hipStream_t s) {
+template <class Tab, class VF, class T, int Ctl, bool X>
+static int launch_adjoint_v(const ErkKArgs& a, const ErkKArgs& ra, const DiscreteAdjArgs& b,
+                            hipStream_t s) {
   const int64_t nb = (a.B + 63) / 64;
-  hipLaunchKernelGGL((erk_adjoint_kernel<Tab, VF, T>), dim3((unsigned)nb), dim3(64), 0, s, a, b);
+  hipLaunchKernelGGL((erk_adjoint_kernel<Tab, VF, T, Ctl, X>), dim3((unsigned)nb), dim3(64), 0, s,
+                     a, ra, b);
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }
 
+// The replay must repeat the forward's controller arithmetic: the default integral
+// controller (and ConstantStepSize) on PID_FIFTH, any other PID on PID_GENERIC, and with
+// step clipping the extras instantiation on PID_GENERIC (whose exponent-0.2 power is the
+// same correctly rounded fifth root, with factors 1 for the p and d terms, so it gives
+// PID_FIFTH's bits too).
+template <class Tab, class VF, class T>
+static int launch_adjoint(const ErkKArgs& a, const ErkKArgs& ra, const DiscreteAdjArgs& b,
+                          hipStream_t s) {
+  const bool x = a.n_step_ts > 0 || a.n_jump_ts > 0;
+  const bool fifth = a.controller != DFX_PID || pid_is_fifth(a.pid);
+  if (x) return launch_adjoint_v<Tab, VF, T, PID_GENERIC, true>(a, ra, b, s);
+  if (fifth) return launch_adjoint_v<Tab, VF, T, PID_FIFTH, false>(a, ra, b, s);
+  return launch_adjoint_v<Tab, VF, T, PID_GENERIC, false>(a, ra, b, s);
+}
+
 template <class Tab, class T>
-static int dispatch_adjoint_vf(int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
-                               hipStream_t s) {
+static int dispatch_adjoint_vf(int vf, int d, const ErkKArgs& a, const ErkKArgs& ra,
+                               const DiscreteAdjArgs& b, hipStream_t s) {
   switch (vf) {
     case VF_LV:
-      if (d == 2) return launch_adjoint<Tab, LotkaVolterraVF<T>, T>(a, b, s);
+      if (d == 2) return launch_adjoint<Tab, LotkaVolterraVF<T>, T>(a, ra, b, s);
       break;
     case VF_VDP:
-      if (d == 2) return launch_adjoint<Tab, VanDerPolVF<T>, T>(a, b, s);
+      if (d == 2) return launch_adjoint<Tab, VanDerPolVF<T>, T>(a, ra, b, s);
       break;
     case VF_LINEAR:
       switch (d) {
-        case 1: return launch_adjoint<Tab, LinearVF<T, 1>, T>(a, b, s);
-        case 2: return launch_adjoint<Tab, LinearVF<T, 2>, T>(a, b, s);
-        case 3: return launch_adjoint<Tab, LinearVF<T, 3>, T>(a, b, s);
-        case 4: return launch_adjoint<Tab, LinearVF<T, 4>, T>(a, b, s);
+        case 1: return launch_adjoint<Tab, LinearVF<T, 1>, T>(a, ra, b, s);
+        case 2: return launch_adjoint<Tab, LinearVF<T, 2>, T>(a, ra, b, s);
+        case 3: return launch_adjoint<Tab, LinearVF<T, 3>, T>(a, ra, b, s);
+        case 4: return launch_adjoint<Tab, LinearVF<T, 4>, T>(a, ra, b, s);
         default: break;
       }
       break;
@@ -282,19 +319,19 @@ static int dispatch_adjoint_vf(int vf, int d, const ErkKArgs& a, const DiscreteA
 }
 
 template <class T>
-static int dispatch_adjoint(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
-                            hipStream_t s) {
+static int dispatch_adjoint(int solver, int vf, int d, const ErkKArgs& a, const ErkKArgs& ra,
+                            const DiscreteAdjArgs& b, hipStream_t s) {
   switch (solver) {
-    case SOLVER_TSIT5: return dispatch_adjoint_vf<Tsit5Tab, T>(vf, d, a, b, s);
-    case SOLVER_DOPRI5: return dispatch_adjoint_vf<Dopri5Tab, T>(vf, d, a, b, s);
-    case SOLVER_HEUN: return dispatch_adjoint_vf<HeunTab, T>(vf, d, a, b, s);
+    case SOLVER_TSIT5: return dispatch_adjoint_vf<Tsit5Tab, T>(vf, d, a, ra, b, s);
+    case SOLVER_DOPRI5: return dispatch_adjoint_vf<Dopri5Tab, T>(vf, d, a, ra, b, s);
+    case SOLVER_HEUN: return dispatch_adjoint_vf<HeunTab, T>(vf, d, a, ra, b, s);
     default: return DFX_E_ARG;
   }
 }
 
-int erk_adjoint_f64(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
-                    hipStream_t s);
-int erk_adjoint_f32(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
-                    hipStream_t s);
+int erk_adjoint_f64(int solver, int vf, int d, const ErkKArgs& a, const ErkKArgs& ra,
+                    const DiscreteAdjArgs& b, hipStream_t s);
+int erk_adjoint_f32(int solver, int vf, int d, const ErkKArgs& a, const ErkKArgs& ra,
+                    const DiscreteAdjArgs& b, hipStream_t s);
 
 }  // namespace dfx

@@ -251,21 +251,27 @@ __host__ __device__ inline double pow_ctl(double x, double c) {
   return __builtin_nan("");
 }
 
-#if defined(__HIPCC__)  // device builtins: HIP compilation only (not a host C++ build)
 // rms_norm for D >= 2 with the division by c = sqrt(D) (rounded) written out: the float64
 // division a / c is d = div_scale(c), n = div_scale(a), r = rcp(d) refined by two steps
-// r += r (1 - d r), q = n r, q + r (n - d q) (div_fmas) and div_fixup; for a = sqrt(s) in
-// {0} or [2^-537, 2^512) the scalings leave their operands unchanged and the fixup returns
-// q unchanged, so the same steps with r = rc (computed once from c) give the same bits.
-// sqrt(s) = +inf is selected; NaN propagates.
-__device__ __forceinline__ double refined_rcp(double b) {
+// r += r (1 - d r), q = n r, q + r (n - d q) (div_fmas) and div_fixup.  Here r = rc is
+// computed once from c (refined_rcp: the correctly rounded 1/c for D = 2..16), q = a rc,
+// and q + rc (a - c q) is the correctly rounded a / c whenever the residual a - c q is
+// exact, which holds for a >= 2^-969.  a = sqrt(s) of a float64 s is 0, +inf or in
+// [2^-537, 2^512] (s >= 2^-1074 when nonzero), so every reachable a is covered; +inf is
+// selected and NaN propagates.  The host build (tests) seeds the reciprocal with a float
+// division instead of v_rcp_f64; the two refinement steps give the same r from either.
+__host__ __device__ __forceinline__ double refined_rcp(double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
   double r = __builtin_amdgcn_rcp(b);
+#else
+  double r = (double)(1.0f / (float)b);
+#endif
   r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
   r = __builtin_fma(r, __builtin_fma(-b, r, 1.0), r);
   return r;
 }
 template <int D>
-__device__ __forceinline__ double rms_norm_rc(const double (&e)[D], double c, double rc) {
+__host__ __device__ __forceinline__ double rms_norm_rc(const double (&e)[D], double c, double rc) {
   static_assert(D >= 2, "D = 1 is |e| (no division)");
   double s = e[0] * e[0];
 #pragma unroll
@@ -275,7 +281,6 @@ __device__ __forceinline__ double rms_norm_rc(const double (&e)[D], double c, do
   const double out = __builtin_fma(__builtin_fma(-c, q, a), rc, q);
   return (a == __builtin_inf()) ? a : out;
 }
-#endif
 
 // ----------------------------------------------------------------------------------
 // Controller parameters (PIDController fields, pid.py:299-311) and state.

@@ -26,8 +26,10 @@ struct ErkKArgs {
   double* ts_out;
   int32_t* stats;
   int32_t* result;
-  double* grid;       // [B, max_steps, 2] normalised (t_start, t_end) of accepted steps
-  void* grid_y;       // [B, max_steps, D] start state of each accepted step (with grid)
+  double* grid;       // [B, grid_stride, 2] normalised (t_start, t_end) of accepted steps
+  void* grid_y;       // [B, grid_stride, D] start state of each accepted step (with grid)
+  double* ckpt;       // [B, n_ckpt, D + 8] lane state every ckpt_every accepted steps
+                      // (ErkLane::write_ckpt), the restart points of the discrete adjoint
   const double* rtol_b;  // per-instance PIDController tolerances [B], or null for pid.rtol
   const double* atol_b;
   double* dense_ts;   // SaveAt(dense=True): [B, max_steps + 1] accepted-step ends (t0 first)
@@ -40,6 +42,8 @@ struct ErkKArgs {
   int32_t* event_mask;         // [B, n_events] out
   unsigned int* counter;
   int64_t B, t1_stride, ts_stride, param_stride;
+  int64_t grid_stride;  // records per instance in grid / grid_y
+  int32_t ckpt_every, n_ckpt;
   int32_t n_step_ts, n_jump_ts, n_events, event_root_find;
   int32_t n_ts, n_out, save_t0, save_t1, save_steps, max_steps, has_dt0, controller;
   double t0, dt0, error_order;
@@ -55,21 +59,27 @@ __device__ __forceinline__ double in_vgpr(double x) {
   asm volatile("v_mov_b64 %0, 0" : "=v"(z));
   return x + z;
 }
+__device__ __forceinline__ float in_vgpr(float x) {
+  float z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return x + z;
+}
 
-// Per-lane copies of a tableau's nonzero coefficients (zeros stay compile-time zeros).
-template <class Tab>
+// Per-lane copies of a tableau's nonzero coefficients (zeros stay compile-time zeros), as
+// C (float64, or the float32 values of the coefficients for float32 stage algebra).
+template <class Tab, class C = double>
 struct TabRegs {
   static constexpr int S = Tab::S;
-  double a[S][S];
-  double b[S];
-  double berr[S];
+  C a[S][S];
+  C b[S];
+  C berr[S];
   __device__ __forceinline__ void load() {
 #pragma unroll
     for (int i = 0; i < S; ++i) {
 #pragma unroll
-      for (int m = 0; m < S; ++m) a[i][m] = (Tab::a[i][m] != 0.0) ? in_vgpr(Tab::a[i][m]) : 0.0;
-      b[i] = (Tab::b[i] != 0.0) ? in_vgpr(Tab::b[i]) : 0.0;
-      berr[i] = (Tab::berr[i] != 0.0) ? in_vgpr(Tab::berr[i]) : 0.0;
+      for (int m = 0; m < S; ++m) a[i][m] = (Tab::a[i][m] != 0.0) ? in_vgpr(C(Tab::a[i][m])) : C(0);
+      b[i] = (Tab::b[i] != 0.0) ? in_vgpr(C(Tab::b[i])) : C(0);
+      berr[i] = (Tab::berr[i] != 0.0) ? in_vgpr(C(Tab::berr[i])) : C(0);
     }
   }
 };
@@ -92,8 +102,8 @@ __device__ __forceinline__ double fma_t(double a, double b, double c) { return _
 // dot product the reference's tensordot, _solver/base.py:37-41, is restated as).  Zero
 // entries, and the choice of the first term, are resolved at compile time.  `vals` holds
 // the coefficient values (in VGPRs).
-template <class Tab, int WHICH, int ROW, int M, int END, bool STARTED, class T, int S>
-__device__ __forceinline__ T comb(const double* vals, const T (&k)[S], T acc) {
+template <class Tab, int WHICH, int ROW, int M, int END, bool STARTED, class T, int S, class C>
+__device__ __forceinline__ T comb(const C* vals, const T (&k)[S], T acc) {
   if constexpr (M >= END) {
     return acc;
   } else {
@@ -343,6 +353,8 @@ struct ErkLane {
   int32_t cs_completed, cs_num;
   double cs_t0, cs_t1;
   int32_t num_steps, num_acc, num_rej, result;
+  int32_t grid_base;  // accepted steps before the first grid record (a replayed segment's start)
+  int32_t ck_left;    // accepted steps until the next checkpoint
   int32_t save_index, ts_index;
   double next_ts;    // normalised save time ts[ts_index], +inf when none left
   double after_raw;  // ts[min(ts_index + 1, n_ts - 1)] as stored, loaded one save ahead
@@ -418,15 +430,35 @@ struct ErkLane {
     norm_rc = refined_rcp(norm_c);
   }
 
+  // The parts of a solve's set-up that depend only on its span: direction, normalised
+  // end, the 100-ulp end-clipping floor (_integrate.py:319-322) and ConstantStepSize's
+  // grid (constant.py:30-54); returns the normalised t0.
+  __device__ double setup_span(const ErkKArgs& a, double t0r, double t1r) {
+    dir = (t0r < t1r) ? 1 : -1;
+    const double T0 = t0r * dir;
+    T1 = t1r * dir;
+    if (a.controller == DFX_PID) {
+      cs_num = 0; cs_t0 = T0; cs_t1 = T1;
+    } else {
+      const double DT0 = a.dt0 * dir;
+      const double qn = ceil((T1 - T0) / nextup(DT0));
+      cs_num = __builtin_isfinite(T1) ? (int32_t)qn : -1;
+      cs_t1 = __builtin_isfinite(T1) ? T1 : DT0;
+      cs_t0 = T0;
+    }
+    double fl = T1;
+    for (int r = 0; r < 100; ++r) fl = prevbefore(fl);
+    floor_t1 = fl;
+    return T0;
+  }
+
   // Time normalisation and controller initialisation of a solve from t0r to t1r for the
   // state in y (_integrate.py:1156-1265, pid.py:316-392, constant.py:30-54, clip.py:243-299);
   // returns the normalised t0.
   __device__ double begin_times(const ErkKArgs& a, double t0r, double t1r) {
 #pragma unroll
     for (int j = 0; j < D; ++j) fsal_f[j] = T(0);
-    dir = (t0r < t1r) ? 1 : -1;
-    const double T0 = t0r * dir;
-    T1 = t1r * dir;
+    const double T0 = setup_span(a, t0r, t1r);
     first = true;
     made_jump = false;
     step_index = jump_index = 0;
@@ -436,7 +468,7 @@ struct ErkLane {
       if (a.pid.has_dtmax) dt0 = jmin(dt0, a.pid.dtmax);
       if (a.pid.has_dtmin) dt0 = jmax(dt0, a.pid.dtmin);
       tnext = T0 + dt0;
-      cs_completed = 0; cs_num = 0; cs_t0 = T0; cs_t1 = T1;
+      cs_completed = 0;
       if (Extras && a.n_step_ts > 0) {
         const ClipTimes c{a.step_ts, a.n_step_ts, dir};
         step_index = c.find(T0);
@@ -448,20 +480,14 @@ struct ErkLane {
         tnext = c.clip(tnext, jump_index, true);
       }
     } else {
-      const double DT0 = a.dt0 * dir;
-      const double qn = ceil((T1 - T0) / nextup(DT0));
-      cs_num = __builtin_isfinite(T1) ? (int32_t)qn : -1;
-      cs_t1 = __builtin_isfinite(T1) ? T1 : DT0;
       cs_completed = 1;
-      cs_t0 = T0;
-      tnext = T0 + DT0;
+      tnext = T0 + a.dt0 * dir;
     }
     tnext = jmin(tnext, T1);
-    double fl = T1;
-    for (int r = 0; r < 100; ++r) fl = prevbefore(fl);
-    floor_t1 = fl;
     tprev = T0;
     num_steps = num_acc = num_rej = 0;
+    grid_base = 0;
+    ck_left = a.ckpt_every;
     result = DFX_SUCCESSFUL;
     if constexpr (Tab::fsal && !Extras) {
       // The first step's stage 0 (runge_kutta.py:687: z0 = y0 + 0 at t0 + 0*dt), evaluated
@@ -490,6 +516,90 @@ struct ErkLane {
     if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
   }
 
+  // ---- checkpoints of the discrete adjoint ----------------------------------------
+  // Record c (CW = D + 8 float64 words) is the lane state after c * ckpt_every accepted
+  // steps: y, tprev, tnext, the PID state, the end time of the last accepted step (tl,
+  // -inf for c = 0), then six int32 (at_dtmin | made_jump << 1, num_steps, num_rej,
+  // step_index, jump_index, cs_completed).  Together with the instance's inputs it
+  // restarts the step loop exactly where the forward was.
+  static constexpr int CW = D + 8;
+  __device__ void write_ckpt(const ErkKArgs& a, int c, double tl) const {
+    if (c >= a.n_ckpt) return;
+    double* r = a.ckpt + (traj * a.n_ckpt + c) * CW;
+#pragma unroll
+    for (int j = 0; j < D; ++j) r[j] = (double)y[j];
+    r[D] = tprev;
+    r[D + 1] = tnext;
+    r[D + 2] = pid.inv;
+    r[D + 3] = pid.pinv;
+    r[D + 4] = tl;
+    int32_t* ri = reinterpret_cast<int32_t*>(r + D + 5);
+    ri[0] = pid.at_dtmin | (made_jump ? 2 : 0);
+    ri[1] = num_steps;
+    ri[2] = num_rej;
+    ri[3] = step_index;
+    ri[4] = jump_index;
+    ri[5] = cs_completed;
+  }
+
+  // The per-instance set-up a replay needs (no initial step, no saves, no events).
+  __device__ void replay_setup(const ErkKArgs& a, int64_t id) {
+    tr.load();
+    traj = id;
+    load_params(a, id);
+    setup_span(a, a.t0, a.t1[id * a.t1_stride]);
+    save_index = 0;
+    ts_index = 0;
+    ts_row = a.t1;
+    next_ts = after_raw = __builtin_inf();
+    ev_mask = 0;
+  }
+
+  // Restore the state of checkpoint c of the forward solve (same Tab, VF, T, controller
+  // arithmetic and clipping, so the steps that follow repeat the forward's bit for bit);
+  // grid records of the steps that follow start at index 0.  Returns the end time of the
+  // accepted step before the checkpoint (-inf for c = 0).
+  __device__ double resume(const ErkKArgs& a, const double* ckpt, int c) {
+    const double* r = ckpt + (traj * a.n_ckpt + c) * CW;
+#pragma unroll
+    for (int j = 0; j < D; ++j) y[j] = T(r[j]);
+    tprev = r[D];
+    tnext = r[D + 1];
+    pid.inv = r[D + 2];
+    pid.pinv = r[D + 3];
+    const double tl = r[D + 4];
+    const int32_t* ri = reinterpret_cast<const int32_t*>(r + D + 5);
+    pid.at_dtmin = ri[0] & 1;
+    made_jump = (ri[0] >> 1) & 1;
+    num_steps = ri[1];
+    num_rej = ri[2];
+    step_index = ri[3];
+    jump_index = ri[4];
+    cs_completed = ri[5];
+    num_acc = c * a.ckpt_every;
+    grid_base = num_acc;
+    ck_left = a.ckpt_every;
+    result = DFX_SUCCESSFUL;
+    // FSAL: the first stage of the next step.  Without extras the forward evaluated it at
+    // set-up (c = 0, begin_times) or as the previous step's last stage, f(t1, y1) with
+    // t1 = tprev and y1 = y (SSAL); with extras a first step or one after a jump
+    // re-evaluates it inside step().
+    first = Extras && c == 0;
+    if constexpr (Tab::fsal) {
+      if (!(Extras && (first || made_jump))) {
+        if (c == 0) {
+          T z[D];
+#pragma unroll
+          for (int j = 0; j < D; ++j) z[j] = y[j] + T(0);
+          eval_f((tprev + 0.0 * (tnext - tprev)) * dir, z, fsal_f);
+        } else {
+          eval_f(tprev * dir, y, fsal_f);
+        }
+      }
+    }
+    return tl;
+  }
+
   __device__ void start(const ErkKArgs& a, int64_t id) {
     tr.load();
     traj = id;
@@ -498,6 +608,7 @@ struct ErkLane {
     for (int j = 0; j < D; ++j) y[j] = y0p[j];
     load_params(a, id);
     const double T0 = begin_times(a, a.t0, a.t1[id * a.t1_stride]);
+    if (a.ckpt) write_ckpt(a, 0, -__builtin_inf());
     save_index = 0;
     ts_index = 0;
     ts_row = a.ts ? a.ts + id * a.ts_stride : a.t1;
@@ -681,11 +792,12 @@ struct ErkLane {
 
     if (keep) {
       if (a.grid) {
-        double* g = a.grid + (traj * a.max_steps + (num_acc - 1)) * 2;
+        const int64_t rec = traj * a.grid_stride + (num_acc - 1 - grid_base);
+        double* g = a.grid + rec * 2;
         g[0] = ts0;
         g[1] = ts1;
-        if (a.grid_y) {  // the accepted step's start state, for the discrete adjoint
-          T* gy = reinterpret_cast<T*>(a.grid_y) + (traj * a.max_steps + (num_acc - 1)) * D;
+        if (a.grid_y) {  // the accepted step's start state (the discrete adjoint's replay)
+          T* gy = reinterpret_cast<T*>(a.grid_y) + rec * D;
 #pragma unroll
           for (int j = 0; j < D; ++j) gy[j] = y[j];
         }
@@ -728,6 +840,11 @@ struct ErkLane {
     }
     tprev = ntprev;
     tnext = ntnext;
+    if (a.ckpt && keep && --ck_left == 0) {
+      // every ckpt_every accepted steps: the discrete adjoint's restart point
+      ck_left = a.ckpt_every;
+      write_ckpt(a, num_acc / a.ckpt_every, ts1);
+    }
   }
 
   __device__ __forceinline__ void interp(double ts0, double ts1, const T (&y0)[D],

@@ -13,8 +13,8 @@ int erk_backsolve_f64(int solver, int vf, int d, const ErkKArgs& a, const Backso
                       hipStream_t s) {
   return dispatch_backsolve<double>(solver, vf, d, a, b, s);
 }
-int erk_adjoint_f64(int solver, int vf, int d, const ErkKArgs& a, const DiscreteAdjArgs& b,
-                    hipStream_t s) {
-  return dispatch_adjoint<double>(solver, vf, d, a, b, s);
+int erk_adjoint_f64(int solver, int vf, int d, const ErkKArgs& a, const ErkKArgs& ra,
+                    const DiscreteAdjArgs& b, hipStream_t s) {
+  return dispatch_adjoint<double>(solver, vf, d, a, ra, b, s);
 }
 }  // namespace dfx

@@ -71,6 +71,12 @@ typedef struct dfx_erk_problem {
    * direction-normalised time, as the reference evaluates cond_fn; event_root_find = 1
    * locates the event time on the triggering step's interpolant. */
   int32_t n_events, event_root_find;
+  /* RecursiveCheckpointAdjoint (_adjoint.py:264-329): K > 0 makes dfx_erk_solve store the
+   * lane state every K accepted steps in dfx_erk_io.ckpt (the restart points of
+   * dfx_erk_adjoint); 0 stores none.  The Python package uses
+   * K = ceil(max_steps / checkpoints), checkpoints = ceil(sqrt(max_steps)) by default. */
+  int32_t ckpt_every;
+  int32_t reserved_;    /* 0 */
 } dfx_erk_problem;
 
 /* ---- device buffers of one solve --------------------------------------------------- */
@@ -99,10 +105,12 @@ typedef struct dfx_erk_io {
                                  (0 either sign change, 1 upcrossing, 2 downcrossing)      */
   int32_t* event_mask;        /* out [B, n_events]: the condition that ended the solve    */
   void* grid_y;         /* optional out [B, max_steps, d] state dtype: each accepted step's
-                           start state (needs grid; input of dfx_erk_adjoint)          */
+                           start state (needs grid)                                    */
   const double* rtol;   /* optional [B]: per-instance PIDController rtol (a vmapped
                            tolerance); NULL uses dfx_erk_problem.rtol                  */
   const double* atol;   /* optional [B]: per-instance atol; NULL uses the problem's     */
+  double* ckpt;         /* out when ckpt_every > 0: [B, n_ckpt, words] (dfx_erk_ckpt_layout)
+                           lane state after 0, K, 2K, ... accepted steps; NULL otherwise */
 } dfx_erk_io;
 
 /* Launch shape (NULL: block 64, one lane per instance).  Fewer lanes than instances
@@ -164,31 +172,42 @@ typedef struct dfx_erk_backsolve_io {
  * Replaces: BacksolveAdjoint (_adjoint.py:539-873) with AdjointTerm (_term.py:767-917). */
 int dfx_erk_backsolve(const dfx_erk_problem* p, const dfx_erk_backsolve_io* io, void* stream);
 
-/* Discrete adjoint (RecursiveCheckpointAdjoint): buffers of a finished dfx_erk_solve run
- * with grid and grid_y, and the cotangents of its saved rows. */
+/* Discrete adjoint (RecursiveCheckpointAdjoint): the checkpoints and counters of a
+ * finished dfx_erk_solve run with ckpt_every > 0, and the cotangents of its saved rows. */
 typedef struct dfx_erk_adjoint_io {
-  const void* params;      /* as given to dfx_erk_solve (state dtype)                        */
-  const double* t1;        /* as given to dfx_erk_solve                                      */
-  const double* ts;        /* as given to dfx_erk_solve (NULL when n_ts == 0)                */
-  const double* grid;      /* [B, max_steps, 2] written by dfx_erk_solve                     */
-  const void* grid_y;      /* [B, max_steps, d] written by dfx_erk_solve                     */
+  const double* ckpt;      /* [B, n_ckpt, words] written by dfx_erk_solve                    */
   const int32_t* stats;    /* [B, 3] written by dfx_erk_solve                                */
   const void* ys_bar;      /* [B, n_out, d] cotangents of the saved rows                     */
   void* y0_bar;            /* out [B, d]: dL/dy0                                             */
   void* params_bar;        /* out [B, P]: dL/dparams per instance (sum them if shared)       */
 } dfx_erk_adjoint_io;
 
+/* Checkpoint layout of a problem with ckpt_every = K > 0: n_ckpt = ceil(max_steps / K)
+ * records per instance of `words` float64 words (d + 8: y, t_prev, t_next, the PID state,
+ * the end of the previous accepted step and six int32 counters). */
+int dfx_erk_ckpt_layout(const dfx_erk_problem* p, int32_t* n_ckpt, int32_t* words);
+/* Workspace of dfx_erk_adjoint (bytes, 256-aligned): a K-record scratch per instance
+ * holding one replayed segment's accepted steps, (t_start, t_end) float64 and the start
+ * state. */
+int dfx_erk_adjoint_workspace_bytes(const dfx_erk_problem* p, size_t* bytes);
+
 /* Gradients of a loss of the saved rows with respect to y0 and the vector-field
- * parameters, by reverse mode through the forward's accepted steps: per accepted step,
- * newest first, the stages are recomputed from the recorded start state and transposed,
- * together with the interpolation of the rows saved inside the step.  Step sizes are
- * constants of the gradient and rejected steps contribute nothing, as in the reference.
- * p is the forward problem (LV, VdP, linear with d <= 4; no events).
+ * parameters, by reverse mode through the forward's accepted steps.  Segment by segment,
+ * newest first, the forward step loop is restarted from the segment's checkpoint and
+ * repeats its accepted steps (rejected ones included) bit for bit; then, per accepted
+ * step, newest first, the stages are recomputed from the replayed start state and
+ * transposed, together with the interpolation of the rows saved inside the step.  Step
+ * sizes are constants of the gradient and rejected steps contribute nothing, as in the
+ * reference.  p and fwd are the forward's problem and buffers (its inputs are read: t1,
+ * ts, params, rtol / atol, step_ts / jump_ts; its outputs are not touched); p: LV, VdP,
+ * linear with d <= 4, no events, no dense output, ckpt_every > 0.
  * Replaces: RecursiveCheckpointAdjoint.loop (_adjoint.py:264-312) differentiating the
  * accepted steps of _integrate.py:302-662 (AbstractRungeKutta.step,
  * _solver/runge_kutta.py:446-1203, and the save interpolation, _integrate.py:456-491),
  * with the stop-gradient sites of pid.py:374, 498-530. */
-int dfx_erk_adjoint(const dfx_erk_problem* p, const dfx_erk_adjoint_io* io, void* stream);
+int dfx_erk_adjoint(const dfx_erk_problem* p, const dfx_erk_io* fwd,
+                    const dfx_erk_adjoint_io* io, void* workspace, size_t workspace_bytes,
+                    void* stream);
 
 /* Query side of SaveAt(dense=True): buffers a dfx_erk_solve wrote, and query times. */
 typedef struct dfx_erk_dense_io {

@@ -130,22 +130,50 @@ def test_backsolve_rejects_bad_arguments_without_launching():
 def test_adjoint_rejects_bad_arguments_without_launching():
     lib = _lib.load()
     z = ctypes.c_void_p(8)    # never dereferenced: every case fails validation first
-    io = _lib.AdjointIO(params=z, t1=z, ts=z, grid=z, grid_y=z, stats=z, ys_bar=z, y0_bar=z,
-                        params_bar=z)
+    fwd = _lib.ErkIO(y0=z, t1=z, ts=z, params=z, ys=z, ts_out=z, stats=z, result=z)
+    io = _lib.AdjointIO(ckpt=z, stats=z, ys_bar=z, y0_bar=z, params_bar=z)
 
     def prob(**kw):
         base = dict(solver=_lib.TSIT5, controller=_lib.PID, vf=_lib.VF_LV, precision=_lib.F64,
-                    d=2, batch=4, max_steps=10, save_t1=1)
+                    d=2, batch=4, max_steps=10, save_t1=1, ckpt_every=4)
         base.update(kw)
         return _lib.ErkProblem(**base)
 
-    call = lambda p, i=io: lib.dfx_erk_adjoint(ctypes.byref(p), ctypes.byref(i), None)
+    def call(p, i=io, f=fwd, ws=z, nbytes=1 << 20):
+        return lib.dfx_erk_adjoint(ctypes.byref(p), ctypes.byref(f), ctypes.byref(i), ws,
+                                   ctypes.c_size_t(nbytes), None)
+
     for bad in (dict(vf=_lib.VF_LINEAR, d=10), dict(vf=_lib.VF_MLP, d=64), dict(vf=_lib.VF_VDP, d=3),
-                dict(n_events=1), dict(precision=_lib.BF16_F32), dict(solver=9), dict(batch=-1)):
-        assert call(prob(**bad)) == -1, bad
-    missing = _lib.AdjointIO(params=z, t1=z, stats=z, ys_bar=z, y0_bar=z, params_bar=z)
-    assert call(prob(), missing) == -1                     # no recorded grid
-    assert call(prob(batch=0), _lib.AdjointIO()) == 0      # empty batch: nothing to do
+                dict(n_events=1), dict(precision=_lib.BF16_F32), dict(solver=9), dict(batch=-1),
+                dict(ckpt_every=0), dict(ckpt_every=-2), dict(n_ts=3, ts_stride=0)):
+        f = _lib.ErkIO(t1=z, params=z) if "n_ts" in bad else fwd   # n_ts > 0 without ts
+        assert call(prob(**bad), f=f) == -1, bad
+    assert call(prob(), _lib.AdjointIO(stats=z, ys_bar=z, y0_bar=z, params_bar=z)) == -1  # no ckpt
+    assert call(prob(), f=_lib.ErkIO(t1=z)) == -1             # forward inputs missing
+    nbytes = ctypes.c_size_t(0)
+    assert lib.dfx_erk_adjoint_workspace_bytes(ctypes.byref(prob()), ctypes.byref(nbytes)) == 0
+    assert nbytes.value == 256 + 256       # [4, 4, 2] + [4, 4, 2] float64, each 256-aligned
+    assert call(prob(), nbytes=nbytes.value - 1) == -3      # workspace too small
+    assert call(prob(), ws=None) == -3
+    assert call(prob(batch=0), _lib.AdjointIO(), _lib.ErkIO()) == 0   # empty batch
+    n_ck, words = ctypes.c_int32(0), ctypes.c_int32(0)
+    assert lib.dfx_erk_ckpt_layout(ctypes.byref(prob(max_steps=10, ckpt_every=4)),
+                                   ctypes.byref(n_ck), ctypes.byref(words)) == 0
+    assert (n_ck.value, words.value) == (3, 10)             # ceil(10 / 4) records of d + 8
+    assert lib.dfx_erk_ckpt_layout(ctypes.byref(prob(ckpt_every=0)), ctypes.byref(n_ck),
+                                   ctypes.byref(words)) == -1
+
+
+def test_solve_needs_checkpoint_buffer():
+    """ckpt_every > 0 needs dfx_erk_io.ckpt (rejected before any launch)."""
+    lib = _lib.load()
+    z = ctypes.c_void_p(8)
+    p = _lib.ErkProblem(solver=_lib.TSIT5, controller=_lib.PID, vf=_lib.VF_LV,
+                        precision=_lib.F64, d=2, batch=4, max_steps=10, save_t1=1,
+                        ckpt_every=4)
+    io = _lib.ErkIO(y0=z, t1=z, ts=z, params=z, ys=z, ts_out=z, stats=z, result=z)
+    assert lib.dfx_erk_solve(ctypes.byref(p), ctypes.byref(io), z, ctypes.c_size_t(256), None,
+                             None) == -1
 
 
 def test_backsolve_host_guards():

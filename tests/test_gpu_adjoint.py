@@ -1,13 +1,15 @@
 """GPU tests of the discrete adjoint of the small-system solve (the default
 RecursiveCheckpointAdjoint, _adjoint.py:264-312): torch.autograd through diffeqsolve,
-computed by dfx_erk_adjoint from the accepted steps the forward recorded.
+computed by dfx_erk_adjoint, which replays each segment of accepted steps from the
+forward's checkpoints and sweeps it in reverse.
 
 - float64: y0 and args gradients bit-identical to the oracle's discrete adjoint
   (oracle.adjoint) on the oracle's own recorded steps, for every field the kernel covers
   and each solver, with SaveAt t0 + ts + t1, steps, per-instance save times, reverse time,
   step clipping and a zero-length solve (the oracle is pinned against torch autograd of an
-  independent replay and against closed forms on the CPU);
-- the recorded grid and start states equal the oracle's bit for bit;
+  independent replay and against closed forms on the CPU), for checkpoint spacings from
+  one accepted step to the whole solve;
+- the checkpoints hold the oracle's recorded steps bit for bit (start state, times);
 - shared args: the per-instance gradients summed over the batch (1e-13, torch's order);
 - float32-state form within 2e-4 relative of the float64 oracle;
 - the reference's test_against recipe (test_adjoint.py:34-214) against the closed form at
@@ -50,13 +52,13 @@ def _args(field, qt, d):
 
 
 def _run(field, solver, y0, q, w_fn, *, t0=0.0, t1=3.0, dt0=0.05, saveat, ctrl,
-         dtype=torch.float64, cuda=None):
+         dtype=torch.float64, cuda=None, adjoint=dfx.RecursiveCheckpointAdjoint()):
     cls, d = FIELDS[field]
     y0t = torch.tensor(y0, dtype=dtype, device=cuda, requires_grad=True)
     qt = torch.tensor(q, dtype=dtype, device=cuda, requires_grad=True)
     sol = dfx.diffeqsolve(dfx.ODETerm(cls()), SOLVERS[solver](), t0, t1, dt0, y0t,
                           _args(field, qt, d), saveat=saveat, stepsize_controller=ctrl,
-                          max_steps=2000)
+                          max_steps=2000, adjoint=adjoint)
     w = w_fn(tuple(sol.ys.shape))
     wt = torch.tensor(w, dtype=dtype, device=cuda)
     gy, gq = torch.autograd.grad((sol.ys * torch.nan_to_num(wt)).sum(), (y0t, qt))
@@ -91,7 +93,7 @@ def test_bitwise_against_oracle(cuda, field, solver, save):
     assert np.array_equal(gq, qb)
 
 
-def test_recorded_steps_match_oracle(cuda):
+def test_checkpoints_match_oracle_steps(cuda):
     rng = np.random.default_rng(1)
     _, d, y0, q = _case("vdp", 64, rng)
     ts = np.linspace(0.0, 3.0, 7)
@@ -99,7 +101,10 @@ def test_recorded_steps_match_oracle(cuda):
         dfx.ODETerm(dfx.VanDerPol()), dfx.Tsit5(), 0.0, 3.0, 0.05,
         torch.tensor(y0), torch.tensor(q[:, 0]), dfx.SaveAt(ts=ts),
         dfx.PIDController(rtol=1e-6, atol=1e-8), 2000, None)
-    plan.record_steps()
+    plan.record_grid()
+    plan.record_checkpoints(checkpoints=400)     # every 5 accepted steps
+    K = int(plan.problem.ckpt_every)
+    assert K == 5 and tuple(plan.t["ckpt"].shape) == (64, 400, d + 8)
     plan.launch()
     torch.cuda.synchronize()
     fwd = oracle.solve(y0, 3.0, q, ts=ts, record_grid=True, solver="tsit5", vf="vdp",
@@ -108,12 +113,49 @@ def test_recorded_steps_match_oracle(cuda):
     assert np.array_equal(n, fwd["stats"][:, 1])
     g = plan.t["grid"].cpu().numpy()
     gy = plan.t["grid_y"].cpu().numpy()
+    ck = plan.t["ckpt"].cpu().numpy()
     for b in range(64):
         assert np.array_equal(g[b, :n[b]], fwd["grid"][b, :n[b]])
         assert np.array_equal(gy[b, :n[b]], fwd["grid_y"][b, :n[b]])
+        for c in range((n[b] - 1) // K + 1):
+            s = c * K
+            assert np.array_equal(ck[b, c, :d], fwd["grid_y"][b, s])      # start state
+            assert ck[b, c, d] == fwd["grid"][b, s, 0]                   # t_prev
+            tl = fwd["grid"][b, s - 1, 1] if c > 0 else -np.inf
+            assert ck[b, c, d + 4] == tl                                 # previous step's end
+        ints = ck[b, :, d + 5:].copy().view(np.int32)
+        assert (ints[0, :] == 0).all()    # no steps, no rejections, indices 0 at c = 0
 
 
-def test_reverse_time_per_instance_ts_clip(cuda):
+@pytest.mark.parametrize("checkpoints", [1, 3, 45, 700, 2000])
+def test_checkpoint_spacing_bitwise(cuda, checkpoints):
+    """Segments of 2,000 (one segment), 667, 45, 3 and 1 accepted steps: the same
+    gradients bit for bit."""
+    rng = np.random.default_rng(21)
+    B = 64
+    _, d, y0, q = _case("vdp", B, rng)
+    ts = np.array([0.0, 0.7, 1.9, 3.0])
+    y0t = torch.tensor(y0, device=cuda, requires_grad=True)
+    qt = torch.tensor(q, device=cuda, requires_grad=True)
+    sol = dfx.diffeqsolve(dfx.ODETerm(dfx.VanDerPol()), dfx.Dopri5(), 0.0, 3.0, 0.05, y0t,
+                          qt[:, 0], saveat=dfx.SaveAt(t0=True, ts=ts, t1=True),
+                          stepsize_controller=dfx.PIDController(rtol=1e-7, atol=1e-9),
+                          adjoint=dfx.RecursiveCheckpointAdjoint(checkpoints=checkpoints),
+                          max_steps=2000)
+    w = rng.normal(size=tuple(sol.ys.shape))
+    gy, gq = torch.autograd.grad((sol.ys * torch.tensor(w, device=cuda)).sum(), (y0t, qt))
+    kw = dict(solver="dopri5", vf="vdp", dt0=0.05, rtol=1e-7, atol=1e-9, max_steps=2000,
+              save_t0=True, save_t1=True)
+    fwd = oracle.solve(y0, 3.0, q, ts=ts, record_grid=True, **kw)
+    yb, qb = oracle.adjoint(3.0, q, fwd, w, ts=ts, **kw)
+    assert np.array_equal(gy.cpu().numpy(), yb)
+    assert np.array_equal(gq.cpu().numpy(), qb)
+
+
+@pytest.mark.parametrize("checkpoints", [None, 2000])
+def test_reverse_time_per_instance_ts_clip(cuda, checkpoints):
+    """Step clipping and jumps; with 2,000 checkpoints every accepted step is a restart
+    point, so restarts right after a jump (made_jump) are covered."""
     rng = np.random.default_rng(12)
     B = 64
     _, d, y0, q = _case("lv", B, rng)
@@ -121,7 +163,8 @@ def test_reverse_time_per_instance_ts_clip(cuda):
     ctrl = dfx.ClipStepSizeController(dfx.PIDController(rtol=1e-6, atol=1e-8),
                                       step_ts=[1.0, 2.0, 3.0], jump_ts=[2.5])
     sol, w, gy, gq = _run("lv", "dopri5", y0, q, lambda s: rng.normal(size=s), t0=5.0, t1=0.0,
-                          dt0=-0.1, saveat=dfx.SaveAt(ts=ts, t1=True), ctrl=ctrl, cuda=cuda)
+                          dt0=-0.1, saveat=dfx.SaveAt(ts=ts, t1=True), ctrl=ctrl, cuda=cuda,
+                          adjoint=dfx.RecursiveCheckpointAdjoint(checkpoints=checkpoints))
     kw = dict(solver="dopri5", vf="lv", t0=5.0, dt0=-0.1, rtol=1e-6, atol=1e-8, save_t1=True,
               step_ts=[1.0, 2.0, 3.0], jump_ts=[2.5], max_steps=2000)
     fwd = oracle.solve(y0, 0.0, q, ts=ts, record_grid=True, **kw)
