@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
   const int64_t traj = (int64_t)blockIdx.x * MLP_TILE + orow;
   const bool valid = traj < a.B;
 
-  TabRegs<Tab> tr;
+  TabRegs<Tab, float> tr;  // float32 stage algebra: the coefficients' float32 values
   tr.load();
 
   // ---- set-up (diffeqsolve / PIDController.init) ----
@@ -251,6 +251,27 @@ __global__ void mlp_pack_kernel(const float* __restrict__ W, typename P::act_t* 
   out[e] = P::to_act(v);
 }
 
+// Three-piece bf16 packing (PolicyBF16x3): fragment element e of plane p holds piece p of
+// split3(v); the planes are `plane` elements apart.
+__global__ void mlp_pack3_kernel(const float* __restrict__ W, __bf16* __restrict__ out, int N,
+                                 int K, int forward, int64_t plane) {
+  using P = PolicyBF16x3;
+  const int64_t total = (int64_t)N * K;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int NKS = K / P::KSTEP;
+  const int j = (int)(e % P::KA);
+  const int64_t fr = e / P::KA;
+  const int lane = (int)(fr % 64);
+  const int64_t f2 = fr / 64;
+  const int ks = (int)(f2 % NKS);
+  const int nt = (int)(f2 / NKS);
+  const int n = nt * 32 + (lane & 31);
+  const int k = ks * P::KSTEP + (lane >> 5) * P::KA + j;
+  const float v = forward ? W[(int64_t)n * K + k] : W[(int64_t)k * N + n];
+  split3(v, out[e], out[e + plane], out[e + 2 * plane]);
+}
+
 }  // namespace dfx
 
 using namespace dfx;
@@ -259,13 +280,8 @@ namespace {
 
 size_t act_bytes(int32_t precision) { return precision == DFX_BF16_F32 ? 2 : 4; }
 
-// element offsets of the six packed matrices inside one packed buffer
-void pack_offsets(int64_t (&off)[7]) {
-  const int64_t D = MLP_D, H = MLP_H;
-  const int64_t sz[6] = {H * D, H * H, D * H, D * H, H * H, H * D};
-  off[0] = 0;
-  for (int i = 0; i < 6; ++i) off[i + 1] = off[i] + sz[i];
-}
+// bf16 buffers hold the three split3 pieces of every weight, float32 buffers one copy
+int pieces_of(int32_t precision) { return precision == DFX_BF16_F32 ? 3 : 1; }
 
 int mlp_check(const dfx_mlp_problem* p) {
   if (!p) return DFX_E_ARG;
@@ -278,14 +294,7 @@ int mlp_check(const dfx_mlp_problem* p) {
 
 template <class P>
 MlpWeights<P> weights_from(const void* packed, const float* b1, const float* b2, const float* b3) {
-  int64_t off[7];
-  pack_offsets(off);
-  const typename P::act_t* base = reinterpret_cast<const typename P::act_t*>(packed);
-  MlpWeights<P> W;
-  W.w1 = base + off[0]; W.w2 = base + off[1]; W.w3 = base + off[2];
-  W.w1t = base + off[3]; W.w2t = base + off[4]; W.w3t = base + off[5];
-  W.b1 = b1; W.b2 = b2; W.b3 = b3;
-  return W;
+  return mlp_weights_from<P>(packed, P::PIECES, b1, b2, b3);
 }
 
 }  // namespace
@@ -294,9 +303,7 @@ extern "C" {
 
 int dfx_mlp_pack_bytes(int32_t precision, size_t* bytes) {
   if (!bytes || (precision != DFX_F32 && precision != DFX_BF16_F32)) return DFX_E_ARG;
-  int64_t off[7];
-  pack_offsets(off);
-  *bytes = (size_t)off[6] * act_bytes(precision);
+  *bytes = (size_t)mlp_pack_layout(pieces_of(precision)).total * act_bytes(precision);
   return DFX_OK;
 }
 
@@ -304,8 +311,7 @@ int dfx_mlp_pack(int32_t precision, const float* w1, const float* w2, const floa
                  void* stream) {
   if (!w1 || !w2 || !w3 || !packed) return DFX_E_ARG;
   if (precision != DFX_F32 && precision != DFX_BF16_F32) return DFX_E_ARG;
-  int64_t off[7];
-  pack_offsets(off);
+  const MlpPackLayout lay = mlp_pack_layout(pieces_of(precision));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   struct Job { const float* W; int N, K, fwd; };
   const Job jobs[6] = {{w1, MLP_H, MLP_D, 1}, {w2, MLP_H, MLP_H, 1}, {w3, MLP_D, MLP_H, 1},
@@ -314,11 +320,11 @@ int dfx_mlp_pack(int32_t precision, const float* w1, const float* w2, const floa
     const int64_t n = (int64_t)jobs[i].N * jobs[i].K;
     const unsigned nb = (unsigned)((n + 255) / 256);
     if (precision == DFX_BF16_F32) {
-      __bf16* o = reinterpret_cast<__bf16*>(packed) + off[i];
-      hipLaunchKernelGGL(mlp_pack_kernel<PolicyBF16>, dim3(nb), dim3(256), 0, s, jobs[i].W, o,
-                         jobs[i].N, jobs[i].K, jobs[i].fwd);
+      __bf16* o = reinterpret_cast<__bf16*>(packed) + lay.off[i];
+      hipLaunchKernelGGL(mlp_pack3_kernel, dim3(nb), dim3(256), 0, s, jobs[i].W, o, jobs[i].N,
+                         jobs[i].K, jobs[i].fwd, lay.size[i]);
     } else {
-      float* o = reinterpret_cast<float*>(packed) + off[i];
+      float* o = reinterpret_cast<float*>(packed) + lay.off[i];
       hipLaunchKernelGGL(mlp_pack_kernel<PolicyF32>, dim3(nb), dim3(256), 0, s, jobs[i].W, o,
                          jobs[i].N, jobs[i].K, jobs[i].fwd);
     }
@@ -335,11 +341,11 @@ int dfx_mlp_eval(int32_t precision, const void* packed, const float* b1, const f
   const unsigned nb = (unsigned)((rows + MLP_TILE - 1) / MLP_TILE);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if (precision == DFX_BF16_F32) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_eval_kernel<PolicyBF16>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<PolicyBF16>));
-    hipLaunchKernelGGL(mlp_eval_kernel<PolicyBF16>, dim3(nb), dim3(MLP_THREADS),
-                       sizeof(MlpLds<PolicyBF16>), s, y, f, rows,
-                       weights_from<PolicyBF16>(packed, b1, b2, b3));
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_eval_kernel<PolicyBF16x3>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<PolicyBF16x3>));
+    hipLaunchKernelGGL(mlp_eval_kernel<PolicyBF16x3>, dim3(nb), dim3(MLP_THREADS),
+                       sizeof(MlpLds<PolicyBF16x3>), s, y, f, rows,
+                       weights_from<PolicyBF16x3>(packed, b1, b2, b3));
   } else {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_eval_kernel<PolicyF32>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<PolicyF32>));
@@ -398,7 +404,7 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   } while (0)
   const bool g = p->ckpt_every > 0;
   if (p->precision == DFX_BF16_F32) {
-    if (g) DFX_MLP_LAUNCH(PolicyBF16, true); else DFX_MLP_LAUNCH(PolicyBF16, false);
+    if (g) DFX_MLP_LAUNCH(PolicyBF16x3, true); else DFX_MLP_LAUNCH(PolicyBF16x3, false);
   } else {
     if (g) DFX_MLP_LAUNCH(PolicyF32, true); else DFX_MLP_LAUNCH(PolicyF32, false);
   }

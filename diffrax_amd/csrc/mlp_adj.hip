@@ -427,21 +427,12 @@ int64_t seg_chunks(int64_t B, int32_t seg_steps) {
   return (seg_rows(B, seg_steps) + ADJ_CHUNK - 1) / ADJ_CHUNK;
 }
 
+// The adjoint's products take plane 0 of a bf16 buffer (the bf16-rounded weights; the
+// buffer holds three split3 planes for the forward) and the single plane of a float32 one.
 template <class P>
-MlpWeights<P> weights_of(const void* packed, const float* b1, const float* b2, const float* b3) {
-  const int64_t D = MLP_D, H = MLP_H;
-  const int64_t sz[6] = {H * D, H * H, D * H, D * H, H * H, H * D};
-  const typename P::act_t* base = reinterpret_cast<const typename P::act_t*>(packed);
-  MlpWeights<P> W;
-  int64_t o = 0;
-  W.w1 = base + o; o += sz[0];
-  W.w2 = base + o; o += sz[1];
-  W.w3 = base + o; o += sz[2];
-  W.w1t = base + o; o += sz[3];
-  W.w2t = base + o; o += sz[4];
-  W.w3t = base + o;
-  W.b1 = b1; W.b2 = b2; W.b3 = b3;
-  return W;
+MlpWeights<P> weights_of(const void* packed, int32_t precision, const float* b1, const float* b2,
+                         const float* b3) {
+  return mlp_weights_from<P>(packed, precision == DFX_BF16_F32 ? 3 : 1, b1, b2, b3);
 }
 
 template <class P>
@@ -457,7 +448,7 @@ int run_segment(const dfx_mlp_problem* p, const void* packed, const float* b1, c
   a.Cs = seg_steps; a.seg = seg; a.n_ts = p->n_ts;
   a.n_out = (p->save_t0 ? 1 : 0) + p->n_ts + (p->save_t1 ? 1 : 0);
   a.save_t0 = p->save_t0; a.t0 = p->t0; a.t1 = p->t1;
-  const MlpWeights<P> W = weights_of<P>(packed, b1, b2, b3);
+  const MlpWeights<P> W = weights_of<P>(packed, p->precision, b1, b2, b3);
   RowBufs<P> R;
   const int64_t rows_alloc = seg_chunks(p->batch, seg_steps) * ADJ_CHUNK;
   typename P::act_t* base = reinterpret_cast<typename P::act_t*>(io->rows);

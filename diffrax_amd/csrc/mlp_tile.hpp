@@ -41,6 +41,7 @@ struct PolicyBF16 {
   // estimate then measures rounding noise instead of truncation error.  Each layer costs two
   // bfloat16 MFMA passes (float32 accumulate); the weights themselves are bfloat16.
   static constexpr bool split_input = true;
+  static constexpr int PIECES = 1;  // weight planes read (plane 0 of a three-piece buffer)
   __device__ __forceinline__ static frag_t load_a(const act_t* p) {
     return *reinterpret_cast<const frag_t*>(p);
   }
@@ -64,6 +65,7 @@ struct PolicyF32 {
   // and the forward runs 2x faster, its VALU work having been the larger part of a step.
   static constexpr bool fast_math = true;
   static constexpr bool split_input = false;
+  static constexpr int PIECES = 1;
   __device__ __forceinline__ static frag_t load_a(const act_t* p) { return *p; }
   __device__ __forceinline__ static frag_t load_b(const act_t* packed, int frag) {
     return packed[frag];
@@ -73,6 +75,46 @@ struct PolicyF32 {
   }
   __device__ __forceinline__ static act_t to_act(float x) { return x; }
 };
+
+// BF16 MFMA (float32 accumulate) with every operand carried as three bfloat16 pieces,
+// v = p0 + p1 + p2 exactly for a float32 v (split3).  The six products p_i q_j with
+// i + j <= 2 give each layer sum to ~2^-24 relative, float32 MFMA accuracy, from six bf16
+// MFMA passes: the float32-weight model itself is evaluated (the weights are split, not
+// rounded), and the products of the leading pieces accumulate apart from the five
+// correction products.  bf16 MFMA has 16x the float32 MFMA rate on gfx950.
+struct PolicyBF16x3 : PolicyBF16 {
+  static constexpr bool split_input = false;
+  static constexpr int PIECES = 3;
+};
+
+// v = p0 + p1 + p2 with bfloat16 pieces: each residual is exact in float32 and the last
+// one has at most 8 significant bits.
+__device__ __forceinline__ void split3(float v, __bf16& p0, __bf16& p1, __bf16& p2) {
+  p0 = (__bf16)v;
+  const float r1 = v - (float)p0;
+  p1 = (__bf16)r1;
+  p2 = (__bf16)(r1 - (float)p1);
+}
+
+// Packed weight buffer: six matrices (w1, w2, w3 forward; w1t, w2t, w3t transposed), each
+// `pieces` planes of its fragment array (bf16: the three split3 pieces, plane 0 the
+// rounded weight; float32: one plane).  Element offsets of plane 0 and plane sizes.
+struct MlpPackLayout {
+  int64_t off[6], size[6], total;
+};
+__host__ __device__ inline MlpPackLayout mlp_pack_layout(int pieces) {
+  const int64_t D = MLP_D, H = MLP_H;
+  const int64_t sz[6] = {H * D, H * H, D * H, D * H, H * H, H * D};
+  MlpPackLayout l;
+  int64_t o = 0;
+  for (int i = 0; i < 6; ++i) {
+    l.off[i] = o;
+    l.size[i] = sz[i];
+    o += sz[i] * pieces;
+  }
+  l.total = o;
+  return l;
+}
 
 // Packed weights of one solve.  Forward fragments: B[k][n] = W[n][k].  Backward fragments
 // (transposed products): B[k][n] = W[k][n].
@@ -87,7 +129,24 @@ struct MlpWeights {
   const float* b1;
   const float* b2;
   const float* b3;
+  int64_t plane1, plane2, plane3;  // elements between the pieces of w1, w2, w3 (PIECES > 1)
 };
+
+// The weights of a packed buffer (mlp_pack_layout) for a policy; a one-piece policy on a
+// three-piece buffer reads plane 0.
+template <class P>
+__host__ __device__ inline MlpWeights<P> mlp_weights_from(const void* packed, int pieces,
+                                                          const float* b1, const float* b2,
+                                                          const float* b3) {
+  const MlpPackLayout l = mlp_pack_layout(pieces);
+  const typename P::act_t* base = reinterpret_cast<const typename P::act_t*>(packed);
+  MlpWeights<P> W;
+  W.w1 = base + l.off[0]; W.w2 = base + l.off[1]; W.w3 = base + l.off[2];
+  W.w1t = base + l.off[3]; W.w2t = base + l.off[4]; W.w3t = base + l.off[5];
+  W.b1 = b1; W.b2 = b2; W.b3 = b3;
+  W.plane1 = l.size[0]; W.plane2 = l.size[1]; W.plane3 = l.size[2];
+  return W;
+}
 
 template <class P>
 struct MlpLds {
@@ -272,6 +331,155 @@ __device__ __forceinline__ void mlp_forward(const MlpWeights<P>& W, MlpLds<P>& L
     s += L.part[(2 * MLP_TILE + owner_row) * L.LDP + c];
     s += L.part[(3 * MLP_TILE + owner_row) * L.LDP + c];
     fout[j] = tanh_f<F>(s + W.b3[c]);
+  }
+}
+
+// ---- the three-piece bf16 form --------------------------------------------------------
+// LDS: the three pieces of the layer-1 input and of both hidden activations (115 KB).  The
+// split-K partials of layer 3 reuse the h1 planes, which nobody reads after layer 2.
+template <>
+struct MlpLds<PolicyBF16x3> {
+  static constexpr int LDX = MLP_D + PolicyBF16x3::PAD;
+  static constexpr int LDH = MLP_H + PolicyBF16x3::PAD;
+  static constexpr int LDP = MLP_D + 4;
+  __bf16 x[3][MLP_TILE * LDX];
+  __bf16 h1[3][MLP_TILE * LDH];
+  __bf16 h2[3][MLP_TILE * LDH];
+  __device__ __forceinline__ float* part() { return reinterpret_cast<float*>(&h1[0][0]); }
+};
+static_assert(4 * MLP_TILE * MlpLds<PolicyBF16x3>::LDP * sizeof(float) <=
+                  sizeof(MlpLds<PolicyBF16x3>::h1),
+              "layer-3 partials fit in the h1 planes");
+
+template <>
+__device__ __forceinline__ void store_input<PolicyBF16x3>(MlpLds<PolicyBF16x3>& L, int r, int c,
+                                                          float v) {
+  const int i = r * L.LDX + c;
+  split3(v, L.x[0][i], L.x[1][i], L.x[2][i]);
+}
+
+// acc[nt] (32x32 f32) = sum over k-steps [ks0, ks1) of A . B with A and B three-piece
+// operands: A from LDS planes a0..a2, B fragment (nt0 + nt, ks) from planes wp + p * plane.
+// The leading products a0 b0 accumulate in `big`, the five corrections (a0 b1, a1 b0, a0 b2,
+// a1 b1, a2 b0) in `sml`, added at the end.  B fragments are loaded PF k-steps ahead.
+template <int NKS, int NT, int PF = 1>
+__device__ __forceinline__ void wave_gemm3(const __bf16* a0, const __bf16* a1, const __bf16* a2,
+                                           int lda, const __bf16* wp, int64_t plane, int nt0,
+                                           int ks0, int ks1, floatx16_t (&acc)[NT], int lane) {
+  using P = PolicyBF16x3;
+  using frag_t = P::frag_t;
+  const int row = lane & 31;
+  const int kh = (lane >> 5) * P::KA;
+  floatx16_t big[NT], sml[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) { big[t] = floatx16_t{}; sml[t] = floatx16_t{}; }
+  auto bfrag = [&](int pc, int t, int k) {
+    return P::load_b(wp + pc * plane, ((nt0 + t) * NKS + k) * 64 + lane);
+  };
+  frag_t buf[PF][3][NT];
+#pragma unroll
+  for (int p = 0; p < PF; ++p) {
+    const int kp = (ks0 + p < ks1) ? ks0 + p : ks1 - 1;
+#pragma unroll
+    for (int pc = 0; pc < 3; ++pc) {
+#pragma unroll
+      for (int t = 0; t < NT; ++t) buf[p][pc][t] = bfrag(pc, t, kp);
+    }
+  }
+  // one PF-step group per iteration: a fully unrolled K loop (seven inlined evaluations per
+  // step) lets the scheduler hoist loads until the registers spill
+#pragma unroll 1
+  for (int ks = ks0; ks < ks1; ks += PF) {
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      const int k = ks + p;
+      if (k < ks1) {
+        frag_t b[3][NT];
+#pragma unroll
+        for (int pc = 0; pc < 3; ++pc) {
+#pragma unroll
+          for (int t = 0; t < NT; ++t) b[pc][t] = buf[p][pc][t];
+        }
+        if (k + PF < ks1) {
+#pragma unroll
+          for (int pc = 0; pc < 3; ++pc) {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) buf[p][pc][t] = bfrag(pc, t, k + PF);
+          }
+        }
+        const int ao = row * lda + k * P::KSTEP + kh;
+        const frag_t x0 = P::load_a(a0 + ao), x1 = P::load_a(a1 + ao), x2 = P::load_a(a2 + ao);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) big[t] = P::mfma(x0, b[0][t], big[t]);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          sml[t] = P::mfma(x0, b[1][t], sml[t]);
+          sml[t] = P::mfma(x1, b[0][t], sml[t]);
+          sml[t] = P::mfma(x0, b[2][t], sml[t]);
+          sml[t] = P::mfma(x1, b[1][t], sml[t]);
+          sml[t] = P::mfma(x2, b[0][t], sml[t]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = big[t] + sml[t];
+}
+
+template <>
+__device__ __forceinline__ void mlp_forward<PolicyBF16x3>(const MlpWeights<PolicyBF16x3>& W,
+                                                          MlpLds<PolicyBF16x3>& L, int wave,
+                                                          int lane, int owner_row, int owner_c0,
+                                                          float (&fout)[8]) {
+  constexpr int KS1 = MLP_D / 16, KS2 = MLP_H / 16;
+  floatx16_t acc[2];
+  // layer 1: [32 x 64] . W1^T -> [32 x 256], wave's columns [64w, 64w + 64)
+  wave_gemm3<KS1, 2>(L.x[0], L.x[1], L.x[2], L.LDX, W.w1, W.plane1, 2 * wave, 0, KS1, acc, lane);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int col = 64 * wave + 32 * t + (lane & 31);
+    const float bias = W.b1[col];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int i = acc_row(lane, v) * L.LDH + col;
+      split3(softplus_f<true>(acc[t][v] + bias), L.h1[0][i], L.h1[1][i], L.h1[2][i]);
+    }
+  }
+  __syncthreads();
+  // layer 2: [32 x 256] . W2^T -> [32 x 256]
+  wave_gemm3<KS2, 2>(L.h1[0], L.h1[1], L.h1[2], L.LDH, W.w2, W.plane2, 2 * wave, 0, KS2, acc,
+                     lane);
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int col = 64 * wave + 32 * t + (lane & 31);
+    const float bias = W.b2[col];
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+      const int i = acc_row(lane, v) * L.LDH + col;
+      split3(softplus_f<true>(acc[t][v] + bias), L.h2[0][i], L.h2[1][i], L.h2[2][i]);
+    }
+  }
+  __syncthreads();
+  // layer 3 (split-K): wave w sums K in [64w, 64w + 64) for all 64 outputs; the partials go
+  // to the h1 planes (layer 2, their last reader, finished before the barrier above)
+  wave_gemm3<KS2, 2>(L.h2[0], L.h2[1], L.h2[2], L.LDH, W.w3, W.plane3, 0, wave * (KS2 / 4),
+                     (wave + 1) * (KS2 / 4), acc, lane);
+  float* part = L.part();
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int col = 32 * t + (lane & 31);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int c = owner_c0 + j;
+    float s = part[(0 * MLP_TILE + owner_row) * L.LDP + c];
+    s += part[(1 * MLP_TILE + owner_row) * L.LDP + c];
+    s += part[(2 * MLP_TILE + owner_row) * L.LDP + c];
+    s += part[(3 * MLP_TILE + owner_row) * L.LDP + c];
+    fout[j] = tanh_f<true>(s + W.b3[c]);
   }
 }
 

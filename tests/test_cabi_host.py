@@ -244,7 +244,7 @@ def test_mlp_sizes_and_argument_checks():
     assert lib.dfx_mlp_pack_bytes(_lib.F32, ctypes.byref(nb)) == 0
     assert nb.value == 2 * (256 * 64 + 256 * 256 + 64 * 256) * 4   # forward + transposed
     assert lib.dfx_mlp_pack_bytes(_lib.BF16_F32, ctypes.byref(nb)) == 0
-    assert nb.value == 2 * (256 * 64 + 256 * 256 + 64 * 256) * 2
+    assert nb.value == 3 * 2 * (256 * 64 + 256 * 256 + 64 * 256) * 2   # three bf16 pieces
     assert lib.dfx_mlp_pack_bytes(_lib.F64, ctypes.byref(nb)) == -1
     p = _mlp.MlpProblem(solver=_lib.TSIT5, precision=_lib.F32, d=64, hidden=256, n_ts=32,
                         max_steps=4096, ckpt_every=64, batch=8192)
