@@ -60,7 +60,7 @@ def make_inputs(seed=0, B=65536, n_ts=64):
     return mu, y0, t1, ts
 
 
-def prepare(dtype, mu, y0, t1, ts):
+def prepare(dtype, mu, y0, t1, ts, rtol=1e-6, atol=1e-9):
     import torch
     import diffrax_amd as dfx
     from diffrax_amd import _integrate
@@ -70,7 +70,41 @@ def prepare(dtype, mu, y0, t1, ts):
         torch.tensor(t1, dtype=torch.float64, device=dev), 0.1,
         torch.tensor(y0, dtype=dtype, device=dev), torch.tensor(mu, dtype=dtype, device=dev),
         dfx.SaveAt(ts=torch.tensor(ts, dtype=torch.float64)),
-        dfx.PIDController(rtol=1e-6, atol=1e-9), 16384, None)
+        dfx.PIDController(rtol=rtol, atol=atol), 16384, None)
+
+
+def matched_error(mu, y0, t1, ts, tight, target, steps, warmup, sub):
+    """The float32-state form at the loosest tolerance (rtol r, atol r / 1000; r = 1e-6
+    halved until it matches) whose normwise error against the tight FP64 solve of the
+    checked subset is at most `target`, the FP64 form's at rtol 1e-6; timed there."""
+    import numpy as np
+    import torch
+    scale = float(np.max(np.abs(tight)))
+    tried = []
+    r = 1e-6
+    for _ in range(6):
+        plan = prepare(torch.float32, mu, y0, t1, ts, rtol=r, atol=r * 1e-3)
+        plan.launch()
+        torch.cuda.synchronize()
+        ys = plan.t["ys"][:sub].double().cpu().numpy()
+        ok = bool((plan.t["result"] == 0).all())
+        err = float(np.max(np.abs(ys - tight)) / scale)
+        tried.append({"rtol": r, "normwise_err": err, "all_successful": ok})
+        if err <= target and ok:
+            t = time_form(plan, steps, warmup)
+            B = y0.shape[0]
+            res = {"matched": True, "rtol": r, "atol": r * 1e-3, "normwise_err": err,
+                   "ms_per_step": round(t["ms_per_step"], 4),
+                   "kernel_ms_mean": round(t["kernel_ms_mean"], 4), "launch": t["launch"],
+                   "solves_per_s": round(B / (t["ms_per_step"] / 1e3), 1), "tried": tried}
+            plan.t.clear()
+            return res
+        plan.t.clear()
+        r *= 0.5
+    return {"matched": False, "tried": tried,
+            "note": "no tolerance down to rtol 3.1e-8 brings the float32-state form to the "
+                    "FP64 form's error: its float32 rounding, amplified by the neutrally "
+                    "stable limit-cycle phase, sets the floor"}
 
 
 def time_form(plan, steps, warmup):
@@ -195,6 +229,43 @@ def mlp_inputs(seed=0, B=8192, n_ts=32):
     return [W1, np.zeros(256), W2, np.zeros(256), W3, np.zeros(64)], y0, ts, targets
 
 
+def mlp_grad_error(gplan, W, y0, ts, check):
+    """Gradient check on `check` rows: the device adjoint of sum(w * ys) with w nonzero on
+    those rows only (weights gradients then come from them alone), against torch float64
+    autograd through a replay of the device's accepted steps of those rows."""
+    import numpy as np
+    import torch
+    from diffrax_amd import _mlp
+    from oracle import pyoracle
+    rng = np.random.default_rng(17)
+    B = y0.shape[0]
+    rows = np.sort(rng.choice(B, size=check, replace=False))
+    w = torch.zeros(gplan.ys.shape, dtype=torch.float32, device=gplan.ys.device)
+    wr = rng.normal(size=(check,) + tuple(gplan.ys.shape[1:])).astype(np.float32)
+    w[torch.as_tensor(rows, device=w.device)] = torch.tensor(wr, device=w.device)
+    gplan.launch()
+    g = _mlp.backward_mlp(gplan, w)     # (y0_bar, W1, b1, W2, b2, W3, b3)
+    torch.cuda.synchronize()
+    grid = gplan.grid.cpu().numpy()[rows]
+    nacc = gplan.stats[:, 1].cpu().numpy()[rows]
+    Pd = [torch.tensor(np.asarray(x, dtype=np.float32).astype(np.float64), requires_grad=True)
+          for x in W]
+    W1, b1, W2, b2, W3, b3 = Pd
+    sp = torch.nn.functional.softplus
+    f = lambda t, y: torch.tanh(sp(sp(y @ W1.T + b1) @ W2.T + b2) @ W3.T + b3)
+    Y0 = torch.tensor(y0[rows].astype(np.float32).astype(np.float64), requires_grad=True)
+    loss = 0.0
+    for i in range(check):
+        steps = [tuple(grid[i, n]) for n in range(nacc[i])]
+        out = pyoracle.replay(f, "tsit5", steps, Y0[i], list(ts))
+        loss = loss + (torch.stack(out) * torch.tensor(wr[i].astype(np.float64))).sum()
+    loss.backward()
+    got = [g[0][torch.as_tensor(rows, device=g[0].device)]] + list(g[1:])
+    errs = [float((a.double().cpu() - r).abs().max() / r.abs().max())
+            for a, r in zip(got, [Y0.grad] + [p.grad for p in Pd])]
+    return dict(grad_max_rel_err_vs_fp64_replay=max(errs), grad_check_rows=check)
+
+
 def bench_mlp(steps, warmup, with_cpu, check=256):
     """Neural-ODE forward and forward+gradient, float32 and bfloat16-MFMA forms."""
     import numpy as np
@@ -258,10 +329,21 @@ def bench_mlp(steps, warmup, with_cpu, check=256):
             out["g"] = _mlp.backward_mlp(gplan, ys_bar)
 
         k_mean, k_med, wall = _event_time(fwd_grad, steps, warmup)
+        gst = gplan.stats.cpu().numpy()
+        acc = gst[:, 1].astype(np.float64)
+        # MFMA work of one gradient: the forward's evaluations, then per accepted step the
+        # segment recompute (6 per step + the first stage), the stage VJPs (7, each the
+        # three transposed products) and the weight-gradient GEMM rows (7)
+        gflop = float((((6 * gst[:, 0] + 1) + (6 * acc + 1) + 7 * acc + 7 * acc)
+                       * MLP_FLOP_PER_EVAL).sum())
+        gtf = gflop / (k_mean * 1e-3) / 1e12
         grad[matmul] = dict(solves_per_s=round(B / (wall * 1e-3), 1), ms_per_solve=round(wall, 4),
                             event_ms_mean=round(k_mean, 4),
                             checkpoint_every=int(gplan.prob.ckpt_every),
-                            grad_finite=bool(all(torch.isfinite(x).all().item() for x in out["g"])))
+                            grad_finite=bool(all(torch.isfinite(x).all().item() for x in out["g"])),
+                            algorithmic_tflops=round(gtf, 3), mfma_peak_tflops=peak,
+                            frac=round(gtf / peak, 5),
+                            **mlp_grad_error(gplan, W, y0, ts, check))
         del plan, gplan, g
     cpu = None
     if cpu_times:
@@ -319,6 +401,33 @@ def bench_sde(steps, warmup, with_cpu, P=262144, d=16, N=1000, cpu_paths=16384):
                                algorithmic_bytes=2 * alg_bytes,
                                achieved_gbs=round(2 * alg_bytes / (wall * 1e-3) / 1e9, 1))
     out.pop("g", None)
+    # Errors on the last 512 paths (dW offsets past 2^31): the forward against the FP64
+    # oracle, and the adjoint of sum(w * y(1)) with w nonzero on those paths only against
+    # torch float64 autograd through the same fixed-step replay.
+    last = slice(P - 512, P)
+    dW_last = dW[:, last].double().cpu().numpy()
+    q = np.concatenate([theta, mu, sigma])
+    y0n = np.zeros((512, d))
+    ref = oracle.solve(y0n, 1.0, q, dW=dW_last, solver="heun", vf="ou", controller="constant",
+                       dt0=dt, save_t1=True, max_steps=100000)
+    gs.forward()
+    ys_last = gs.ys[last].double().cpu().numpy()
+    out["forward"]["normwise_err_vs_fp64_oracle_last512"] = float(
+        np.max(np.abs(ys_last - ref["ys"])) / np.max(np.abs(ref["ys"])))
+    w = torch.zeros_like(gs.ys)
+    wr = np.random.default_rng(9).normal(size=(512, 1, d))
+    w[last] = torch.tensor(wr, dtype=torch.float32, device=dev)
+    _, thb, mub, sgb = gs.adjoint(w, False)
+    from oracle import pyoracle
+    TH, MU, SG = (torch.tensor(a, requires_grad=True) for a in (theta, mu, sigma))
+    dWt = torch.tensor(dW_last)
+    g64 = grid.cpu().numpy()
+    rep = pyoracle.replay(lambda t, y: TH * (MU - y), "heun", list(zip(g64[:-1], g64[1:])),
+                          torch.tensor(y0n), [], g_dw=lambda n, t, y: SG * dWt[n], save_t1=True)
+    (rep[-1] * torch.tensor(wr[:, 0])).sum().backward()
+    out["forward_grad"]["grad_normwise_err_vs_fp64_replay_last512"] = max(
+        float((a.cpu() - r).abs().max() / r.abs().max())
+        for a, r in ((thb, TH.grad), (mub, MU.grad), (sgb, SG.grad)))
     if with_cpu:
         cores = min(len(os.sched_getaffinity(0)), 16)
         dWc = dW[:, :cpu_paths].double().cpu().numpy()
@@ -585,8 +694,25 @@ def main():
         for form in forms:
             forms[form]["fp64_oracle_normwise_err_vs_tight_subset"] = ref_err
 
-    eligible = [f for f in forms if forms[f]["max_rel_err_vs_fp64"] <= 1e-6]
+    # Matched error: the FP64 form at the workload tolerance sets the error (normwise, vs
+    # the tight FP64 solve of the subset); the float32-state form is run at the tolerance
+    # that matches it.  `value` is the faster of the forms at matched error; the stricter
+    # gate (max relative error <= 1e-6 against the plain FP64 solve) is reported beside it.
+    matched = None
+    if "f64" in forms and "f32" in forms:
+        target = forms["f64"]["normwise_err_vs_tight_fp64_subset"]
+        matched = {"target_normwise_err": target,
+                   "f64": {"rtol": 1e-6, "normwise_err": target,
+                           "solves_per_s": round(forms["f64"]["solves_per_s"], 1),
+                           "ms_per_step": round(forms["f64"]["ms_per_step"], 4)},
+                   "f32": matched_error(mu, y0, t1, ts, tight, target, args.steps,
+                                        args.warmup, sub)}
+    gate = {f: forms[f]["max_rel_err_vs_fp64"] <= 1e-6 for f in forms}
+    eligible = [f for f in forms if gate[f]]
     head = max(eligible, key=lambda f: forms[f]["solves_per_s"]) if eligible else "f64"
+    if matched is not None and matched["f32"].get("matched") and \
+            matched["f32"]["solves_per_s"] > forms[head]["solves_per_s"]:
+        head = "f32"
     h = forms[head]
     peak = FP64_PEAK_TFLOPS
     roof = {"bound": "valu", "achieved": round(h["achieved_tflops"], 4), "peak": peak,
@@ -614,7 +740,15 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "forms": {f: {k: v for k, v in forms[f].items() if k != "tune_ms"} for f in forms},
+        "matched_error": matched,
+        "gate_max_rel_err_le_1e-6_vs_fp64": gate,
     }
+    if head == "f32" and matched is not None:
+        # the float32-state form at its matched tolerance carries the headline
+        line["value"] = matched["f32"]["solves_per_s"]
+        line["ms_per_step"] = matched["f32"]["ms_per_step"]
+        line["config"]["workload"] = line["config"]["workload"].replace(
+            "PID(1e-6, 1e-9)", f"PID({matched['f32']['rtol']:.3g}, {matched['f32']['atol']:.3g})")
     for p in plans.values():
         p.t.clear()
     plans.clear()

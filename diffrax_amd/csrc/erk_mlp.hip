@@ -49,8 +49,7 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
   const int64_t traj = (int64_t)blockIdx.x * MLP_TILE + orow;
   const bool valid = traj < a.B;
 
-  TabRegs<Tab, float> tr;  // float32 stage algebra: the coefficients' float32 values
-  tr.load();
+  using TL = TabLit<Tab, float>;  // float32 stage algebra: the coefficients as literals
 
   // ---- set-up (diffeqsolve / PIDController.init) ----
   float y[8], fsal[8];
@@ -106,7 +105,7 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
         float kc[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
-        z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, 0.0f);
+        z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(TL::v.a[i], kc, 0.0f);
         store_input<P>(L, orow, c0 + j, z[j]);
       }
       __syncthreads();
@@ -123,7 +122,7 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
       float kc[S];
 #pragma unroll
       for (int m = 0; m < S; ++m) kc[m] = k[m][j];
-      yerr[j] = comb<Tab, 2, 0, 0, S, false>(tr.berr, kc, 0.0f);
+      yerr[j] = comb<Tab, 2, 0, 0, S, false>(TL::v.berr, kc, 0.0f);
       anynan = anynan || (z[j] != z[j]);
     }
     // group reduction in owner order (identical in all 8 owners)

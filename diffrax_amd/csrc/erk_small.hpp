@@ -84,6 +84,29 @@ struct TabRegs {
   }
 };
 
+// The tableau as compile-time constants of type C: with C = float the stage algebra takes
+// the coefficients as literal operands (v_fmac_f32 with a 32-bit literal), so they occupy
+// no registers, which the MFMA kernels need for their fragments.
+template <class Tab, class C>
+struct TabLit {
+  static constexpr int S = Tab::S;
+  struct Arr {
+    C a[S][S];
+    C b[S];
+    C berr[S];
+  };
+  static constexpr Arr make() {
+    Arr r{};
+    for (int i = 0; i < S; ++i) {
+      for (int m = 0; m < S; ++m) r.a[i][m] = C(Tab::a[i][m]);
+      r.b[i] = C(Tab::b[i]);
+      r.berr[i] = C(Tab::berr[i]);
+    }
+    return r;
+  }
+  static constexpr Arr v = make();
+};
+
 // Compile-time tableau access: WHICH 0 = a[ROW], 1 = b, 2 = b_error, 3 = Dopri5 c_mid.
 template <class Tab, int WHICH, int ROW, int M>
 constexpr double tab_coef() {

@@ -1,6 +1,8 @@
 // erk_small_f64.hip -- float64 parity build of the small-system kernel.  Compiled with
-// -ffp-contract=off so every a*b+c is the rounded product then the rounded sum, as the
-// reference formulas are written.
+// -ffp-contract=off: the compiler fuses nothing, so the only fused multiply-adds are the
+// explicit ones (the stage, b, b_err and interpolant sums, written as fma chains in index
+// order and restated the same way by the oracle); every other a*b+c is the rounded
+// product then the rounded sum.
 #include "erk_adjoint.hpp"
 #include "erk_small_dispatch.hpp"
 

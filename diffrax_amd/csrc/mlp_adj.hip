@@ -92,8 +92,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
   int steps_here = nacc - n0;
   steps_here = steps_here < 0 ? 0 : (steps_here > a.Cs ? a.Cs : steps_here);
   const int dir = (a.t0 < a.t1) ? 1 : -1;
-  TabRegs<Tab> tr;
-  tr.load();
+  using TL = TabLit<Tab, float>;  // float32 stage algebra: the coefficients as literals
   float* fs_tile = a.zs + (int64_t)tile * a.Cs * S * MLP_TILE * MLP_D;
   auto fslot = [&](int q, int i) { return fs_tile + (((int64_t)q * S + i) * MLP_TILE + orow) * MLP_D + c0; };
   // Rows of stage (q, i) in the segment's row buffers.
@@ -141,7 +140,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
         float kc[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
-        z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, 0.0f);
+        z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(TL::v.a[i], kc, 0.0f);
         store_input<P>(L, orow, c0 + j, z[j]);
       }
       __syncthreads();
@@ -184,7 +183,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
 #pragma unroll
       for (int m = 0; m < S - 1; ++m) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) kb[m][j] += (float)tr.a[S - 1][m] * yb[j];
+        for (int j = 0; j < 8; ++j) kb[m][j] += TL::v.a[S - 1][m] * yb[j];
       }
       // SaveAt(ts) values interpolated inside this step
       int jlo = 0, jhi = 0;
@@ -275,7 +274,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       for (int j = 0; j < 8; ++j) {
         ybn[j] += zb[j];
 #pragma unroll
-        for (int m = 0; m < i; ++m) kb[m][j] += (float)tr.a[i][m] * zb[j];
+        for (int m = 0; m < i; ++m) kb[m][j] += TL::v.a[i][m] * zb[j];
       }
       __syncthreads();
     });
