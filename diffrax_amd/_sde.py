@@ -38,7 +38,11 @@ _SIGS = {
     "dfx_sde_heun_adjoint": ([ctypes.POINTER(SdeProblem), ctypes.POINTER(SdeIO),
                               ctypes.POINTER(SdeGradIO), ctypes.c_void_p, ctypes.c_size_t,
                               ctypes.c_void_p], ctypes.c_int),
+    "dfx_sde_heun_adjoint_static": ([ctypes.POINTER(SdeProblem), ctypes.POINTER(SdeIO),
+                                     ctypes.POINTER(SdeGradIO), ctypes.c_void_p], ctypes.c_int),
 }
+# DFX_SDE_STATIC_SCRATCH_BYTES: the static partial-sum array of dfx_sde_heun_adjoint_static
+SDE_STATIC_SCRATCH_BYTES = 8 << 20
 _lib._SIGS.update(_SIGS)
 
 

@@ -126,6 +126,7 @@ const char* dfx_strerror(int code) {
     case DFX_E_ARG: return "invalid argument or unsupported solver/vector-field/precision combination";
     case DFX_E_HIP: return "HIP runtime error";
     case DFX_E_WORKSPACE: return "workspace missing or too small";
+    case DFX_E_TOO_LARGE: return "problem needs more scratch than the library's static array holds";
     default: return "unknown error";
   }
 }

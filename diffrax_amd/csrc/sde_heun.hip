@@ -4,6 +4,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "../../include/diffrax_amd.h"
 #include "sde_heun.hpp"
 
@@ -52,6 +54,35 @@ int sde_check(const dfx_sde_problem* p) {
 }
 
 int64_t sde_threads(const dfx_sde_problem* p, int V) { return p->paths * (p->d / V); }
+
+int sde_adjoint_impl(const dfx_sde_problem* p, const dfx_sde_io* io, const dfx_sde_grad_io* g,
+                     void* workspace, size_t workspace_bytes, void* stream);
+}  // namespace
+
+// Scratch of dfx_sde_heun_adjoint_static: the adjoint's per-block partial sums, one copy per
+// GPU.  Host code uses the device address looked up per device (static_scratch), never the
+// array's name.
+static __device__ __align__(256) unsigned char scratch[DFX_SDE_STATIC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on first use.
+int static_scratch(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return DFX_E_HIP;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+      return DFX_E_HIP;
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return DFX_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -92,6 +123,25 @@ int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* str
 int dfx_sde_heun_adjoint(const dfx_sde_problem* p, const dfx_sde_io* io,
                          const dfx_sde_grad_io* g, void* workspace, size_t workspace_bytes,
                          void* stream) {
+  return sde_adjoint_impl(p, io, g, workspace, workspace_bytes, stream);
+}
+
+int dfx_sde_heun_adjoint_static(const dfx_sde_problem* p, const dfx_sde_io* io,
+                                const dfx_sde_grad_io* g, void* stream) {
+  size_t need = 0;
+  if (dfx_sde_workspace_bytes(p, &need) != DFX_OK) return DFX_E_ARG;
+  if (need > DFX_SDE_STATIC_SCRATCH_BYTES) return DFX_E_TOO_LARGE;
+  void* ptr = nullptr;
+  const int rc = static_scratch(&ptr);
+  if (rc != DFX_OK) return rc;
+  return sde_adjoint_impl(p, io, g, ptr, DFX_SDE_STATIC_SCRATCH_BYTES, stream);
+}
+
+}  // extern "C"
+
+namespace {
+int sde_adjoint_impl(const dfx_sde_problem* p, const dfx_sde_io* io, const dfx_sde_grad_io* g,
+                     void* workspace, size_t workspace_bytes, void* stream) {
   int rc = sde_check(p);
   if (rc != DFX_OK) return rc;
   if (!io || !g || !io->y0 || !io->theta || !io->mu || !io->sigma || !io->grid || !g->ys_bar)
@@ -125,5 +175,4 @@ int dfx_sde_heun_adjoint(const dfx_sde_problem* p, const dfx_sde_io* io,
   }
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }
-
-}  // extern "C"
+}  // namespace

@@ -12,7 +12,8 @@
  *  - Plain C: pointers are device pointers (hipMalloc / torch CUDA tensors), sizes are
  *    element counts, `stream` is a hipStream_t passed as void*.
  *  - The library allocates no device memory.  Scratch memory the kernels need is a
- *    caller-provided workspace sized by the matching *_workspace_bytes call.
+ *    caller-provided workspace sized by the matching *_workspace_bytes call; the one
+ *    exception, dfx_sde_heun_adjoint_static, uses a fixed static device array instead.
  *  - Every call is stream ordered, makes no host synchronisation and may be captured
  *    into a hipGraph.
  *  - Return value: 0 on success, a negative DFX_E* code otherwise (dfx_strerror).
@@ -44,6 +45,7 @@ enum { DFX_SUCCESSFUL = 0, DFX_MAX_STEPS_REACHED = 1, DFX_DT_MIN_REACHED = 2,
 #define DFX_E_ARG (-1)        /* invalid argument / unsupported combination */
 #define DFX_E_HIP (-2)        /* a HIP runtime call failed                  */
 #define DFX_E_WORKSPACE (-3)  /* workspace missing or too small             */
+#define DFX_E_TOO_LARGE (-4)  /* problem needs more than the static scratch */
 
 /* ---- problem description (host memory) ------------------------------------------- */
 /* Fields mirror diffeqsolve's arguments (_integrate.py:890-911), SaveAt
@@ -278,6 +280,21 @@ int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* str
 int dfx_sde_heun_adjoint(const dfx_sde_problem* p, const dfx_sde_io* io,
                          const dfx_sde_grad_io* g, void* workspace, size_t workspace_bytes,
                          void* stream);
+
+/* Allocation-free variant of dfx_sde_heun_adjoint.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The per-block partial sums live in a
+ * library-owned static device array of DFX_SDE_STATIC_SCRATCH_BYTES on each GPU (held until
+ * the process ends) instead of a caller workspace; the caller still passes every input and
+ * output.  A problem whose dfx_sde_workspace_bytes exceeds the array returns
+ * DFX_E_TOO_LARGE (262,144 paths x 16 components need 6.3 MB).  The current device must be
+ * the stream's device; the array's address is looked up on the first call on a device.
+ * All calls on one GPU share the array: calls that overlap on one GPU (two streams, two
+ * host threads) race.  Every partial is written before it is read, so nothing is cleared
+ * between calls.  Results are those of dfx_sde_heun_adjoint bit for bit. */
+#define DFX_SDE_STATIC_SCRATCH_BYTES (8u << 20)
+int dfx_sde_heun_adjoint_static(const dfx_sde_problem* p, const dfx_sde_io* io,
+                                const dfx_sde_grad_io* g, void* stream);
 
 /* ---- neural ODE: dy/dt = MLP(y), d = 64, width 256, depth 2 ------------------------- */
 /* f(y) = tanh(W3 softplus(W2 softplus(W1 y + b1) + b2) + b3), W1 [256,64], W2 [256,256],
