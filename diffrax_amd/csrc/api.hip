@@ -196,7 +196,7 @@ int dfx_erk_ckpt_layout(const dfx_erk_problem* p, int32_t* n_ckpt, int32_t* word
 
 int dfx_erk_adjoint_workspace_bytes(const dfx_erk_problem* p, size_t* bytes) {
   if (!p || !bytes || p->ckpt_every <= 0 || p->batch < 0 || p->d < 1) return DFX_E_ARG;
-  const size_t recs = (size_t)p->batch * (size_t)p->ckpt_every;
+  const size_t recs = (size_t)p->batch * 2 * (size_t)p->ckpt_every;  // two segments each
   const size_t elt = p->precision == DFX_F64 ? 8 : 4;
   *bytes = align256(recs * 2 * sizeof(double)) + align256(recs * (size_t)p->d * elt);
   if (*bytes < 256) *bytes = 256;
@@ -312,17 +312,17 @@ int dfx_erk_adjoint(const dfx_erk_problem* p, const dfx_erk_io* fwd,
   a.ckpt_every = p->ckpt_every;
   a.n_ckpt = ckpt_count(p);
   // The replay's arguments: no saves, no checkpoint writes; accepted steps recorded into
-  // the per-instance segment scratch of ckpt_every records.
+  // the per-instance scratch of two segments (2 ckpt_every records).
   ErkKArgs ra = a;
   ra.ts = nullptr;
   ra.n_ts = 0;
   ra.n_out = 0;
   ra.save_t0 = ra.save_t1 = ra.save_steps = 0;
   ra.ckpt = nullptr;
-  const size_t recs = (size_t)p->batch * (size_t)p->ckpt_every;
+  const size_t recs = (size_t)p->batch * 2 * (size_t)p->ckpt_every;
   ra.grid = reinterpret_cast<double*>(workspace);
   ra.grid_y = reinterpret_cast<char*>(workspace) + align256(recs * 2 * sizeof(double));
-  ra.grid_stride = p->ckpt_every;
+  ra.grid_stride = 2 * (int64_t)p->ckpt_every;
   DiscreteAdjArgs b;
   b.ckpt = io->ckpt;
   b.stats = io->stats;

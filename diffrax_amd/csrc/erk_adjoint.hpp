@@ -7,8 +7,9 @@
 // takes the segments newest first: it restarts the forward step loop from the segment's
 // checkpoint (ErkLane::resume; same code and arithmetic, so the accepted steps repeat
 // bit for bit, rejected ones included), writes each accepted step's (t_start, t_end) and
-// start state to a K-record per-lane scratch (ErkKArgs::grid / grid_y of the replay
-// arguments) and sweeps those records in reverse.  The step-size controller's factors are
+// start state to one half of a 2K-record per-instance scratch (ErkKArgs::grid / grid_y of
+// the replay arguments) and sweeps those records in reverse, while the next (older)
+// segment is replayed into the other half by a second wave.  The step-size controller's factors are
 // constants of the gradient (pid.py:374, 498-530) and rejected steps contribute nothing,
 // so only the accepted steps enter the sweep.
 //
@@ -83,39 +84,52 @@ __device__ __forceinline__ void erk_interp_vjp(double ts0, double ts1, double t,
   }
 }
 
-// a: the forward's arguments (save layout, ts, t1, params); ra: the same with saves,
-// checkpoint writes and events off, and grid / grid_y pointing at the per-lane segment
-// scratch (grid_stride = ckpt_every records per lane).
-template <class Tab, class VF, class T, int Ctl, bool Extras>
-__global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const ErkKArgs ra,
-                                                         const DiscreteAdjArgs b) {
-  constexpr int S = Tab::S, D = VF::D, P = VF::P;
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= a.B) return;
-  ErkLane<Tab, VF, T, Ctl, Extras> L;
-  L.replay_setup(ra, id);
-  T gq[P];
-#pragma unroll
-  for (int m = 0; m < P; ++m) gq[m] = T(0);
-  const int dir = L.dir;
-  const double T0 = a.t0 * dir, T1 = L.T1;
-  const int n = min(max(b.stats[id * 3 + 1], 0), a.max_steps);
-  const int K = a.ckpt_every;
-  const double* sg = ra.grid + id * ra.grid_stride * 2;
-  const T* sy = reinterpret_cast<const T*>(ra.grid_y) + id * ra.grid_stride * D;
-  const T* g = reinterpret_cast<const T*>(b.ys_bar) + id * (int64_t)a.n_out * D;
-  const double* tsr = a.ts ? a.ts + id * a.ts_stride : nullptr;
-  const int ksave = a.save_steps;
-
+// The reverse sweep of one instance: the cotangent state carried from segment to segment
+// (newest first) and the transpose of every accepted step of a segment, read from the
+// segment's records.  a: the forward's arguments (save layout, ts, t1, params).
+template <class Tab, class VF, class T>
+struct AdjSweep {
+  static constexpr int S = Tab::S, D = VF::D, P = VF::P;
+  static constexpr int CW = D + 8;  // checkpoint record words (ErkLane::write_ckpt)
+  TabRegs<Tab> tr;
+  T q[P], gq[P];
+  int64_t id;
+  int dir, n, c_last, ksave;
+  double T0, T1;
+  const T* g;
+  const double* tsr;
   T ab[D];    // cotangent of the state at the end of the step being processed
   T grow[D];  // cotangent of save row r, loaded one row ahead
+  int r, ti;
+  double tsv;
+
+  __device__ void setup(const ErkKArgs& a, const DiscreteAdjArgs& b, int64_t id_, int n_,
+                        int K) {
+    tr.load();
+    id = id_;
+    n = n_;
+    c_last = n > 0 ? (n - 1) / K : -1;
+    ksave = a.save_steps;
+    const T* qp = reinterpret_cast<const T*>(a.params) + id * a.param_stride;
 #pragma unroll
-  for (int j = 0; j < D; ++j) { ab[j] = T(0); grow[j] = T(0); }
-  int r = -1, ti = -1;
-  double tsv = -__builtin_inf();
+    for (int m = 0; m < P; ++m) { q[m] = qp[m]; gq[m] = T(0); }
+    const double t1r = a.t1[id * a.t1_stride];
+    dir = (a.t0 < t1r) ? 1 : -1;
+    T0 = a.t0 * dir;
+    T1 = t1r * dir;
+    g = reinterpret_cast<const T*>(b.ys_bar) + id * (int64_t)a.n_out * D;
+    tsr = a.ts ? a.ts + id * a.ts_stride : nullptr;
+#pragma unroll
+    for (int j = 0; j < D; ++j) { ab[j] = T(0); grow[j] = T(0); }
+    r = -1;
+    ti = -1;
+    tsv = -__builtin_inf();
+    if (n == 0) setup_rows(a, -__builtin_inf());
+  }
+
   // SaveAt(ts) rows the forward wrote: all of them for a zero-length solve, else those at
   // or before the end of the last accepted step (a prefix: ts is sorted); then the t1 row.
-  auto setup_rows = [&](double tend) {
+  __device__ void setup_rows(const ErkKArgs& a, double tend) {
     int n_saved = 0;
     if (T0 == T1) {
       n_saved = a.n_ts;
@@ -134,31 +148,22 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
       for (int j = 0; j < D; ++j) grow[j] = g[(int64_t)r * D + j];
     }
     tsv = (ti >= 0) ? tsr[ti] * dir : -__builtin_inf();
-  };
-  if (n == 0) setup_rows(-__builtin_inf());
+  }
 
-  const int c_last = n > 0 ? (n - 1) / K : -1;
-  for (int c = c_last; c >= 0; --c) {
-    // replay the segment's accepted steps [lo, target) from its checkpoint
-    const double tl = L.resume(ra, b.ckpt, c);
+  // Accepted steps [c K, min((c + 1) K, n)) of segment c, newest first, from their records
+  // sg (t_start, t_end) and sy (start state); tl is the end of the step before the segment.
+  __device__ void segment(const ErkKArgs& a, int c, int K, const double* sg, const T* sy,
+                          double tl) {
     const int lo = c * K;
-    const int target = min(lo + K, n);
-    while (L.num_acc < target && L.running(ra)) L.step(ra);
-    const int hi = L.num_acc;  // == target: the replay repeats the forward
-    if (c == c_last) setup_rows(hi > lo ? sg[(int64_t)(hi - 1 - lo) * 2 + 1] : -__builtin_inf());
-
+    const int hi = min(lo + K, n);
+    if (c == c_last) setup_rows(a, sg[(int64_t)(hi - 1 - lo) * 2 + 1]);
     // The records are read one step ahead: a lane walks them serially, so a load issued
     // where it is used would expose its full latency once per step.
-    double nts0 = 0.0, nts1 = 0.0;
+    double nts0 = sg[(int64_t)(hi - 1 - lo) * 2];
+    double nts1 = sg[(int64_t)(hi - 1 - lo) * 2 + 1];
     T ny0[D];
 #pragma unroll
-    for (int j = 0; j < D; ++j) ny0[j] = T(0);
-    if (hi > lo) {
-      nts0 = sg[(int64_t)(hi - 1 - lo) * 2];
-      nts1 = sg[(int64_t)(hi - 1 - lo) * 2 + 1];
-#pragma unroll
-      for (int j = 0; j < D; ++j) ny0[j] = sy[(int64_t)(hi - 1 - lo) * D + j];
-    }
+    for (int j = 0; j < D; ++j) ny0[j] = sy[(int64_t)(hi - 1 - lo) * D + j];
     for (int s = hi - 1; s >= lo; --s) {
       const int i = s - lo;
       const double ts0 = nts0, ts1 = nts1;
@@ -182,12 +187,12 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
           T kc[S];
 #pragma unroll
           for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : T(0);
-          zs[i][j] = y0[j] + comb<Tab, 0, i, 0, i, false>(L.tr.a[i], kc, T(0));
+          zs[i][j] = y0[j] + comb<Tab, 0, i, 0, i, false>(tr.a[i], kc, T(0));
         }
         constexpr double ci = Tab::c[i];
         const double tsi = (ci == 1.0) ? ts1 : ts0 + ci * dt;
         T f[D];
-        VF::eval(tsi * dir, zs[i], L.q, f);
+        VF::eval(tsi * dir, zs[i], q, f);
 #pragma unroll
         for (int j = 0; j < D; ++j) k[i][j] = control * f[j];
       });
@@ -235,7 +240,7 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
         T w[D], gz[D], gqi[P];
 #pragma unroll
         for (int j = 0; j < D; ++j) w[j] = control * gk[i][j];
-        VF::vjp(zs[i], L.q, w, gz, gqi);
+        VF::vjp(zs[i], q, w, gz, gqi);
 #pragma unroll
         for (int m = 0; m < P; ++m) gq[m] = gq[m] + gqi[m];
 #pragma unroll
@@ -252,30 +257,89 @@ __global__ void __launch_bounds__(64) erk_adjoint_kernel(const ErkKArgs a, const
       for (int j = 0; j < D; ++j) ab[j] = gy0[j];
     }
   }
-  if (T0 == T1) {  // a zero-length solve saves y0 in every ts row
-    for (int rr = a.save_t0; rr < a.save_t0 + a.n_ts; ++rr) {
+
+  __device__ void finish(const ErkKArgs& a, const DiscreteAdjArgs& b) {
+    if (T0 == T1) {  // a zero-length solve saves y0 in every ts row
+      for (int rr = a.save_t0; rr < a.save_t0 + a.n_ts; ++rr) {
 #pragma unroll
-      for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[(int64_t)rr * D + j];
+        for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[(int64_t)rr * D + j];
+      }
     }
+    if (a.save_t0) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[j];
+    }
+    T* yb = reinterpret_cast<T*>(b.y0_bar) + id * D;
+    T* qb = reinterpret_cast<T*>(b.q_bar) + id * P;
+#pragma unroll
+    for (int j = 0; j < D; ++j) yb[j] = ab[j];
+#pragma unroll
+    for (int m = 0; m < P; ++m) qb[m] = gq[m];
   }
-  if (a.save_t0) {
-#pragma unroll
-    for (int j = 0; j < D; ++j) ab[j] = ab[j] + g[j];
+};
+
+// One workgroup of two waves per 64 instances: wave 0 replays, wave 1 sweeps, one segment
+// apart.  At stage s the replay wave restarts segment c_last - s of each of its instances
+// from the checkpoint and records its accepted steps in half (c & 1) of the instance's
+// 2K-record scratch, while the sweep wave transposes segment c_last - s + 1 from the
+// other half; a barrier ends every stage.  Each instance's segments are still swept
+// newest first and every record is written one stage before it is read, so the result is
+// that of replaying and sweeping each segment in turn; the two waves' dependency chains
+// share a SIMD's issue slots, which one wave leaves more than half idle.
+// ra: the forward's arguments with saves, checkpoint writes and events off and grid /
+// grid_y pointing at the scratch (grid_stride = 2K records per instance).
+template <class Tab, class VF, class T, int Ctl, bool Extras>
+__global__ void __launch_bounds__(128) erk_adjoint_kernel(const ErkKArgs a, const ErkKArgs ra,
+                                                          const DiscreteAdjArgs b) {
+  constexpr int D = VF::D;
+  const int role = threadIdx.x >> 6;  // 0: replay, 1: sweep (wave-uniform)
+  const int64_t id = (int64_t)blockIdx.x * 64 + (threadIdx.x & 63);
+  const bool live = id < a.B;
+  const int K = a.ckpt_every;
+  const int n = live ? min(max(b.stats[id * 3 + 1], 0), a.max_steps) : 0;
+  const int c_last = n > 0 ? (n - 1) / K : -1;
+  __shared__ int s_cmax;
+  if (threadIdx.x == 0) s_cmax = -1;
+  __syncthreads();
+  if (role == 0 && c_last >= 0) atomicMax(&s_cmax, c_last);
+  __syncthreads();
+  const int nstages = s_cmax + 2;  // the replay of the oldest segment, then its sweep
+  if (role == 0) {
+    ErkLane<Tab, VF, T, Ctl, Extras> L;
+    if (live) L.replay_setup(ra, id);
+    for (int s = 0; s < nstages; ++s) {
+      const int c = c_last - s;
+      if (live && c >= 0) {
+        const int base = (c & 1) * K;  // records at [base, base + K)
+        L.resume(ra, b.ckpt, c, base);
+        const int target = base + min(K, n - c * K);
+        while (L.num_acc < target && L.running(ra)) L.step(ra);
+      }
+      __syncthreads();
+    }
+  } else {
+    AdjSweep<Tab, VF, T> W;
+    if (live) W.setup(a, b, id, n, K);
+    for (int s = 0; s < nstages; ++s) {
+      const int c = c_last - s + 1;
+      if (live && c >= 0 && c <= c_last) {
+        const int64_t base = id * ra.grid_stride + (int64_t)(c & 1) * K;
+        const double tl = b.ckpt[(id * a.n_ckpt + c) * (D + 8) + D + 4];
+        W.segment(a, c, K, ra.grid + base * 2, reinterpret_cast<const T*>(ra.grid_y) + base * D,
+                  tl);
+      }
+      __syncthreads();
+    }
+    if (live) W.finish(a, b);
   }
-  T* yb = reinterpret_cast<T*>(b.y0_bar) + id * D;
-  T* qb = reinterpret_cast<T*>(b.q_bar) + id * P;
-#pragma unroll
-  for (int j = 0; j < D; ++j) yb[j] = ab[j];
-#pragma unroll
-  for (int m = 0; m < P; ++m) qb[m] = gq[m];
 }
 
 template <class Tab, class VF, class T, int Ctl, bool X>
 static int launch_adjoint_v(const ErkKArgs& a, const ErkKArgs& ra, const DiscreteAdjArgs& b,
                             hipStream_t s) {
   const int64_t nb = (a.B + 63) / 64;
-  hipLaunchKernelGGL((erk_adjoint_kernel<Tab, VF, T, Ctl, X>), dim3((unsigned)nb), dim3(64), 0, s,
-                     a, ra, b);
+  hipLaunchKernelGGL((erk_adjoint_kernel<Tab, VF, T, Ctl, X>), dim3((unsigned)nb), dim3(128), 0,
+                     s, a, ra, b);
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }
 

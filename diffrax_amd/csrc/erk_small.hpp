@@ -349,7 +349,10 @@ __device__ __forceinline__ unsigned int lane_rank(unsigned long long mask) {
 // ConstantStepSize).  Extras = the solve uses SaveAt(dense=True) or
 // ClipStepSizeController; the plain instantiation carries none of that code in its step
 // loop.
-template <class Tab, class VF, class T, int Ctl, bool Extras>
+// Ck: the step loop writes the discrete adjoint's checkpoints when a.ckpt is set.  The
+// plain kernels are built without and with it, so a forward solve that records none runs
+// no code of theirs; the extras kernels always carry it.
+template <class Tab, class VF, class T, int Ctl, bool Extras, bool Ck = Extras>
 struct ErkLane {
   static constexpr int D = VF::D;
   static constexpr int P = VF::P;
@@ -376,8 +379,7 @@ struct ErkLane {
   int32_t cs_completed, cs_num;
   double cs_t0, cs_t1;
   int32_t num_steps, num_acc, num_rej, result;
-  int32_t grid_base;  // accepted steps before the first grid record (a replayed segment's start)
-  int32_t ck_left;    // accepted steps until the next checkpoint
+  int32_t ck_left;    // accepted steps until the next checkpoint (INT32_MAX: none recorded)
   int32_t save_index, ts_index;
   double next_ts;    // normalised save time ts[ts_index], +inf when none left
   double after_raw;  // ts[min(ts_index + 1, n_ts - 1)] as stored, loaded one save ahead
@@ -509,8 +511,7 @@ struct ErkLane {
     tnext = jmin(tnext, T1);
     tprev = T0;
     num_steps = num_acc = num_rej = 0;
-    grid_base = 0;
-    ck_left = a.ckpt_every;
+    ck_left = (Ck && a.ckpt) ? a.ckpt_every : INT32_MAX;
     result = DFX_SUCCESSFUL;
     if constexpr (Tab::fsal && !Extras) {
       // The first step's stage 0 (runge_kutta.py:687: z0 = y0 + 0 at t0 + 0*dt), evaluated
@@ -579,10 +580,12 @@ struct ErkLane {
   }
 
   // Restore the state of checkpoint c of the forward solve (same Tab, VF, T, controller
-  // arithmetic and clipping, so the steps that follow repeat the forward's bit for bit);
-  // grid records of the steps that follow start at index 0.  Returns the end time of the
-  // accepted step before the checkpoint (-inf for c = 0).
-  __device__ double resume(const ErkKArgs& a, const double* ckpt, int c) {
+  // arithmetic and clipping, so the steps that follow repeat the forward's bit for bit).
+  // The accepted-step counter restarts at acc0, so the grid records of the steps that
+  // follow go to indices acc0, acc0 + 1, ... of the instance's grid row; nothing else in
+  // a replay (no saves, dense output or checkpoints) reads the counter.  Returns the end
+  // time of the accepted step before the checkpoint (-inf for c = 0).
+  __device__ double resume(const ErkKArgs& a, const double* ckpt, int c, int32_t acc0) {
     const double* r = ckpt + (traj * a.n_ckpt + c) * CW;
 #pragma unroll
     for (int j = 0; j < D; ++j) y[j] = T(r[j]);
@@ -599,9 +602,8 @@ struct ErkLane {
     step_index = ri[3];
     jump_index = ri[4];
     cs_completed = ri[5];
-    num_acc = c * a.ckpt_every;
-    grid_base = num_acc;
-    ck_left = a.ckpt_every;
+    num_acc = acc0;
+    ck_left = INT32_MAX;
     result = DFX_SUCCESSFUL;
     // FSAL: the first stage of the next step.  Without extras the forward evaluated it at
     // set-up (c = 0, begin_times) or as the previous step's last stage, f(t1, y1) with
@@ -631,7 +633,7 @@ struct ErkLane {
     for (int j = 0; j < D; ++j) y[j] = y0p[j];
     load_params(a, id);
     const double T0 = begin_times(a, a.t0, a.t1[id * a.t1_stride]);
-    if (a.ckpt) write_ckpt(a, 0, -__builtin_inf());
+    if (Ck && a.ckpt) write_ckpt(a, 0, -__builtin_inf());
     save_index = 0;
     ts_index = 0;
     ts_row = a.ts ? a.ts + id * a.ts_stride : a.t1;
@@ -815,7 +817,7 @@ struct ErkLane {
 
     if (keep) {
       if (a.grid) {
-        const int64_t rec = traj * a.grid_stride + (num_acc - 1 - grid_base);
+        const int64_t rec = traj * a.grid_stride + (num_acc - 1);
         double* g = a.grid + rec * 2;
         g[0] = ts0;
         g[1] = ts1;
@@ -863,10 +865,14 @@ struct ErkLane {
     }
     tprev = ntprev;
     tnext = ntnext;
-    if (a.ckpt && keep && --ck_left == 0) {
-      // every ckpt_every accepted steps: the discrete adjoint's restart point
-      ck_left = a.ckpt_every;
-      write_ckpt(a, num_acc / a.ckpt_every, ts1);
+    if constexpr (Ck) {
+      if (keep && --ck_left == 0) {
+        // every ckpt_every accepted steps: the discrete adjoint's restart point (without
+        // checkpoints the count starts at INT32_MAX and the test below keeps max_steps =
+        // INT32_MAX safe)
+        ck_left = a.ckpt_every;
+        if (a.ckpt) write_ckpt(a, num_acc / a.ckpt_every, ts1);
+      }
     }
   }
 
@@ -985,11 +991,11 @@ __global__ void __launch_bounds__(256) erk_backsolve_kernel(const ErkKArgs a, co
 // One trajectory per lane, no work queue: used when the launch has a lane for every
 // instance.  Lanes whose trajectory has finished idle (masked) until the wave's last lane
 // is done.
-template <class Tab, class VF, class T, int Ctl, bool Extras>
+template <class Tab, class VF, class T, int Ctl, bool Extras, bool Ck = Extras>
 __global__ void __launch_bounds__(256) erk_small_kernel(const ErkKArgs a) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= a.B) return;
-  ErkLane<Tab, VF, T, Ctl, Extras> L;
+  ErkLane<Tab, VF, T, Ctl, Extras, Ck> L;
   L.start(a, id);
   while (L.running(a)) L.step(a);
   L.finish(a);

@@ -7,10 +7,16 @@ namespace dfx {
 
 template <class Tab, class VF, class T, int G, bool X>
 static void launch_small_g(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
+  if (!X && a.ckpt) {  // plain path with checkpoints: its own build, one lane per instance
+    const int64_t nb = (a.B + block - 1) / block;
+    hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T, G, X, true>), dim3((unsigned)nb), dim3(block),
+                       0, s, a);
+    return;
+  }
   const int64_t nblocks = (lanes + block - 1) / block;
   if (lanes >= a.B) {
-    hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T, G, X>), dim3((unsigned)nblocks), dim3(block), 0, s,
-                       a);
+    hipLaunchKernelGGL((erk_small_kernel<Tab, VF, T, G, X>), dim3((unsigned)nblocks), dim3(block),
+                       0, s, a);
   } else {
     hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(1), 0, s, a.counter);
     hipLaunchKernelGGL((erk_small_queue_kernel<Tab, VF, T, G, X>), dim3((unsigned)nblocks), dim3(block),
