@@ -17,7 +17,8 @@ from ._solver import (AbstractAdaptiveSolver, AbstractERK, AbstractSolver, Butch
                       Dopri5, Heun, Tsit5)
 from ._step_size_controller import (AbstractAdaptiveStepSizeController,
                                     AbstractStepSizeController, ClipStepSizeController,
-                                    ConstantStepSize, PIDController, rms_norm)
+                                    ConstantStepSize, PIDController, adjoint_rms_seminorm,
+                                    rms_norm)
 from ._term import (AbstractPath, AbstractTerm, ControlTerm, IncrementPath, MultiTerm,
                     ODETerm)
 from ._vector_fields import (DiagonalNoise, LinearField, LotkaVolterra, MLPField, OUDrift,

@@ -15,17 +15,23 @@ _CACHE: dict = {}
 
 
 def _key(plan):
-    """Everything that selects a different kernel or changes its memory traffic: the
-    (solver, field, precision, d) kernel, the controller kind (PID coefficients), the
-    dense / clipping / event variant, step recording, and the batch and device."""
+    """Everything that selects a different kernel or changes its work or memory traffic:
+    the (solver, field, precision, d) kernel, the controller kind (PID coefficients) and
+    tolerances (shared or per instance), the save layout (ts count and sharing, t0 / t1 /
+    every-k-th-step rows), max_steps, the dense / clipping / event variant, step and
+    checkpoint recording, and the batch and device."""
     p = plan.problem
     io = getattr(plan, "io", None)
+    has = lambda f: int(bool(io is not None and getattr(io, f)))
     name = torch.cuda.get_device_name(torch.cuda.current_device())
     pid = f"{p.pcoeff}/{p.icoeff}/{p.dcoeff}/{p.error_order}"
-    extras = (f"{p.n_step_ts}/{p.n_jump_ts}/{p.n_events}/"
-              f"{int(bool(io is not None and io.dense_ts))}/{int(bool(io is not None and io.grid))}")
-    return (f"erk:{p.solver}:{p.vf}:{p.precision}:{p.d}:{p.controller}:{pid}:{extras}:"
-            f"{p.batch}:{name}")
+    tol = f"{p.rtol!r}/{p.atol!r}/{has('rtol')}/{has('atol')}"
+    save = (f"{p.n_ts}/{int(p.ts_stride != 0)}/{int(p.t1_stride != 0)}/{p.save_t0}/"
+            f"{p.save_t1}/{p.save_steps}/{p.max_steps}")
+    extras = (f"{p.n_step_ts}/{p.n_jump_ts}/{p.n_events}/{has('dense_ts')}/{has('grid')}/"
+              f"{p.ckpt_every}")
+    return (f"erk:{p.solver}:{p.vf}:{p.precision}:{p.d}:{p.controller}:{pid}:{tol}:{save}:"
+            f"{extras}:{p.batch}:{name}")
 
 
 def _load_file():

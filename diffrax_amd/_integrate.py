@@ -19,7 +19,8 @@ from ._adjoint import AbstractAdjoint, BacksolveAdjoint, RecursiveCheckpointAdjo
 from ._saveat import SaveAt, SubSaveAt, save_y
 from ._solution import RESULTS, Solution
 from ._solver import AbstractERK, AbstractSolver
-from ._step_size_controller import ClipStepSizeController, ConstantStepSize, PIDController
+from ._step_size_controller import (ClipStepSizeController, ConstantStepSize, PIDController,
+                                    rms_norm)
 from ._term import ControlTerm, IncrementPath, MultiTerm, ODETerm
 from ._vector_fields import DiagonalNoise, MLPField, OUDrift, RegisteredVectorField
 
@@ -49,6 +50,43 @@ def _scalar_time(x, name):
     return float(x)
 
 
+class _HostCopies:
+    """Host values of device-resident inputs that the host checks (t1, SaveAt ts).
+
+    A CPU tensor is read in place.  A device tensor is copied to pinned host memory behind
+    the work already queued on the current stream, so the solve can be queued before the
+    checks run: ``get`` then waits for those copies only, never for the solve."""
+
+    def __init__(self):
+        self._vals = {}
+        self._event = None
+        self._pending = False
+
+    def add(self, name, t):
+        if t.device.type == "cpu":
+            self._vals[name] = t.numpy()
+            return
+        h = torch.empty(tuple(t.shape), dtype=t.dtype, pin_memory=True)
+        h.copy_(t, non_blocking=True)
+        self._vals[name] = h
+        self._pending = True
+
+    @property
+    def deferred(self):
+        return self._pending
+
+    def seal(self):
+        if self._pending:
+            self._event = torch.cuda.Event()
+            self._event.record()
+
+    def get(self, name):
+        if self._event is not None:
+            self._event.synchronize()
+        v = self._vals[name]
+        return v.numpy() if isinstance(v, torch.Tensor) else v
+
+
 class _Workspace:
     """Per-device scratch buffers owned by the caller side (torch allocator)."""
 
@@ -73,13 +111,13 @@ class ErkPlan:
     ``launch()`` enqueues the solve on the current stream (no host sync), so a plan can be
     replayed or captured into a CUDA/HIP graph by the caller."""
 
-    def __init__(self, problem, tensors, n_out, batched, direction_sign, launch_cfg=None):
+    def __init__(self, problem, tensors, n_out, batched, launch_cfg=None):
         self.problem = problem
         self.t = tensors
         self.n_out = n_out
         self.batched = batched
         self.launch_cfg = launch_cfg
-        self.direction_sign = direction_sign
+        self.deferred_checks = None
         io = _lib.ErkIO()
         self.io = io
         self._bind()
@@ -187,6 +225,10 @@ def _controller_fields(stepsize_controller, dt0f):
     """The dfx_erk_problem fields of a PIDController or ConstantStepSize."""
     if isinstance(stepsize_controller, PIDController):
         c = stepsize_controller
+        if c.norm is not rms_norm:
+            raise NotImplementedError(
+                "adjoint_rms_seminorm is provided as the norm of the neural-ODE "
+                "BacksolveAdjoint's controller; the small-system kernels use rms_norm")
         per_instance = np.ndim(c.rtol) > 0 or np.ndim(c.atol) > 0
         return dict(controller=_lib.PID, pcoeff=float(c.pcoeff), icoeff=float(c.icoeff),
                     dcoeff=float(c.dcoeff), rtol=0.0 if per_instance else c.rtol,
@@ -227,27 +269,30 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
     t0f = _scalar_time(t0, "t0")
     if t0f is None:
         raise ValueError("t0 must be a scalar")
-    t1f = _scalar_time(t1, "t1")
-    if t1f is not None:
-        t1_t = torch.tensor([t1f], dtype=torch.float64)
-        t1_stride = 0
-        t1_host = np.array([t1f])
-    else:
-        t1_t = torch.as_tensor(t1, dtype=torch.float64).reshape(-1)
-        if t1_t.numel() != B:
+    host = _HostCopies()
+    if isinstance(t1, torch.Tensor) and t1.device.type != "cpu":
+        # device-resident t1 (one value or one per instance): read by the kernel in place
+        t1_t = t1.to(torch.float64).reshape(-1)
+        if t1_t.numel() not in (1, B):
             raise ValueError(f"t1 must be a scalar or have shape [{B}]")
-        t1_stride = 1
-        t1_host = t1_t.cpu().numpy()
-    directions = np.where(t0f < t1_host, 1.0, -1.0)
+        t1_stride = 0 if t1_t.numel() == 1 else 1
+        host.add("t1", t1_t)
+    else:
+        t1f = _scalar_time(t1, "t1")
+        if t1f is not None:
+            t1_t = torch.tensor([t1f], dtype=torch.float64)
+            t1_stride = 0
+        else:
+            t1_t = torch.as_tensor(t1, dtype=torch.float64).reshape(-1)
+            if t1_t.numel() != B:
+                raise ValueError(f"t1 must be a scalar or have shape [{B}]")
+            t1_stride = 1
+        host.add("t1", t1_t)
 
     if dt0 is not None:
         dt0f = _scalar_time(dt0, "dt0")
         if dt0f is None:
             raise NotImplementedError("dt0 must be a scalar")
-        if bool(np.any((t1_host - t0f) * dt0f < 0)):
-            raise RuntimeError(
-                f"Must have (t1 - t0) * dt0 >= 0, we instead got t1 with value {t1}, t0 "
-                f"with value {t0}, dt0 with value {dt0}")
     else:
         dt0f = None
 
@@ -281,7 +326,21 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
             ts_stride = n_ts
         else:
             raise ValueError(f"SaveAt(ts) must be [n] or [{B}, n]")
-        _check_ts(ts_t.cpu().numpy(), t0f, t1_host, directions)
+        host.add("ts", ts_t)
+
+    def check_times():
+        # _integrate.py:1107-1112 (dt0 sign) and 1222-1235 (ts order and range)
+        t1_host = host.get("t1")
+        if dt0f is not None and bool(np.any((t1_host - t0f) * dt0f < 0)):
+            raise RuntimeError(
+                f"Must have (t1 - t0) * dt0 >= 0, we instead got t1 with value {t1}, t0 "
+                f"with value {t0}, dt0 with value {dt0}")
+        if ts_t is not None:
+            _check_ts(host.get("ts"), t0f, t1_host, np.where(t0f < t1_host, 1.0, -1.0))
+
+    host.seal()
+    if not host.deferred:
+        check_times()
 
     clip = None
     if isinstance(stepsize_controller, ClipStepSizeController):
@@ -347,10 +406,15 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
             dense_ts=torch.empty((B, int(max_steps) + 1), dtype=torch.float64, device=device),
             dense_ys=torch.empty((B, int(max_steps) + 1, d), dtype=y0b.dtype, device=device),
             dense_k=torch.empty((B, int(max_steps), S, d), dtype=y0b.dtype, device=device))
-    return ErkPlan(prob, tensors, n_out, batched, directions, launch)
+    plan = ErkPlan(prob, tensors, n_out, batched, launch)
+    # checks of device-resident times, run by diffeqsolve once the solve is queued
+    plan.deferred_checks = check_times if host.deferred else None
+    return plan
 
 
 def _finish(plan, t0, t1, max_steps, throw, ys=None):
+    if plan.deferred_checks is not None:
+        plan.deferred_checks()
     T = plan.t
     stats = {
         "num_steps": T["stats"][:, 0],

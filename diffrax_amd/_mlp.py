@@ -37,6 +37,11 @@ class MlpAdjIO(ctypes.Structure):
                 ("grid", "ckpt", "stats", "ts", "ys_bar", "ybar", "zs", "rows", "partial", "dw")]
 
 
+class MlpBacksolveIO(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("ys", "ts_out", "ys_bar", "state", "rows", "partial", "dw", "ybar", "result")]
+
+
 class MlpIO(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
                 ("y0", "ts", "ys", "ts_out", "stats", "result", "grid", "ckpt")]
@@ -58,6 +63,13 @@ _SIGS.update({
     "dfx_mlp_adj_segment": ([_P(MlpProblem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                              ctypes.c_void_p, _P(MlpAdjIO), ctypes.c_int32, ctypes.c_int32,
                              ctypes.c_void_p], ctypes.c_int),
+    "dfx_mlp_backsolve_workspace": ([_P(MlpProblem), ctypes.c_int32, _P(ctypes.c_size_t),
+                                     _P(ctypes.c_size_t), _P(ctypes.c_size_t)], ctypes.c_int),
+    "dfx_mlp_backsolve": ([_P(MlpProblem), ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                           ctypes.c_void_p, _P(MlpBacksolveIO), ctypes.c_int32, ctypes.c_int32,
+                           ctypes.c_void_p], ctypes.c_int),
+    "dfx_mlp_backsolve_status": ([_P(MlpProblem), ctypes.c_int32, ctypes.c_void_p,
+                                  _P(ctypes.c_void_p)], ctypes.c_int),
 })
 _lib._SIGS.update(_SIGS)
 
@@ -132,6 +144,22 @@ class MlpPlan:
                                *(ctypes.c_void_p(b.data_ptr()) for b in self.b),
                                ctypes.byref(self.io), _stream())
         _lib.check(rc, "dfx_mlp_solve")
+        self._nmax_event = None   # a copy noted for an earlier launch is stale
+
+    def note_max_accepted(self):
+        """Queue a copy of the largest accepted-step count to pinned host memory behind the
+        solve; the backward pass sizes its segments from it (max_accepted) without
+        draining the stream."""
+        self._nmax_host = torch.empty((), dtype=torch.int32, pin_memory=True)
+        self._nmax_host.copy_(self.stats[:, 1].amax(), non_blocking=True)
+        self._nmax_event = torch.cuda.Event()
+        self._nmax_event.record()
+
+    def max_accepted(self):
+        if getattr(self, "_nmax_event", None) is None:
+            self.note_max_accepted()
+        self._nmax_event.synchronize()
+        return int(self._nmax_host)
 
 
 class _Probe:
@@ -147,6 +175,10 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
     if not isinstance(stepsize_controller, PIDController):
         raise NotImplementedError("the neural-ODE kernel implements PIDController")
     c = stepsize_controller
+    from ._step_size_controller import rms_norm
+    if c.norm is not rms_norm:
+        raise NotImplementedError("the forward solve's controller uses rms_norm; "
+                                  "adjoint_rms_seminorm is for BacksolveAdjoint's controller")
     if float(c.pcoeff) != 0.0 or float(c.dcoeff) != 0.0:
         raise NotImplementedError("the neural-ODE kernel implements the integral controller "
                                   "(pcoeff = dcoeff = 0)")
@@ -159,21 +191,34 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
     batched = y0.ndim == 2
     y0b = y0 if batched else y0.reshape(1, -1)
     ts = None
+    check = None
     if sub.ts is not None:
+        # device-resident ts are checked from a pinned copy once the solve is queued
+        from ._integrate import _HostCopies
         ts = torch.as_tensor(sub.ts, dtype=torch.float64).reshape(-1)
+        host = _HostCopies()
+        host.add("ts", ts)
+        host.seal()
         t0f, t1f = float(t0), float(t1)
-        dirn = 1.0 if t0f < t1f else -1.0
-        tn = ts.numpy() * dirn
-        if len(tn) > 1 and (tn[1:] < tn[:-1]).any():
-            raise RuntimeError("saveat.ts must be increasing or decreasing.")
-        if ((tn > t1f * dirn) | (tn < t0f * dirn)).any():
-            raise RuntimeError("saveat.ts must lie between t0 and t1.")
+
+        def check():
+            dirn = 1.0 if t0f < t1f else -1.0
+            tn = host.get("ts") * dirn
+            if len(tn) > 1 and (tn[1:] < tn[:-1]).any():
+                raise RuntimeError("saveat.ts must be increasing or decreasing.")
+            if ((tn > t1f * dirn) | (tn < t0f * dirn)).any():
+                raise RuntimeError("saveat.ts must lie between t0 and t1.")
+        if not host.deferred:
+            check()
+            check = None
     if not allocate:
         probe = _Probe()
         probe.ts_host = ts
+        probe.deferred_checks = check
         return probe
     plan = MlpPlan(terms.vector_field, t0, t1, dt0, y0b, ts, c, sub, max_steps, ckpt_every, device)
     plan.batched = batched
+    plan.deferred_checks = check
     return plan
 
 
@@ -182,6 +227,8 @@ _ = (math, np)
 
 def _solution(plan, t0, t1, max_steps, throw, ys):
     from ._solution import RESULTS, Solution
+    if getattr(plan, "deferred_checks", None) is not None:
+        plan.deferred_checks()
     st = plan.stats
     stats = {"num_steps": st[:, 0], "num_accepted_steps": st[:, 1],
              "num_rejected_steps": st[:, 2], "max_steps": max_steps}
@@ -210,8 +257,7 @@ def backward_mlp(plan, ys_bar):
     dev = plan.ys.device
     B = plan.ys.shape[0]
     p = plan.prob
-    nacc = plan.stats[:, 1]
-    n_max = int(nacc.max().item()) if B else 0
+    n_max = plan.max_accepted() if B else 0
     ys_bar = ys_bar.to(device=dev, dtype=torch.float32).contiguous()
     n_ts = p.n_ts
     ybar = (ys_bar[:, -1, :].clone() if p.save_t1 else torch.zeros((B, 64), dtype=torch.float32, device=dev))
@@ -263,6 +309,7 @@ class _MlpSolve(torch.autograd.Function):
                        cfg["ctrl"], cfg["sub"], cfg["max_steps"],
                        cfg["ckpt_every"] if need else 0, cfg["device"])
         plan.launch()
+        plan.note_max_accepted()
         ctx.plan = plan
         ctx.dtypes = [t.dtype for t in (y0, W1, b1, W2, b2, W3, b3)]
         ctx.mark_non_differentiable(plan.stats, plan.result, plan.ts_out)
@@ -271,6 +318,139 @@ class _MlpSolve(torch.autograd.Function):
     @staticmethod
     def backward(ctx, ys_bar, *_):
         grads = backward_mlp(ctx.plan, ys_bar)
+        out = [g.to(dt) if need else None
+               for g, dt, need in zip(grads, ctx.dtypes, ctx.needs_input_grad[:7])]
+        return (*out, None)
+
+
+# Lock-step iterations per backsolve launch: the row buffers hold chunk * 7 stages rows of
+# every trajectory (4.6 KB each in float32), and the host checks for unfinished rows after
+# every launch.
+BACKSOLVE_CHUNK = 8
+
+
+def backsolve_problem(plan, adjoint):
+    """The problem of the backward solves: the forward's, with BacksolveAdjoint's kwargs
+    applied as the reference applies them (_adjoint.py:623-632).  The neural-ODE kernel
+    solves them with the adjoint seminorm (_misc.py:85-116), Tsit5 and integral control."""
+    from ._solver import Tsit5
+    from ._step_size_controller import PIDController, adjoint_rms_seminorm
+    kw = adjoint.kwargs
+    ctrl = kw.get("stepsize_controller")
+    if not (isinstance(ctrl, PIDController) and ctrl.norm is adjoint_rms_seminorm):
+        raise NotImplementedError(
+            "BacksolveAdjoint on the neural-ODE kernel solves with the adjoint seminorm: pass "
+            "BacksolveAdjoint(stepsize_controller=PIDController(rtol, atol, "
+            "norm=adjoint_rms_seminorm)).  Under the default norm every trajectory's "
+            "98,880 weight adjoints would enter its error control, which is not provided.")
+    if not isinstance(kw.get("solver", Tsit5()), Tsit5):
+        raise NotImplementedError("the neural-ODE backsolve runs Tsit5")
+    if float(ctrl.pcoeff) != 0.0 or float(ctrl.dcoeff) != 0.0:
+        raise NotImplementedError("the neural-ODE backsolve implements the integral controller "
+                                  "(pcoeff = dcoeff = 0)")
+    if "dt0" in kw and kw["dt0"] is None:
+        raise NotImplementedError("pass dt0 for the neural-ODE backsolve")
+    nan = float("nan")
+    f = plan.prob
+    back = MlpProblem()
+    ctypes.memmove(ctypes.byref(back), ctypes.byref(f), ctypes.sizeof(f))
+    back.precision = _lib.F32   # float32 MFMA for the backward products (both matmul forms)
+    back.ckpt_every = 0
+    back.dt0 = float(kw["dt0"]) if "dt0" in kw else -float(f.dt0)
+    back.max_steps = int(kw.get("max_steps", f.max_steps))
+    back.rtol, back.atol = _scalar_tol(ctrl.rtol), _scalar_tol(ctrl.atol)
+    back.pcoeff, back.icoeff, back.dcoeff = float(ctrl.pcoeff), float(ctrl.icoeff), float(ctrl.dcoeff)
+    back.dtmin = nan if ctrl.dtmin is None else float(ctrl.dtmin)
+    back.dtmax = nan if ctrl.dtmax is None else float(ctrl.dtmax)
+    back.force_dtmin = int(ctrl.force_dtmin)
+    back.factormin, back.factormax, back.safety = ctrl.factormin, ctrl.factormax, ctrl.safety
+    back.error_order = nan if ctrl.error_order is None else float(ctrl.error_order)
+    return back
+
+
+def backsolve_mlp(plan, back, ys_bar, chunk=BACKSOLVE_CHUNK):
+    """Gradients (y0, W1, b1, W2, b2, W3, b3) of sum(ys * ys_bar) by the continuous adjoint
+    (dfx_mlp_backsolve), and the backward solves' result codes.  The weights pack for
+    float32 products (the backward runs them in float32 MFMA for either matmul form)."""
+    lib = _lib.load()
+    dev = plan.ys.device
+    B = plan.ys.shape[0]
+    ys_bar = ys_bar.to(device=dev, dtype=torch.float32).contiguous()
+    ybar = torch.zeros((B, 64), dtype=torch.float32, device=dev)
+    result = torch.zeros((B,), dtype=torch.int32, device=dev)
+    dw = torch.zeros(98880, dtype=torch.float32, device=dev)
+    if B == 0:
+        return (ybar,) + _split_dw(dw), result
+    W1, _, W2, _, W3, _ = plan.vfo.params
+    packed, _ = pack_weights(W1, W2, W3, "f32", dev)
+    st_b, rows_b, part_b = (ctypes.c_size_t(0) for _ in range(3))
+    _lib.check(lib.dfx_mlp_backsolve_workspace(ctypes.byref(back), chunk, ctypes.byref(st_b),
+                                               ctypes.byref(rows_b), ctypes.byref(part_b)),
+               "dfx_mlp_backsolve_workspace")
+    state = torch.zeros(int(st_b.value), dtype=torch.uint8, device=dev)
+    rows = torch.zeros(int(rows_b.value) // 4, dtype=torch.float32, device=dev)
+    part = torch.empty(int(part_b.value) // 4, dtype=torch.float32, device=dev)
+    io = MlpBacksolveIO(ys=plan.ys.data_ptr(), ts_out=plan.ts_out.data_ptr(),
+                        ys_bar=ys_bar.data_ptr(), state=state.data_ptr(), rows=rows.data_ptr(),
+                        partial=part.data_ptr(), dw=dw.data_ptr(), ybar=ybar.data_ptr(),
+                        result=result.data_ptr())
+    cptr = ctypes.c_void_p(0)
+    _lib.check(lib.dfx_mlp_backsolve_status(ctypes.byref(back), chunk,
+                                            ctypes.c_void_p(state.data_ptr()), ctypes.byref(cptr)),
+               "dfx_mlp_backsolve_status")
+    off = int(cptr.value) - state.data_ptr()
+    counters = state[off:off + 8].view(torch.int32)
+    host = torch.empty(2, dtype=torch.int32, pin_memory=True)
+    first = 1
+    while True:
+        _lib.check(lib.dfx_mlp_backsolve(ctypes.byref(back), ctypes.c_void_p(packed.data_ptr()),
+                                         *(ctypes.c_void_p(b.data_ptr()) for b in plan.b),
+                                         ctypes.byref(io), chunk, first, _stream()),
+                   "dfx_mlp_backsolve")
+        first = 0
+        host.copy_(counters, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+        if int(host[0]) == 0:
+            break
+    return (ybar,) + _split_dw(dw), result
+
+
+def _split_dw(dw):
+    """dW1 | dW2 | dW3 | db1 | db2 | db3 views of the flat accumulator."""
+    o = 0
+    Ws = []
+    for r, c in ((256, 64), (256, 256), (64, 256)):
+        Ws.append(dw[o:o + r * c].view(r, c))
+        o += r * c
+    return Ws[0], dw[o:o + 256], Ws[1], dw[o + 256:o + 512], Ws[2], dw[o + 512:o + 576]
+
+
+class _MlpBacksolve(torch.autograd.Function):
+    """ys of a neural-ODE solve with the continuous adjoint (dfx_mlp_backsolve) as its
+    backward."""
+
+    @staticmethod
+    def forward(ctx, y0, W1, b1, W2, b2, W3, b3, cfg):
+        params = _Params([t.detach() for t in (W1, b1, W2, b2, W3, b3)], cfg["matmul"])
+        plan = MlpPlan(params, cfg["t0"], cfg["t1"], cfg["dt0"], y0.detach(), cfg["ts"],
+                       cfg["ctrl"], cfg["sub"], cfg["max_steps"], 0, cfg["device"])
+        plan.launch()
+        ctx.plan = plan
+        ctx.back = backsolve_problem(plan, cfg["adjoint"])
+        ctx.throw = cfg["throw"]
+        ctx.dtypes = [t.dtype for t in (y0, W1, b1, W2, b2, W3, b3)]
+        ctx.mark_non_differentiable(plan.stats, plan.result, plan.ts_out)
+        return plan.ys, plan.stats, plan.result, plan.ts_out
+
+    @staticmethod
+    def backward(ctx, ys_bar, *_):
+        from ._solution import RESULTS
+        ys_bar = torch.zeros_like(ctx.plan.ys) if ys_bar is None else ys_bar
+        grads, result = backsolve_mlp(ctx.plan, ctx.back, ys_bar)
+        if ctx.throw:
+            bad = result != 0
+            if bool(bad.any()):
+                raise RuntimeError(RESULTS[int(result[bad][0].item())])
         out = [g.to(dt) if need else None
                for g, dt, need in zip(grads, ctx.dtypes, ctx.needs_input_grad[:7])]
         return (*out, None)
@@ -296,23 +476,29 @@ def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
                            max_steps, device)
         plan.launch()
         return _solution(plan, t0, t1, max_steps, throw, plan.ys)
-    from ._adjoint import RecursiveCheckpointAdjoint
-    if not isinstance(adjoint, RecursiveCheckpointAdjoint):
-        raise NotImplementedError("gradients are provided through RecursiveCheckpointAdjoint")
+    from ._adjoint import BacksolveAdjoint, RecursiveCheckpointAdjoint
+    if not isinstance(adjoint, (RecursiveCheckpointAdjoint, BacksolveAdjoint)):
+        raise NotImplementedError("gradients are provided through RecursiveCheckpointAdjoint "
+                                  "and BacksolveAdjoint")
     # validation and shapes via a throwaway-free preparation of the configuration
     probe = prepare_mlp(terms, solver, t0, t1, dt0, y0t.detach(), saveat, stepsize_controller,
                         max_steps, device, ckpt_every=0, allocate=False)
     cfg = dict(matmul=vfo.matmul, t0=t0, t1=t1, dt0=dt0, ts=probe.ts_host, ctrl=stepsize_controller,
-               sub=saveat.subs, max_steps=max_steps, ckpt_every=checkpoint_every(adjoint, max_steps),
-               device=device)
+               sub=saveat.subs, max_steps=max_steps, device=device)
     batched = y0t.ndim == 2
     y0b = y0t if batched else y0t.reshape(1, -1)
-    ys, stats, result, ts_out = _MlpSolve.apply(y0b.to(device), *vfo.params, cfg)
+    if isinstance(adjoint, BacksolveAdjoint):
+        cfg.update(adjoint=adjoint, throw=bool(adjoint.kwargs.get("throw", throw)))
+        ys, stats, result, ts_out = _MlpBacksolve.apply(y0b.to(device), *vfo.params, cfg)
+    else:
+        cfg.update(ckpt_every=checkpoint_every(adjoint, max_steps))
+        ys, stats, result, ts_out = _MlpSolve.apply(y0b.to(device), *vfo.params, cfg)
 
     class _P:
         pass
     pl = _P()
     pl.stats, pl.result, pl.ts_out, pl.batched = stats, result, ts_out, batched
+    pl.deferred_checks = probe.deferred_checks
     return _solution(pl, t0, t1, max_steps, throw, ys)
 
 

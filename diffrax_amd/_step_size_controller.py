@@ -32,6 +32,17 @@ def rms_norm(x):
     return torch.sqrt((x * x).sum()) / math.sqrt(x.numel())
 
 
+def adjoint_rms_seminorm(x):
+    """The adjoint seminorm (_misc.py:85-116): the RMS norm of (y, a_y) of a backward
+    state (y, a_y, a_args, a_terms), leaving the parameter adjoints out of the error
+    control.  For PIDController(norm=adjoint_rms_seminorm) as the stepsize_controller of
+    BacksolveAdjoint; the neural-ODE backsolve kernel implements it."""
+    assert isinstance(x, tuple) and len(x) == 4
+    import torch
+    y, a_y = (torch.as_tensor(v).reshape(-1) for v in x[:2])
+    return rms_norm(torch.cat([y, a_y]))
+
+
 class AbstractStepSizeController:
     def wrap(self, direction):
         return self
@@ -74,8 +85,9 @@ class PIDController(AbstractAdaptiveStepSizeController, metaclass=_MetaPID):
                 "version 0.1.0. (As the choice of tolerance is nearly always "
                 "something that you, as an end user, should make an explicit choice "
                 "about.)")
-        if norm is not rms_norm:
-            raise NotImplementedError("diffrax_amd kernels implement rms_norm only")
+        if norm is not rms_norm and norm is not adjoint_rms_seminorm:
+            raise NotImplementedError(
+                "diffrax_amd kernels implement rms_norm and adjoint_rms_seminorm only")
         self.rtol = _tolerance(rtol, "rtol")
         self.atol = _tolerance(atol, "atol")
         self.norm = norm

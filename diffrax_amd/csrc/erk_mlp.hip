@@ -381,16 +381,7 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   a.n_ckpt = dfx_mlp_n_ckpt(p);
   a.has_dt0 = !isnan(p->dt0);
   a.t0 = p->t0; a.t1 = p->t1; a.dt0 = p->dt0; a.error_order = error_order;
-  PidParams& q = a.pid;
-  q.rtol = p->rtol; q.atol = p->atol;
-  q.coeff1 = (p->icoeff + p->pcoeff + p->dcoeff) / error_order;
-  q.coeff2 = -(p->pcoeff + 2.0 * p->dcoeff) / error_order;
-  q.coeff3 = p->dcoeff / error_order;
-  q.dtmin = p->dtmin; q.dtmax = p->dtmax;
-  q.has_dtmin = !isnan(p->dtmin); q.has_dtmax = !isnan(p->dtmax);
-  q.force_dtmin = p->force_dtmin; q.factormin = p->factormin; q.factormax = p->factormax;
-  q.safety = p->safety;
-  if (!pid_is_integral(q)) return DFX_E_ARG;  // the neural-ODE kernel implements the I controller
+  if (!mlp_pid_params(p, a.pid)) return DFX_E_ARG;  // the neural-ODE kernel implements the I controller
   if (!a.has_dt0) return DFX_E_ARG;           // dt0 must be given
   const unsigned nblocks = (unsigned)((p->batch + MLP_TILE - 1) / MLP_TILE);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

@@ -312,11 +312,16 @@ __host__ __device__ inline DwLayout dw_layout() {
 // chunk back to back, which then read the chunk's rows through one L2 instead of eight.
 constexpr int kXcds = 8;
 
-template <class P>
+// SC: the delta rows are multiplied by rscale[row] as they load (the backsolve's per-row
+// weights dt * b_i, zero for rejected steps and idle rows; a row of zero weight is dropped
+// from both operands, so a non-finite value in it contributes nothing).  Float32 rows only.
+template <class P, bool SC = false>
 __global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R,
                                                          float* __restrict__ partial,
                                                          float* __restrict__ partial_db,
-                                                         int nch) {
+                                                         int nch,
+                                                         const float* __restrict__ rscale = nullptr) {
+  static_assert(!SC || P::KA == 1, "row scales need one row per fragment element");
   const DwLayout lay = dw_layout();
   const int ntile = lay.first[3];
   const int xcd = (int)(blockIdx.x % kXcds), slot = (int)(blockIdx.x / kXcds);
@@ -346,6 +351,11 @@ __global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R,
       const int64_t rb = (r0 + u * KS) / KA + (lane >> 5);
       fa[u] = P::load_a(A + (rb * WA + ca) * KA);
       fb[u] = P::load_a(Bm + (rb * WB + cb) * KA);
+      if constexpr (SC) {
+        const float w = rscale[rb];
+        fa[u] = (w != 0.0f) ? fa[u] * w : 0.0f;
+        fb[u] = (w != 0.0f) ? fb[u] : 0.0f;
+      }
     }
   };
   auto mma_group = [&](const frag_t (&fa)[U], const frag_t (&fb)[U]) {
@@ -410,6 +420,311 @@ __global__ void mlp_dw_reduce_kernel(const float* __restrict__ partial,
   dw_acc[e] += s;
 }
 
+// ---- BacksolveAdjoint of the neural ODE ---------------------------------------------
+// The continuous adjoint (_adjoint.py:539-873) with the adjoint seminorm
+// (_misc.py:85-116): between consecutive save times, from the last one back to t0, the
+// augmented state (y, a_y, a_theta) is integrated backwards by Tsit5 with the integral PID
+// controller, after a_y takes that save's cotangent and y restarts from the forward's saved
+// state, under the adjoint vector field (AdjointTerm, _term.py:767-917):
+//   dy/dt = f(y),  da_y/dt = vjp(f, y)(-a_y),  da_theta/dt = vjp(f, theta)(-a_y).
+// The seminorm leaves a_theta out of the error norm (RMS over the 128 components of y and
+// a_y), so a_theta only accumulates: a_theta(t0) = sum over accepted steps and stages of
+// dt b_i vjp_theta(z_i).  Each stage's vjp_theta is the set of outer products delta_l x
+// input_l; the kernel writes those rows (unscaled) to the row buffers and their weights
+// dt b_i (0 for rejected steps and idle rows) to rscale, and the split-K GEMM above forms
+// the sum over all rows (trajectories, steps and stages) for the launch.
+// One workgroup per tile of 32 rows; each launch runs at most Q lock-step iterations (one
+// attempted step of every row) and leaves unfinished rows' state in BsRow / ya for the
+// next launch.  Each iteration evaluates stage 0 afresh: f(y) at an unchanged y gives the
+// same value as the FSAL stage, and fresh evaluation also covers the first step after
+// each save (made_jump).
+struct BsRow {
+  double tprev, tnext, T1, floor_t1, inv, pinv;
+  int32_t at_dtmin, s, nsteps, dir, result, res_total, done, pad_;
+};
+
+struct BsArgs {
+  const float* ys;       // [B, n_out, 64] forward saved states
+  const double* ts_out;  // [B, n_out] their times
+  const float* ys_bar;   // [B, n_out, 64] cotangents
+  float* ya;             // [B_pad, 128] y then a_y of each row between launches
+  BsRow* rows;           // [B_pad]
+  float* rscale;         // [Q * 7 * B_pad] weights of the weight-gradient rows
+  float* ybar;           // out [B, 64] cotangent of y0
+  int32_t* result;       // out [B]
+  int32_t* counters;     // [2]: rows unfinished after the launch, iterations run (max)
+  int64_t B, B_pad;
+  int32_t n_out, save_t0, max_steps, Q, first;
+  double t0, dt0;        // the forward t0; dt0 of the backward solves
+  PidParams pid;         // the backward controller
+};
+
+template <class P>
+__global__ void __launch_bounds__(MLP_THREADS, 1)
+mlp_backsolve_kernel(const BsArgs a, const MlpWeights<P> W, const RowBufs<P> R) {
+  static_assert(P::KA == 1, "float32 rows");
+  extern __shared__ __align__(16) unsigned char smem[];
+  MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
+  constexpr bool F = P::fast_math;
+  using Tab = Tsit5Tab;
+  constexpr int S = Tab::S;
+  using TL = TabLit<Tab, float>;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int orow = tid >> 3, c0 = (tid & 7) * 8, gbase = lane & ~7;
+  const bool lead = (tid & 7) == 0;
+  const int tile = blockIdx.x;
+  const int64_t traj = (int64_t)tile * MLP_TILE + orow;
+  const bool valid = traj < a.B;
+  const int s_end = a.save_t0 ? 1 : 0;
+
+  float y[8], av[8];
+  BsRow st;
+  // Start the backward solve over save interval s: y from the forward's row s, a_y plus
+  // that row's cotangent, span [ts_out[s], ts_out[s - 1] or t0] (_scan_fun).
+  auto begin = [&](int s) {
+    const float* ysr = a.ys + (traj * a.n_out + s) * MLP_D + c0;
+    const float* gr = a.ys_bar + (traj * a.n_out + s) * MLP_D + c0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) { y[j] = ysr[j]; av[j] = av[j] + gr[j]; }
+    const double t0r = a.ts_out[traj * a.n_out + s];
+    const double t1r = s > 0 ? a.ts_out[traj * a.n_out + s - 1] : a.t0;
+    st.s = s;
+    st.dir = (t0r < t1r) ? 1 : -1;
+    const double T0 = t0r * st.dir;
+    st.T1 = t1r * st.dir;
+    double fl = st.T1;
+    for (int r = 0; r < 100; ++r) fl = prevbefore(fl);
+    st.floor_t1 = fl;
+    double dt0v = a.dt0 * st.dir;
+    if (a.pid.has_dtmax) dt0v = jmin(dt0v, a.pid.dtmax);
+    if (a.pid.has_dtmin) dt0v = jmax(dt0v, a.pid.dtmin);
+    st.tprev = T0;
+    st.tnext = jmin(T0 + dt0v, st.T1);
+    st.inv = 1.0; st.pinv = 1.0; st.at_dtmin = 0;
+    st.nsteps = 0;
+    st.result = DFX_SUCCESSFUL;
+  };
+  // Move past finished intervals; after the last one the row's a_y (plus the t0 row's
+  // cotangent when t0 was saved) is dL/dy0.
+  auto settle = [&]() {
+    while (!st.done) {
+      if (st.tprev < st.T1 && st.result == DFX_SUCCESSFUL && st.nsteps < a.max_steps) return;
+      if (st.tprev < st.T1 && st.result == DFX_SUCCESSFUL) st.result = DFX_MAX_STEPS_REACHED;
+      if (st.res_total == DFX_SUCCESSFUL) st.res_total = st.result;
+      if (st.s - 1 >= s_end) {
+        begin(st.s - 1);
+      } else {
+        if (a.save_t0 && a.n_out > 0) {
+          const float* g0 = a.ys_bar + traj * a.n_out * MLP_D + c0;
+#pragma unroll
+          for (int j = 0; j < 8; ++j) av[j] = av[j] + g0[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a.ybar[traj * MLP_D + c0 + j] = av[j];
+        if (lead) a.result[traj] = st.res_total;
+        st.done = 1;
+      }
+    }
+  };
+
+  if (!valid) {
+    st.done = 1;
+  } else if (a.first) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) av[j] = 0.0f;
+    st.res_total = DFX_SUCCESSFUL;
+    st.done = 0;
+    if (a.n_out - 1 >= s_end) {
+      begin(a.n_out - 1);
+    } else {
+      st.tprev = st.T1 = 0.0;  // nothing saved after t0: no interval
+      st.s = s_end;
+      st.result = DFX_SUCCESSFUL;
+    }
+    settle();
+  } else {
+    st = a.rows[traj];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      y[j] = a.ya[traj * 128 + c0 + j];
+      av[j] = a.ya[traj * 128 + 64 + c0 + j];
+    }
+  }
+
+  const int64_t rb0 = (int64_t)tile * MLP_TILE + orow;
+  auto stage_row0 = [&](int q, int i) { return ((int64_t)q * S + i) * a.B_pad + (int64_t)tile * MLP_TILE; };
+  int q = 0;
+  for (; q < a.Q; ++q) {
+    const bool run = !st.done;
+    if (!__syncthreads_or(run)) break;
+    const double ts0 = st.tprev, ts1 = st.tnext;
+    const float control = run ? (float)((double)st.dir * (ts1 - ts0)) : 0.0f;
+    float ky[S][8], ka[S][8], zy[8], za[8];
+    static_for<0, S>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if constexpr (i == 0) {
+          zy[j] = y[j];
+          za[j] = av[j];
+        } else {
+          float cy[S], ca[S];
+#pragma unroll
+          for (int m = 0; m < S; ++m) {
+            cy[m] = (m < i) ? ky[m][j] : 0.0f;
+            ca[m] = (m < i) ? ka[m][j] : 0.0f;
+          }
+          zy[j] = y[j] + comb<Tab, 0, i, 0, i, false>(TL::v.a[i], cy, 0.0f);
+          za[j] = av[j] + comb<Tab, 0, i, 0, i, false>(TL::v.a[i], ca, 0.0f);
+        }
+        if (!run) { zy[j] = 0.0f; za[j] = 0.0f; }  // idle rows: zero deltas
+        store_input<P>(L, orow, c0 + j, zy[j]);
+      }
+      __syncthreads();
+      float f[8];
+      mlp_forward<P>(W, L, wave, lane, orow, c0, f);
+      const int64_t row0 = stage_row0(q, i);
+      rows_out<P>(L.x, L.LDX, MLP_D, R.b[1], row0);
+      rows_out<P>(L.h1, L.LDH, MLP_H, R.b[3], row0);
+      rows_out<P>(L.h2, L.LDH, MLP_H, R.b[5], row0);
+      // vjp with cotangent w = -a_y of the stage: delta3 = w (1 - f^2)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) L.d3[orow * L.LDX + c0 + j] = P::to_act(-za[j] * (1.0f - f[j] * f[j]));
+      __syncthreads();
+      rows_out<P>(L.d3, L.LDX, MLP_D, R.b[4], row0);
+      // delta2 = (delta3 . W3) softplus'(h2)
+      floatx16_t acc[2];
+      wave_gemm<P, MLP_D / P::KSTEP, 2>(L.d3, L.d3, L.LDX, W.w3t, 2 * wave, 0, MLP_D / P::KSTEP, acc, lane);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 64 * wave + 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = acc_row(lane, v);
+          const float h = (float)L.h2[r * L.LDH + col];
+          L.d[r * L.LDH + col] = P::to_act(acc[t][v] * dsoftplus_from_value<F>(h));
+        }
+      }
+      __syncthreads();
+      // delta1 = (delta2 . W2) softplus'(h1), into the h2 buffer (h2 is in its row buffer)
+      wave_gemm<P, MLP_H / P::KSTEP, 2>(L.d, L.d, L.LDH, W.w2t, 2 * wave, 0, MLP_H / P::KSTEP, acc, lane);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 64 * wave + 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = acc_row(lane, v);
+          const float h = (float)L.h1[r * L.LDH + col];
+          L.h2[r * L.LDH + col] = P::to_act(acc[t][v] * dsoftplus_from_value<F>(h));
+        }
+      }
+      __syncthreads();
+      rows_out<P>(L.d, L.LDH, MLP_H, R.b[2], row0);
+      rows_out<P>(L.h2, L.LDH, MLP_H, R.b[0], row0);
+      // vjp_y = delta1 . W1 (split-K over the 256 hidden units)
+      constexpr int KS1 = MLP_H / P::KSTEP;
+      wave_gemm<P, KS1, 2>(L.h2, L.h2, L.LDH, W.w1t, 0, wave * (KS1 / 4), (wave + 1) * (KS1 / 4), acc, lane);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) L.part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int c = c0 + j;
+        float sv = L.part[(0 * MLP_TILE + orow) * L.LDP + c];
+        sv += L.part[(1 * MLP_TILE + orow) * L.LDP + c];
+        sv += L.part[(2 * MLP_TILE + orow) * L.LDP + c];
+        sv += L.part[(3 * MLP_TILE + orow) * L.LDP + c];
+        ky[i][j] = control * f[j];
+        ka[i][j] = control * sv;
+      }
+      __syncthreads();
+    });
+    // y1, a1 = the last stage (SSAL); embedded error over the 128 components of (y, a_y)
+    double part = 0.0;
+    bool anynan = false;
+    float ey[8], ea[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      float cy[S], ca[S];
+#pragma unroll
+      for (int m = 0; m < S; ++m) { cy[m] = ky[m][j]; ca[m] = ka[m][j]; }
+      ey[j] = comb<Tab, 2, 0, 0, S, false>(TL::v.berr, cy, 0.0f);
+      ea[j] = comb<Tab, 2, 0, 0, S, false>(TL::v.berr, ca, 0.0f);
+      anynan = anynan || (zy[j] != zy[j]) || (za[j] != za[j]);
+    }
+    const unsigned long long nanmask = __ballot(anynan);
+    const bool group_nan = ((nanmask >> gbase) & 0xffull) != 0;
+    auto acc_err = [&](float e0, float v0, float v1) {
+      const double v1c = group_nan ? (double)v0 : (double)v1;
+      const double yy = jmax(fabs((double)v0), fabs(v1c));
+      double e = (double)e0;
+      if (e != e) e = __builtin_inf();
+      e = e / (a.pid.atol + yy * a.pid.rtol);
+      part += e * e;
+    };
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc_err(ey[j], y[j], zy[j]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc_err(ea[j], av[j], za[j]);
+    double ssum = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) ssum += __shfl(part, gbase + k);
+    const double scaled = sqrt(ssum / 128.0);
+    PidState npid;
+    npid.inv = st.inv; npid.pinv = st.pinv; npid.at_dtmin = st.at_dtmin;
+    double ntprev, ntnext;
+    int code = 0;
+    const bool keep = pid_adapt<PID_INTEGRAL>(a.pid, scaled, ts0, ts1, npid, ntprev, ntnext, code);
+    ntprev = jmin(ntprev, st.T1);
+    if (ntnext > st.floor_t1) ntnext = keep ? st.T1 : ntprev + 0.5 * (st.T1 - ntprev);
+    // weights of this iteration's weight-gradient rows: dt b_i for an accepted step
+    if (lead) {
+#pragma unroll
+      for (int i = 0; i < S; ++i)
+        a.rscale[((int64_t)q * S + i) * a.B_pad + rb0] = (run && keep) ? control * TL::v.b[i] : 0.0f;
+    }
+    if (run) {
+      st.inv = npid.inv; st.pinv = npid.pinv; st.at_dtmin = npid.at_dtmin;
+      ++st.nsteps;
+      if (st.result == DFX_SUCCESSFUL) st.result = code;
+      if (keep) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) { y[j] = zy[j]; av[j] = za[j]; }
+      }
+      st.tprev = ntprev;
+      st.tnext = ntnext;
+      settle();
+    }
+  }
+  // rows of the iterations this tile did not run carry no weight
+  if (lead) {
+    for (int qq = q; qq < a.Q; ++qq) {
+#pragma unroll
+      for (int i = 0; i < S; ++i) a.rscale[((int64_t)qq * S + i) * a.B_pad + rb0] = 0.0f;
+    }
+  }
+  if (tid == 0 && q > 0) atomicMax(&a.counters[1], q);
+  // Every row's state goes back, finished rows included: the next launch must see their
+  // done flag (or it would run their intervals again from an older state).
+  if (valid) {
+    if (!st.done) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        a.ya[traj * 128 + c0 + j] = y[j];
+        a.ya[traj * 128 + 64 + c0 + j] = av[j];
+      }
+    }
+    if (lead) {
+      a.rows[traj] = st;
+      if (!st.done) atomicAdd(&a.counters[0], 1);
+    }
+  }
+}
+
 }  // namespace dfx
 
 using namespace dfx;
@@ -424,6 +739,29 @@ int64_t seg_rows(int64_t B, int32_t seg_steps) { return (int64_t)seg_steps * 7 *
 
 int64_t seg_chunks(int64_t B, int32_t seg_steps) {
   return (seg_rows(B, seg_steps) + ADJ_CHUNK - 1) / ADJ_CHUNK;
+}
+
+// The backsolve's state buffer: ya [B_pad, 128] floats, rows [B_pad] BsRow, rscale [nrow]
+// floats (every row of the allocated row buffers), counters [2] int32; 256-byte aligned.
+struct BsStateLayout {
+  size_t ya, rows, rscale, counters, total;
+};
+BsStateLayout bs_state_layout(int64_t Bp, int64_t nrow) {
+  auto up = [](size_t x) { return (x + 255) / 256 * 256; };
+  BsStateLayout l;
+  l.ya = 0;
+  l.rows = up(l.ya + (size_t)Bp * 128 * sizeof(float));
+  l.rscale = up(l.rows + (size_t)Bp * sizeof(BsRow));
+  l.counters = up(l.rscale + (size_t)nrow * sizeof(float));
+  l.total = up(l.counters + 2 * sizeof(int32_t));
+  return l;
+}
+
+// Zero the launch counters and the row weights past the kernel's rows (the padding of the
+// last split-K chunk), which the kernel never writes.
+__global__ void bs_reset_kernel(int32_t* counters, float* rscale, int64_t used, int64_t nrow) {
+  if (threadIdx.x < 2) counters[threadIdx.x] = 0;
+  for (int64_t r = used + threadIdx.x; r < nrow; r += blockDim.x) rscale[r] = 0.0f;
 }
 
 // The adjoint's products take plane 0 of a bf16 buffer (the bf16-rounded weights; the
@@ -488,6 +826,91 @@ int dfx_mlp_adj_workspace(const dfx_mlp_problem* p, int32_t seg_steps, size_t* z
   *rows_bytes = (size_t)(rows_alloc * wsum) * act_size(p->precision);
   const DwLayout lay = dw_layout();
   *partial_bytes = (size_t)(seg_chunks(p->batch, seg_steps) * ((int64_t)lay.first[3] * 1024 + 2 * MLP_H + MLP_D)) * sizeof(float);
+  return DFX_OK;
+}
+
+int dfx_mlp_backsolve_workspace(const dfx_mlp_problem* p, int32_t chunk_iters,
+                                size_t* state_bytes, size_t* rows_bytes, size_t* partial_bytes) {
+  if (!p || chunk_iters <= 0 || !state_bytes || !rows_bytes || !partial_bytes) return DFX_E_ARG;
+  const int64_t Bp = padded_batch(p->batch);
+  const int64_t nrow = seg_chunks(p->batch, chunk_iters) * ADJ_CHUNK;
+  *state_bytes = bs_state_layout(Bp, nrow).total;
+  int64_t wsum = 0;
+  const int widths[6] = {MLP_H, MLP_D, MLP_H, MLP_H, MLP_D, MLP_H};
+  for (int i = 0; i < 6; ++i) wsum += widths[i];
+  *rows_bytes = (size_t)(nrow * wsum) * sizeof(float);
+  const DwLayout lay = dw_layout();
+  *partial_bytes = (size_t)(seg_chunks(p->batch, chunk_iters) *
+                            ((int64_t)lay.first[3] * 1024 + 2 * MLP_H + MLP_D)) * sizeof(float);
+  return DFX_OK;
+}
+
+int dfx_mlp_backsolve(const dfx_mlp_problem* back, const void* packed, const float* b1,
+                      const float* b2, const float* b3, const dfx_mlp_backsolve_io* io,
+                      int32_t chunk_iters, int32_t first, void* stream) {
+  if (!back || !packed || !b1 || !b2 || !b3 || !io) return DFX_E_ARG;
+  if (back->d != MLP_D || back->hidden != MLP_H || back->solver != DFX_TSIT5) return DFX_E_ARG;
+  if (back->precision != DFX_F32 && back->precision != DFX_BF16_F32) return DFX_E_ARG;
+  if (chunk_iters <= 0 || isnan(back->dt0) || back->max_steps <= 0) return DFX_E_ARG;
+  const int32_t n_out = (back->save_t0 ? 1 : 0) + back->n_ts + (back->save_t1 ? 1 : 0);
+  if (!io->ys || !io->ts_out || !io->ys_bar || !io->state || !io->rows || !io->partial ||
+      !io->dw || !io->ybar || !io->result)
+    return DFX_E_ARG;
+  if (back->batch == 0) return DFX_OK;
+  BsArgs a;
+  memset(&a, 0, sizeof(a));
+  if (!mlp_pid_params(back, a.pid)) return DFX_E_ARG;
+  const int64_t Bp = padded_batch(back->batch);
+  const int nch = (int)seg_chunks(back->batch, chunk_iters);
+  const BsStateLayout sl = bs_state_layout(Bp, (int64_t)nch * ADJ_CHUNK);
+  unsigned char* sb = reinterpret_cast<unsigned char*>(io->state);
+  a.ys = io->ys; a.ts_out = io->ts_out; a.ys_bar = io->ys_bar;
+  a.ya = reinterpret_cast<float*>(sb + sl.ya);
+  a.rows = reinterpret_cast<BsRow*>(sb + sl.rows);
+  a.rscale = reinterpret_cast<float*>(sb + sl.rscale);
+  a.counters = reinterpret_cast<int32_t*>(sb + sl.counters);
+  a.ybar = io->ybar; a.result = io->result;
+  a.B = back->batch; a.B_pad = Bp;
+  a.n_out = n_out; a.save_t0 = back->save_t0; a.max_steps = back->max_steps;
+  a.Q = chunk_iters; a.first = first ? 1 : 0;
+  a.t0 = back->t0; a.dt0 = back->dt0;
+  // The products run in float32 MFMA for both matmul forms: the bfloat16 form's forward
+  // evaluates the float32-weight model (three-piece products), and so does this.
+  using P = PolicyF32;
+  const MlpWeights<P> W = mlp_weights_from<P>(packed, back->precision == DFX_BF16_F32 ? 3 : 1,
+                                              b1, b2, b3);
+  RowBufs<P> R;
+  float* base = reinterpret_cast<float*>(io->rows);
+  int64_t off = 0;
+  const int widths[6] = {MLP_H, MLP_D, MLP_H, MLP_H, MLP_D, MLP_H};
+  for (int i = 0; i < 6; ++i) { R.b[i] = base + off; off += (int64_t)nch * ADJ_CHUNK * widths[i]; }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(bs_reset_kernel, dim3(1), dim3(64), 0, s, a.counters, a.rscale,
+                     (int64_t)chunk_iters * 7 * Bp, (int64_t)nch * ADJ_CHUNK);
+  const unsigned ntiles = (unsigned)(Bp / MLP_TILE);
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_backsolve_kernel<P>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<P>));
+  hipLaunchKernelGGL(mlp_backsolve_kernel<P>, dim3(ntiles), dim3(MLP_THREADS), sizeof(MlpLds<P>),
+                     s, a, W, R);
+  const DwLayout lay = dw_layout();
+  float* partial = io->partial;
+  float* partial_db = io->partial + (int64_t)nch * lay.first[3] * 1024;
+  const unsigned gemm_blocks = (unsigned)(kXcds * lay.first[3] * ((nch + kXcds - 1) / kXcds));
+  hipLaunchKernelGGL((mlp_dw_gemm_kernel<P, true>), dim3(gemm_blocks), dim3(64), 0, s, R, partial,
+                     partial_db, nch, (const float*)a.rscale);
+  const int nout = MLP_H * MLP_D + MLP_H * MLP_H + MLP_D * MLP_H + 2 * MLP_H + MLP_D;
+  hipLaunchKernelGGL(mlp_dw_reduce_kernel, dim3((nout + 255) / 256), dim3(256), 0, s, partial,
+                     partial_db, nch, io->dw);
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+int dfx_mlp_backsolve_status(const dfx_mlp_problem* back, int32_t chunk_iters,
+                             const void* state, const int32_t** counters) {
+  if (!back || chunk_iters <= 0 || !state || !counters) return DFX_E_ARG;
+  const int64_t Bp = padded_batch(back->batch);
+  const BsStateLayout sl = bs_state_layout(Bp, seg_chunks(back->batch, chunk_iters) * ADJ_CHUNK);
+  *counters = reinterpret_cast<const int32_t*>(reinterpret_cast<const unsigned char*>(state) +
+                                               sl.counters);
   return DFX_OK;
 }
 

@@ -16,7 +16,11 @@
 // f32 throughout (v_mfma_f32_32x32x2f32).  Activation functions are evaluated in f32.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
+
+#include "../../include/diffrax_amd.h"
+#include "erk_common.hpp"
 
 namespace dfx {
 
@@ -481,6 +485,23 @@ __device__ __forceinline__ void mlp_forward<PolicyBF16x3>(const MlpWeights<Polic
     s += part[(3 * MLP_TILE + owner_row) * L.LDP + c];
     fout[j] = tanh_f<true>(s + W.b3[c]);
   }
+}
+
+// The PID parameters of a neural-ODE problem (PIDController.__init__, pid.py:299-311, with
+// its coefficients divided by the error order); false unless the controller is the integral
+// one the neural-ODE kernels implement.
+inline bool mlp_pid_params(const dfx_mlp_problem* p, PidParams& q) {
+  const double error_order = isnan(p->error_order) ? 5.0 : p->error_order;
+  q.rtol = p->rtol; q.atol = p->atol;
+  q.coeff1 = (p->icoeff + p->pcoeff + p->dcoeff) / error_order;
+  q.coeff2 = -(p->pcoeff + 2.0 * p->dcoeff) / error_order;
+  q.coeff3 = p->dcoeff / error_order;
+  q.dtmin = p->dtmin; q.dtmax = p->dtmax;
+  q.has_dtmin = !isnan(p->dtmin); q.has_dtmax = !isnan(p->dtmax);
+  q.force_dtmin = p->force_dtmin; q.factormin = p->factormin; q.factormax = p->factormax;
+  q.safety = p->safety;
+  q.pad_ = 0;
+  return pid_is_integral(q);
 }
 
 }  // namespace dfx

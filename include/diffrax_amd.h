@@ -188,9 +188,9 @@ typedef struct dfx_erk_adjoint_io {
  * records per instance of `words` float64 words (d + 8: y, t_prev, t_next, the PID state,
  * the end of the previous accepted step and six int32 counters). */
 int dfx_erk_ckpt_layout(const dfx_erk_problem* p, int32_t* n_ckpt, int32_t* words);
-/* Workspace of dfx_erk_adjoint (bytes, 256-aligned): a K-record scratch per instance
- * holding one replayed segment's accepted steps, (t_start, t_end) float64 and the start
- * state. */
+/* Workspace of dfx_erk_adjoint (bytes, 256-aligned): a 2K-record scratch per instance
+ * holding two replayed segments' accepted steps (one being swept while the next is
+ * replayed), (t_start, t_end) float64 and the start state. */
 int dfx_erk_adjoint_workspace_bytes(const dfx_erk_problem* p, size_t* bytes);
 
 /* Gradients of a loss of the saved rows with respect to y0 and the vector-field
@@ -376,6 +376,41 @@ int dfx_mlp_adj_workspace(const dfx_mlp_problem* p, int32_t seg_steps, size_t* z
 int dfx_mlp_adj_segment(const dfx_mlp_problem* p, const void* packed, const float* b1,
                         const float* b2, const float* b3, const dfx_mlp_adj_io* io,
                         int32_t segment, int32_t seg_steps, void* stream);
+
+/* BacksolveAdjoint of the neural-ODE solve with the adjoint seminorm. */
+typedef struct dfx_mlp_backsolve_io {
+  const float* ys;          /* [B, n_out, 64] the forward's saved states                   */
+  const double* ts_out;     /* [B, n_out] their times                                      */
+  const float* ys_bar;      /* [B, n_out, 64] cotangent of the saved states                 */
+  void* state;              /* scratch, dfx_mlp_backsolve_workspace state_bytes             */
+  void* rows;               /* scratch, rows_bytes; zero it once before the first launch   */
+  float* partial;           /* scratch, partial_bytes                                      */
+  float* dw;                /* in/out accumulator: dW1 | dW2 | dW3 | db1 | db2 | db3
+                               (float32, row-major, 98880 values); zero it first           */
+  float* ybar;              /* out [B, 64] dL/dy0                                          */
+  int32_t* result;          /* out [B] the first failure of the backward solves            */
+} dfx_mlp_backsolve_io;
+
+/* Scratch sizes for launches of chunk_iters lock-step iterations. */
+int dfx_mlp_backsolve_workspace(const dfx_mlp_problem* back, int32_t chunk_iters,
+                                size_t* state_bytes, size_t* rows_bytes,
+                                size_t* partial_bytes);
+/* Up to chunk_iters attempted steps of every row of the backward solves (first = 1 on the
+ * first launch), then the launch's weight gradients added to io->dw.  `back` is the
+ * forward's problem with the backward controller's rtol / atol / max_steps and dt0 (the
+ * sign the backward solves use: the negated forward dt0 unless BacksolveAdjoint was given
+ * its own).  Repeat while the counter from dfx_mlp_backsolve_status (rows unfinished) is
+ * nonzero after a launch; a launch after the last does no work.
+ * Replaces: _loop_backsolve_bwd (_adjoint.py:581-735) under PIDController(norm=
+ * adjoint_rms_seminorm) (_misc.py:85-116) for the MLP vector field (AdjointTerm,
+ * _term.py:767-917). */
+int dfx_mlp_backsolve(const dfx_mlp_problem* back, const void* packed, const float* b1,
+                      const float* b2, const float* b3, const dfx_mlp_backsolve_io* io,
+                      int32_t chunk_iters, int32_t first, void* stream);
+/* Device address of the two int32 launch counters in the state buffer: rows unfinished
+ * after the last launch, and the lock-step iterations it ran. */
+int dfx_mlp_backsolve_status(const dfx_mlp_problem* back, int32_t chunk_iters,
+                             const void* state, const int32_t** counters);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,8 @@ Independent of ``erk_oracle.c`` (written separately, same reference lines):
 * interpolation ................... _solver/tsit5.py:116-156, _local_interpolation.py:50-139
 * PID / constant controllers ...... _step_size_controller/pid.py:33-81, 316-567,
                                     _step_size_controller/constant.py:30-104
+* continuous adjoint (backsolve) .. _adjoint.py:646-735, _term.py:767-917; the adjoint
+                                    seminorm _misc.py:85-116
 
 The vector field is any Python callable ``f(t, y) -> array`` over numpy (real or
 complex) or torch tensors, so the same code serves DETEST problems, complex-step
@@ -153,7 +155,18 @@ def _lincomb(coeffs, ks):
     return acc
 
 
-def rk_step(tab, prod, n, ts0, ts1, direction, y0, f0=None):
+def _lincomb_plain(coeffs, ks):
+    """The same sum in index order with separately rounded products and sums (numpy speed
+    for long states, where the float32 kernels it checks round differently anyway)."""
+    acc = None
+    for a, k in zip(coeffs, ks):
+        if a == 0:
+            continue
+        acc = a * k if acc is None else acc + a * k
+    return acc
+
+
+def rk_step(tab, prod, n, ts0, ts1, direction, y0, f0=None, lincomb=_lincomb):
     """One ERK step.  prod(t, y, control, n) returns the k-value (control * f, plus
     the diffusion increment for SDEs).  f0: FSAL value (k0 = control * f0).
     Returns (y1, y_err, ks, f_last)."""
@@ -166,7 +179,7 @@ def rk_step(tab, prod, n, ts0, ts1, direction, y0, f0=None):
         if i == 0 and f0 is not None and tab.fsal:
             ks.append(control * f0)
             continue
-        inc = _lincomb(tab.a[i], ks) if i > 0 else None
+        inc = lincomb(tab.a[i], ks) if i > 0 else None
         z = y0 if inc is None else y0 + inc
         ti = ts1 if tab.c[i] == 1 else ts0 + tab.c[i] * dt
         k, f = prod(ti * direction, z, control, n)
@@ -176,8 +189,8 @@ def rk_step(tab, prod, n, ts0, ts1, direction, y0, f0=None):
     if tab.ssal:
         y1 = z
     else:
-        y1 = y0 + _lincomb(tab.b, ks)
-    y_err = _lincomb(tab.berr, ks)
+        y1 = y0 + lincomb(tab.b, ks)
+    y_err = lincomb(tab.berr, ks)
     return y1, y_err, ks, f_last
 
 
@@ -272,9 +285,13 @@ def select_initial_step(f, t0, y0, direction, rtol, atol, error_order):
 def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller="pid",
           save_t0=False, save_t1=False, max_steps=4096, pcoeff=0.0, icoeff=1.0, dcoeff=0.0,
           dtmin=None, dtmax=None, force_dtmin=True, factormin=0.2, factormax=10.0,
-          safety=0.9, g_dw=None):
+          safety=0.9, g_dw=None, norm_n=None, fused=True):
     """FP64 numpy solve of one trajectory.  Returns dict(ys, ts, stats, result, grid,
-    direction).  g_dw(n, t, y) adds a diffusion increment for SDEs."""
+    direction).  g_dw(n, t, y) adds a diffusion increment for SDEs.  norm_n: the error
+    norm is the RMS over the first norm_n components only (adjoint_rms_seminorm,
+    _misc.py:85-116, on a state (y, a_y, a_params) with norm_n = 2 d).  fused=False sums
+    the stages with plain arithmetic (_lincomb_plain)."""
+    lincomb = _lincomb if fused else _lincomb_plain
     tab = TABLEAUX[solver]
     y = np.array(y0, dtype=np.float64)
     direction = 1 if t0 < t1 else -1
@@ -319,14 +336,15 @@ def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller
     grid = []
     ts_index = 0
     while tprev < T1 and result == 0 and n_steps < max_steps:
-        y1, yerr, ks, flast = rk_step(tab, prod, n_steps, tprev, tnext, direction, y, fsal)
+        y1, yerr, ks, flast = rk_step(tab, prod, n_steps, tprev, tnext, direction, y, fsal,
+                                      lincomb)
         yerr = np.where(np.isnan(yerr), np.inf, yerr)
         step_result = 0
         if controller == "pid":
             prev_dt = tnext - tprev
             y1c = y if np.isnan(y1).any() else y1
             e = yerr / (atol + np.maximum(np.abs(y), np.abs(y1c)) * rtol)
-            scaled = _rms(e)
+            scaled = _rms(e if norm_n is None else e[:norm_n])
             keep = scaled < 1
             if dtmin is not None:
                 keep = keep or at_dtmin
@@ -399,6 +417,35 @@ def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller
         ys[i] = v
     return dict(ys=ys, ts=tso, stats=(n_steps, n_acc, n_rej), result=result, grid=grid,
                 direction=direction)
+
+
+def backsolve(f_aug, d, n_q, solver, t0, dt0, ts, ys, ys_bar, *, save_t0, rtol, atol,
+              max_steps=4096, seminorm=False, fused=True):
+    """The continuous adjoint of one trajectory (_loop_backsolve_bwd, _adjoint.py:646-735):
+    from the last saved time back to t0, solve the augmented state z = (y, a_y, a_q) under
+    f_aug(t, z) = (f, vjp_y(-a_y), vjp_q(-a_y)) (AdjointTerm, _term.py:767-917) between
+    consecutive saves, y restarting from the saved state ys[s] and a_y taking the save's
+    cotangent ys_bar[s]; each solve starts from dt0 as given (the negated forward dt0 by
+    default) with a fresh first stage.  seminorm: adjoint_rms_seminorm (error norm over
+    (y, a_y) only).  Returns (a_y(t0) [+ ys_bar[0] when t0 was saved], a_q(t0), worst
+    result code, backward steps per interval)."""
+    a_y = np.zeros(d)
+    a_q = np.zeros(n_q)
+    s_end = 1 if save_t0 else 0
+    worst = 0
+    steps = []
+    for s in range(len(ts) - 1, s_end - 1, -1):
+        lo = ts[s - 1] if s > 0 else t0
+        z0 = np.concatenate([np.asarray(ys[s], dtype=np.float64), a_y + ys_bar[s], a_q])
+        r = solve(f_aug, solver, ts[s], lo, dt0, z0, rtol=rtol, atol=atol, save_t1=True,
+                  max_steps=max_steps, norm_n=2 * d if seminorm else None, fused=fused)
+        z1 = r["ys"][-1]
+        a_y, a_q = z1[d:2 * d], z1[2 * d:]
+        worst = worst or r["result"]
+        steps.append(r["stats"])
+    if save_t0 and len(ts) > 0:
+        a_y = a_y + ys_bar[0]
+    return a_y, a_q, worst, steps
 
 
 def replay(f, solver, grid, y0, ts, direction=1, g_dw=None, save_t1=False, save_steps=False):

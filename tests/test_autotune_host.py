@@ -12,7 +12,10 @@ def _plan(**kw):
                         icoeff=1.0, error_order=5.0)
     io = _lib.ErkIO()
     for k, v in kw.items():
-        setattr(p if hasattr(p, k) and k not in ("dense_ts", "grid") else io, k, v)
+        if k.startswith("io_"):
+            setattr(io, k[3:], v)
+        else:
+            setattr(p if hasattr(p, k) and k not in ("dense_ts", "grid") else io, k, v)
     return types.SimpleNamespace(problem=p, io=io)
 
 
@@ -21,7 +24,10 @@ def test_key_separates_kernel_variants(monkeypatch):
     monkeypatch.setattr(torch.cuda, "current_device", lambda: 0)
     base = _autotune._key(_plan())
     variants = [_plan(n_events=1), _plan(n_step_ts=3), _plan(dense_ts=4096),
-                _plan(grid=4096), _plan(pcoeff=0.3), _plan(batch=1024), _plan(precision=1)]
+                _plan(grid=4096), _plan(pcoeff=0.3), _plan(batch=1024), _plan(precision=1),
+                _plan(n_ts=64), _plan(ts_stride=64), _plan(save_t1=1), _plan(save_steps=1),
+                _plan(max_steps=4096), _plan(rtol=1e-8), _plan(atol=1e-12),
+                _plan(io_rtol=4096), _plan(io_atol=4096), _plan(ckpt_every=128)]
     keys = [_autotune._key(v) for v in variants]
     assert base not in keys and len(set(keys)) == len(keys)
     assert _autotune._key(_plan()) == base

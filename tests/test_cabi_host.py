@@ -152,7 +152,7 @@ def test_adjoint_rejects_bad_arguments_without_launching():
     assert call(prob(), f=_lib.ErkIO(t1=z)) == -1             # forward inputs missing
     nbytes = ctypes.c_size_t(0)
     assert lib.dfx_erk_adjoint_workspace_bytes(ctypes.byref(prob()), ctypes.byref(nbytes)) == 0
-    assert nbytes.value == 256 + 256       # [4, 4, 2] + [4, 4, 2] float64, each 256-aligned
+    assert nbytes.value == 512 + 512       # [4, 2 * 4, 2] + [4, 2 * 4, 2] float64, 256-aligned
     assert call(prob(), nbytes=nbytes.value - 1) == -3      # workspace too small
     assert call(prob(), ws=None) == -3
     assert call(prob(batch=0), _lib.AdjointIO(), _lib.ErkIO()) == 0   # empty batch
@@ -267,3 +267,32 @@ def test_mlp_sizes_and_argument_checks():
     assert zs.value == 8192 * 4 * 7 * 64 * 4       # f of every stage of the segment
     chunks = -(-(4 * 7 * 8192) // 4096)
     assert rows.value == chunks * 4096 * (256 + 64 + 256 + 256 + 64 + 256) * 4
+
+
+def test_mlp_backsolve_sizes_and_argument_checks():
+    from diffrax_amd import _mlp
+    lib = _lib.load()
+    p = _mlp.MlpProblem(solver=_lib.TSIT5, precision=_lib.F32, d=64, hidden=256, n_ts=32,
+                        max_steps=4096, batch=8192, dt0=-0.1, rtol=1e-6, atol=1e-8, icoeff=1.0,
+                        dtmin=float("nan"), dtmax=float("nan"), factormin=0.2, factormax=10.0,
+                        safety=0.9, error_order=float("nan"))
+    st, rows, part = (ctypes.c_size_t(0) for _ in range(3))
+    assert lib.dfx_mlp_backsolve_workspace(ctypes.byref(p), 8, ctypes.byref(st),
+                                           ctypes.byref(rows), ctypes.byref(part)) == 0
+    nrow = -(-(8 * 7 * 8192) // 4096) * 4096
+    assert rows.value == nrow * (256 + 64 + 256 + 256 + 64 + 256) * 4
+    # y and a_y per row, the row state, a weight per row-buffer row, two counters
+    assert st.value >= 8192 * 128 * 4 + 8192 * 64 + nrow * 4 + 8
+    assert lib.dfx_mlp_backsolve_workspace(ctypes.byref(p), 0, ctypes.byref(st),
+                                           ctypes.byref(rows), ctypes.byref(part)) == -1
+    io = _mlp.MlpBacksolveIO()
+    z = ctypes.c_void_p(0)
+    assert lib.dfx_mlp_backsolve(ctypes.byref(p), z, z, z, z, ctypes.byref(io), 8, 1, None) == -1
+    q = _mlp.MlpProblem()
+    ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(p))
+    q.dt0 = float("nan")        # the backward solves need dt0
+    assert lib.dfx_mlp_backsolve(ctypes.byref(q), z, z, z, z, ctypes.byref(io), 8, 1, None) == -1
+    cp = ctypes.c_void_p(0)
+    assert lib.dfx_mlp_backsolve_status(ctypes.byref(p), 8, ctypes.c_void_p(4096),
+                                        ctypes.byref(cp)) == 0
+    assert cp.value > 4096 and (cp.value - 4096) % 256 == 0

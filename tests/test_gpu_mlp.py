@@ -82,3 +82,26 @@ def test_mlp_solution_accuracy_matched_tolerance(cuda, matmul):
           f"{st.min()}-{st.max()} (oracle {ref['stats'][:, 0].min()}-{ref['stats'][:, 0].max()})")
     assert np.isfinite(ys).all()
     assert e_dev <= e_ora + 1.0
+
+
+def test_device_resident_ts(cuda):
+    """SaveAt(ts) as a device tensor: the same rows as host ts, bit for bit, through the
+    solve and its gradient (the backward pass reads its segment count from a pinned copy
+    queued behind the forward); an unordered device ts still raises."""
+    W = mlp_weights(1)
+    y0 = torch.tensor(np.random.default_rng(2).normal(size=(40, 64)), dtype=torch.float32)
+    ts = torch.linspace(0.0, 1.0, 5, dtype=torch.float64)
+    ctrl = dfx.PIDController(rtol=RTOL, atol=ATOL)
+
+    def solve(ts_, y0_):
+        return dfx.diffeqsolve(dfx.ODETerm(_vf(W, "f32")), dfx.Tsit5(), 0.0, 1.0, 0.1, y0_,
+                               saveat=dfx.SaveAt(ts=ts_), stepsize_controller=ctrl,
+                               max_steps=64)
+    a, b = solve(ts, y0), solve(ts.to(cuda), y0)
+    assert torch.equal(a.ys, b.ys)
+    ga, gb = (y0.clone().requires_grad_(True) for _ in range(2))
+    solve(ts, ga).ys.square().sum().backward()
+    solve(ts.to(cuda), gb).ys.square().sum().backward()
+    assert torch.equal(ga.grad, gb.grad)
+    with pytest.raises(RuntimeError, match="increasing or decreasing"):
+        solve(torch.tensor([0.0, 0.5, 0.25, 0.75, 1.0], dtype=torch.float64, device=cuda), y0)

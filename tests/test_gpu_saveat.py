@@ -89,3 +89,35 @@ def test_steps_k_against_oracle(cuda, k):
                        rtol=1e-6, atol=1e-8, save_steps=k, save_t1=True, max_steps=501)
     assert np.array_equal(sol.ys.cpu().numpy(), ref["ys"])
     assert np.array_equal(sol.ts.cpu().numpy(), ref["ts"])
+
+
+def test_device_resident_times_match_host_and_are_checked(cuda):
+    """t1 and SaveAt(ts) given as device tensors: the kernel reads them in place and the
+    reference's checks (_integrate.py:1107-1112, 1222-1235) run from pinned copies after
+    the solve is queued; results equal those of host inputs, bit for bit, and invalid
+    times still raise the reference's messages."""
+    y0 = _y0(B=16, seed=3)
+    t1 = torch.tensor(np.random.default_rng(4).uniform(3.0, 5.0, size=16))
+    ts = torch.linspace(0.0, 1.0, 9, dtype=torch.float64)[None] * t1[:, None]
+    ctrl = dfx.PIDController(rtol=1e-6, atol=1e-8)
+    term = dfx.ODETerm(dfx.LotkaVolterra())
+
+    def solve(t1_, ts_, dt0=0.1):
+        return dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, t1_, dt0, y0, Q,
+                               saveat=dfx.SaveAt(ts=ts_, t1=True), stepsize_controller=ctrl)
+    host = solve(t1, ts)
+    dev = solve(t1.to(cuda), ts.to(cuda))
+    assert torch.equal(host.ys, dev.ys) and torch.equal(host.ts, dev.ts)
+    # one device-resident t1 for every instance
+    shared = solve(torch.tensor([4.0], dtype=torch.float64, device=cuda),
+                   torch.linspace(0.0, 4.0, 9, dtype=torch.float64, device=cuda))
+    ref = solve(4.0, np.linspace(0.0, 4.0, 9))
+    assert torch.equal(shared.ys, ref.ys)
+    bad = ts.clone()
+    bad[3, 4] = bad[3, 6]   # above its successor
+    with pytest.raises(RuntimeError, match="increasing or decreasing"):
+        solve(t1.to(cuda), bad.to(cuda))
+    with pytest.raises(RuntimeError, match="between t0 and t1"):
+        solve(t1.to(cuda), (ts * 1.5).to(cuda))
+    with pytest.raises(RuntimeError, match=r"\(t1 - t0\) \* dt0 >= 0"):
+        solve(t1.to(cuda), ts.to(cuda), dt0=-0.1)

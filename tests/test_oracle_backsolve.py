@@ -131,3 +131,85 @@ def test_rejects_unsupported():
     with pytest.raises(ValueError):   # SaveAt(steps=True)
         oracle.backsolve(y0, 1.0, np.ones(4), np.ones((1, 4097, 2)), np.ones((1, 4097)),
                          np.ones((1, 4097, 2)), vf="lv", dt0=0.1, save_steps=True)
+
+
+# ---- pyoracle.backsolve: the generic continuous adjoint that checks the neural-ODE kernel
+
+from oracle import pyoracle  # noqa: E402
+
+
+def _lv_aug(q):
+    """Augmented Lotka-Volterra field (f, vjp_y(-a), vjp_q(-a)) over numpy float64."""
+    def f(t, z):
+        x, v = z[0], z[1]
+        w0, w1 = -z[2], -z[3]
+        fy = [q[0] * x - q[1] * x * v, q[3] * x * v - q[2] * v]
+        gy = [w0 * (q[0] - q[1] * v) + w1 * (q[3] * v), w0 * (-(q[1] * x)) + w1 * (q[3] * x - q[2])]
+        gq = [w0 * x, -(w0 * (x * v)), -(w1 * v), w1 * (x * v)]
+        return np.array(fy + gy + gq)
+    return f
+
+
+def test_pyoracle_backsolve_matches_c_oracle():
+    """pyoracle.backsolve (full norm, fused sums) against orc_backsolve on Lotka-Volterra:
+    the same algorithm written twice agrees to rounding."""
+    y0 = np.array([1.2, 0.7])
+    q = np.array([1.5, 1.0, 3.0, 1.0])
+    ts = np.array([0.5, 1.5, 3.0])
+    kw = dict(solver="tsit5", vf="lv", t0=0.0, dt0=0.1, rtol=1e-7, atol=1e-9, save_t0=True,
+              save_t1=True)
+    f = oracle.solve(y0[None], 3.5, q, ts=ts, **kw)
+    w = np.random.default_rng(3).normal(size=f["ys"].shape)
+    yb, qb, res = oracle.backsolve(y0[None], 3.5, q, f["ys"], f["ts"], w, ts=ts, **kw)
+    ay, aq, worst, _ = pyoracle.backsolve(_lv_aug(q), 2, 4, "tsit5", 0.0, -0.1, f["ts"][0],
+                                          f["ys"][0], w[0], save_t0=True, rtol=1e-7, atol=1e-9)
+    assert worst == 0 and res[0] == 0
+    np.testing.assert_allclose(ay, yb[0], rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(aq, qb[0], rtol=1e-11, atol=1e-12)
+
+
+def _linear_aug(A):
+    d = A.shape[0]
+
+    def f(t, z):
+        y, w = z[:d], -z[d:2 * d]
+        return np.concatenate([A @ y, A.T @ w, np.outer(w, y).ravel()])
+    return f
+
+
+@pytest.mark.parametrize("seminorm", [False, True])
+def test_pyoracle_backsolve_seminorm_closed_form(seminorm):
+    """The seminorm leaves a_A out of the error control but not out of the solution: both
+    norms give the closed-form gradient at the reference's backsolve tolerance, and the
+    seminorm takes no more steps (its norm is over a subset of the components)."""
+    ts = np.array([1.0, 4.0, 9.5])
+    f = oracle.solve(Y0_REF[None], 9.5, A_REF.ravel(), ts=ts, solver="tsit5", vf="linear",
+                     t0=0.3, dt0=0.1, rtol=1e-8, atol=1e-8)
+    w = np.ones_like(f["ys"][0])
+    runs = {}
+    for sn in (False, True):
+        runs[sn] = pyoracle.backsolve(_linear_aug(A_REF), 2, 4, "tsit5", 0.3, -0.1, f["ts"][0],
+                                      f["ys"][0], w, save_t0=False, rtol=1e-8, atol=1e-8,
+                                      seminorm=sn, fused=False)
+    ay, aq, worst, steps = runs[seminorm]
+    gy, gA = _closed_form(A_REF, Y0_REF, 0.3, f["ts"][0], w)
+    assert worst == 0
+    np.testing.assert_allclose(ay, gy, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(aq.reshape(2, 2), gA, rtol=1e-5, atol=1e-5)
+    n_semi = sum(s[0] for s in runs[True][3])
+    n_full = sum(s[0] for s in runs[False][3])
+    assert n_semi <= n_full
+
+
+def test_pyoracle_backsolve_reference_seminorm_recipe():
+    """test_adjoint.py::test_adjoint_seminorm: y' = -y, y0 = 2 on [0, 1], Tsit5, forward
+    PIDController(1e-3, 1e-6), backward the same tolerances with the seminorm; the gradient
+    of y(1) is exp(-1)."""
+    fwd = pyoracle.solve(lambda t, y: -y, "tsit5", 0.0, 1.0, None, np.array([2.0]),
+                         rtol=1e-3, atol=1e-6, save_t1=True)
+    aug = lambda t, z: np.array([-z[0], z[1]])    # (f, vjp_y(-a)); no parameters
+    ay, aq, worst, _ = pyoracle.backsolve(aug, 1, 0, "tsit5", 0.0, None, fwd["ts"], fwd["ys"],
+                                          np.ones((1, 1)), save_t0=False, rtol=1e-3,
+                                          atol=1e-6, seminorm=True)
+    assert worst == 0 and aq.size == 0
+    assert abs(ay[0] - np.exp(-1.0)) < 1e-3
