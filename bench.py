@@ -344,6 +344,8 @@ def bench_mlp(steps, warmup, with_cpu, check=256):
                             algorithmic_tflops=round(gtf, 3), mfma_peak_tflops=peak,
                             frac=round(gtf / peak, 5),
                             **mlp_grad_error(gplan, W, y0, ts, check))
+        if matmul == "f32":
+            grad["f32_backsolve"] = bench_mlp_backsolve(plan, tgt, out["g"], steps)
         del plan, gplan, g
     cpu = None
     if cpu_times:
@@ -356,6 +358,37 @@ def bench_mlp(steps, warmup, with_cpu, check=256):
                                         error_subset=check),
                 neural_ode_grad=dict(batch=B, forms=grad,
                                      loss="mean((ys - targets)^2), targets ~ N(0, 0.25)"))
+
+
+def bench_mlp_backsolve(plan, tgt, g_discrete, steps):
+    """Forward + BacksolveAdjoint (adjoint seminorm, the forward's tolerances) of the same
+    loss; wall clock, since the host reads one counter per backsolve launch."""
+    import torch
+    import diffrax_amd as dfx
+    from diffrax_amd import _mlp
+    adj = dfx.BacksolveAdjoint(stepsize_controller=dfx.PIDController(
+        rtol=1e-5, atol=1e-7, norm=dfx.adjoint_rms_seminorm))
+    back = _mlp.backsolve_problem(plan, adj)
+
+    def fwd_back():
+        plan.launch()
+        return _mlp.backsolve_mlp(plan, back, (plan.ys - tgt) * (2.0 / plan.ys.numel()))
+    g, res = fwd_back()
+    torch.cuda.synchronize()
+    n = max(2, steps // 4)
+    t = time.perf_counter()
+    for _ in range(n):
+        g, res = fwd_back()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t) * 1e3 / n
+    B = plan.ys.shape[0]
+    diff = max(float((a - b).abs().max() / b.abs().max()) for a, b in zip(g, g_discrete))
+    return dict(ms_per_solve=round(ms, 4), solves_per_s=round(B / (ms * 1e-3), 1),
+                adjoint="BacksolveAdjoint(stepsize_controller=PIDController(1e-5, 1e-7, "
+                        "norm=adjoint_rms_seminorm))",
+                iterations_per_launch=_mlp.BACKSOLVE_CHUNK,
+                all_successful=bool((res == 0).all().item()),
+                grad_normwise_diff_vs_discrete_adjoint=diff)
 
 
 def bench_sde(steps, warmup, with_cpu, P=262144, d=16, N=1000, cpu_paths=16384):
