@@ -375,3 +375,22 @@ def test_python_fma_matches_libm():
             assert math.isnan(got), (x, y, z)
         else:
             assert got == ref and math.copysign(1.0, got) == math.copysign(1.0, ref), (x, y, z, got, ref)
+
+
+def test_mlp_vector_field_against_torch_float64():
+    """The C oracle's MLP field (eqx.nn.MLP(64, 64, width 256, depth 2, softplus, final
+    tanh)) against torch float64: the first two Tsit5 stages of one unit step are
+    f(y0) and f(y0 + a10 k0), evaluated independently by torch."""
+    import torch
+    rng = np.random.default_rng(41)
+    xav = lambda o, i: rng.normal(0.0, np.sqrt(2.0 / (i + o)), size=(o, i))
+    W = [xav(256, 64), rng.normal(0, 0.1, 256), xav(256, 256), rng.normal(0, 0.1, 256),
+         xav(64, 256), rng.normal(0, 0.1, 64)]
+    y0 = rng.normal(size=64)
+    _, _, k = oracle.single_step("tsit5", "mlp", y0, 0.0, 1.0, oracle.mlp_pack(*W), hidden=256)
+    W1, b1, W2, b2, W3, b3 = (torch.tensor(w) for w in W)
+    sp = torch.nn.functional.softplus
+    f = lambda y: torch.tanh(W3 @ sp(W2 @ sp(W1 @ y + b1) + b2) + b3).numpy()
+    a10 = oracle.tableau("tsit5")[0][1][0]
+    np.testing.assert_allclose(k[0], f(torch.tensor(y0)), rtol=0, atol=1e-14)
+    np.testing.assert_allclose(k[1], f(torch.tensor(y0 + a10 * k[0])), rtol=0, atol=1e-14)
