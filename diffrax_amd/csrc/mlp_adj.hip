@@ -289,6 +289,241 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
   }
 }
 
+// ---- the three-piece bfloat16 form's adjoint ----------------------------------------------
+// The recompute and reverse sweep of mlp_adj_segment_kernel with every product on the
+// six-pass three-piece MFMA (wave_gemm3): the deltas are split like the activations, so the
+// gradients are those of the float32-weight model that the forward evaluates.  The row
+// buffers hold float32 values (layer inputs and deltas written before their split into
+// pieces) for the float32 weight-gradient GEMM.  LDS: the forward planes;
+// in the sweep the x planes take delta3, the h1 planes delta2 and then the split-K partials,
+// the h2 planes delta1 (the activations come back from the row buffers).
+
+__global__ void __launch_bounds__(MLP_THREADS, 1)
+mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
+                        const RowBufs<PolicyF32> R) {
+  using P = PolicyBF16x3;
+  extern __shared__ __align__(16) unsigned char smem[];
+  MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
+  using Tab = Tsit5Tab;
+  constexpr int S = Tab::S;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int orow = tid >> 3, c0 = (tid & 7) * 8;
+  const int tile = blockIdx.x;
+  const int64_t traj = (int64_t)tile * MLP_TILE + orow;
+  const bool valid = traj < a.B;
+  const int nacc = valid ? a.stats[traj * 3 + 1] : 0;
+  const int n0 = a.seg * a.C;
+  int steps_here = nacc - n0;
+  steps_here = steps_here < 0 ? 0 : (steps_here > a.Cs ? a.Cs : steps_here);
+  const int dir = (a.t0 < a.t1) ? 1 : -1;
+  using TL = TabLit<Tab, float>;
+  float* fs_tile = a.zs + (int64_t)tile * a.Cs * S * MLP_TILE * MLP_D;
+  auto fslot = [&](int q, int i) { return fs_tile + (((int64_t)q * S + i) * MLP_TILE + orow) * MLP_D + c0; };
+  auto stage_row0 = [&](int q, int i) { return ((int64_t)q * S + i) * a.B_pad + (int64_t)tile * MLP_TILE; };
+  // The layer inputs of stage (q, i) go to the row buffers as they are formed: the stage
+  // value by its owner threads, the activations from the forward's epilogues.  The last
+  // stage's rows are also the next step's stage-0 rows (FSAL: the same input).
+  auto sink_of = [&](int q, int i) {
+    MlpRowSink k;
+    k.h1 = R.b[3];
+    k.h2 = R.b[5];
+    k.row0 = stage_row0(q, i);
+    k.row0b = (i == S - 1 && q + 1 < a.Cs) ? stage_row0(q + 1, 0) : -1;
+    return k;
+  };
+  auto put_x = [&](const MlpRowSink& k, int j, float v) {
+    R.b[1][(k.row0 + orow) * MLP_D + c0 + j] = v;
+    if (k.row0b >= 0) R.b[1][(k.row0b + orow) * MLP_D + c0 + j] = v;
+  };
+
+  // ---- 1. recompute the segment (as mlp_adj_segment_kernel) ----
+  float y[8], fl[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) y[j] = (steps_here > 0) ? a.ckpt[(traj * a.n_ckpt + a.seg) * MLP_D + c0 + j] : 0.0f;
+  for (int q = 0; q < a.Cs; ++q) {
+    const bool act = q < steps_here;
+    const int n = n0 + q;
+    const double ta = act ? a.grid[(traj * a.max_steps + n) * 2 + 0] : 0.0;
+    const double tb = act ? a.grid[(traj * a.max_steps + n) * 2 + 1] : 0.0;
+    const float control = (float)((double)dir * (tb - ta));
+    float k[S][8], z[8], f[8];
+    if (q == 0) {
+      const MlpRowSink k0 = sink_of(0, 0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        store_input<P>(L, orow, c0 + j, y[j]);
+        put_x(k0, j, y[j]);
+      }
+      __syncthreads();
+      mlp_forward3(W, L, wave, lane, orow, c0, fl, k0);
+      __syncthreads();
+    }
+    {
+      float* fp = fslot(q, 0);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) { fp[j] = fl[j]; k[0][j] = control * fl[j]; }
+    }
+    static_for<1, S>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const MlpRowSink ki = sink_of(q, i);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        float kc[S];
+#pragma unroll
+        for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
+        z[j] = y[j] + comb<Tab, 0, i, 0, i, false>(TL::v.a[i], kc, 0.0f);
+        store_input<P>(L, orow, c0 + j, z[j]);
+        put_x(ki, j, z[j]);
+      }
+      __syncthreads();
+      mlp_forward3(W, L, wave, lane, orow, c0, f, ki);
+      __syncthreads();
+      float* fp = fslot(q, i);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) { fp[j] = f[j]; k[i][j] = control * f[j]; }
+    });
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (act) y[j] = z[j];
+      fl[j] = f[j];
+    }
+  }
+
+  // ---- 2. reverse sweep ----
+  float yb[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) yb[j] = valid ? a.ybar[traj * MLP_D + c0 + j] : 0.0f;
+  for (int q = a.Cs - 1; q >= 0; --q) {
+    const bool act = q < steps_here;
+    const int n = n0 + q;
+    const double ta = act ? a.grid[(traj * a.max_steps + n) * 2 + 0] : 0.0;
+    const double tb = act ? a.grid[(traj * a.max_steps + n) * 2 + 1] : 0.0;
+    const float control = (float)((double)dir * (tb - ta));
+    float kb[S][8], ybn[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      ybn[j] = yb[j];
+#pragma unroll
+      for (int m = 0; m < S; ++m) kb[m][j] = 0.0f;
+    }
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < S - 1; ++m) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) kb[m][j] += TL::v.a[S - 1][m] * yb[j];
+      }
+      int jlo = 0, jhi = 0;
+      for (int s = 0; s < a.n_ts; ++s) {
+        const double tsn = a.ts[s] * dir;
+        if (n > 0 && tsn <= ta) jlo = s + 1;
+        if (tsn <= tb) jhi = s + 1;
+      }
+      for (int s = jlo; s < jhi; ++s) {
+        const double th = linear_rescale(ta, a.ts[s] * dir, tb);
+        double bb[7];
+        tsit5_weights(th, bb);
+        const float* g = a.ys_bar + (traj * a.n_out + (a.save_t0 ? 1 : 0) + s) * MLP_D + c0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const float gv = g[j];
+          ybn[j] += gv;
+#pragma unroll
+          for (int m = 0; m < 7; ++m) kb[m][j] += (float)bb[m] * gv;
+        }
+      }
+    }
+    static_for<0, S>([&](auto ic) {
+      constexpr int i = S - 1 - decltype(ic)::value;
+      const int64_t row0 = stage_row0(q, i);
+      const float* fp = fslot(q, i);
+      // delta3 = fbar (1 - f^2), fbar = control * kbar_i: its row, and its pieces in x
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const float fb = act ? control * kb[i][j] : 0.0f;
+        const float fv = fp[j];
+        const float d3 = fb * (1.0f - fv * fv);
+        R.b[4][(row0 + orow) * MLP_D + c0 + j] = d3;
+        const int li = orow * L.LDX + c0 + j;
+        split3(d3, L.x[0][li], L.x[1][li], L.x[2][li]);
+      }
+      __syncthreads();
+      // delta2 = (delta3 . W3) softplus'(h2): its row, and its pieces in h1
+      floatx16_t acc[2];
+      wave_gemm3<MLP_D / 16, 2>(L.x[0], L.x[1], L.x[2], L.LDX, W.w3t, W.plane3, 2 * wave, 0,
+                                MLP_D / 16, acc, lane);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 64 * wave + 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = acc_row(lane, v);
+          const float h = R.b[5][(row0 + r) * MLP_H + col];
+          const float d2 = acc[t][v] * dsoftplus_from_value<true>(h);
+          R.b[2][(row0 + r) * MLP_H + col] = d2;
+          const int li = r * L.LDH + col;
+          split3(d2, L.h1[0][li], L.h1[1][li], L.h1[2][li]);
+        }
+      }
+      __syncthreads();
+      // delta1 = (delta2 . W2) softplus'(h1): its row, and its pieces in h2
+      wave_gemm3<MLP_H / 16, 2>(L.h1[0], L.h1[1], L.h1[2], L.LDH, W.w2t, W.plane2, 2 * wave, 0,
+                                MLP_H / 16, acc, lane);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 64 * wave + 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int r = acc_row(lane, v);
+          const float h = R.b[3][(row0 + r) * MLP_H + col];
+          const float d1 = acc[t][v] * dsoftplus_from_value<true>(h);
+          R.b[0][(row0 + r) * MLP_H + col] = d1;
+          const int li = r * L.LDH + col;
+          split3(d1, L.h2[0][li], L.h2[1][li], L.h2[2][li]);
+        }
+      }
+      __syncthreads();
+      // zbar = delta1 . W1 (split-K over the 256 hidden units); the partials take the h1
+      // planes, whose last readers (the delta1 products) are behind the barrier above
+      constexpr int KS1 = MLP_H / 16;
+      wave_gemm3<KS1, 2>(L.h2[0], L.h2[1], L.h2[2], L.LDH, W.w1t, W.plane1, 0, wave * (KS1 / 4),
+                         (wave + 1) * (KS1 / 4), acc, lane);
+      float* part = L.part();
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int col = 32 * t + (lane & 31);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
+      }
+      __syncthreads();
+      float zb[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int c = c0 + j;
+        float sv = part[(0 * MLP_TILE + orow) * L.LDP + c];
+        sv += part[(1 * MLP_TILE + orow) * L.LDP + c];
+        sv += part[(2 * MLP_TILE + orow) * L.LDP + c];
+        sv += part[(3 * MLP_TILE + orow) * L.LDP + c];
+        zb[j] = act ? sv : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        ybn[j] += zb[j];
+#pragma unroll
+        for (int m = 0; m < i; ++m) kb[m][j] += TL::v.a[i][m] * zb[j];
+      }
+      __syncthreads();
+    });
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < 8; ++j) yb[j] = ybn[j];
+    }
+  }
+  if (valid) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a.ybar[traj * MLP_D + c0 + j] = yb[j];
+  }
+}
+
 // ---- weight gradients of one segment ---------------------------------------------------
 // Output tiles: matrix 0 = dW1 [256 x 64] (D1, Z), 1 = dW2 [256 x 256] (D2, H1),
 // 2 = dW3 [64 x 256] (D3, H2); 32x32 tiles numbered matrix by matrix, row-major.
@@ -731,8 +966,6 @@ using namespace dfx;
 
 namespace {
 
-size_t act_size(int32_t precision) { return precision == DFX_BF16_F32 ? 2 : 4; }
-
 int64_t padded_batch(int64_t B) { return (B + MLP_TILE - 1) / MLP_TILE * MLP_TILE; }
 
 int64_t seg_rows(int64_t B, int32_t seg_steps) { return (int64_t)seg_steps * 7 * padded_batch(B); }
@@ -764,18 +997,14 @@ __global__ void bs_reset_kernel(int32_t* counters, float* rscale, int64_t used, 
   for (int64_t r = used + threadIdx.x; r < nrow; r += blockDim.x) rscale[r] = 0.0f;
 }
 
-// The adjoint's products take plane 0 of a bf16 buffer (the bf16-rounded weights; the
-// buffer holds three split3 planes for the forward) and the single plane of a float32 one.
-template <class P>
-MlpWeights<P> weights_of(const void* packed, int32_t precision, const float* b1, const float* b2,
-                         const float* b3) {
-  return mlp_weights_from<P>(packed, precision == DFX_BF16_F32 ? 3 : 1, b1, b2, b3);
-}
-
-template <class P>
-int run_segment(const dfx_mlp_problem* p, const void* packed, const float* b1, const float* b2,
-                const float* b3, const dfx_mlp_adj_io* io, int32_t seg, int32_t seg_steps,
-                hipStream_t s) {
+// One segment: the recompute + sweep kernel `kern` with the weights of policy P (float32:
+// mlp_adj_segment_kernel on a float32 buffer; the three-piece bfloat16 form:
+// mlp_adj_segment3_kernel on the three split3 planes of a bf16 buffer), then the float32
+// weight-gradient GEMM over its float32 row buffers and the fixed-order reduction.
+template <class P, class Kern>
+int run_segment(Kern kern, const dfx_mlp_problem* p, const void* packed, const float* b1,
+                const float* b2, const float* b3, const dfx_mlp_adj_io* io, int32_t seg,
+                int32_t seg_steps, hipStream_t s) {
   AdjArgs a;
   memset(&a, 0, sizeof(a));
   a.grid = io->grid; a.ckpt = io->ckpt; a.stats = io->stats; a.ts = io->ts; a.ys_bar = io->ys_bar;
@@ -785,25 +1014,24 @@ int run_segment(const dfx_mlp_problem* p, const void* packed, const float* b1, c
   a.Cs = seg_steps; a.seg = seg; a.n_ts = p->n_ts;
   a.n_out = (p->save_t0 ? 1 : 0) + p->n_ts + (p->save_t1 ? 1 : 0);
   a.save_t0 = p->save_t0; a.t0 = p->t0; a.t1 = p->t1;
-  const MlpWeights<P> W = weights_of<P>(packed, p->precision, b1, b2, b3);
-  RowBufs<P> R;
+  const MlpWeights<P> W = mlp_weights_from<P>(packed, P::PIECES, b1, b2, b3);
+  RowBufs<PolicyF32> R;
   const int64_t rows_alloc = seg_chunks(p->batch, seg_steps) * ADJ_CHUNK;
-  typename P::act_t* base = reinterpret_cast<typename P::act_t*>(io->rows);
+  float* base = reinterpret_cast<float*>(io->rows);
   int64_t off = 0;
   const int widths[6] = {MLP_H, MLP_D, MLP_H, MLP_H, MLP_D, MLP_H};
   for (int i = 0; i < 6; ++i) { R.b[i] = base + off; off += rows_alloc * widths[i]; }
   const unsigned ntiles = (unsigned)(a.B_pad / MLP_TILE);
-  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&mlp_adj_segment_kernel<P>),
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<P>));
-  hipLaunchKernelGGL(mlp_adj_segment_kernel<P>, dim3(ntiles), dim3(MLP_THREADS), sizeof(MlpLds<P>),
-                     s, a, W, R);
+  hipLaunchKernelGGL(kern, dim3(ntiles), dim3(MLP_THREADS), sizeof(MlpLds<P>), s, a, W, R);
   const int nch = (int)seg_chunks(p->batch, seg_steps);
   const DwLayout lay = dw_layout();
   float* partial = io->partial;
   float* partial_db = io->partial + (int64_t)nch * lay.first[3] * 1024;
   const unsigned gemm_blocks = (unsigned)(kXcds * lay.first[3] * ((nch + kXcds - 1) / kXcds));
-  hipLaunchKernelGGL(mlp_dw_gemm_kernel<P>, dim3(gemm_blocks), dim3(64), 0, s, R, partial,
-                     partial_db, nch);
+  hipLaunchKernelGGL((mlp_dw_gemm_kernel<PolicyF32>), dim3(gemm_blocks), dim3(64), 0, s, R,
+                     partial, partial_db, nch, (const float*)nullptr);
   const int nout = MLP_H * MLP_D + MLP_H * MLP_H + MLP_D * MLP_H + 2 * MLP_H + MLP_D;
   hipLaunchKernelGGL(mlp_dw_reduce_kernel, dim3((nout + 255) / 256), dim3(256), 0, s, partial,
                      partial_db, nch, io->dw);
@@ -823,7 +1051,7 @@ int dfx_mlp_adj_workspace(const dfx_mlp_problem* p, int32_t seg_steps, size_t* z
   int64_t wsum = 0;
   const int widths[6] = {MLP_H, MLP_D, MLP_H, MLP_H, MLP_D, MLP_H};
   for (int i = 0; i < 6; ++i) wsum += widths[i];
-  *rows_bytes = (size_t)(rows_alloc * wsum) * act_size(p->precision);
+  *rows_bytes = (size_t)(rows_alloc * wsum) * sizeof(float);   // float32 rows for both forms
   const DwLayout lay = dw_layout();
   *partial_bytes = (size_t)(seg_chunks(p->batch, seg_steps) * ((int64_t)lay.first[3] * 1024 + 2 * MLP_H + MLP_D)) * sizeof(float);
   return DFX_OK;
@@ -850,7 +1078,7 @@ int dfx_mlp_backsolve(const dfx_mlp_problem* back, const void* packed, const flo
                       int32_t chunk_iters, int32_t first, void* stream) {
   if (!back || !packed || !b1 || !b2 || !b3 || !io) return DFX_E_ARG;
   if (back->d != MLP_D || back->hidden != MLP_H || back->solver != DFX_TSIT5) return DFX_E_ARG;
-  if (back->precision != DFX_F32 && back->precision != DFX_BF16_F32) return DFX_E_ARG;
+  if (back->precision != DFX_F32) return DFX_E_ARG;  // float32 products on a float32 packing
   if (chunk_iters <= 0 || isnan(back->dt0) || back->max_steps <= 0) return DFX_E_ARG;
   const int32_t n_out = (back->save_t0 ? 1 : 0) + back->n_ts + (back->save_t1 ? 1 : 0);
   if (!io->ys || !io->ts_out || !io->ys_bar || !io->state || !io->rows || !io->partial ||
@@ -874,11 +1102,10 @@ int dfx_mlp_backsolve(const dfx_mlp_problem* back, const void* packed, const flo
   a.n_out = n_out; a.save_t0 = back->save_t0; a.max_steps = back->max_steps;
   a.Q = chunk_iters; a.first = first ? 1 : 0;
   a.t0 = back->t0; a.dt0 = back->dt0;
-  // The products run in float32 MFMA for both matmul forms: the bfloat16 form's forward
-  // evaluates the float32-weight model (three-piece products), and so does this.
+  // The products run in float32 MFMA for both matmul forms (the caller packs the weights
+  // for DFX_F32): the bfloat16 form's forward evaluates the float32-weight model too.
   using P = PolicyF32;
-  const MlpWeights<P> W = mlp_weights_from<P>(packed, back->precision == DFX_BF16_F32 ? 3 : 1,
-                                              b1, b2, b3);
+  const MlpWeights<P> W = mlp_weights_from<P>(packed, 1, b1, b2, b3);
   RowBufs<P> R;
   float* base = reinterpret_cast<float*>(io->rows);
   int64_t off = 0;
@@ -929,8 +1156,12 @@ int dfx_mlp_adj_segment(const dfx_mlp_problem* p, const void* packed, const floa
   if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  if (p->precision == DFX_BF16_F32) return run_segment<PolicyBF16>(p, packed, b1, b2, b3, io, segment, seg_steps, s);
-  if (p->precision == DFX_F32) return run_segment<PolicyF32>(p, packed, b1, b2, b3, io, segment, seg_steps, s);
+  if (p->precision == DFX_BF16_F32)
+    return run_segment<PolicyBF16x3>(&mlp_adj_segment3_kernel, p, packed, b1, b2, b3, io, segment,
+                                      seg_steps, s);
+  if (p->precision == DFX_F32)
+    return run_segment<PolicyF32>(&mlp_adj_segment_kernel<PolicyF32>, p, packed, b1, b2, b3, io,
+                                  segment, seg_steps, s);
   return DFX_E_ARG;
 }
 

@@ -430,11 +430,36 @@ __device__ __forceinline__ void wave_gemm3(const __bf16* a0, const __bf16* a1, c
   for (int t = 0; t < NT; ++t) acc[t] = big[t] + sml[t];
 }
 
+// Float32 copies of the hidden activations for the adjoint's row buffers ([rows][256] each,
+// the tile's rows from row0, and again from row0b when row0b >= 0): written in the layer
+// epilogues, before layer 3's partials reuse the h1 planes.
+struct MlpRowSink {
+  float* h1 = nullptr;
+  float* h2 = nullptr;
+  int64_t row0 = 0, row0b = -1;
+  __device__ __forceinline__ void put(float* dst, int r, int col, float v) const {
+    dst[(row0 + r) * MLP_H + col] = v;
+    if (row0b >= 0) dst[(row0b + r) * MLP_H + col] = v;
+  }
+};
+
+__device__ __forceinline__ void mlp_forward3(const MlpWeights<PolicyBF16x3>& W,
+                                             MlpLds<PolicyBF16x3>& L, int wave, int lane,
+                                             int owner_row, int owner_c0, float (&fout)[8],
+                                             const MlpRowSink& sink);
+
 template <>
 __device__ __forceinline__ void mlp_forward<PolicyBF16x3>(const MlpWeights<PolicyBF16x3>& W,
                                                           MlpLds<PolicyBF16x3>& L, int wave,
                                                           int lane, int owner_row, int owner_c0,
                                                           float (&fout)[8]) {
+  mlp_forward3(W, L, wave, lane, owner_row, owner_c0, fout, MlpRowSink{});
+}
+
+__device__ __forceinline__ void mlp_forward3(const MlpWeights<PolicyBF16x3>& W,
+                                             MlpLds<PolicyBF16x3>& L, int wave, int lane,
+                                             int owner_row, int owner_c0, float (&fout)[8],
+                                             const MlpRowSink& sink) {
   constexpr int KS1 = MLP_D / 16, KS2 = MLP_H / 16;
   floatx16_t acc[2];
   // layer 1: [32 x 64] . W1^T -> [32 x 256], wave's columns [64w, 64w + 64)
@@ -446,7 +471,9 @@ __device__ __forceinline__ void mlp_forward<PolicyBF16x3>(const MlpWeights<Polic
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int i = acc_row(lane, v) * L.LDH + col;
-      split3(softplus_f<true>(acc[t][v] + bias), L.h1[0][i], L.h1[1][i], L.h1[2][i]);
+      const float h = softplus_f<true>(acc[t][v] + bias);
+      if (sink.h1) sink.put(sink.h1, acc_row(lane, v), col, h);
+      split3(h, L.h1[0][i], L.h1[1][i], L.h1[2][i]);
     }
   }
   __syncthreads();
@@ -460,7 +487,9 @@ __device__ __forceinline__ void mlp_forward<PolicyBF16x3>(const MlpWeights<Polic
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
       const int i = acc_row(lane, v) * L.LDH + col;
-      split3(softplus_f<true>(acc[t][v] + bias), L.h2[0][i], L.h2[1][i], L.h2[2][i]);
+      const float h = softplus_f<true>(acc[t][v] + bias);
+      if (sink.h2) sink.put(sink.h2, acc_row(lane, v), col, h);
+      split3(h, L.h2[0][i], L.h2[1][i], L.h2[2][i]);
     }
   }
   __syncthreads();

@@ -396,7 +396,9 @@ int dfx_mlp_backsolve_workspace(const dfx_mlp_problem* back, int32_t chunk_iters
                                 size_t* state_bytes, size_t* rows_bytes,
                                 size_t* partial_bytes);
 /* Up to chunk_iters attempted steps of every row of the backward solves (first = 1 on the
- * first launch), then the launch's weight gradients added to io->dw.  `back` is the
+ * first launch), then the launch's weight gradients added to io->dw.  The products run in
+ * float32: `packed` holds the weights packed for DFX_F32 and back->precision is DFX_F32
+ * whatever the forward's form.  `back` is the
  * forward's problem with the backward controller's rtol / atol / max_steps and dt0 (the
  * sign the backward solves use: the negated forward dt0 unless BacksolveAdjoint was given
  * its own).  Repeat while the counter from dfx_mlp_backsolve_status (rows unfinished) is

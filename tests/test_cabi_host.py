@@ -292,6 +292,8 @@ def test_mlp_backsolve_sizes_and_argument_checks():
     ctypes.memmove(ctypes.byref(q), ctypes.byref(p), ctypes.sizeof(p))
     q.dt0 = float("nan")        # the backward solves need dt0
     assert lib.dfx_mlp_backsolve(ctypes.byref(q), z, z, z, z, ctypes.byref(io), 8, 1, None) == -1
+    q.dt0, q.precision = -0.1, _lib.BF16_F32   # float32 products on a float32 packing only
+    assert lib.dfx_mlp_backsolve(ctypes.byref(q), z, z, z, z, ctypes.byref(io), 8, 1, None) == -1
     cp = ctypes.c_void_p(0)
     assert lib.dfx_mlp_backsolve_status(ctypes.byref(p), 8, ctypes.c_void_p(4096),
                                         ctypes.byref(cp)) == 0

@@ -16,8 +16,8 @@ Neural ODE, 8,192 trajectories, MLP 64 -> 256 -> 256 -> 64, Tsit5 + PID(1e-5, 1e
 32 saves, sqrt(N) checkpoints: the cotangent of ys is nonzero on 256 rows spread over the
 batch, so the weight gradients come from those rows alone; y0 and every weight gradient
 against torch float64 autograd through a replay of the device's accepted steps of those
-rows (pinned by central differences in test_gpu_mlp_grad.py): float32 form 1e-4, bfloat16
-form 5e-2 (the bars of the small-batch test)."""
+rows (pinned by central differences in test_gpu_mlp_grad.py): 1e-4 for both forms (the bar
+of the small-batch test)."""
 import math
 
 import numpy as np
@@ -117,7 +117,7 @@ def _mlp_weights():
             xav(64, 256), rng.normal(0, 0.1, 64)]
 
 
-@pytest.mark.parametrize("matmul,tol", [("f32", 1e-4), ("bf16", 5e-2)])
+@pytest.mark.parametrize("matmul,tol", [("f32", 1e-4), ("bf16", 1e-4)])
 def test_neural_ode_full_batch_gradient(cuda, matmul, tol):
     B, n_ts = 8192, 32
     Wn = _mlp_weights()
@@ -162,7 +162,7 @@ def test_neural_ode_full_batch_gradient(cuda, matmul, tol):
             p.grad for p in Pd]):
         rel = float((g - r).abs().max() / r.abs().max())
         print(f"full-batch neural ODE {matmul} gradient {name}: rel err {rel:.3e}")
-        worst = max(worst, rel)
+        worst = max(worst, rel) if rel == rel else float("inf")   # NaN fails
     assert worst < tol
     # rows without a cotangent get none
     others = np.setdiff1d(np.arange(B), rows)

@@ -124,7 +124,7 @@ def test_backsolve_against_discrete_adjoint(cuda):
         ew = np.max(np.abs(a - b) / (1.0 + np.abs(b)))
         print(f"backsolve vs discrete adjoint, {nm}: normwise {e:.3e}, "
               f"elementwise |diff| / (1 + |g|) {ew:.3e}")
-        worst = max(worst, e)
+        worst = max(worst, e) if e == e else float("inf")   # NaN fails
     assert worst < 1e-5
 
 

@@ -3,8 +3,9 @@
 Reference gradient: torch float64 autograd through a replay of the same accepted step grid
 (the reference's gradient is the derivative of exactly that replay: step-size factors are
 stop_gradient, pid.py:498-530).  The replay itself is pinned by central differences along
-random parameter directions.  Bars: float32 form 1e-4 relative (max |g - g_ref| /
-max |g_ref| per parameter), bfloat16 form 5e-2 (8-bit operands in the MLP products)."""
+random parameter directions.  Bar: 1e-4 relative (max |g - g_ref| / max |g_ref| per
+parameter) for both forms; the bfloat16 form's adjoint runs its products on three-piece
+bf16 operands (the float32-weight model, as its forward)."""
 import numpy as np
 import pytest
 import torch
@@ -76,7 +77,7 @@ def _replay_grads(Wn, y0, ts, wloss, grid, nacc):
     return [Y0.grad] + [p.grad for p in P], loss.item()
 
 
-@pytest.mark.parametrize("matmul,tol", [("f32", 1e-4), ("bf16", 5e-2)])
+@pytest.mark.parametrize("matmul,tol", [("f32", 1e-4), ("bf16", 1e-4)])
 def test_mlp_adjoint_against_fp64_replay(cuda, matmul, tol):
     C = 2   # checkpoint every 2 accepted steps: several segments per trajectory
     Wn, y0, ts, wloss = _setup(matmul, C=C)
@@ -89,7 +90,7 @@ def test_mlp_adjoint_against_fp64_replay(cuda, matmul, tol):
     for name, g, r in zip(names, grads, ref):
         rel = float((g - r).abs().max() / r.abs().max())
         print(f"  {name}: rel err {rel:.3e}")
-        worst = max(worst, rel)
+        worst = max(worst, rel) if rel == rel else float("inf")   # NaN fails
     assert worst < tol
 
 
