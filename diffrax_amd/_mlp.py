@@ -111,7 +111,8 @@ class MlpPlan:
             solver=_lib.TSIT5, precision=PRECISION[vfo.matmul], d=64, hidden=256, n_ts=n_ts,
             save_t0=int(sub.t0), save_t1=int(sub.t1), max_steps=int(max_steps),
             ckpt_every=int(ckpt_every), force_dtmin=int(ctrl.force_dtmin), batch=B,
-            t0=float(t0), t1=float(t1), dt0=float(dt0), rtol=_scalar_tol(ctrl.rtol),
+            t0=float(t0), t1=float(t1), dt0=nan if dt0 is None else float(dt0),
+            rtol=_scalar_tol(ctrl.rtol),
             atol=_scalar_tol(ctrl.atol),
             pcoeff=float(ctrl.pcoeff), icoeff=float(ctrl.icoeff), dcoeff=float(ctrl.dcoeff),
             dtmin=nan if ctrl.dtmin is None else float(ctrl.dtmin),
@@ -179,11 +180,6 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
     if c.norm is not rms_norm:
         raise NotImplementedError("the forward solve's controller uses rms_norm; "
                                   "adjoint_rms_seminorm is for BacksolveAdjoint's controller")
-    if float(c.pcoeff) != 0.0 or float(c.dcoeff) != 0.0:
-        raise NotImplementedError("the neural-ODE kernel implements the integral controller "
-                                  "(pcoeff = dcoeff = 0)")
-    if dt0 is None:
-        raise NotImplementedError("pass dt0 for the neural-ODE kernel")
     sub = saveat.subs
     if sub is None or sub.steps or saveat.dense:
         raise NotImplementedError("the neural-ODE kernel saves SaveAt(t0, t1, ts)")
@@ -348,8 +344,9 @@ def backsolve_problem(plan, adjoint):
     if float(ctrl.pcoeff) != 0.0 or float(ctrl.dcoeff) != 0.0:
         raise NotImplementedError("the neural-ODE backsolve implements the integral controller "
                                   "(pcoeff = dcoeff = 0)")
-    if "dt0" in kw and kw["dt0"] is None:
-        raise NotImplementedError("pass dt0 for the neural-ODE backsolve")
+    if ("dt0" in kw and kw["dt0"] is None) or ("dt0" not in kw and math.isnan(plan.prob.dt0)):
+        raise NotImplementedError("the neural-ODE backsolve needs a dt0: pass it to diffeqsolve "
+                                  "or to BacksolveAdjoint")
     nan = float("nan")
     f = plan.prob
     back = MlpProblem()

@@ -36,7 +36,9 @@ struct MlpArgs {
   PidParams pid;
 };
 
-template <class P, class Tab, bool Grad>
+// Ctl: PID_INTEGRAL (the integral controller, the default) or PID_GENERIC (any p, i, d
+// coefficients).
+template <class P, class Tab, bool Grad, int Ctl>
 __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a, const MlpWeights<P> W) {
   extern __shared__ __align__(16) unsigned char smem[];
   MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
@@ -57,10 +59,6 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
   for (int j = 0; j < 8; ++j) y[j] = valid ? a.y0[traj * MLP_D + c0 + j] : 0.0f;
   const int dir = (a.t0 < a.t1) ? 1 : -1;
   const double T0 = a.t0 * dir, T1 = a.t1 * dir;
-  double dt0v = a.dt0 * dir;
-  if (a.pid.has_dtmax) dt0v = jmin(dt0v, a.pid.dtmax);
-  if (a.pid.has_dtmin) dt0v = jmax(dt0v, a.pid.dtmin);
-  double tnext = jmin(T0 + dt0v, T1);
   double floor_t1 = T1;
   for (int r = 0; r < 100; ++r) floor_t1 = prevbefore(floor_t1);
   double tprev = T0;
@@ -84,6 +82,51 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
   for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, y[j]);
   __syncthreads();
   mlp_forward<P>(W, L, wave, lane, orow, c0, fsal);
+
+  // ---- the first step size: dt0, or _select_initial_step (pid.py:33-81) ----
+  // RMS norms over the 64 components sum in owner order, as the error norm does.
+  auto group_rms = [&](double part) {
+    double ssum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ssum += __shfl(part, gbase + q);
+    return sqrt(ssum) / 8.0;
+  };
+  double dt0v;
+  if (a.has_dt0) {
+    dt0v = a.dt0 * dir;
+  } else {
+    double scale[8], p0 = 0.0, p1 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      scale[j] = a.pid.atol + fabs((double)y[j]) * a.pid.rtol;
+      const double e0 = (double)y[j] / scale[j], e1 = (double)fsal[j] / scale[j];
+      p0 += e0 * e0;
+      p1 += e1 * e1;
+    }
+    const double d0 = group_rms(p0), d1 = group_rms(p1);
+    const bool cond = (d0 < 1e-5) || (d1 < 1e-5);
+    const double h0 = cond ? 1e-6 : 0.01 * (d0 / (cond ? 1.0 : d1));
+    __syncthreads();  // every wave is done with the seed's layer-3 partials
+#pragma unroll
+    for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, y[j] + (float)h0 * fsal[j]);
+    __syncthreads();
+    float f1[8];
+    mlp_forward<P>(W, L, wave, lane, orow, c0, f1);
+    double p2 = 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const double e2 = ((double)f1[j] - (double)fsal[j]) / scale[j];
+      p2 += e2 * e2;
+    }
+    const double d2 = group_rms(p2) / h0;
+    const double max_d = jmax(d1, d2);
+    const double h1 = (max_d <= 1e-15) ? jmax(1e-6, h0 * 1e-3)
+                                       : pow_ctl<true>(0.01 / max_d, 1.0 / a.error_order);
+    dt0v = jmin(100.0 * h0, h1);
+  }
+  if (a.pid.has_dtmax) dt0v = jmin(dt0v, a.pid.dtmax);
+  if (a.pid.has_dtmin) dt0v = jmax(dt0v, a.pid.dtmin);
+  double tnext = jmin(T0 + dt0v, T1);
 
   auto running = [&]() {
     return valid && tprev < T1 && result == DFX_SUCCESSFUL && num_steps < a.max_steps;
@@ -144,7 +187,7 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
     double ntprev, ntnext;
     int code = 0;
     PidState npid = pid;
-    bool keep = pid_adapt<PID_INTEGRAL>(a.pid, scaled, ts0, ts1, npid, ntprev, ntnext, code);
+    bool keep = pid_adapt<Ctl>(a.pid, scaled, ts0, ts1, npid, ntprev, ntnext, code);
     ntprev = jmin(ntprev, T1);
     if (ntnext > floor_t1) ntnext = keep ? T1 : ntprev + 0.5 * (T1 - ntprev);
     if (run) {
@@ -381,23 +424,29 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   a.n_ckpt = dfx_mlp_n_ckpt(p);
   a.has_dt0 = !isnan(p->dt0);
   a.t0 = p->t0; a.t1 = p->t1; a.dt0 = p->dt0; a.error_order = error_order;
-  if (!mlp_pid_params(p, a.pid)) return DFX_E_ARG;  // the neural-ODE kernel implements the I controller
-  if (!a.has_dt0) return DFX_E_ARG;           // dt0 must be given
+  // the integral controller (the default) has its own build; any p / d terms take the
+  // generic one
+  const bool integral = mlp_pid_params(p, a.pid);
   const unsigned nblocks = (unsigned)((p->batch + MLP_TILE - 1) / MLP_TILE);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-#define DFX_MLP_LAUNCH(POL, G)                                                               \
-  do {                                                                                       \
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&erk_mlp_kernel<POL, Tsit5Tab, G>), \
-                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<POL>)); \
-    hipLaunchKernelGGL((erk_mlp_kernel<POL, Tsit5Tab, G>), dim3(nblocks), dim3(MLP_THREADS),   \
-                       sizeof(MlpLds<POL>), s, a, weights_from<POL>(packed, b1, b2, b3));      \
+#define DFX_MLP_LAUNCH(POL, G, C)                                                                \
+  do {                                                                                           \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&erk_mlp_kernel<POL, Tsit5Tab, G, C>), \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<POL>)); \
+    hipLaunchKernelGGL((erk_mlp_kernel<POL, Tsit5Tab, G, C>), dim3(nblocks), dim3(MLP_THREADS),    \
+                       sizeof(MlpLds<POL>), s, a, weights_from<POL>(packed, b1, b2, b3));          \
+  } while (0)
+#define DFX_MLP_CTL(POL, G)                                                                      \
+  do {                                                                                           \
+    if (integral) DFX_MLP_LAUNCH(POL, G, PID_INTEGRAL); else DFX_MLP_LAUNCH(POL, G, PID_GENERIC); \
   } while (0)
   const bool g = p->ckpt_every > 0;
   if (p->precision == DFX_BF16_F32) {
-    if (g) DFX_MLP_LAUNCH(PolicyBF16x3, true); else DFX_MLP_LAUNCH(PolicyBF16x3, false);
+    if (g) DFX_MLP_CTL(PolicyBF16x3, true); else DFX_MLP_CTL(PolicyBF16x3, false);
   } else {
-    if (g) DFX_MLP_LAUNCH(PolicyF32, true); else DFX_MLP_LAUNCH(PolicyF32, false);
+    if (g) DFX_MLP_CTL(PolicyF32, true); else DFX_MLP_CTL(PolicyF32, false);
   }
+#undef DFX_MLP_CTL
 #undef DFX_MLP_LAUNCH
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }

@@ -17,6 +17,7 @@ import pytest
 import torch
 
 import diffrax_amd as dfx
+from diffrax_amd import _mlp
 import oracle
 
 pytestmark = pytest.mark.gpu
@@ -105,3 +106,57 @@ def test_device_resident_ts(cuda):
     assert torch.equal(ga.grad, gb.grad)
     with pytest.raises(RuntimeError, match="increasing or decreasing"):
         solve(torch.tensor([0.0, 0.5, 0.25, 0.75, 1.0], dtype=torch.float64, device=cuda), y0)
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+@pytest.mark.parametrize("ctrl_kw", [dict(), dict(pcoeff=0.3, icoeff=0.4),
+                                     dict(pcoeff=0.1, icoeff=0.3, dcoeff=0.05)],
+                         ids=["I", "PI", "PID"])
+def test_mlp_initial_step_and_pid_terms(cuda, matmul, ctrl_kw):
+    """dt0=None (_select_initial_step, pid.py:33-81) and PID p / d terms (pid.py:394-567)
+    on the neural-ODE kernel, against the FP64 oracle with the same controller: the first
+    accepted step within 1e-4 of the oracle's, and the solution's rtol-scaled error against
+    a tight solve at most the oracle's plus the kernel's own floor plus one unit (the two
+    errors add).  The floor is the kernel's error at rtol 1e-9: the float32 form evaluates
+    f to ~8e-7 relative (hardware exp2 / log2), ~1e-6 absolute over [0, 1], which the atol
+    of 1e-7 scales to several units on small components, where a PI or PID controller
+    takes the FP64 oracle well below one."""
+    W = mlp_weights(4)
+    y0 = np.random.default_rng(5).normal(size=(48, 64))
+    ts = np.linspace(0.0, 1.0, 16)
+    ctrl = dfx.PIDController(rtol=RTOL, atol=ATOL, **ctrl_kw)
+    sol = dfx.diffeqsolve(dfx.ODETerm(_vf(W, matmul)), dfx.Tsit5(), 0.0, 1.0, None,
+                          torch.tensor(y0, dtype=torch.float32), saveat=dfx.SaveAt(ts=ts),
+                          stepsize_controller=ctrl)
+    q = oracle.mlp_pack(*W)
+    y0m = y0.astype(np.float32).astype(np.float64)
+    kw = dict(ts=ts, solver="tsit5", vf="mlp", hidden=256, dt0=None, rtol=RTOL, atol=ATOL,
+              pcoeff=ctrl_kw.get("pcoeff", 0.0), icoeff=ctrl_kw.get("icoeff", 1.0),
+              dcoeff=ctrl_kw.get("dcoeff", 0.0))
+    ref = oracle.solve(y0m, 1.0, q, record_grid=True, **kw)
+    true = oracle.solve(y0m, 1.0, q, ts=ts, solver="tsit5", vf="mlp", hidden=256, dt0=0.1,
+                        rtol=1e-12, atol=1e-12, max_steps=100000)["ys"]
+    ys = sol.ys.double().cpu().numpy()
+    assert np.isfinite(ys).all() and int(sol.result.abs().max()) == 0
+    e_dev, e_ora = _scaled(ys, true), _scaled(ref["ys"], true)
+    # the first step size of every trajectory (the oracle's grid row 0) against the kernel's
+    plan = _mlp.prepare_mlp(
+        dfx.ODETerm(_vf(W, matmul)), dfx.Tsit5(), 0.0, 1.0, None,
+        torch.tensor(y0, dtype=torch.float32), dfx.SaveAt(ts=ts), ctrl, 4096,
+        torch.device("cuda", 0), ckpt_every=64)
+    plan.launch()
+    g = plan.grid[:, 0].cpu().numpy()
+    dt_dev = g[:, 1] - g[:, 0]
+    dt_ora = ref["grid"][:, 0, 1] - ref["grid"][:, 0, 0]
+    rel = np.max(np.abs(dt_dev - dt_ora) / dt_ora)
+    st = sol.stats["num_steps"].cpu().numpy()
+    print(f"{matmul} {ctrl_kw}: first step rel diff {rel:.2e}; rtol-scaled err {e_dev:.3f} "
+          f"(oracle {e_ora:.3f}); steps {st.min()}-{st.max()} (oracle "
+          f"{ref['stats'][:, 0].min()}-{ref['stats'][:, 0].max()})")
+    floor = dfx.diffeqsolve(dfx.ODETerm(_vf(W, matmul)), dfx.Tsit5(), 0.0, 1.0, None,
+                            torch.tensor(y0, dtype=torch.float32), saveat=dfx.SaveAt(ts=ts),
+                            stepsize_controller=dfx.PIDController(rtol=1e-9, atol=1e-11))
+    e_floor = _scaled(floor.ys.double().cpu().numpy(), true)
+    print(f"  kernel floor (rtol 1e-9): rtol-scaled err {e_floor:.3f}")
+    assert rel < 1e-4
+    assert e_dev <= e_ora + e_floor + 1.0
