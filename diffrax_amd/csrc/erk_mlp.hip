@@ -1,8 +1,8 @@
 // erk_mlp.hip -- persistent Runge-Kutta solve of the neural ODE dy/dt = MLP(y) (d = 64,
 // hidden 256) for batches of trajectories, and weight packing for the MFMA fragments.
 //
-// One 256-thread workgroup integrates a tile of 32 trajectories: every stage of every step
-// is one MLP evaluation of the whole tile on MFMA (mlp_tile.hpp), while time, step size,
+// One workgroup of four or eight waves integrates a tile of 32 trajectories: every stage of
+// every step is one MLP evaluation of the whole tile on MFMA (mlp_tile.hpp), while time, step size,
 // error norm, accept/reject, saves and counters stay per trajectory (owner threads, float64
 // controller as in erk_common.hpp).  Reference behaviour: the same loop as the small-system
 // kernel (_integrate.py:302-885, _solver/runge_kutta.py:446-1203,
@@ -37,26 +37,30 @@ struct MlpArgs {
 };
 
 // Ctl: PID_INTEGRAL (the integral controller, the default) or PID_GENERIC (any p, i, d
-// coefficients).
-template <class P, class Tab, bool Grad, int Ctl>
-__global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a, const MlpWeights<P> W) {
+// coefficients).  NW: waves per tile (4 or 8; MlpWaves); each trajectory has 64 / CPT owner
+// threads of CPT components each.  The results do not depend on NW.
+template <class P, class Tab, bool Grad, int Ctl, int NW>
+__global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, const MlpWeights<P> W) {
   extern __shared__ __align__(16) unsigned char smem[];
   MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
   constexpr int S = Tab::S;
+  constexpr int CPT = MlpWaves<NW>::CPT;  // components per owner
+  constexpr int GRP = MLP_D / CPT;         // owners per trajectory
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
-  const int orow = tid >> 3;          // trajectory within the tile
-  const int c0 = (tid & 7) * 8;       // owner's first component
-  const int gbase = lane & ~7;        // first lane of this trajectory's owner group
+  const int orow = tid / GRP;             // trajectory within the tile
+  const int c0 = (tid % GRP) * CPT;       // owner's first component
+  const int gbase = lane & ~(GRP - 1);    // first lane of this trajectory's owner group
+  const bool lead = (tid % GRP) == 0;
   const int64_t traj = (int64_t)blockIdx.x * MLP_TILE + orow;
   const bool valid = traj < a.B;
 
   using TL = TabLit<Tab, float>;  // float32 stage algebra: the coefficients as literals
 
   // ---- set-up (diffeqsolve / PIDController.init) ----
-  float y[8], fsal[8];
+  float y[CPT], fsal[CPT];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) y[j] = valid ? a.y0[traj * MLP_D + c0 + j] : 0.0f;
+  for (int j = 0; j < CPT; ++j) y[j] = valid ? a.y0[traj * MLP_D + c0 + j] : 0.0f;
   const int dir = (a.t0 < a.t1) ? 1 : -1;
   const double T0 = a.t0 * dir, T1 = a.t1 * dir;
   double floor_t1 = T1;
@@ -69,56 +73,70 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
   double next_ts = (a.n_ts > 0) ? a.ts[0] * dir : __builtin_inf();
   float* ys_row = valid ? a.ys + traj * (int64_t)a.n_out * MLP_D : nullptr;
   double* tso_row = valid ? a.ts_out + traj * (int64_t)a.n_out : nullptr;
-  auto write_save = [&](double tn, const float (&v)[8]) {
+  auto write_save = [&](double tn, const float (&v)[CPT]) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) ys_row[save_index * MLP_D + c0 + j] = v[j];
-    if ((tid & 7) == 0) tso_row[save_index] = tn * dir;
+    for (int j = 0; j < CPT; ++j) ys_row[save_index * MLP_D + c0 + j] = v[j];
+    if (lead) tso_row[save_index] = tn * dir;
     ++save_index;
   };
   if (valid && a.save_t0) write_save(T0, y);
 
   // ---- first stage of the first step (FSAL seed): f(t0, y0) ----
 #pragma unroll
-  for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, y[j]);
+  for (int j = 0; j < CPT; ++j) store_input<P>(L, orow, c0 + j, y[j]);
   __syncthreads();
-  mlp_forward<P>(W, L, wave, lane, orow, c0, fsal);
+  mlp_forward<P, NW>(W, L, wave, lane, orow, c0, fsal);
 
   // ---- the first step size: dt0, or _select_initial_step (pid.py:33-81) ----
-  // RMS norms over the 64 components sum in owner order, as the error norm does.
-  auto group_rms = [&](double part) {
+  // RMS over the 64 components of per-component squares t: eight partial sums of eight
+  // components each, in component order, then the eight partials in order.  With four
+  // components per owner the odd owner of a pair continues its even neighbour's partial,
+  // so both wave counts round identically.
+  auto group_rms = [&](const double (&t)[CPT]) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) s += t[j];
+    int stride = 1, first = 0;
+    if constexpr (CPT == 4) {
+      s = __shfl(s, lane & ~1);
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) s += t[j];
+      stride = 2;
+      first = 1;
+    }
     double ssum = 0.0;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) ssum += __shfl(part, gbase + q);
+    for (int q = 0; q < 8; ++q) ssum += __shfl(s, gbase + stride * q + first);
     return sqrt(ssum) / 8.0;
   };
   double dt0v;
   if (a.has_dt0) {
     dt0v = a.dt0 * dir;
   } else {
-    double scale[8], p0 = 0.0, p1 = 0.0;
+    double scale[CPT], t0[CPT], t1[CPT];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CPT; ++j) {
       scale[j] = a.pid.atol + fabs((double)y[j]) * a.pid.rtol;
       const double e0 = (double)y[j] / scale[j], e1 = (double)fsal[j] / scale[j];
-      p0 += e0 * e0;
-      p1 += e1 * e1;
+      t0[j] = e0 * e0;
+      t1[j] = e1 * e1;
     }
-    const double d0 = group_rms(p0), d1 = group_rms(p1);
+    const double d0 = group_rms(t0), d1 = group_rms(t1);
     const bool cond = (d0 < 1e-5) || (d1 < 1e-5);
     const double h0 = cond ? 1e-6 : 0.01 * (d0 / (cond ? 1.0 : d1));
     __syncthreads();  // every wave is done with the seed's layer-3 partials
 #pragma unroll
-    for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, y[j] + (float)h0 * fsal[j]);
+    for (int j = 0; j < CPT; ++j) store_input<P>(L, orow, c0 + j, y[j] + (float)h0 * fsal[j]);
     __syncthreads();
-    float f1[8];
-    mlp_forward<P>(W, L, wave, lane, orow, c0, f1);
-    double p2 = 0.0;
+    float f1[CPT];
+    mlp_forward<P, NW>(W, L, wave, lane, orow, c0, f1);
+    double t2[CPT];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CPT; ++j) {
       const double e2 = ((double)f1[j] - (double)fsal[j]) / scale[j];
-      p2 += e2 * e2;
+      t2[j] = e2 * e2;
     }
-    const double d2 = group_rms(p2) / h0;
+    const double d2 = group_rms(t2) / h0;
     const double max_d = jmax(d1, d2);
     const double h1 = (max_d <= 1e-15) ? jmax(1e-6, h0 * 1e-3)
                                        : pow_ctl<true>(0.01 / max_d, 1.0 / a.error_order);
@@ -137,14 +155,14 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
     const double ts0 = tprev, ts1 = tnext;
     const double dt = ts1 - ts0;
     const float control = (float)((double)dir * dt);
-    float k[S][8];
-    float z[8], f[8];
+    float k[S][CPT];
+    float z[CPT], f[CPT];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) k[0][j] = control * fsal[j];
+    for (int j = 0; j < CPT; ++j) k[0][j] = control * fsal[j];
     static_for<1, S>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         float kc[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
@@ -152,38 +170,35 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
         store_input<P>(L, orow, c0 + j, z[j]);
       }
       __syncthreads();
-      mlp_forward<P>(W, L, wave, lane, orow, c0, f);
+      mlp_forward<P, NW>(W, L, wave, lane, orow, c0, f);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) k[i][j] = control * f[j];
+      for (int j = 0; j < CPT; ++j) k[i][j] = control * f[j];
     });
     // y1 = last stage (SSAL); embedded error
-    float yerr[8];
-    double part = 0.0;
+    float yerr[CPT];
+    double sq[CPT];
     bool anynan = false;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CPT; ++j) {
       float kc[S];
 #pragma unroll
       for (int m = 0; m < S; ++m) kc[m] = k[m][j];
       yerr[j] = comb<Tab, 2, 0, 0, S, false>(TL::v.berr, kc, 0.0f);
       anynan = anynan || (z[j] != z[j]);
     }
-    // group reduction in owner order (identical in all 8 owners)
+    // group reduction in owner order (identical in all owners of the trajectory)
     unsigned long long nanmask = __ballot(anynan);
-    const bool group_nan = ((nanmask >> gbase) & 0xffull) != 0;
+    const bool group_nan = ((nanmask >> gbase) & ((1ull << GRP) - 1)) != 0;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CPT; ++j) {
       const double y1c = group_nan ? (double)y[j] : (double)z[j];
       const double yy = jmax(fabs((double)y[j]), fabs(y1c));
       double e = (double)yerr[j];
       if (e != e) e = __builtin_inf();
       e = e / (a.pid.atol + yy * a.pid.rtol);
-      part += e * e;
+      sq[j] = e * e;
     }
-    double ssum = 0.0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) ssum += __shfl(part, gbase + q);
-    const double scaled = sqrt(ssum) / 8.0;  // rms over d = 64 components
+    const double scaled = group_rms(sq);
     double ntprev, ntnext;
     int code = 0;
     PidState npid = pid;
@@ -198,23 +213,23 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
       if (keep) {
         if constexpr (Grad) {
           const int n = num_acc - 1;
-          if ((tid & 7) == 0) {
+          if (lead) {
             a.grid[(traj * a.max_steps + n) * 2 + 0] = ts0;
             a.grid[(traj * a.max_steps + n) * 2 + 1] = ts1;
           }
           if (n % a.ckpt_every == 0 && n / a.ckpt_every < a.n_ckpt) {
             float* c = a.ckpt + (traj * a.n_ckpt + n / a.ckpt_every) * MLP_D + c0;
 #pragma unroll
-            for (int j = 0; j < 8; ++j) c[j] = y[j];
+            for (int j = 0; j < CPT; ++j) c[j] = y[j];
           }
         }
         while (next_ts <= ts1) {
           const double th = linear_rescale(ts0, next_ts, ts1);
           double bb[7];
           tsit5_weights(th, bb);
-          float v[8];
+          float v[CPT];
 #pragma unroll
-          for (int j = 0; j < 8; ++j) {
+          for (int j = 0; j < CPT; ++j) {
             float acc = (float)bb[0] * k[0][j];
 #pragma unroll
             for (int m = 1; m < 7; ++m) acc = acc + (float)bb[m] * k[m][j];
@@ -225,7 +240,7 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
           next_ts = (ts_index < a.n_ts) ? a.ts[ts_index] * dir : __builtin_inf();
         }
 #pragma unroll
-        for (int j = 0; j < 8; ++j) { y[j] = z[j]; fsal[j] = f[j]; }
+        for (int j = 0; j < CPT; ++j) { y[j] = z[j]; fsal[j] = f[j]; }
       }
       tprev = ntprev;
       tnext = ntnext;
@@ -239,10 +254,10 @@ __global__ void __launch_bounds__(MLP_THREADS, 1) erk_mlp_kernel(const MlpArgs a
   if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
   for (int s = save_index; s < a.n_out; ++s) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) ys_row[s * MLP_D + c0 + j] = __builtin_inff();
-    if ((tid & 7) == 0) tso_row[s] = __builtin_inf();  // (dir * inf) * dir: _integrate.py:1296, 1481
+    for (int j = 0; j < CPT; ++j) ys_row[s * MLP_D + c0 + j] = __builtin_inff();
+    if (lead) tso_row[s] = __builtin_inf();  // (dir * inf) * dir: _integrate.py:1296, 1481
   }
-  if ((tid & 7) == 0) {
+  if (lead) {
     a.stats[traj * 3 + 0] = num_steps;
     a.stats[traj * 3 + 1] = num_acc;
     a.stats[traj * 3 + 2] = num_rej;
@@ -431,9 +446,9 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
 #define DFX_MLP_LAUNCH(POL, G, C)                                                                \
   do {                                                                                           \
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&erk_mlp_kernel<POL, Tsit5Tab, G, C>), \
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&erk_mlp_kernel<POL, Tsit5Tab, G, C, NWV>), \
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<POL>)); \
-    hipLaunchKernelGGL((erk_mlp_kernel<POL, Tsit5Tab, G, C>), dim3(nblocks), dim3(MLP_THREADS),    \
+    hipLaunchKernelGGL((erk_mlp_kernel<POL, Tsit5Tab, G, C, NWV>), dim3(nblocks), dim3(64 * NWV), \
                        sizeof(MlpLds<POL>), s, a, weights_from<POL>(packed, b1, b2, b3));          \
   } while (0)
 #define DFX_MLP_CTL(POL, G)                                                                      \
@@ -441,9 +456,14 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
     if (integral) DFX_MLP_LAUNCH(POL, G, PID_INTEGRAL); else DFX_MLP_LAUNCH(POL, G, PID_GENERIC); \
   } while (0)
   const bool g = p->ckpt_every > 0;
+  // waves per tile: eight for the three-piece form (its 436 registers per lane at four
+  // waves leave one wave per SIMD; at eight, two), four for float32, which runs faster
+  // at four (0.49 vs 0.52 ms per solve of 8,192 trajectories)
   if (p->precision == DFX_BF16_F32) {
+    constexpr int NWV = 8;
     if (g) DFX_MLP_CTL(PolicyBF16x3, true); else DFX_MLP_CTL(PolicyBF16x3, false);
   } else {
+    constexpr int NWV = 4;
     if (g) DFX_MLP_CTL(PolicyF32, true); else DFX_MLP_CTL(PolicyF32, false);
   }
 #undef DFX_MLP_CTL

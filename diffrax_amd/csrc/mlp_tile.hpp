@@ -282,62 +282,6 @@ __device__ __forceinline__ void wave_gemm(const typename P::act_t* a_lds,
 // row 8i + (l/32)*4 + j, column l%32 of the 32x32 tile.
 __device__ __forceinline__ int acc_row(int lane, int v) { return 8 * (v >> 2) + (lane >> 5) * 4 + (v & 3); }
 
-// Forward of the MLP on the tile: reads L.x (stage values), leaves softplus activations in
-// L.h1 / L.h2 and returns f for the owner's 8 components in fout.
-template <class P>
-__device__ __forceinline__ void mlp_forward(const MlpWeights<P>& W, MlpLds<P>& L, int wave,
-                                            int lane, int owner_row, int owner_c0,
-                                            float (&fout)[8]) {
-  constexpr bool F = P::fast_math;
-  floatx16_t acc[2];
-  // layer 1: [32 x 64] . W1^T -> [32 x 256], wave's columns [64w, 64w + 64)
-  constexpr bool S = P::split_input;
-  wave_gemm<P, MLP_D / P::KSTEP, 2, S>(L.x, L.xlo, L.LDX, W.w1, 2 * wave, 0, MLP_D / P::KSTEP, acc,
-                                       lane);
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int col = 64 * wave + 32 * t + (lane & 31);
-    const float bias = W.b1[col];
-#pragma unroll
-    for (int v = 0; v < 16; ++v)
-      store_hidden<P>(L.h1, L.h1lo, acc_row(lane, v) * L.LDH + col, softplus_f<F>(acc[t][v] + bias));
-  }
-  __syncthreads();
-  // layer 2: [32 x 256] . W2^T -> [32 x 256]
-  wave_gemm<P, MLP_H / P::KSTEP, 2, S, (P::KA == 1 ? 0 : 8)>(L.h1, L.h1lo, L.LDH, W.w2,
-                                                             2 * wave, 0, MLP_H / P::KSTEP,
-                                                             acc, lane);
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int col = 64 * wave + 32 * t + (lane & 31);
-    const float bias = W.b2[col];
-#pragma unroll
-    for (int v = 0; v < 16; ++v)
-      store_hidden<P>(L.h2, L.h2lo, acc_row(lane, v) * L.LDH + col, softplus_f<F>(acc[t][v] + bias));
-  }
-  __syncthreads();
-  // layer 3 (split-K): wave w sums K in [64w, 64w + 64) for all 64 outputs
-  constexpr int KS3 = MLP_H / P::KSTEP;
-  wave_gemm<P, KS3, 2, S>(L.h2, L.h2lo, L.LDH, W.w3, 0, wave * (KS3 / 4), (wave + 1) * (KS3 / 4),
-                          acc, lane);
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int col = 32 * t + (lane & 31);
-#pragma unroll
-    for (int v = 0; v < 16; ++v) L.part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
-  }
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int c = owner_c0 + j;
-    float s = L.part[(0 * MLP_TILE + owner_row) * L.LDP + c];
-    s += L.part[(1 * MLP_TILE + owner_row) * L.LDP + c];
-    s += L.part[(2 * MLP_TILE + owner_row) * L.LDP + c];
-    s += L.part[(3 * MLP_TILE + owner_row) * L.LDP + c];
-    fout[j] = tanh_f<F>(s + W.b3[c]);
-  }
-}
-
 // ---- the three-piece bf16 form --------------------------------------------------------
 // LDS: the three pieces of the layer-1 input and of both hidden activations (115 KB).  The
 // split-K partials of layer 3 reuse the h1 planes, which nobody reads after layer 2.
@@ -443,30 +387,35 @@ struct MlpRowSink {
   }
 };
 
+// The tile's waves split the work: layers 1 and 2 by output columns (NT12 tiles of 32 per
+// wave), layer 3 by K quarters (four waves: both 32-column tiles of one quarter; eight
+// waves: one tile of one quarter), into the same four partial slabs.  The owners of a row
+// hold 32 / NW of its components.
+template <int NW>
+struct MlpWaves {
+  static_assert(NW == 4 || NW == 8, "four or eight waves per tile");
+  static constexpr int NT12 = 8 / NW;
+  static constexpr int NT3 = NW == 4 ? 2 : 1;
+  static constexpr int CPT = 32 / NW;
+  __device__ static int nt3(int wave) { return NW == 4 ? 0 : (wave & 1); }
+  __device__ static int quarter(int wave) { return NW == 4 ? wave : (wave >> 1); }
+};
+
+template <int NW = 4>
 __device__ __forceinline__ void mlp_forward3(const MlpWeights<PolicyBF16x3>& W,
                                              MlpLds<PolicyBF16x3>& L, int wave, int lane,
-                                             int owner_row, int owner_c0, float (&fout)[8],
-                                             const MlpRowSink& sink);
-
-template <>
-__device__ __forceinline__ void mlp_forward<PolicyBF16x3>(const MlpWeights<PolicyBF16x3>& W,
-                                                          MlpLds<PolicyBF16x3>& L, int wave,
-                                                          int lane, int owner_row, int owner_c0,
-                                                          float (&fout)[8]) {
-  mlp_forward3(W, L, wave, lane, owner_row, owner_c0, fout, MlpRowSink{});
-}
-
-__device__ __forceinline__ void mlp_forward3(const MlpWeights<PolicyBF16x3>& W,
-                                             MlpLds<PolicyBF16x3>& L, int wave, int lane,
-                                             int owner_row, int owner_c0, float (&fout)[8],
+                                             int owner_row, int owner_c0,
+                                             float (&fout)[MlpWaves<NW>::CPT],
                                              const MlpRowSink& sink) {
+  using WV = MlpWaves<NW>;
+  constexpr int NT = WV::NT12;
   constexpr int KS1 = MLP_D / 16, KS2 = MLP_H / 16;
-  floatx16_t acc[2];
+  floatx16_t acc[NT];
   // layer 1: [32 x 64] . W1^T -> [32 x 256], wave's columns [64w, 64w + 64)
-  wave_gemm3<KS1, 2>(L.x[0], L.x[1], L.x[2], L.LDX, W.w1, W.plane1, 2 * wave, 0, KS1, acc, lane);
+  wave_gemm3<KS1, NT>(L.x[0], L.x[1], L.x[2], L.LDX, W.w1, W.plane1, NT * wave, 0, KS1, acc, lane);
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int col = 64 * wave + 32 * t + (lane & 31);
+  for (int t = 0; t < NT; ++t) {
+    const int col = 32 * (NT * wave + t) + (lane & 31);
     const float bias = W.b1[col];
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
@@ -478,11 +427,11 @@ __device__ __forceinline__ void mlp_forward3(const MlpWeights<PolicyBF16x3>& W,
   }
   __syncthreads();
   // layer 2: [32 x 256] . W2^T -> [32 x 256]
-  wave_gemm3<KS2, 2>(L.h1[0], L.h1[1], L.h1[2], L.LDH, W.w2, W.plane2, 2 * wave, 0, KS2, acc,
-                     lane);
+  wave_gemm3<KS2, NT>(L.h1[0], L.h1[1], L.h1[2], L.LDH, W.w2, W.plane2, NT * wave, 0, KS2, acc,
+                      lane);
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int col = 64 * wave + 32 * t + (lane & 31);
+  for (int t = 0; t < NT; ++t) {
+    const int col = 32 * (NT * wave + t) + (lane & 31);
     const float bias = W.b2[col];
 #pragma unroll
     for (int v = 0; v < 16; ++v) {
@@ -495,24 +444,93 @@ __device__ __forceinline__ void mlp_forward3(const MlpWeights<PolicyBF16x3>& W,
   __syncthreads();
   // layer 3 (split-K): wave w sums K in [64w, 64w + 64) for all 64 outputs; the partials go
   // to the h1 planes (layer 2, their last reader, finished before the barrier above)
-  wave_gemm3<KS2, 2>(L.h2[0], L.h2[1], L.h2[2], L.LDH, W.w3, W.plane3, 0, wave * (KS2 / 4),
-                     (wave + 1) * (KS2 / 4), acc, lane);
+  constexpr int NT3 = WV::NT3;
+  const int qk = WV::quarter(wave), n3 = WV::nt3(wave);
+  floatx16_t acc3[NT3];
+  wave_gemm3<KS2, NT3>(L.h2[0], L.h2[1], L.h2[2], L.LDH, W.w3, W.plane3, n3, qk * (KS2 / 4),
+                       (qk + 1) * (KS2 / 4), acc3, lane);
   float* part = L.part();
 #pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int col = 32 * t + (lane & 31);
+  for (int t = 0; t < NT3; ++t) {
+    const int col = 32 * (n3 + t) + (lane & 31);
 #pragma unroll
-    for (int v = 0; v < 16; ++v) part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
+    for (int v = 0; v < 16; ++v) part[(qk * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc3[t][v];
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < 8; ++j) {
+  for (int j = 0; j < WV::CPT; ++j) {
     const int c = owner_c0 + j;
     float s = part[(0 * MLP_TILE + owner_row) * L.LDP + c];
     s += part[(1 * MLP_TILE + owner_row) * L.LDP + c];
     s += part[(2 * MLP_TILE + owner_row) * L.LDP + c];
     s += part[(3 * MLP_TILE + owner_row) * L.LDP + c];
     fout[j] = tanh_f<true>(s + W.b3[c]);
+  }
+}
+
+// Forward of the MLP on the tile: reads L.x (stage values), leaves softplus activations in
+// L.h1 / L.h2 and returns f for the owner's 32 / NW components in fout (NW waves per tile;
+// MlpWaves).  The three-piece bf16 form runs mlp_forward3.
+template <class P, int NW = 4>
+__device__ __forceinline__ void mlp_forward(const MlpWeights<P>& W, MlpLds<P>& L, int wave,
+                                            int lane, int owner_row, int owner_c0,
+                                            float (&fout)[MlpWaves<NW>::CPT]) {
+  if constexpr (P::PIECES == 3) {
+    mlp_forward3<NW>(W, L, wave, lane, owner_row, owner_c0, fout, MlpRowSink{});
+  } else {
+  using WV = MlpWaves<NW>;
+  constexpr int NT = WV::NT12;
+  constexpr bool F = P::fast_math;
+  floatx16_t acc[NT];
+  // layer 1: [32 x 64] . W1^T -> [32 x 256], wave's columns [64w, 64w + 64)
+  constexpr bool S = P::split_input;
+  wave_gemm<P, MLP_D / P::KSTEP, NT, S>(L.x, L.xlo, L.LDX, W.w1, NT * wave, 0, MLP_D / P::KSTEP,
+                                        acc, lane);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int col = 32 * (NT * wave + t) + (lane & 31);
+    const float bias = W.b1[col];
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      store_hidden<P>(L.h1, L.h1lo, acc_row(lane, v) * L.LDH + col, softplus_f<F>(acc[t][v] + bias));
+  }
+  __syncthreads();
+  // layer 2: [32 x 256] . W2^T -> [32 x 256]
+  wave_gemm<P, MLP_H / P::KSTEP, NT, S, (P::KA == 1 ? 0 : 8)>(L.h1, L.h1lo, L.LDH, W.w2,
+                                                              NT * wave, 0, MLP_H / P::KSTEP,
+                                                              acc, lane);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int col = 32 * (NT * wave + t) + (lane & 31);
+    const float bias = W.b2[col];
+#pragma unroll
+    for (int v = 0; v < 16; ++v)
+      store_hidden<P>(L.h2, L.h2lo, acc_row(lane, v) * L.LDH + col, softplus_f<F>(acc[t][v] + bias));
+  }
+  __syncthreads();
+  // layer 3 (split-K): wave w sums K in [64w, 64w + 64) for all 64 outputs
+  constexpr int KS3 = MLP_H / P::KSTEP;
+  constexpr int NT3 = WV::NT3;
+  const int qk = WV::quarter(wave), n3 = WV::nt3(wave);
+  floatx16_t acc3[NT3];
+  wave_gemm<P, KS3, NT3, S>(L.h2, L.h2lo, L.LDH, W.w3, n3, qk * (KS3 / 4), (qk + 1) * (KS3 / 4),
+                            acc3, lane);
+#pragma unroll
+  for (int t = 0; t < NT3; ++t) {
+    const int col = 32 * (n3 + t) + (lane & 31);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) L.part[(qk * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc3[t][v];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < WV::CPT; ++j) {
+    const int c = owner_c0 + j;
+    float s = L.part[(0 * MLP_TILE + owner_row) * L.LDP + c];
+    s += L.part[(1 * MLP_TILE + owner_row) * L.LDP + c];
+    s += L.part[(2 * MLP_TILE + owner_row) * L.LDP + c];
+    s += L.part[(3 * MLP_TILE + owner_row) * L.LDP + c];
+    fout[j] = tanh_f<F>(s + W.b3[c]);
+  }
   }
 }
 
