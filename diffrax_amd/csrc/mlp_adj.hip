@@ -296,9 +296,12 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
 // buffers hold float32 values (layer inputs and deltas written before their split into
 // pieces) for the float32 weight-gradient GEMM.  LDS: the forward planes;
 // in the sweep the x planes take delta3, the h1 planes delta2 and then the split-K partials,
-// the h2 planes delta1 (the activations come back from the row buffers).
+// the h2 planes delta1 (the activations come back from the row buffers).  NW waves per tile
+// share the products as in the forward (MlpWaves); the arithmetic of every element does not
+// depend on NW.
 
-__global__ void __launch_bounds__(MLP_THREADS, 1)
+template <int NW>
+__global__ void __launch_bounds__(64 * NW, 1)
 mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
                         const RowBufs<PolicyF32> R) {
   using P = PolicyBF16x3;
@@ -306,8 +309,10 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
   MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
   using Tab = Tsit5Tab;
   constexpr int S = Tab::S;
+  using WV = MlpWaves<NW>;
+  constexpr int CPT = WV::CPT, GRP = MLP_D / CPT, NT = WV::NT12, NT3 = WV::NT3;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int orow = tid >> 3, c0 = (tid & 7) * 8;
+  const int orow = tid / GRP, c0 = (tid % GRP) * CPT;
   const int tile = blockIdx.x;
   const int64_t traj = (int64_t)tile * MLP_TILE + orow;
   const bool valid = traj < a.B;
@@ -337,37 +342,37 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
   };
 
   // ---- 1. recompute the segment (as mlp_adj_segment_kernel) ----
-  float y[8], fl[8];
+  float y[CPT], fl[CPT];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) y[j] = (steps_here > 0) ? a.ckpt[(traj * a.n_ckpt + a.seg) * MLP_D + c0 + j] : 0.0f;
+  for (int j = 0; j < CPT; ++j) y[j] = (steps_here > 0) ? a.ckpt[(traj * a.n_ckpt + a.seg) * MLP_D + c0 + j] : 0.0f;
   for (int q = 0; q < a.Cs; ++q) {
     const bool act = q < steps_here;
     const int n = n0 + q;
     const double ta = act ? a.grid[(traj * a.max_steps + n) * 2 + 0] : 0.0;
     const double tb = act ? a.grid[(traj * a.max_steps + n) * 2 + 1] : 0.0;
     const float control = (float)((double)dir * (tb - ta));
-    float k[S][8], z[8], f[8];
+    float k[S][CPT], z[CPT], f[CPT];
     if (q == 0) {
       const MlpRowSink k0 = sink_of(0, 0);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         store_input<P>(L, orow, c0 + j, y[j]);
         put_x(k0, j, y[j]);
       }
       __syncthreads();
-      mlp_forward3(W, L, wave, lane, orow, c0, fl, k0);
+      mlp_forward3<NW>(W, L, wave, lane, orow, c0, fl, k0);
       __syncthreads();
     }
     {
       float* fp = fslot(q, 0);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) { fp[j] = fl[j]; k[0][j] = control * fl[j]; }
+      for (int j = 0; j < CPT; ++j) { fp[j] = fl[j]; k[0][j] = control * fl[j]; }
     }
     static_for<1, S>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const MlpRowSink ki = sink_of(q, i);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         float kc[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
@@ -376,32 +381,32 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
         put_x(ki, j, z[j]);
       }
       __syncthreads();
-      mlp_forward3(W, L, wave, lane, orow, c0, f, ki);
+      mlp_forward3<NW>(W, L, wave, lane, orow, c0, f, ki);
       __syncthreads();
       float* fp = fslot(q, i);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) { fp[j] = f[j]; k[i][j] = control * f[j]; }
+      for (int j = 0; j < CPT; ++j) { fp[j] = f[j]; k[i][j] = control * f[j]; }
     });
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CPT; ++j) {
       if (act) y[j] = z[j];
       fl[j] = f[j];
     }
   }
 
   // ---- 2. reverse sweep ----
-  float yb[8];
+  float yb[CPT];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) yb[j] = valid ? a.ybar[traj * MLP_D + c0 + j] : 0.0f;
+  for (int j = 0; j < CPT; ++j) yb[j] = valid ? a.ybar[traj * MLP_D + c0 + j] : 0.0f;
   for (int q = a.Cs - 1; q >= 0; --q) {
     const bool act = q < steps_here;
     const int n = n0 + q;
     const double ta = act ? a.grid[(traj * a.max_steps + n) * 2 + 0] : 0.0;
     const double tb = act ? a.grid[(traj * a.max_steps + n) * 2 + 1] : 0.0;
     const float control = (float)((double)dir * (tb - ta));
-    float kb[S][8], ybn[8];
+    float kb[S][CPT], ybn[CPT];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CPT; ++j) {
       ybn[j] = yb[j];
 #pragma unroll
       for (int m = 0; m < S; ++m) kb[m][j] = 0.0f;
@@ -410,7 +415,7 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
 #pragma unroll
       for (int m = 0; m < S - 1; ++m) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) kb[m][j] += TL::v.a[S - 1][m] * yb[j];
+        for (int j = 0; j < CPT; ++j) kb[m][j] += TL::v.a[S - 1][m] * yb[j];
       }
       int jlo = 0, jhi = 0;
       for (int s = 0; s < a.n_ts; ++s) {
@@ -424,7 +429,7 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
         tsit5_weights(th, bb);
         const float* g = a.ys_bar + (traj * a.n_out + (a.save_t0 ? 1 : 0) + s) * MLP_D + c0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < CPT; ++j) {
           const float gv = g[j];
           ybn[j] += gv;
 #pragma unroll
@@ -438,7 +443,7 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
       const float* fp = fslot(q, i);
       // delta3 = fbar (1 - f^2), fbar = control * kbar_i: its row, and its pieces in x
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         const float fb = act ? control * kb[i][j] : 0.0f;
         const float fv = fp[j];
         const float d3 = fb * (1.0f - fv * fv);
@@ -448,12 +453,12 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
       }
       __syncthreads();
       // delta2 = (delta3 . W3) softplus'(h2): its row, and its pieces in h1
-      floatx16_t acc[2];
-      wave_gemm3<MLP_D / 16, 2>(L.x[0], L.x[1], L.x[2], L.LDX, W.w3t, W.plane3, 2 * wave, 0,
-                                MLP_D / 16, acc, lane);
+      floatx16_t acc[NT];
+      wave_gemm3<MLP_D / 16, NT>(L.x[0], L.x[1], L.x[2], L.LDX, W.w3t, W.plane3, NT * wave, 0,
+                                 MLP_D / 16, acc, lane);
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int col = 64 * wave + 32 * t + (lane & 31);
+      for (int t = 0; t < NT; ++t) {
+        const int col = 32 * (NT * wave + t) + (lane & 31);
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const int r = acc_row(lane, v);
@@ -466,11 +471,11 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
       }
       __syncthreads();
       // delta1 = (delta2 . W2) softplus'(h1): its row, and its pieces in h2
-      wave_gemm3<MLP_H / 16, 2>(L.h1[0], L.h1[1], L.h1[2], L.LDH, W.w2t, W.plane2, 2 * wave, 0,
-                                MLP_H / 16, acc, lane);
+      wave_gemm3<MLP_H / 16, NT>(L.h1[0], L.h1[1], L.h1[2], L.LDH, W.w2t, W.plane2, NT * wave, 0,
+                                 MLP_H / 16, acc, lane);
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int col = 64 * wave + 32 * t + (lane & 31);
+      for (int t = 0; t < NT; ++t) {
+        const int col = 32 * (NT * wave + t) + (lane & 31);
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const int r = acc_row(lane, v);
@@ -485,19 +490,21 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
       // zbar = delta1 . W1 (split-K over the 256 hidden units); the partials take the h1
       // planes, whose last readers (the delta1 products) are behind the barrier above
       constexpr int KS1 = MLP_H / 16;
-      wave_gemm3<KS1, 2>(L.h2[0], L.h2[1], L.h2[2], L.LDH, W.w1t, W.plane1, 0, wave * (KS1 / 4),
-                         (wave + 1) * (KS1 / 4), acc, lane);
+      const int qk = WV::quarter(wave), n3 = WV::nt3(wave);
+      floatx16_t acc3[NT3];
+      wave_gemm3<KS1, NT3>(L.h2[0], L.h2[1], L.h2[2], L.LDH, W.w1t, W.plane1, n3,
+                           qk * (KS1 / 4), (qk + 1) * (KS1 / 4), acc3, lane);
       float* part = L.part();
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int col = 32 * t + (lane & 31);
+      for (int t = 0; t < NT3; ++t) {
+        const int col = 32 * (n3 + t) + (lane & 31);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
+        for (int v = 0; v < 16; ++v) part[(qk * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc3[t][v];
       }
       __syncthreads();
-      float zb[8];
+      float zb[CPT];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         const int c = c0 + j;
         float sv = part[(0 * MLP_TILE + orow) * L.LDP + c];
         sv += part[(1 * MLP_TILE + orow) * L.LDP + c];
@@ -506,7 +513,7 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
         zb[j] = act ? sv : 0.0f;
       }
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         ybn[j] += zb[j];
 #pragma unroll
         for (int m = 0; m < i; ++m) kb[m][j] += TL::v.a[i][m] * zb[j];
@@ -515,12 +522,12 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
     });
     if (act) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) yb[j] = ybn[j];
+      for (int j = 0; j < CPT; ++j) yb[j] = ybn[j];
     }
   }
   if (valid) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) a.ybar[traj * MLP_D + c0 + j] = yb[j];
+    for (int j = 0; j < CPT; ++j) a.ybar[traj * MLP_D + c0 + j] = yb[j];
   }
 }
 
@@ -1002,9 +1009,9 @@ __global__ void bs_reset_kernel(int32_t* counters, float* rscale, int64_t used, 
 // mlp_adj_segment3_kernel on the three split3 planes of a bf16 buffer), then the float32
 // weight-gradient GEMM over its float32 row buffers and the fixed-order reduction.
 template <class P, class Kern>
-int run_segment(Kern kern, const dfx_mlp_problem* p, const void* packed, const float* b1,
-                const float* b2, const float* b3, const dfx_mlp_adj_io* io, int32_t seg,
-                int32_t seg_steps, hipStream_t s) {
+int run_segment(Kern kern, int threads, const dfx_mlp_problem* p, const void* packed,
+                const float* b1, const float* b2, const float* b3, const dfx_mlp_adj_io* io,
+                int32_t seg, int32_t seg_steps, hipStream_t s) {
   AdjArgs a;
   memset(&a, 0, sizeof(a));
   a.grid = io->grid; a.ckpt = io->ckpt; a.stats = io->stats; a.ts = io->ts; a.ys_bar = io->ys_bar;
@@ -1024,7 +1031,7 @@ int run_segment(Kern kern, const dfx_mlp_problem* p, const void* packed, const f
   const unsigned ntiles = (unsigned)(a.B_pad / MLP_TILE);
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(MlpLds<P>));
-  hipLaunchKernelGGL(kern, dim3(ntiles), dim3(MLP_THREADS), sizeof(MlpLds<P>), s, a, W, R);
+  hipLaunchKernelGGL(kern, dim3(ntiles), dim3(threads), sizeof(MlpLds<P>), s, a, W, R);
   const int nch = (int)seg_chunks(p->batch, seg_steps);
   const DwLayout lay = dw_layout();
   float* partial = io->partial;
@@ -1156,12 +1163,15 @@ int dfx_mlp_adj_segment(const dfx_mlp_problem* p, const void* packed, const floa
   if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // the three-piece form at eight waves per tile: two waves per SIMD (at four it held 512
+  // registers per lane, one wave per SIMD; the backward of 8,192 trajectories took 2.51-2.55
+  // ms there, 2.16-2.17 ms at eight, with bit-identical gradients)
   if (p->precision == DFX_BF16_F32)
-    return run_segment<PolicyBF16x3>(&mlp_adj_segment3_kernel, p, packed, b1, b2, b3, io, segment,
-                                      seg_steps, s);
+    return run_segment<PolicyBF16x3>(&mlp_adj_segment3_kernel<8>, 64 * 8, p, packed, b1, b2, b3,
+                                      io, segment, seg_steps, s);
   if (p->precision == DFX_F32)
-    return run_segment<PolicyF32>(&mlp_adj_segment_kernel<PolicyF32>, p, packed, b1, b2, b3, io,
-                                  segment, seg_steps, s);
+    return run_segment<PolicyF32>(&mlp_adj_segment_kernel<PolicyF32>, MLP_THREADS, p, packed, b1,
+                                  b2, b3, io, segment, seg_steps, s);
   return DFX_E_ARG;
 }
 
