@@ -58,7 +58,7 @@ __device__ __forceinline__ void rows_out(const typename P::act_t* lds, int ld, i
                                          typename P::act_t* dst, int64_t row0) {
   constexpr int KA = P::KA;
   const int pairs = (MLP_TILE / KA) * Wd;
-  for (int e = threadIdx.x; e < pairs; e += MLP_THREADS) {
+  for (int e = threadIdx.x; e < pairs; e += blockDim.x) {
     const int rb = e / Wd, col = e - rb * Wd;
     typename P::act_t* o = dst + ((row0 / KA + rb) * Wd + col) * KA;
 #pragma unroll
@@ -74,16 +74,18 @@ __device__ __forceinline__ float row_val(const typename P::act_t* buf, int Wd, i
   return (float)buf[((row / KA) * Wd + col) * KA + row % KA];
 }
 
-template <class P>
-__global__ void __launch_bounds__(MLP_THREADS, 1)
+template <class P, int NW>
+__global__ void __launch_bounds__(64 * NW, 1)
 mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> R) {
   extern __shared__ __align__(16) unsigned char smem[];
   MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
   constexpr bool F = P::fast_math;
   using Tab = Tsit5Tab;
   constexpr int S = Tab::S;
+  using WV = MlpWaves<NW>;
+  constexpr int CPT = WV::CPT, GRP = MLP_D / CPT, NT = WV::NT12, NT3 = WV::NT3;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const int orow = tid >> 3, c0 = (tid & 7) * 8;
+  const int orow = tid / GRP, c0 = (tid % GRP) * CPT;
   const int tile = blockIdx.x;
   const int64_t traj = (int64_t)tile * MLP_TILE + orow;
   const bool valid = traj < a.B;
@@ -108,21 +110,21 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
 
   // ---- 1. recompute the segment: keep every stage's f (scratch) and its layer inputs (row
   //      buffers), so that the reverse sweep needs no forward re-evaluation ----
-  float y[8], fl[8];
+  float y[CPT], fl[CPT];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) y[j] = (steps_here > 0) ? a.ckpt[(traj * a.n_ckpt + a.seg) * MLP_D + c0 + j] : 0.0f;
+  for (int j = 0; j < CPT; ++j) y[j] = (steps_here > 0) ? a.ckpt[(traj * a.n_ckpt + a.seg) * MLP_D + c0 + j] : 0.0f;
   for (int q = 0; q < a.Cs; ++q) {
     const bool act = q < steps_here;
     const int n = n0 + q;
     const double ta = act ? a.grid[(traj * a.max_steps + n) * 2 + 0] : 0.0;
     const double tb = act ? a.grid[(traj * a.max_steps + n) * 2 + 1] : 0.0;
     const float control = (float)((double)dir * (tb - ta));
-    float k[S][8], z[8], f[8];
+    float k[S][CPT], z[CPT], f[CPT];
     if (q == 0) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) store_input<P>(L, orow, c0 + j, y[j]);
+      for (int j = 0; j < CPT; ++j) store_input<P>(L, orow, c0 + j, y[j]);
       __syncthreads();
-      mlp_forward<P>(W, L, wave, lane, orow, c0, fl);
+      mlp_forward<P, NW>(W, L, wave, lane, orow, c0, fl);
       stash_rows(0, 0);
       __syncthreads();
     }
@@ -131,12 +133,12 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       // (same input, z_6 = y_{n+1}) or the evaluation above wrote
       float* fp = fslot(q, 0);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) { fp[j] = fl[j]; k[0][j] = control * fl[j]; }
+      for (int j = 0; j < CPT; ++j) { fp[j] = fl[j]; k[0][j] = control * fl[j]; }
     }
     static_for<1, S>([&](auto ic) {
       constexpr int i = decltype(ic)::value;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         float kc[S];
 #pragma unroll
         for (int m = 0; m < S; ++m) kc[m] = (m < i) ? k[m][j] : 0.0f;
@@ -144,7 +146,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
         store_input<P>(L, orow, c0 + j, z[j]);
       }
       __syncthreads();
-      mlp_forward<P>(W, L, wave, lane, orow, c0, f);
+      mlp_forward<P, NW>(W, L, wave, lane, orow, c0, f);
       stash_rows(q, i);
       if constexpr (i == S - 1) {
         if (q + 1 < a.Cs) stash_rows(q + 1, 0);
@@ -152,28 +154,28 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       __syncthreads();
       float* fp = fslot(q, i);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) { fp[j] = f[j]; k[i][j] = control * f[j]; }
+      for (int j = 0; j < CPT; ++j) { fp[j] = f[j]; k[i][j] = control * f[j]; }
     });
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CPT; ++j) {
       if (act) y[j] = z[j];
       fl[j] = f[j];
     }
   }
 
   // ---- 2. reverse sweep ----
-  float yb[8];
+  float yb[CPT];
 #pragma unroll
-  for (int j = 0; j < 8; ++j) yb[j] = valid ? a.ybar[traj * MLP_D + c0 + j] : 0.0f;
+  for (int j = 0; j < CPT; ++j) yb[j] = valid ? a.ybar[traj * MLP_D + c0 + j] : 0.0f;
   for (int q = a.Cs - 1; q >= 0; --q) {
     const bool act = q < steps_here;
     const int n = n0 + q;
     const double ta = act ? a.grid[(traj * a.max_steps + n) * 2 + 0] : 0.0;
     const double tb = act ? a.grid[(traj * a.max_steps + n) * 2 + 1] : 0.0;
     const float control = (float)((double)dir * (tb - ta));
-    float kb[S][8], ybn[8];
+    float kb[S][CPT], ybn[CPT];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < CPT; ++j) {
       ybn[j] = yb[j];
 #pragma unroll
       for (int m = 0; m < S; ++m) kb[m][j] = 0.0f;
@@ -183,7 +185,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
 #pragma unroll
       for (int m = 0; m < S - 1; ++m) {
 #pragma unroll
-        for (int j = 0; j < 8; ++j) kb[m][j] += TL::v.a[S - 1][m] * yb[j];
+        for (int j = 0; j < CPT; ++j) kb[m][j] += TL::v.a[S - 1][m] * yb[j];
       }
       // SaveAt(ts) values interpolated inside this step
       int jlo = 0, jhi = 0;
@@ -198,7 +200,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
         tsit5_weights(th, bb);
         const float* g = a.ys_bar + (traj * a.n_out + (a.save_t0 ? 1 : 0) + s) * MLP_D + c0;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
+        for (int j = 0; j < CPT; ++j) {
           const float gv = g[j];
           ybn[j] += gv;
 #pragma unroll
@@ -213,7 +215,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       const float* fp = fslot(q, i);
       // delta3 = fbar * (1 - f^2), fbar = control * kbar_i
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         const float fb = act ? control * kb[i][j] : 0.0f;
         const float fv = fp[j];
         L.d3[orow * L.LDX + c0 + j] = P::to_act(fb * (1.0f - fv * fv));
@@ -221,11 +223,12 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       __syncthreads();
       rows_out<P>(L.d3, L.LDX, MLP_D, R.b[4], row0);
       // h2bar = d3 . W3 ; delta2 = h2bar * softplus'(.) (h2 from its row buffer)
-      floatx16_t acc[2];
-      wave_gemm<P, MLP_D / P::KSTEP, 2>(L.d3, L.d3, L.LDX, W.w3t, 2 * wave, 0, MLP_D / P::KSTEP, acc, lane);
+      floatx16_t acc[NT];
+      wave_gemm<P, MLP_D / P::KSTEP, NT>(L.d3, L.d3, L.LDX, W.w3t, NT * wave, 0, MLP_D / P::KSTEP,
+                                         acc, lane);
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int col = 64 * wave + 32 * t + (lane & 31);
+      for (int t = 0; t < NT; ++t) {
+        const int col = 32 * (NT * wave + t) + (lane & 31);
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const int r = acc_row(lane, v);
@@ -235,10 +238,11 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       }
       __syncthreads();
       // h1bar = d2 . W2 ; delta1 = h1bar * softplus'(.), kept in L.h2
-      wave_gemm<P, MLP_H / P::KSTEP, 2>(L.d, L.d, L.LDH, W.w2t, 2 * wave, 0, MLP_H / P::KSTEP, acc, lane);
+      wave_gemm<P, MLP_H / P::KSTEP, NT>(L.d, L.d, L.LDH, W.w2t, NT * wave, 0, MLP_H / P::KSTEP,
+                                         acc, lane);
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int col = 64 * wave + 32 * t + (lane & 31);
+      for (int t = 0; t < NT; ++t) {
+        const int col = 32 * (NT * wave + t) + (lane & 31);
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const int r = acc_row(lane, v);
@@ -251,17 +255,20 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       rows_out<P>(L.h2, L.LDH, MLP_H, R.b[0], row0);
       // zbar = d1 . W1 (split-K over the 256 hidden units)
       constexpr int KS1 = MLP_H / P::KSTEP;
-      wave_gemm<P, KS1, 2>(L.h2, L.h2, L.LDH, W.w1t, 0, wave * (KS1 / 4), (wave + 1) * (KS1 / 4), acc, lane);
+      const int qk = WV::quarter(wave), n3 = WV::nt3(wave);
+      floatx16_t acc3[NT3];
+      wave_gemm<P, KS1, NT3>(L.h2, L.h2, L.LDH, W.w1t, n3, qk * (KS1 / 4), (qk + 1) * (KS1 / 4),
+                             acc3, lane);
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int col = 32 * t + (lane & 31);
+      for (int t = 0; t < NT3; ++t) {
+        const int col = 32 * (n3 + t) + (lane & 31);
 #pragma unroll
-        for (int v = 0; v < 16; ++v) L.part[(wave * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc[t][v];
+        for (int v = 0; v < 16; ++v) L.part[(qk * MLP_TILE + acc_row(lane, v)) * L.LDP + col] = acc3[t][v];
       }
       __syncthreads();
-      float zb[8];
+      float zb[CPT];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         const int c = c0 + j;
         float s = L.part[(0 * MLP_TILE + orow) * L.LDP + c];
         s += L.part[(1 * MLP_TILE + orow) * L.LDP + c];
@@ -271,7 +278,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       }
       // z_i = y_n + sum_{m<i} a_im k_m
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
+      for (int j = 0; j < CPT; ++j) {
         ybn[j] += zb[j];
 #pragma unroll
         for (int m = 0; m < i; ++m) kb[m][j] += TL::v.a[i][m] * zb[j];
@@ -280,12 +287,12 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
     });
     if (act) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) yb[j] = ybn[j];
+      for (int j = 0; j < CPT; ++j) yb[j] = ybn[j];
     }
   }
   if (valid) {
 #pragma unroll
-    for (int j = 0; j < 8; ++j) a.ybar[traj * MLP_D + c0 + j] = yb[j];
+    for (int j = 0; j < CPT; ++j) a.ybar[traj * MLP_D + c0 + j] = yb[j];
   }
 }
 
@@ -1163,14 +1170,15 @@ int dfx_mlp_adj_segment(const dfx_mlp_problem* p, const void* packed, const floa
   if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  // the three-piece form at eight waves per tile: two waves per SIMD (at four it held 512
-  // registers per lane, one wave per SIMD; the backward of 8,192 trajectories took 2.51-2.55
-  // ms there, 2.16-2.17 ms at eight, with bit-identical gradients)
+  // eight waves per tile, two per SIMD: at four the kernels held 415 (float32) and 512 + 93
+  // spilled (three-piece) registers per lane, one wave per SIMD.  The backward of 8,192
+  // trajectories went from 3.11 to 2.86 ms (float32) and from 2.51-2.55 to 2.16-2.17 ms
+  // (three-piece), with bit-identical gradients.
   if (p->precision == DFX_BF16_F32)
     return run_segment<PolicyBF16x3>(&mlp_adj_segment3_kernel<8>, 64 * 8, p, packed, b1, b2, b3,
                                       io, segment, seg_steps, s);
   if (p->precision == DFX_F32)
-    return run_segment<PolicyF32>(&mlp_adj_segment_kernel<PolicyF32>, MLP_THREADS, p, packed, b1,
+    return run_segment<PolicyF32>(&mlp_adj_segment_kernel<PolicyF32, 8>, 64 * 8, p, packed, b1,
                                   b2, b3, io, segment, seg_steps, s);
   return DFX_E_ARG;
 }
