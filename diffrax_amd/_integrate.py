@@ -133,7 +133,7 @@ class ErkPlan:
         for name in ("y0", "t1", "ts", "params", "ys", "ts_out", "stats", "result", "grid",
                      "dense_ts", "dense_ys", "dense_k", "step_ts", "jump_ts",
                      "event_coeffs", "event_flags", "event_mask", "grid_y", "rtol", "atol",
-                     "ckpt"):
+                     "ckpt", "state_in", "state_out"):
             v = self.t.get(name)
             setattr(self.io, name, v.data_ptr() if v is not None else None)
 
@@ -246,8 +246,81 @@ def _controller_fields(stepsize_controller, dt0f):
     raise NotImplementedError(f"stepsize controller {stepsize_controller!r}")
 
 
+def _state_record(solver, stepsize_controller, B, d, batched, solver_state, controller_state,
+                  made_jump):
+    """The resume mask and the [B, d + 9] float64 state records of dfx_erk_io.state_in
+    (the record layout of DFX_ERK_STATE_WORDS) from diffeqsolve's solver_state,
+    controller_state and made_jump (_integrate.py:906-908, 1252-1270, 1309); each value is
+    a scalar shared by the batch or has the batch as its leading dimension."""
+    rec = torch.zeros((B, d + 9), dtype=torch.float64)
+    mask = 0
+
+    def col(v, name):
+        t = torch.as_tensor(v).detach().to("cpu", torch.float64).reshape(-1)
+        if t.numel() not in (1, B):
+            raise ValueError(f"{name} must be a scalar or have shape [{B}]")
+        return t.expand(B)
+
+    if solver_state is not None and solver.tableau.fsal:
+        if not (isinstance(solver_state, (tuple, list)) and len(solver_state) == 2):
+            raise ValueError("solver_state must be the (first_step, f0) pair of a previous "
+                             "solve's SaveAt(solver_state=True)")
+        first, f0 = solver_state
+        f0 = torch.as_tensor(f0).detach().to("cpu", torch.float64)
+        if f0.shape not in ((B, d), (d,)) or (f0.ndim == 1 and batched and B > 1):
+            raise ValueError(f"solver_state f0 must have shape [{B}, {d}]" if batched
+                             else f"solver_state f0 must have shape [{d}]")
+        rec[:, :d] = f0.reshape(-1, d).expand(B, d)
+        rec[:, d] = col(first, "solver_state first_step")
+        mask |= _lib.RESUME_SOLVER
+    if made_jump is not None:
+        rec[:, d + 1] = col(made_jump, "made_jump")
+        mask |= _lib.RESUME_MADE_JUMP
+    if controller_state is not None:
+        cs = tuple(controller_state)
+        if isinstance(stepsize_controller, PIDController):
+            if len(cs) != 3:
+                raise ValueError("a PIDController state is (prev_inv_scaled_error, "
+                                 "prev_prev_inv_scaled_error, at_dtmin)")
+            for i, name in enumerate(("prev_inv", "prev_prev_inv", "at_dtmin")):
+                rec[:, d + 2 + i] = col(cs[i], name)
+        else:
+            if len(cs) != 4:
+                raise ValueError("a ConstantStepSize state is (steps_completed, num_steps, "
+                                 "t0_sim, t1_sim_or_dt0)")
+            for i, name in enumerate(("steps_completed", "num_steps", "t0_sim", "t1_sim")):
+                rec[:, d + 5 + i] = col(cs[i], name)
+        mask |= _lib.RESUME_CONTROLLER
+    return mask, rec
+
+
+def _state_outputs(plan, saveat):
+    """Solution.solver_state / controller_state / made_jump from the final state records
+    (_integrate.py:1489-1500), as device tensors with the batch as leading dimension
+    (dropped for an unbatched solve).  The PID state is float64 in both precisions."""
+    rec = plan.t.get("state_out")
+    if rec is None:
+        return None, None, None
+    d = plan.problem.d
+    pick = (lambda v: v) if plan.batched else (lambda v: v[0])
+    solver_state = controller_state = made_jump = None
+    if saveat.solver_state and plan.fsal:
+        solver_state = (pick(rec[:, d] != 0), pick(rec[:, :d].to(plan.t["y0"].dtype)))
+    if saveat.controller_state:
+        if plan.problem.controller == _lib.PID:
+            controller_state = (pick(rec[:, d + 2]), pick(rec[:, d + 3]), pick(rec[:, d + 4] != 0))
+        else:
+            controller_state = (pick(rec[:, d + 5].to(torch.int32)),
+                                pick(rec[:, d + 6].to(torch.int32)), pick(rec[:, d + 7]),
+                                pick(rec[:, d + 8]))
+    if saveat.made_jump:
+        made_jump = pick(rec[:, d + 1] != 0)
+    return solver_state, controller_state, made_jump
+
+
 def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
-                 max_steps, launch, record_grid=False, event=None):
+                 max_steps, launch, record_grid=False, event=None, solver_state=None,
+                 controller_state=None, made_jump=None):
     # Host-side checks first (they need no GPU); device placement at the end.
     if not isinstance(terms, ODETerm) or not isinstance(terms.vector_field, RegisteredVectorField):
         raise NotImplementedError(
@@ -296,13 +369,16 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
     else:
         dt0f = None
 
-    if saveat.solver_state or saveat.controller_state or saveat.made_jump:
-        raise NotImplementedError("saving solver/controller state is not provided by diffrax_amd")
+    want_state = saveat.solver_state or saveat.controller_state or saveat.made_jump
+    resuming = solver_state is not None or controller_state is not None or made_jump is not None
+    if (want_state or resuming) and isinstance(stepsize_controller, ClipStepSizeController):
+        raise NotImplementedError("solver/controller state export and resume are provided "
+                                  "for PIDController and ConstantStepSize")
     sub = saveat.subs
     if sub is None:
-        if not saveat.dense:
+        if not saveat.dense and not want_state:
             raise ValueError("SaveAt must save something")
-        sub = SubSaveAt.__new__(SubSaveAt)   # dense output only: no saved rows
+        sub = SubSaveAt.__new__(SubSaveAt)   # dense output / state only: no saved rows
         sub.t0 = sub.t1 = False
         sub.ts, sub.steps, sub.fn = None, 0, save_y
     if not hasattr(sub, "fn"):
@@ -352,6 +428,9 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
             raise NotImplementedError("store_rejected_steps is not provided by diffrax_amd")
         stepsize_controller = clip.controller
     ctrl = _controller_fields(stepsize_controller, dt0f)
+    if resuming:
+        resume_mask, state_rec = _state_record(solver, stepsize_controller, B, d, batched,
+                                               solver_state, controller_state, made_jump)
 
     device = _require_gpu()
     y0b = y0b.to(device).contiguous()
@@ -400,6 +479,11 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
               if record_grid else None),
         step_ts=clip_t.get("step_ts"), jump_ts=clip_t.get("jump_ts"), **ev, **tol_t,
     )
+    if resuming:
+        prob.resume = resume_mask
+        tensors["state_in"] = state_rec.to(device)
+    if want_state:
+        tensors["state_out"] = torch.empty((B, d + 9), dtype=torch.float64, device=device)
     if saveat.dense:
         S = solver.tableau.num_stages
         tensors.update(
@@ -407,12 +491,13 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
             dense_ys=torch.empty((B, int(max_steps) + 1, d), dtype=y0b.dtype, device=device),
             dense_k=torch.empty((B, int(max_steps), S, d), dtype=y0b.dtype, device=device))
     plan = ErkPlan(prob, tensors, n_out, batched, launch)
+    plan.fsal = bool(solver.tableau.fsal)
     # checks of device-resident times, run by diffeqsolve once the solve is queued
     plan.deferred_checks = check_times if host.deferred else None
     return plan
 
 
-def _finish(plan, t0, t1, max_steps, throw, ys=None):
+def _finish(plan, t0, t1, max_steps, throw, ys=None, saveat=None):
     if plan.deferred_checks is not None:
         plan.deferred_checks()
     T = plan.t
@@ -446,8 +531,11 @@ def _finish(plan, t0, t1, max_steps, throw, ys=None):
         ev = getattr(plan, "event", None)
         event_mask = [m[..., i] for i in range(m.shape[-1])] if ev is not None and ev.is_list \
             else m[..., 0]
+    solver_state, controller_state, made_jump = (
+        _state_outputs(plan, saveat) if saveat is not None else (None, None, None))
     sol = Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=res,
-                   interpolation=interp, event_mask=event_mask)
+                   interpolation=interp, event_mask=event_mask, solver_state=solver_state,
+                   controller_state=controller_state, made_jump=made_jump)
     return sol
 
 
@@ -534,8 +622,12 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
         leaves, spec = torch.utils._pytree.tree_flatten(subs)
         if not leaves or not all(isinstance(x, SubSaveAt) for x in leaves):
             raise ValueError("SaveAt(subs=...) must be a SubSaveAt or a container of them")
+        first_only = lambda i: dict(dense=saveat.dense and i == 0,
+                                    solver_state=saveat.solver_state and i == 0,
+                                    controller_state=saveat.controller_state and i == 0,
+                                    made_jump=saveat.made_jump and i == 0)
         sols = [diffeqsolve(terms, solver, t0, t1, dt0, y0, args,
-                            saveat=SaveAt(subs=leaf, dense=saveat.dense and i == 0), **kw)
+                            saveat=SaveAt(subs=leaf, **first_only(i)), **kw)
                 for i, leaf in enumerate(leaves)]
         first = sols[0]
         first.ts = torch.utils._pytree.tree_unflatten([x.ts for x in sols], spec)
@@ -585,8 +677,12 @@ def _apply_save_fn(fn, ts, ys, args, batched):
 def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_controller,
                      adjoint, event, max_steps, throw, progress_meter, solver_state,
                      controller_state, made_jump, launch):
-    if solver_state is not None or controller_state is not None or made_jump is not None:
-        raise NotImplementedError("resuming from solver/controller state is not provided")
+    resuming = solver_state is not None or controller_state is not None or made_jump is not None
+    if (resuming or saveat.solver_state or saveat.controller_state or saveat.made_jump) and (
+            _is_ou_sde(terms) or (isinstance(terms, ODETerm) and
+                                  isinstance(terms.vector_field, MLPField))):
+        raise NotImplementedError("solver/controller state export and resume are provided "
+                                  "for the small-system kernel only")
     if progress_meter is not None and type(progress_meter).__name__ != "NoProgressMeter":
         raise NotImplementedError("progress meters are not provided by diffrax_amd")
     if event is not None and (_is_ou_sde(terms) or (isinstance(terms, ODETerm) and
@@ -612,14 +708,18 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
                                     stepsize_controller, adjoint, max_steps, throw,
                                     _require_gpu())
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
-                        max_steps, launch, event=event)
+                        max_steps, launch, event=event, solver_state=solver_state,
+                        controller_state=controller_state, made_jump=made_jump)
     plan.event = event
     T = plan.t
     wants_grad = torch.is_grad_enabled() and (T["y0"].requires_grad or
                                               T["params"].requires_grad)
     if not wants_grad:
         plan.launch()
-        return _finish(plan, t0, t1, max_steps, throw)
+        return _finish(plan, t0, t1, max_steps, throw, saveat=saveat)
+    if resuming or T.get("state_out") is not None:
+        raise NotImplementedError("gradients through a solve that resumes or saves "
+                                  "solver/controller state are not provided")
     if isinstance(adjoint, BacksolveAdjoint):
         back = _backsolve_problem(plan.problem, adjoint, stepsize_controller)
         ys = _ErkBacksolve.apply(plan, back, bool(adjoint.kwargs.get("throw", throw)),

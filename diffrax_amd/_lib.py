@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libdiffrax_amd.so")
 # enums (include/diffrax_amd.h)
 TSIT5, DOPRI5, HEUN = 0, 1, 2
 PID, CONSTANT = 0, 1
+RESUME_SOLVER, RESUME_CONTROLLER, RESUME_MADE_JUMP = 1, 2, 4
 VF_LV, VF_VDP, VF_LINEAR, VF_MLP, VF_OU = 0, 1, 2, 3, 4
 F64, F32S_F64T, F32, BF16_F32 = 0, 1, 2, 3
 SUCCESSFUL, MAX_STEPS_REACHED, DT_MIN_REACHED = 0, 1, 2
@@ -38,7 +39,7 @@ class ErkProblem(ctypes.Structure):
         ("safety", ctypes.c_double), ("error_order", ctypes.c_double),
         ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32),
         ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
-        ("ckpt_every", ctypes.c_int32), ("reserved_", ctypes.c_int32),
+        ("ckpt_every", ctypes.c_int32), ("resume", ctypes.c_int32),
     ]
 
 
@@ -52,6 +53,7 @@ class ErkIO(ctypes.Structure):
         ("event_coeffs", ctypes.c_void_p), ("event_flags", ctypes.c_void_p),
         ("event_mask", ctypes.c_void_p), ("grid_y", ctypes.c_void_p),
         ("rtol", ctypes.c_void_p), ("atol", ctypes.c_void_p), ("ckpt", ctypes.c_void_p),
+        ("state_in", ctypes.c_void_p), ("state_out", ctypes.c_void_p),
     ]
 
 

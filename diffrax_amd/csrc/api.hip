@@ -75,6 +75,10 @@ int fill_erk_inputs(const dfx_erk_problem* p, const dfx_erk_io* io, ErkKArgs& a)
   if (p->n_step_ts < 0 || p->n_jump_ts < 0) return DFX_E_ARG;
   if ((p->n_step_ts > 0 || p->n_jump_ts > 0) && p->controller != DFX_PID) return DFX_E_ARG;
   if (p->n_events < 0 || p->n_events > kMaxEvents) return DFX_E_ARG;
+  if ((p->resume & ~(DFX_RESUME_SOLVER | DFX_RESUME_CONTROLLER | DFX_RESUME_MADE_JUMP)) != 0)
+    return DFX_E_ARG;
+  if ((p->resume & DFX_RESUME_CONTROLLER) && (p->n_step_ts > 0 || p->n_jump_ts > 0))
+    return DFX_E_ARG;
   memset(&a, 0, sizeof(a));
   if (p->batch == 0) return DFX_OK;
   if (!io->t1 || !io->params) return DFX_E_ARG;
@@ -234,6 +238,12 @@ int dfx_erk_solve(const dfx_erk_problem* p, const dfx_erk_io* io, void* workspac
     a.n_ckpt = ckpt_count(p);
   }
   a.counter = reinterpret_cast<unsigned int*>(workspace);
+  if (p->resume != 0) {
+    if (!io->state_in || p->ckpt_every > 0) return DFX_E_ARG;
+    a.state_in = io->state_in;
+    a.resume = p->resume;
+  }
+  a.state_out = io->state_out;
 
   int block = 64;
   int64_t lanes = p->batch;
@@ -258,6 +268,7 @@ int dfx_erk_backsolve(const dfx_erk_problem* p, const dfx_erk_backsolve_io* io, 
   if (p->controller != DFX_PID && p->controller != DFX_CONSTANT) return DFX_E_ARG;
   if (p->controller == DFX_CONSTANT && isnan(p->dt0)) return DFX_E_ARG;
   if (p->save_steps || p->n_step_ts > 0 || p->n_jump_ts > 0 || p->n_events > 0) return DFX_E_ARG;
+  if (p->resume != 0) return DFX_E_ARG;
   if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0 || p->param_stride < 0) return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
   if (!io->params || !io->ys_bar || !io->y0_bar || !io->params_bar || !io->result) return DFX_E_ARG;
@@ -300,7 +311,7 @@ int dfx_erk_adjoint(const dfx_erk_problem* p, const dfx_erk_io* fwd,
   const bool vf_ok = (p->vf == DFX_VF_LV || p->vf == DFX_VF_VDP) ? p->d == 2
                      : (p->vf == DFX_VF_LINEAR && p->d >= 1 && p->d <= 4);
   if (!vf_ok) return DFX_E_ARG;
-  if (p->n_events > 0 || p->ckpt_every <= 0) return DFX_E_ARG;
+  if (p->n_events > 0 || p->ckpt_every <= 0 || p->resume != 0) return DFX_E_ARG;
   ErkKArgs a;
   const int rc = fill_erk_inputs(p, fwd, a);
   if (rc != DFX_OK) return rc;

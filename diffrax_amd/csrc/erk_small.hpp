@@ -40,10 +40,12 @@ struct ErkKArgs {
   const double* event_coeffs;  // Event: [n_events, D + 2] affine conditions (c_t, c_y, c_0)
   const int32_t* event_flags;  // bit 0 boolean, bits 1-2 direction (0 any, 1 up, 2 down)
   int32_t* event_mask;         // [B, n_events] out
+  const double* state_in;      // [B, D + 9] solver / controller state to resume from
+  double* state_out;           // [B, D + 9] out: the final state (same layout)
   unsigned int* counter;
   int64_t B, t1_stride, ts_stride, param_stride;
   int64_t grid_stride;  // records per instance in grid / grid_y
-  int32_t ckpt_every, n_ckpt;
+  int32_t ckpt_every, n_ckpt, resume;  // resume: DFX_RESUME_* parts of state_in taken
   int32_t n_step_ts, n_jump_ts, n_events, event_root_find;
   int32_t n_ts, n_out, save_t0, save_t1, save_steps, max_steps, has_dt0, controller;
   double t0, dt0, error_order;
@@ -625,6 +627,60 @@ struct ErkLane {
     return tl;
   }
 
+  // ---- solver / controller state of a resumed solve ------------------------------
+  // Record layout (D + 9 float64 words): f0[D], first_step, made_jump, the PID state
+  // (prev_inv_scaled_error, prev_prev_inv_scaled_error, at_dtmin), the ConstantStepSize
+  // state (steps_completed, num_steps, t0_sim, t1_sim_or_dt0).  Resuming replaces what
+  // begin_times set (_integrate.py:1252-1270): with the controller state and dt0 given the
+  // first step ends at t0 + dt0, without the PID init's dtmin / dtmax clamps; with the
+  // solver state the first step takes f0 as its first stage unless first_step or
+  // made_jump asks for a fresh evaluation (runge_kutta.py:684-690).
+  static constexpr int SW = D + 9;
+  __device__ void resume_state(const ErkKArgs& a, double T0) {
+    const double* r = a.state_in + traj * SW;
+    if (a.resume & DFX_RESUME_SOLVER) {
+#pragma unroll
+      for (int j = 0; j < D; ++j) fsal_f[j] = Tab::fsal ? T(r[j]) : T(0);
+      first = Tab::fsal && r[D] != 0.0;
+    }
+    if (a.resume & DFX_RESUME_MADE_JUMP) made_jump = r[D + 1] != 0.0;
+    if (a.resume & DFX_RESUME_CONTROLLER) {
+      if (a.controller == DFX_PID) {
+        pid.inv = r[D + 2];
+        pid.pinv = r[D + 3];
+        pid.at_dtmin = r[D + 4] != 0.0;
+        if (a.has_dt0) tnext = jmin(T0 + a.dt0 * dir, T1);
+      } else {
+        cs_completed = (int32_t)r[D + 5];
+        cs_num = (int32_t)r[D + 6];
+        cs_t0 = r[D + 7];
+        cs_t1 = r[D + 8];
+      }
+    }
+  }
+  // The final state (SaveAt(solver_state=True, controller_state=True, made_jump=True),
+  // _integrate.py:1489-1500).  Before any accepted step the solver state is still the
+  // initial one: f0 as resumed, or zero (runge_kutta.py:424-442), with first_step set.
+  __device__ void write_state(const ErkKArgs& a) const {
+    double* r = a.state_out + traj * SW;
+    const bool resumed_f0 = (a.resume & DFX_RESUME_SOLVER) != 0;
+#pragma unroll
+    for (int j = 0; j < D; ++j) {
+      double v = 0.0;
+      if (Tab::fsal) v = !first ? (double)fsal_f[j] : (resumed_f0 ? a.state_in[traj * SW + j] : 0.0);
+      r[j] = v;
+    }
+    r[D] = (Tab::fsal && first) ? 1.0 : 0.0;
+    r[D + 1] = made_jump ? 1.0 : 0.0;
+    r[D + 2] = pid.inv;
+    r[D + 3] = pid.pinv;
+    r[D + 4] = pid.at_dtmin ? 1.0 : 0.0;
+    r[D + 5] = (double)cs_completed;
+    r[D + 6] = (double)cs_num;
+    r[D + 7] = cs_t0;
+    r[D + 8] = cs_t1;
+  }
+
   __device__ void start(const ErkKArgs& a, int64_t id) {
     tr.load();
     traj = id;
@@ -633,6 +689,7 @@ struct ErkLane {
     for (int j = 0; j < D; ++j) y[j] = y0p[j];
     load_params(a, id);
     const double T0 = begin_times(a, a.t0, a.t1[id * a.t1_stride]);
+    if (Extras && a.resume) resume_state(a, T0);
     if (Ck && a.ckpt) write_ckpt(a, 0, -__builtin_inf());
     save_index = 0;
     ts_index = 0;
@@ -759,9 +816,12 @@ struct ErkLane {
       ntprev = ts1;
       ntnext = tn;
     }
+    // made_jump of the step: only a jump time sets it (PIDController and ConstantStepSize
+    // return False, pid.py:566, constant.py:98-104); an accepted step takes it
+    // (_integrate.py:425)
+    bool mj = false;
     if (Extras && a.controller == DFX_PID && (a.n_step_ts > 0 || a.n_jump_ts > 0)) {
       // ClipStepSizeController.adapt_step_size (clip.py:301-428)
-      bool mj = false;
       if (a.n_step_ts > 0) {
         const ClipTimes c{a.step_ts, a.n_step_ts, dir};
         bool unused;
@@ -775,8 +835,8 @@ struct ErkLane {
         jump_index = c.find(ntprev);
         ntnext = c.clip(ntnext, jump_index, true);
       }
-      if (keep) made_jump = mj;  // _integrate.py:425
     }
+    if (Extras && keep) made_jump = mj;
     ntprev = jmin(ntprev, T1);
     if (ntnext > floor_t1) ntnext = keep ? T1 : ntprev + 0.5 * (T1 - ntprev);
 
@@ -917,6 +977,7 @@ struct ErkLane {
     a.stats[traj * 3 + 1] = num_acc;
     a.stats[traj * 3 + 2] = num_rej;
     a.result[traj] = result;
+    if (Extras && a.state_out) write_state(a);
   }
 };
 

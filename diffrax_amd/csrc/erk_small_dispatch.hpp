@@ -26,11 +26,13 @@ static void launch_small_g(const ErkKArgs& a, int block, int64_t lanes, hipStrea
 
 // Variants: the controller kind (the default integral controller with exponent 0.2 on
 // PID_FIFTH, every other PID on PID_GENERIC; ConstantStepSize takes PID_FIFTH, whose
-// controller code it never runs) and dense output / step clipping / events (X).
+// controller code it never runs) and dense output / step clipping / events / resumed or
+// exported solver state (X).
 template <class Tab, class VF, class T>
 static int launch_small(const ErkKArgs& a, int block, int64_t lanes, hipStream_t s) {
   const bool g = a.controller == DFX_PID && !pid_is_fifth(a.pid);
-  const bool x = a.dense_ts != nullptr || a.n_step_ts > 0 || a.n_jump_ts > 0 || a.n_events > 0;
+  const bool x = a.dense_ts != nullptr || a.n_step_ts > 0 || a.n_jump_ts > 0 || a.n_events > 0 ||
+                 a.resume != 0 || a.state_out != nullptr;
   if (g) {
     if (x) launch_small_g<Tab, VF, T, PID_GENERIC, true>(a, block, lanes, s);
     else launch_small_g<Tab, VF, T, PID_GENERIC, false>(a, block, lanes, s);

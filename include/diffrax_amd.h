@@ -78,8 +78,32 @@ typedef struct dfx_erk_problem {
    * dfx_erk_adjoint); 0 stores none.  The Python package uses
    * K = ceil(max_steps / checkpoints), checkpoints = ceil(sqrt(max_steps)) by default. */
   int32_t ckpt_every;
-  int32_t reserved_;    /* 0 */
+  /* Resuming a solve: diffeqsolve(solver_state=, controller_state=, made_jump=)
+   * (_integrate.py:906-908, 1252-1270, 1309).  A mask of DFX_RESUME_* naming which parts
+   * of dfx_erk_io.state_in replace the fresh start; 0 starts afresh.  With
+   * DFX_RESUME_CONTROLLER and dt0 given, the first step ends at t0 + dt0 unclamped (the
+   * controller's init is skipped); with dt0 NaN the initial step is selected as usual.
+   * Not with step_ts / jump_ts clipping, checkpoints or the adjoints. */
+  int32_t resume;
 } dfx_erk_problem;
+
+enum { DFX_RESUME_SOLVER = 1, DFX_RESUME_CONTROLLER = 2, DFX_RESUME_MADE_JUMP = 4 };
+
+/* The solver / controller state record of one instance (dfx_erk_io.state_in / state_out),
+ * DFX_ERK_STATE_WORDS(d) float64 words, times direction-normalised:
+ *   [0, d)  solver_state f0: the FSAL stage f(t, y) (runge_kutta.py:415-442, 1199-1200;
+ *           zero for Heun, which keeps no solver state)
+ *   d       solver_state first_step (1 or 0)
+ *   d + 1   made_jump (1 or 0)
+ *   d + 2   PIDController state: prev_inv_scaled_error (pid.py:84-85)
+ *   d + 3   prev_prev_inv_scaled_error
+ *   d + 4   at_dtmin (1 or 0)
+ *   d + 5   ConstantStepSize state: steps_completed (constant.py:14-17, 47-53)
+ *   d + 6   num_steps (-1: infinite t1)
+ *   d + 7   t0_sim
+ *   d + 8   t1_sim_or_dt0
+ * Both controllers' words are written; a resume reads those of the problem's controller. */
+#define DFX_ERK_STATE_WORDS(d) ((d) + 9)
 
 /* ---- device buffers of one solve --------------------------------------------------- */
 typedef struct dfx_erk_io {
@@ -113,6 +137,12 @@ typedef struct dfx_erk_io {
   const double* atol;   /* optional [B]: per-instance atol; NULL uses the problem's     */
   double* ckpt;         /* out when ckpt_every > 0: [B, n_ckpt, words] (dfx_erk_ckpt_layout)
                            lane state after 0, K, 2K, ... accepted steps; NULL otherwise */
+  const double* state_in; /* [B, DFX_ERK_STATE_WORDS(d)] when problem.resume != 0: the
+                             state to resume from; NULL otherwise                      */
+  double* state_out;      /* optional out [B, DFX_ERK_STATE_WORDS(d)]: the final solver /
+                             controller state and made_jump (SaveAt(solver_state=True,
+                             controller_state=True, made_jump=True), _integrate.py:
+                             1489-1500); NULL to skip                                   */
 } dfx_erk_io;
 
 /* Launch shape (NULL: block 64, one lane per instance).  Fewer lanes than instances

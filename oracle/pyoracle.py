@@ -285,12 +285,22 @@ def select_initial_step(f, t0, y0, direction, rtol, atol, error_order):
 def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller="pid",
           save_t0=False, save_t1=False, max_steps=4096, pcoeff=0.0, icoeff=1.0, dcoeff=0.0,
           dtmin=None, dtmax=None, force_dtmin=True, factormin=0.2, factormax=10.0,
-          safety=0.9, g_dw=None, norm_n=None, fused=True):
+          safety=0.9, g_dw=None, norm_n=None, fused=True, solver_state=None,
+          controller_state=None, made_jump=None):
     """FP64 numpy solve of one trajectory.  Returns dict(ys, ts, stats, result, grid,
-    direction).  g_dw(n, t, y) adds a diffusion increment for SDEs.  norm_n: the error
-    norm is the RMS over the first norm_n components only (adjoint_rms_seminorm,
-    _misc.py:85-116, on a state (y, a_y, a_params) with norm_n = 2 d).  fused=False sums
-    the stages with plain arithmetic (_lincomb_plain)."""
+    direction, tprev, tnext, solver_state, controller_state, made_jump).  g_dw(n, t, y)
+    adds a diffusion increment for SDEs.  norm_n: the error norm is the RMS over the first
+    norm_n components only (adjoint_rms_seminorm, _misc.py:85-116, on a state
+    (y, a_y, a_params) with norm_n = 2 d).  fused=False sums the stages with plain
+    arithmetic (_lincomb_plain).
+
+    Resuming (_integrate.py:1252-1270, 1309): solver_state = (first_step, f0) of an FSAL
+    solver (runge_kutta.py:415-442; None for Heun), controller_state = (prev_inv,
+    prev_prev_inv, at_dtmin) for the PID controller (pid.py:84-85) or (steps_completed,
+    num_steps, t0_sim, t1_sim) for ConstantStepSize (constant.py:14-17), in normalised
+    time; with a controller state and dt0 given the first step ends at t0 + dt0 without
+    the PID init's clamps.  The returned states are the final ones (_integrate.py:
+    1489-1500)."""
     lincomb = _lincomb if fused else _lincomb_plain
     tab = TABLEAUX[solver]
     y = np.array(y0, dtype=np.float64)
@@ -320,22 +330,37 @@ def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller
         tnext = T0 + dt0v
         pinv = ppinv = 1.0
         at_dtmin = False
+        if controller_state is not None:
+            pinv, ppinv, at_dtmin = (float(controller_state[0]), float(controller_state[1]),
+                                     bool(controller_state[2]))
+            if dt0 is not None:
+                tnext = T0 + dt0 * direction
     else:
         DT0 = dt0 * direction
         num = int(math.ceil((T1 - T0) / np.nextafter(DT0, math.inf)))
         completed = 1
+        c_t0, c_t1 = T0, T1
         tnext = T0 + DT0
+        if controller_state is not None:
+            completed, num = int(controller_state[0]), int(controller_state[1])
+            c_t0, c_t1 = float(controller_state[2]), float(controller_state[3])
     tnext = _jmin(tnext, T1)
+    # solver state: (first_step, f0); the first stage is evaluated while first_step or
+    # made_jump holds (runge_kutta.py:684-690)
+    first, f0 = True, np.zeros_like(y)
+    if solver_state is not None and tab.fsal:
+        first, f0 = bool(solver_state[0]), np.array(solver_state[1], dtype=np.float64)
+    jumped = bool(made_jump) if made_jump is not None else False
     floor = T1
     for _ in range(100):
         floor = float(np.nextafter(floor, -math.inf))
     tprev = T0
-    fsal = None
     n_steps = n_acc = n_rej = 0
     result = 0
     grid = []
     ts_index = 0
     while tprev < T1 and result == 0 and n_steps < max_steps:
+        fsal = None if (first or jumped or not tab.fsal) else f0
         y1, yerr, ks, flast = rk_step(tab, prod, n_steps, tprev, tnext, direction, y, fsal,
                                       lincomb)
         yerr = np.where(np.isnan(yerr), np.inf, yerr)
@@ -376,9 +401,11 @@ def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller
         else:
             keep = True
             completed += 1
-            tn = T0 + (T1 - T0) * (completed / num)
+            tn = c_t0 + (c_t1 - c_t0) * (completed / num)
             if completed == num:
-                tn = T1
+                tn = c_t1
+            if not num >= 0:
+                tn = tnext + c_t1
             ntprev, ntnext = tnext, tn
         ntprev = _jmin(ntprev, T1)
         if ntnext > floor:
@@ -398,7 +425,8 @@ def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller
                 ts_index += 1
             y = y1
             if tab.fsal:
-                fsal = flast
+                first, f0 = False, flast
+            jumped = False
         tprev, tnext = ntprev, ntnext
     if T0 == T1:
         for _ in ts_n:
@@ -415,8 +443,14 @@ def solve(f, solver, t0, t1, dt0, y0, ts=(), *, rtol=1e-6, atol=1e-8, controller
     for i, (t, v) in enumerate(zip(outs_t, outs_y)):
         tso[i] = t * direction
         ys[i] = v
+    if controller == "pid":
+        cstate = (pinv, ppinv, bool(at_dtmin))
+    else:
+        cstate = (completed, num, c_t0, c_t1)
     return dict(ys=ys, ts=tso, stats=(n_steps, n_acc, n_rej), result=result, grid=grid,
-                direction=direction)
+                direction=direction, tprev=tprev, tnext=tnext,
+                solver_state=(first, f0.copy()) if tab.fsal else None,
+                controller_state=cstate, made_jump=jumped)
 
 
 def backsolve(f_aug, d, n_q, solver, t0, dt0, ts, ys, ys_bar, *, save_t0, rtol, atol,

@@ -77,6 +77,57 @@ def test_solve_rejects_bad_arguments_without_launching():
     assert lib.dfx_strerror(-3).decode().startswith("workspace")
 
 
+def test_resume_rejects_bad_arguments_without_launching():
+    """dfx_erk_problem.resume: unknown bits, a controller state with step clipping, a
+    missing state_in, checkpoints, and the adjoint entries are rejected before any launch."""
+    lib = _lib.load()
+    z = ctypes.c_void_p(8)    # never dereferenced: every case fails validation first
+    ws = ctypes.c_void_p(256)
+
+    def prob(**kw):
+        base = dict(solver=_lib.TSIT5, controller=_lib.PID, vf=_lib.VF_LV, precision=_lib.F64,
+                    d=2, batch=4, max_steps=10, save_t1=1, dt0=0.1, rtol=1e-6, atol=1e-8,
+                    pcoeff=0.0, icoeff=1.0, dcoeff=0.0, dtmin=float("nan"),
+                    dtmax=float("nan"), error_order=float("nan"), factormin=0.2,
+                    factormax=10.0, safety=0.9)
+        base.update(kw)
+        return _lib.ErkProblem(**base)
+
+    io = _lib.ErkIO(y0=z, t1=z, params=z, ys=z, ts_out=z, stats=z, result=z, step_ts=z,
+                    state_in=z, ckpt=z)
+    call = lambda p, i=io: lib.dfx_erk_solve(ctypes.byref(p), ctypes.byref(i), ws, 256, None,
+                                             None)
+    assert call(prob(resume=8)) == -1
+    assert call(prob(resume=_lib.RESUME_CONTROLLER, n_step_ts=1)) == -1
+    assert call(prob(resume=_lib.RESUME_SOLVER, ckpt_every=4)) == -1
+    no_state = _lib.ErkIO(y0=z, t1=z, params=z, ys=z, ts_out=z, stats=z, result=z)
+    assert call(prob(resume=_lib.RESUME_MADE_JUMP), no_state) == -1
+    bio = _lib.BacksolveIO(params=z, ys=z, ts_out=z, ys_bar=z, y0_bar=z, params_bar=z, result=z)
+    assert lib.dfx_erk_backsolve(ctypes.byref(prob(resume=1)), ctypes.byref(bio), None) == -1
+    aio = _lib.AdjointIO(ckpt=z, stats=z, ys_bar=z, y0_bar=z, params_bar=z)
+    assert lib.dfx_erk_adjoint(ctypes.byref(prob(resume=1, ckpt_every=4)), ctypes.byref(io),
+                               ctypes.byref(aio), ws, 1 << 20, None) == -1
+
+
+def test_resume_host_validation():
+    """Malformed states and unsupported combinations fail on the host (no GPU needed)."""
+    with pytest.raises(NotImplementedError, match="PIDController and ConstantStepSize"):
+        _solve(stepsize_controller=dfx.ClipStepSizeController(
+            dfx.PIDController(rtol=1e-6, atol=1e-8), step_ts=[0.5]),
+            saveat=dfx.SaveAt(t1=True, controller_state=True))
+    with pytest.raises(NotImplementedError, match="small-system kernel only"):
+        dfx.diffeqsolve(dfx.MultiTerm(dfx.ODETerm(dfx.OUDrift()), dfx.ControlTerm(
+            dfx.DiagonalNoise(), dfx.IncrementPath(torch.zeros(1, 4, 2)))), dfx.Heun(), 0.0,
+            1.0, 0.25, torch.zeros(1, 2), (torch.ones(2), torch.zeros(2), torch.ones(2)),
+            saveat=dfx.SaveAt(t1=True), made_jump=True)
+    with pytest.raises(ValueError, match="first_step, f0"):
+        _solve(solver_state=torch.zeros(2))
+    with pytest.raises(ValueError, match="prev_inv"):
+        _solve(controller_state=(1.0, 1.0))
+    with pytest.raises(ValueError, match="steps_completed"):
+        _solve(controller_state=(1.0, 1.0, False), stepsize_controller=dfx.ConstantStepSize())
+
+
 def _solve(**kw):
     base = dict(terms=dfx.ODETerm(dfx.LotkaVolterra()), solver=dfx.Tsit5(), t0=0.0, t1=1.0,
                 dt0=0.1, y0=torch.ones(2, dtype=torch.float64), args=(1.5, 1.0, 3.0, 1.0),
