@@ -1,9 +1,12 @@
 """Launch-shape autotuner for the persistent ERK kernels.
 
-For a prepared plan it times a few candidate launch shapes (workgroup size, number of
-persistent trajectory slots) with HIP events and keeps the fastest.  Results are cached
-in memory and, if DIFFRAX_AMD_TUNE_CACHE names a file, as JSON keyed by
-(kernel signature, batch, device name)."""
+For a prepared plan it times a few candidate launch shapes with HIP events and keeps the
+fastest: for the small-system kernel the workgroup size and the number of persistent
+trajectory slots (all instances, or half with the refill queue), for the neural-ODE
+kernel the wavefronts per 32-trajectory tile (4 or 8; the tile's LDS layout is the same,
+the waves split its output columns and state components).  Results are cached in memory
+and, if DIFFRAX_AMD_TUNE_CACHE names a file, as JSON keyed by (kernel signature, batch,
+device name)."""
 from __future__ import annotations
 
 import json
@@ -59,18 +62,23 @@ def candidates(batch):
     return out
 
 
-def time_plan(plan, launch_cfg, reps=3):
-    plan.launch_cfg = launch_cfg
-    plan.launch()
+def time_launch(launch, reps=3):
+    """Mean event time (ms) of reps calls of launch() after one untimed call."""
+    launch()
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
     start.record()
     for _ in range(reps):
-        plan.launch()
+        launch()
     end.record()
     end.synchronize()
     return start.elapsed_time(end) / reps
+
+
+def time_plan(plan, launch_cfg, reps=3):
+    plan.launch_cfg = launch_cfg
+    return time_launch(plan.launch, reps)
 
 
 def tune(plan, verbose=False):
@@ -99,3 +107,38 @@ def tune(plan, verbose=False):
     _CACHE[key] = {"best": [b, l], "ms": timings}
     _save_file()
     return plan.launch_cfg, timings
+
+
+def _key_mlp(plan):
+    """The neural-ODE kernel variant (precision, controller kind, gradient recording, dt0
+    given or selected), its save layout and tolerances, the batch and the device."""
+    p = plan.prob
+    name = torch.cuda.get_device_name(torch.cuda.current_device())
+    pid = f"{p.pcoeff}/{p.icoeff}/{p.dcoeff}/{p.error_order}/{p.rtol!r}/{p.atol!r}"
+    return (f"mlp:{p.precision}:{pid}:{int(p.dt0 == p.dt0)}:{p.n_ts}/{p.save_t0}/{p.save_t1}:"
+            f"{p.max_steps}:{p.ckpt_every}:{p.t0!r}/{p.t1!r}:{p.batch}:{name}")
+
+
+def tune_mlp(plan, verbose=False):
+    """Pick the faster wave count (4 or 8) for a neural-ODE plan; sets plan.prob.waves and
+    returns it with the timings."""
+    if not _CACHE:
+        _load_file()
+    key = _key_mlp(plan)
+    if key in _CACHE:
+        plan.prob.waves = int(_CACHE[key]["best"])
+        return plan.prob.waves, _CACHE[key]["ms"]
+    timings = {}
+    for _ in range(2):
+        for waves in (4, 8):
+            plan.prob.waves = waves
+            ms = time_launch(plan.launch, reps=5)
+            timings[str(waves)] = min(ms, timings.get(str(waves), ms))
+    if verbose:
+        for name, ms in timings.items():
+            print(f"  tune waves {name}: {ms:.3f} ms", flush=True)
+    best = min(timings, key=timings.get)
+    plan.prob.waves = int(best)
+    _CACHE[key] = {"best": int(best), "ms": timings}
+    _save_file()
+    return plan.prob.waves, timings

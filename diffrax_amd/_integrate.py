@@ -490,7 +490,7 @@ def _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controll
             dense_ts=torch.empty((B, int(max_steps) + 1), dtype=torch.float64, device=device),
             dense_ys=torch.empty((B, int(max_steps) + 1, d), dtype=y0b.dtype, device=device),
             dense_k=torch.empty((B, int(max_steps), S, d), dtype=y0b.dtype, device=device))
-    plan = ErkPlan(prob, tensors, n_out, batched, launch)
+    plan = ErkPlan(prob, tensors, n_out, batched, None if launch == "auto" else launch)
     plan.fsal = bool(solver.tableau.fsal)
     # checks of device-resident times, run by diffeqsolve once the solve is queued
     plan.deferred_checks = check_times if host.deferred else None
@@ -603,7 +603,10 @@ def diffeqsolve(terms, solver: AbstractSolver, t0, t1, dt0, y0, args: Any = None
                 launch: Optional[tuple] = None) -> Solution:
     """Solve an initial value problem on the GPU (signature of _integrate.py:890-911).
 
-    Extra keyword ``launch=(block, lanes)`` overrides the persistent kernel's launch shape.
+    Extra keyword ``launch``: ``(block, lanes)`` sets the small-system kernel's launch
+    shape, 4 or 8 the neural-ODE kernel's wavefronts per tile; ``"auto"`` times the
+    candidates once per problem signature and device and keeps the fastest
+    (``_autotune``).
     Batched use: y0 of shape [B, d] solves B independent instances in one launch.
 
     SaveAt(fn=...) is applied to the saved (t, y) rows after the solve, batched with
@@ -706,7 +709,7 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
         from . import _mlp
         return _mlp.diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat,
                                     stepsize_controller, adjoint, max_steps, throw,
-                                    _require_gpu())
+                                    _require_gpu(), launch=launch)
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                         max_steps, launch, event=event, solver_state=solver_state,
                         controller_state=controller_state, made_jump=made_jump)
@@ -714,6 +717,9 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
     T = plan.t
     wants_grad = torch.is_grad_enabled() and (T["y0"].requires_grad or
                                               T["params"].requires_grad)
+    if launch == "auto":
+        from . import _autotune
+        _autotune.tune(plan)
     if not wants_grad:
         plan.launch()
         return _finish(plan, t0, t1, max_steps, throw, saveat=saveat)

@@ -29,7 +29,8 @@ class MlpProblem(ctypes.Structure):
                 ("dcoeff", ctypes.c_double), ("dtmin", ctypes.c_double),
                 ("dtmax", ctypes.c_double), ("factormin", ctypes.c_double),
                 ("factormax", ctypes.c_double), ("safety", ctypes.c_double),
-                ("error_order", ctypes.c_double)]
+                ("error_order", ctypes.c_double), ("waves", ctypes.c_int32),
+                ("reserved_", ctypes.c_int32)]
 
 
 class MlpAdjIO(ctypes.Structure):
@@ -304,6 +305,7 @@ class _MlpSolve(torch.autograd.Function):
         plan = MlpPlan(params, cfg["t0"], cfg["t1"], cfg["dt0"], y0.detach(), cfg["ts"],
                        cfg["ctrl"], cfg["sub"], cfg["max_steps"],
                        cfg["ckpt_every"] if need else 0, cfg["device"])
+        set_waves(plan, cfg.get("launch"))
         plan.launch()
         plan.note_max_accepted()
         ctx.plan = plan
@@ -462,8 +464,25 @@ def checkpoint_every(adjoint, max_steps):
     return max(1, int(math.ceil(max_steps / int(k))))
 
 
+def set_waves(plan, launch):
+    """The tile's wave count from diffeqsolve's ``launch``: None keeps the library default
+    (4 for float32, 8 for the three-piece bf16 form), 4 or 8 selects it, "auto" times both
+    once per problem signature and device (_autotune.tune_mlp).  The results are the same
+    bits either way."""
+    if launch is None:
+        return
+    if launch == "auto":
+        from . import _autotune
+        _autotune.tune_mlp(plan)
+    elif launch in (4, 8):
+        plan.prob.waves = int(launch)
+    else:
+        raise ValueError("launch for the neural-ODE kernel is None, 'auto', 4 or 8 "
+                         "(wavefronts per 32-trajectory tile)")
+
+
 def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, adjoint,
-                    max_steps, throw, device):
+                    max_steps, throw, device, launch=None):
     vfo = terms.vector_field
     y0t = torch.as_tensor(y0)
     needs_grad = torch.is_grad_enabled() and (
@@ -471,6 +490,7 @@ def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
     if not needs_grad:
         plan = prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
                            max_steps, device)
+        set_waves(plan, launch)
         plan.launch()
         return _solution(plan, t0, t1, max_steps, throw, plan.ys)
     from ._adjoint import BacksolveAdjoint, RecursiveCheckpointAdjoint
@@ -481,7 +501,7 @@ def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
     probe = prepare_mlp(terms, solver, t0, t1, dt0, y0t.detach(), saveat, stepsize_controller,
                         max_steps, device, ckpt_every=0, allocate=False)
     cfg = dict(matmul=vfo.matmul, t0=t0, t1=t1, dt0=dt0, ts=probe.ts_host, ctrl=stepsize_controller,
-               sub=saveat.subs, max_steps=max_steps, device=device)
+               sub=saveat.subs, max_steps=max_steps, device=device, launch=launch)
     batched = y0t.ndim == 2
     y0b = y0t if batched else y0t.reshape(1, -1)
     if isinstance(adjoint, BacksolveAdjoint):

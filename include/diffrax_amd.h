@@ -343,6 +343,10 @@ typedef struct dfx_mlp_problem {
   double t0, t1, dt0;
   double rtol, atol, pcoeff, icoeff, dcoeff, dtmin, dtmax;
   double factormin, factormax, safety, error_order;
+  int32_t waves;            /* wavefronts per 32-trajectory workgroup of dfx_mlp_solve: 4
+                               or 8; 0 picks 4 for DFX_F32 and 8 for DFX_BF16_F32 (no
+                               effect on the results, which are the same bits)          */
+  int32_t reserved_;        /* 0 */
 } dfx_mlp_problem;
 
 typedef struct dfx_mlp_io {

@@ -31,3 +31,28 @@ def test_key_separates_kernel_variants(monkeypatch):
     keys = [_autotune._key(v) for v in variants]
     assert base not in keys and len(set(keys)) == len(keys)
     assert _autotune._key(_plan()) == base
+
+
+def _mlp_plan(**kw):
+    from diffrax_amd import _mlp
+    p = _mlp.MlpProblem(solver=_lib.TSIT5, precision=_lib.F32, d=64, hidden=256, n_ts=32,
+                        max_steps=4096, batch=8192, t0=0.0, t1=1.0, dt0=0.1, rtol=1e-5,
+                        atol=1e-7, icoeff=1.0, error_order=float("nan"))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return types.SimpleNamespace(prob=p)
+
+
+def test_mlp_key_separates_kernel_variants(monkeypatch):
+    """The neural-ODE tuning key: precision, controller, dt0 selection, saves, gradient
+    recording, span and batch each select another entry; the chosen wave count does not."""
+    monkeypatch.setattr(torch.cuda, "get_device_name", lambda *a: "dev")
+    monkeypatch.setattr(torch.cuda, "current_device", lambda: 0)
+    base = _autotune._key_mlp(_mlp_plan())
+    variants = [_mlp_plan(precision=_lib.BF16_F32), _mlp_plan(pcoeff=0.3),
+                _mlp_plan(dt0=float("nan")), _mlp_plan(n_ts=8), _mlp_plan(save_t1=1),
+                _mlp_plan(ckpt_every=64), _mlp_plan(t1=2.0), _mlp_plan(batch=1024),
+                _mlp_plan(rtol=1e-6), _mlp_plan(max_steps=100)]
+    keys = [_autotune._key_mlp(v) for v in variants]
+    assert base not in keys and len(set(keys)) == len(keys)
+    assert _autotune._key_mlp(_mlp_plan(waves=8)) == base

@@ -302,7 +302,8 @@ def test_mlp_sizes_and_argument_checks():
     assert lib.dfx_mlp_n_ckpt(ctypes.byref(p)) == 64
     io = _mlp.MlpIO()
     z = ctypes.c_void_p(0)
-    for field, bad in (("d", 32), ("hidden", 128), ("precision", _lib.F64), ("solver", _lib.DOPRI5)):
+    for field, bad in (("d", 32), ("hidden", 128), ("precision", _lib.F64), ("solver", _lib.DOPRI5),
+                       ("waves", 2), ("waves", 16)):
         q = _mlp.MlpProblem(solver=_lib.TSIT5, precision=_lib.F32, d=64, hidden=256, n_ts=32,
                             max_steps=4096, ckpt_every=64, batch=8192)
         setattr(q, field, bad)

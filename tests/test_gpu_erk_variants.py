@@ -153,6 +153,14 @@ def test_refill_queue_kernel_matches_static(cuda):
     _assert_exact(a, ref)
 
 
+def test_autotuned_launch_matches_oracle(cuda):
+    """launch='auto' times the candidate shapes (static and refill, three block sizes) and
+    solves with the fastest: the results stay bit-identical."""
+    ts = np.linspace(0.0, 8.0, 33)
+    a, ref = _run("tsit5", "vdp", VDP_Y0, VDP_MU, 0.0, 8.0, 0.05, ts, launch="auto")
+    _assert_exact(a, ref)
+
+
 def test_mixed_precision_form_error_bound(cuda):
     """float32 state / float64 time on Lotka-Volterra: within 100 * rtol of the FP64 oracle."""
     ts = np.linspace(0.0, 10.0, 100)

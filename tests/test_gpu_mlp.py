@@ -160,3 +160,35 @@ def test_mlp_initial_step_and_pid_terms(cuda, matmul, ctrl_kw):
     print(f"  kernel floor (rtol 1e-9): rtol-scaled err {e_floor:.3f}")
     assert rel < 1e-4
     assert e_dev <= e_ora + e_floor + 1.0
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+def test_wave_count_gives_the_same_bits(cuda, matmul):
+    """Four and eight wavefronts per tile split the same products and the controller norms
+    sum in the same order, so solves (integral and PID control, given and automatic dt0,
+    with gradient recording) and gradients are bit-identical; launch='auto' picks one."""
+    W = mlp_weights(6)
+    y0 = torch.tensor(np.random.default_rng(8).normal(size=(100, 64)), dtype=torch.float32)
+    ts = np.linspace(0.0, 1.0, 9)
+    P = [torch.tensor(w, dtype=torch.float32, device=cuda) for w in W]
+
+    def solve(launch, ctrl, dt0, grad=False):
+        params = [p.clone().requires_grad_(grad) for p in P]
+        y = y0.to(cuda).clone().requires_grad_(grad)
+        sol = dfx.diffeqsolve(dfx.ODETerm(dfx.MLPField(*params, matmul=matmul)), dfx.Tsit5(),
+                              0.0, 1.0, dt0, y, saveat=dfx.SaveAt(ts=ts),
+                              stepsize_controller=ctrl, launch=launch)
+        if not grad:
+            return [sol.ys.detach(), sol.stats["num_steps"]]
+        sol.ys.square().sum().backward()
+        return [sol.ys.detach(), y.grad] + [p.grad for p in params]
+
+    for ctrl, dt0 in ((dfx.PIDController(rtol=RTOL, atol=ATOL), 0.1),
+                      (dfx.PIDController(rtol=RTOL, atol=ATOL, pcoeff=0.2, icoeff=0.5), None)):
+        for grad in (False, True):
+            a, b = solve(4, ctrl, dt0, grad), solve(8, ctrl, dt0, grad)
+            assert all(torch.equal(x, y) for x, y in zip(a, b))
+    c = solve("auto", dfx.PIDController(rtol=RTOL, atol=ATOL), 0.1)
+    assert torch.equal(c[0], solve(4, dfx.PIDController(rtol=RTOL, atol=ATOL), 0.1)[0])
+    with pytest.raises(ValueError, match="wavefronts"):
+        solve(6, dfx.PIDController(rtol=RTOL, atol=ATOL), 0.1)
