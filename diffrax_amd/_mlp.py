@@ -30,7 +30,7 @@ class MlpProblem(ctypes.Structure):
                 ("dtmax", ctypes.c_double), ("factormin", ctypes.c_double),
                 ("factormax", ctypes.c_double), ("safety", ctypes.c_double),
                 ("error_order", ctypes.c_double), ("waves", ctypes.c_int32),
-                ("reserved_", ctypes.c_int32)]
+                ("save_steps", ctypes.c_int32)]
 
 
 class MlpAdjIO(ctypes.Structure):
@@ -45,12 +45,15 @@ class MlpBacksolveIO(ctypes.Structure):
 
 class MlpIO(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
-                ("y0", "ts", "ys", "ts_out", "stats", "result", "grid", "ckpt")]
+                ("y0", "ts", "ys", "ts_out", "stats", "result", "grid", "ckpt", "dense_ts",
+                 "dense_ys", "dense_k")]
 
 
 _P = ctypes.POINTER
 _SIGS = {
     "dfx_mlp_pack_bytes": ([ctypes.c_int32, _P(ctypes.c_size_t)], ctypes.c_int),
+    "dfx_mlp_n_out": ([_P(MlpProblem)], ctypes.c_int),
+    "dfx_mlp_dense_eval": ([_P(MlpProblem), _P(_lib.DenseIO), ctypes.c_void_p], ctypes.c_int),
     "dfx_mlp_pack": ([ctypes.c_int32] + [ctypes.c_void_p] * 5, ctypes.c_int),
     "dfx_mlp_n_ckpt": ([_P(MlpProblem)], ctypes.c_int),
     "dfx_mlp_eval": ([ctypes.c_int32] + [ctypes.c_void_p] * 6 + [ctypes.c_int64, ctypes.c_void_p],
@@ -97,7 +100,8 @@ def pack_weights(W1, W2, W3, matmul, device):
 class MlpPlan:
     """Prepared batched neural-ODE solve (launch() enqueues without host sync)."""
 
-    def __init__(self, vfo, t0, t1, dt0, y0, ts, ctrl, sub, max_steps, ckpt_every, device):
+    def __init__(self, vfo, t0, t1, dt0, y0, ts, ctrl, sub, max_steps, ckpt_every, device,
+                 dense=False):
         lib = _lib.load()
         self.vfo = vfo
         B = y0.shape[0]
@@ -119,8 +123,9 @@ class MlpPlan:
             dtmin=nan if ctrl.dtmin is None else float(ctrl.dtmin),
             dtmax=nan if ctrl.dtmax is None else float(ctrl.dtmax),
             factormin=ctrl.factormin, factormax=ctrl.factormax, safety=ctrl.safety,
-            error_order=nan if ctrl.error_order is None else float(ctrl.error_order))
-        self.n_out = int(sub.t0) + n_ts + int(sub.t1)
+            error_order=nan if ctrl.error_order is None else float(ctrl.error_order),
+            save_steps=int(sub.steps))
+        self.n_out = int(lib.dfx_mlp_n_out(ctypes.byref(self.prob)))
         self.ys = torch.empty((B, self.n_out, 64), dtype=torch.float32, device=device)
         self.ts_out = torch.empty((B, self.n_out), dtype=torch.float64, device=device)
         self.stats = torch.empty((B, 3), dtype=torch.int32, device=device)
@@ -138,6 +143,15 @@ class MlpPlan:
         io.stats, io.result = self.stats.data_ptr(), self.result.data_ptr()
         io.grid = self.grid.data_ptr() if self.grid is not None else None
         io.ckpt = self.ckpt.data_ptr() if self.ckpt is not None else None
+        self.dense = None
+        if dense:
+            M = int(max_steps)
+            self.dense = dict(
+                dense_ts=torch.empty((B, M + 1), dtype=torch.float64, device=device),
+                dense_ys=torch.empty((B, M + 1, 64), dtype=torch.float32, device=device),
+                dense_k=torch.empty((B, M, 7, 64), dtype=torch.float32, device=device))
+            for name, v in self.dense.items():
+                setattr(io, name, v.data_ptr())
         self.io = io
 
     def launch(self):
@@ -182,8 +196,20 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
         raise NotImplementedError("the forward solve's controller uses rms_norm; "
                                   "adjoint_rms_seminorm is for BacksolveAdjoint's controller")
     sub = saveat.subs
-    if sub is None or sub.steps or saveat.dense:
-        raise NotImplementedError("the neural-ODE kernel saves SaveAt(t0, t1, ts)")
+    if sub is None:
+        if not saveat.dense:
+            raise ValueError("SaveAt must save something")
+        from ._saveat import SubSaveAt, save_y
+        sub = SubSaveAt.__new__(SubSaveAt)   # dense output only: no saved rows
+        sub.t0 = sub.t1 = False
+        sub.ts, sub.steps, sub.fn = None, 0, save_y
+    if not hasattr(sub, "fn"):
+        raise NotImplementedError("the neural-ODE kernel takes a single SubSaveAt")
+    if sub.fn is not save_y_fn():
+        raise NotImplementedError("SaveAt(fn=...) is provided on the small-system kernel")
+    if ckpt_every > 0 and (sub.steps or saveat.dense):
+        raise NotImplementedError("gradients through SaveAt(steps=...) or SaveAt(dense=True) "
+                                  "are not provided on the neural-ODE kernel")
     y0 = torch.as_tensor(y0)
     batched = y0.ndim == 2
     y0b = y0 if batched else y0.reshape(1, -1)
@@ -212,8 +238,10 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
         probe = _Probe()
         probe.ts_host = ts
         probe.deferred_checks = check
+        probe.sub = sub
         return probe
-    plan = MlpPlan(terms.vector_field, t0, t1, dt0, y0b, ts, c, sub, max_steps, ckpt_every, device)
+    plan = MlpPlan(terms.vector_field, t0, t1, dt0, y0b, ts, c, sub, max_steps, ckpt_every, device,
+                   dense=saveat.dense)
     plan.batched = batched
     plan.deferred_checks = check
     return plan
@@ -237,7 +265,67 @@ def _solution(plan, t0, t1, max_steps, throw, ys):
         bad = plan.result != 0
         if bool(bad.any()):
             raise RuntimeError(RESULTS[int(plan.result[bad][0].item())])
-    return Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=result)
+    interp = None
+    if getattr(plan, "dense", None) is not None:
+        interp = MlpDenseInterpolation(plan)
+        p = plan.prob
+        if p.n_ts == 0 and not (p.save_t0 or p.save_t1 or p.save_steps):
+            ts = ys = None   # SaveAt(dense=True) alone saves no rows
+    return Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=result,
+                    interpolation=interp)
+
+
+def save_y_fn():
+    from ._saveat import save_y
+    return save_y
+
+
+class MlpDenseInterpolation:
+    """Dense output of a neural-ODE SaveAt(dense=True) solve (DenseInterpolation,
+    _global_interpolation.py:315-400), evaluated on the GPU by dfx_mlp_dense_eval.
+    ``evaluate(t0, t1=None, left=True)`` and ``derivative(t, left=True)`` take a scalar
+    time, [n] times shared by the batch or [B, n] per-trajectory times."""
+
+    def __init__(self, plan):
+        self.plan = plan
+
+    def evaluate(self, t0, t1=None, left: bool = True):
+        if t1 is not None:
+            return self.evaluate(t1, left=left) - self.evaluate(t0, left=left)
+        return self._query(t0, False, left)
+
+    def derivative(self, t, left: bool = True):
+        return self._query(t, True, left)
+
+    def _query(self, t, derivative, left):
+        plan = self.plan
+        dev = plan.y0.device
+        B = plan.y0.shape[0]
+        tq = torch.as_tensor(t, dtype=torch.float64)
+        scalar = tq.ndim == 0
+        if scalar:
+            tq = tq.reshape(1)
+        if tq.ndim == 1:
+            n_q, stride = tq.shape[0], 0
+        elif tq.ndim == 2 and tq.shape[0] == B:
+            n_q, stride = tq.shape[1], tq.shape[1]
+        else:
+            raise ValueError(f"evaluation times must be a scalar, [n] or [{B}, n]")
+        tq = tq.to(dev).contiguous()
+        out = torch.empty((B, n_q, 64), dtype=torch.float32, device=dev)
+        D = plan.dense
+        io = _lib.DenseIO(dense_ts=D["dense_ts"].data_ptr(), dense_ys=D["dense_ys"].data_ptr(),
+                          dense_k=D["dense_k"].data_ptr(), stats=plan.stats.data_ptr(),
+                          t1=None, tq=tq.data_ptr(), n_q=n_q, tq_stride=stride,
+                          left=int(bool(left)), derivative=int(bool(derivative)),
+                          out=out.data_ptr())
+        _lib.check(_lib.load().dfx_mlp_dense_eval(ctypes.byref(plan.prob), ctypes.byref(io),
+                                                  _stream()), "dfx_mlp_dense_eval")
+        if scalar:
+            out = out[:, 0]
+        if not plan.batched:
+            out = out[0]
+        return out
 
 
 class _Params:
@@ -500,6 +588,9 @@ def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
     # validation and shapes via a throwaway-free preparation of the configuration
     probe = prepare_mlp(terms, solver, t0, t1, dt0, y0t.detach(), saveat, stepsize_controller,
                         max_steps, device, ckpt_every=0, allocate=False)
+    if probe.sub.steps or saveat.dense:
+        raise NotImplementedError("gradients through SaveAt(steps=...) or SaveAt(dense=True) "
+                                  "are not provided on the neural-ODE kernel")
     cfg = dict(matmul=vfo.matmul, t0=t0, t1=t1, dt0=dt0, ts=probe.ts_host, ctrl=stepsize_controller,
                sub=saveat.subs, max_steps=max_steps, device=device, launch=launch)
     batched = y0t.ndim == 2

@@ -73,6 +73,72 @@ __global__ void mlp_pack3_kernel(const float* __restrict__ W, __bf16* __restrict
   split3(v, out[e], out[e + plane], out[e + 2 * plane]);
 }
 
+// Dense output of the neural ODE at query times (erk_dense.hpp's evaluation for d = 64):
+// 64 threads per (trajectory, query), one per component; the step holding t, the
+// interpolant's weights and the bounds rule as there, and each component's value with
+// the arithmetic of the solve kernel's SaveAt(ts) interpolation, so evaluating at a saved
+// time returns that saved row bit for bit.
+struct MlpDenseArgs {
+  const double* dense_ts;
+  const float* dense_ys;
+  const float* dense_k;
+  const int32_t* stats;
+  const double* tq;
+  float* out;
+  int64_t B, n_q, tq_stride;
+  int32_t max_steps, left, derivative;
+  double t0, t1;
+};
+
+__global__ void __launch_bounds__(256) mlp_dense_eval_kernel(const MlpDenseArgs a) {
+  constexpr int S = Tsit5Tab::S;
+  const int64_t gid = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int j = threadIdx.x & 63;
+  if (gid >= a.B * a.n_q) return;
+  const int64_t b = gid / a.n_q;
+  const int64_t q = gid - b * a.n_q;
+  const int dir = (a.t0 < a.t1) ? 1 : -1;
+  const double t = a.tq[b * a.tq_stride + q] * dir;
+  const int n = a.stats[b * 3 + 1];
+  const int64_t row0 = b * ((int64_t)a.max_steps + 1);
+  const double* ts = a.dense_ts + row0;
+  const float* ys = a.dense_ys + row0 * MLP_D;
+  float* out = a.out + gid * MLP_D;
+  const double ts_first = ts[0], ts_last = ts[n];
+  if (n < 1 || t < ts_first || t > ts_last || t != t) {
+    const bool trivial = (n == 0) && (t == a.t0) && !a.derivative;
+    out[j] = trivial ? ys[j] : __builtin_nanf("");
+    return;
+  }
+  int lo = 0, hi = n + 1;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const bool before = a.left ? (ts[mid] < t) : (ts[mid] <= t);
+    if (before) lo = mid + 1; else hi = mid;
+  }
+  int idx = lo - 1;
+  idx = idx < 0 ? 0 : (idx > n - 1 ? n - 1 : idx);
+  const double ts0 = ts[idx], ts1 = ts[idx + 1];
+  const float* kp = a.dense_k + (b * (int64_t)a.max_steps + idx) * S * MLP_D + j;
+  const double th = linear_rescale(ts0, t, ts1);
+  if (a.derivative) {
+    double db[7];
+    tsit5_dweights(th, db);
+    const double inv_h = (ts0 == ts1) ? 0.0 : 1.0 / (ts1 - ts0);
+    float acc = (float)db[0] * kp[0];
+#pragma unroll
+    for (int m = 1; m < 7; ++m) acc = acc + (float)db[m] * kp[m * MLP_D];
+    out[j] = (float)dir * (float)((double)acc * inv_h);
+  } else {
+    double bb[7];
+    tsit5_weights(th, bb);
+    float acc = (float)bb[0] * kp[0];
+#pragma unroll
+    for (int m = 1; m < 7; ++m) acc = acc + (float)bb[m] * kp[m * MLP_D];
+    out[j] = ys[(int64_t)idx * MLP_D + j] + acc;
+  }
+}
+
 }  // namespace dfx
 
 using namespace dfx;
@@ -163,13 +229,26 @@ int dfx_mlp_n_ckpt(const dfx_mlp_problem* p) {
   return (p->max_steps + p->ckpt_every - 1) / p->ckpt_every;
 }
 
+int dfx_mlp_n_out(const dfx_mlp_problem* p) {
+  if (!p || p->save_steps < 0 || p->max_steps < 0) return DFX_E_ARG;
+  int n = (p->save_t0 ? 1 : 0) + p->n_ts;
+  if (p->save_steps > 0) n += p->max_steps / p->save_steps;
+  if (p->save_t1 && (p->save_steps <= 0 || p->max_steps % p->save_steps != 0)) n += 1;
+  return n;
+}
+
 int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1, const float* b2,
                   const float* b3, const dfx_mlp_io* io, void* stream) {
   int rc = mlp_check(p);
   if (rc != DFX_OK) return rc;
   if (!packed || !b1 || !b2 || !b3 || !io) return DFX_E_ARG;
+  if (p->save_steps < 0) return DFX_E_ARG;
+  const int dense = (io->dense_ts != nullptr) + (io->dense_ys != nullptr) + (io->dense_k != nullptr);
+  if (dense != 0 && dense != 3) return DFX_E_ARG;
+  // the adjoints replay the ts saves only
+  if (p->ckpt_every > 0 && (dense || p->save_steps > 0)) return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
-  const int n_out = (p->save_t0 ? 1 : 0) + p->n_ts + (p->save_t1 ? 1 : 0);
+  const int n_out = dfx_mlp_n_out(p);
   if (!io->y0 || !io->stats || !io->result) return DFX_E_ARG;
   if (n_out > 0 && (!io->ys || !io->ts_out)) return DFX_E_ARG;
   if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
@@ -179,6 +258,8 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   memset(&a, 0, sizeof(a));
   a.y0 = io->y0; a.ts = io->ts; a.ys = io->ys; a.ts_out = io->ts_out; a.stats = io->stats;
   a.result = io->result; a.grid = io->grid; a.ckpt = io->ckpt;
+  a.dense_ts = io->dense_ts; a.dense_ys = io->dense_ys; a.dense_k = io->dense_k;
+  a.save_steps = p->save_steps;
   a.B = p->batch; a.n_ts = p->n_ts; a.n_out = n_out; a.save_t0 = p->save_t0;
   a.save_t1 = p->save_t1; a.max_steps = p->max_steps; a.ckpt_every = p->ckpt_every;
   a.n_ckpt = dfx_mlp_n_ckpt(p);
@@ -201,6 +282,38 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   else
     launch_erk_mlp_f32(waves, a, weights_from<PolicyF32>(packed, b1, b2, b3), g, integral,
                        nblocks, s);
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+int dfx_mlp_dense_eval(const dfx_mlp_problem* p, const dfx_erk_dense_io* io, void* stream) {
+  if (!p || !io) return DFX_E_ARG;
+  if (p->d != MLP_D || p->solver != DFX_TSIT5 || p->batch < 0 || p->max_steps < 0)
+    return DFX_E_ARG;
+  if (io->n_q < 0 || io->tq_stride < 0) return DFX_E_ARG;
+  if (io->tq_stride != 0 && io->tq_stride < io->n_q) return DFX_E_ARG;
+  if (p->batch == 0 || io->n_q == 0) return DFX_OK;
+  if (!io->dense_ts || !io->dense_ys || !io->dense_k || !io->stats || !io->tq || !io->out)
+    return DFX_E_ARG;
+  MlpDenseArgs a;
+  memset(&a, 0, sizeof(a));
+  a.dense_ts = io->dense_ts;
+  a.dense_ys = reinterpret_cast<const float*>(io->dense_ys);
+  a.dense_k = reinterpret_cast<const float*>(io->dense_k);
+  a.stats = io->stats;
+  a.tq = io->tq;
+  a.out = reinterpret_cast<float*>(io->out);
+  a.B = p->batch;
+  a.n_q = io->n_q;
+  a.tq_stride = io->tq_stride;
+  a.max_steps = p->max_steps;
+  a.left = io->left ? 1 : 0;
+  a.derivative = io->derivative ? 1 : 0;
+  a.t0 = p->t0;
+  a.t1 = p->t1;
+  const int64_t groups = a.B * a.n_q;
+  const unsigned nb = (unsigned)((groups + 3) / 4);
+  hipLaunchKernelGGL(mlp_dense_eval_kernel, dim3(nb), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), a);
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }
 

@@ -31,8 +31,11 @@ struct MlpArgs {
   int32_t* result;
   double* grid;
   float* ckpt;
+  double* dense_ts;  // SaveAt(dense=True): [B, max_steps + 1], [B, max_steps + 1, 64],
+  float* dense_ys;   // [B, max_steps, 7, 64] (or null)
+  float* dense_k;
   int64_t B;
-  int32_t n_ts, n_out, save_t0, save_t1, max_steps, ckpt_every, n_ckpt, has_dt0;
+  int32_t n_ts, n_out, save_t0, save_t1, max_steps, ckpt_every, n_ckpt, has_dt0, save_steps;
   double t0, t1, dt0, error_order;
   PidParams pid;
 };
@@ -81,6 +84,12 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
     ++save_index;
   };
   if (valid && a.save_t0) write_save(T0, y);
+  if (valid && a.dense_ts) {  // the dense output's first knot: (t0, y0)
+    const int64_t row = traj * ((int64_t)a.max_steps + 1);
+    if (lead) a.dense_ts[row] = T0;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) a.dense_ys[row * MLP_D + c0 + j] = y[j];
+  }
 
   // ---- first stage of the first step (FSAL seed): f(t0, y0) ----
 #pragma unroll
@@ -224,6 +233,21 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
             for (int j = 0; j < CPT; ++j) c[j] = y[j];
           }
         }
+        if (a.dense_ts) {
+          // SaveAt(dense=True): the step's end time, end state and stage increments
+          // (_integrate.py:529-540)
+          const int64_t row = traj * ((int64_t)a.max_steps + 1) + num_acc;
+          if (lead) a.dense_ts[row] = ntprev;
+          float* dy = a.dense_ys + row * MLP_D + c0;
+          float* dk = a.dense_k + (traj * (int64_t)a.max_steps + (num_acc - 1)) * S * MLP_D + c0;
+#pragma unroll
+          for (int j = 0; j < CPT; ++j) dy[j] = z[j];
+#pragma unroll
+          for (int m = 0; m < S; ++m) {
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) dk[m * MLP_D + j] = k[m][j];
+          }
+        }
         while (next_ts <= ts1) {
           const double th = linear_rescale(ts0, next_ts, ts1);
           double bb[7];
@@ -242,6 +266,8 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
         }
 #pragma unroll
         for (int j = 0; j < CPT; ++j) { y[j] = z[j]; fsal[j] = f[j]; }
+        // SaveAt(steps=k): every k-th accepted step (_integrate.py:493-527)
+        if (a.save_steps && num_acc % a.save_steps == 0) write_save(ntprev, y);
       }
       tprev = ntprev;
       tnext = ntnext;
@@ -251,7 +277,9 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
   if (T0 == T1) {
     for (int r = 0; r < a.n_ts; ++r) write_save(T0, y);
   }
-  if (a.save_t1) write_save(tprev, y);
+  // _save_t1 (_integrate.py:847-877): no t1 row when the last step was saved via steps
+  const bool via_steps = a.save_steps && num_acc % a.save_steps == 0;
+  if (a.save_t1 && !via_steps && save_index < a.n_out) write_save(tprev, y);
   if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
   for (int s = save_index; s < a.n_out; ++s) {
 #pragma unroll

@@ -346,7 +346,8 @@ typedef struct dfx_mlp_problem {
   int32_t waves;            /* wavefronts per 32-trajectory workgroup of dfx_mlp_solve: 4
                                or 8; 0 picks 4 for DFX_F32 and 8 for DFX_BF16_F32 (no
                                effect on the results, which are the same bits)          */
-  int32_t reserved_;        /* 0 */
+  int32_t save_steps;       /* SaveAt(steps=k): k > 0 also saves every k-th accepted step
+                               (rows in time order with the ts rows; forward only)      */
 } dfx_mlp_problem;
 
 typedef struct dfx_mlp_io {
@@ -358,6 +359,10 @@ typedef struct dfx_mlp_io {
   int32_t* result;          /* out [B]                                                   */
   double* grid;             /* out [B, max_steps, 2] when ckpt_every > 0                  */
   float* ckpt;              /* out [B, dfx_mlp_n_ckpt, 64] when ckpt_every > 0            */
+  /* SaveAt(dense=True), all three or none (forward only), the layout of dfx_erk_io's:   */
+  double* dense_ts;         /* out [B, max_steps + 1]: t0, then each accepted step's end  */
+  float* dense_ys;          /* out [B, max_steps + 1, 64]: the states at dense_ts         */
+  float* dense_k;           /* out [B, max_steps, 7, 64]: each accepted step's k_i        */
 } dfx_mlp_io;
 
 /* Bytes of the packed-weight buffer for a precision. */
@@ -379,6 +384,15 @@ int dfx_mlp_n_ckpt(const dfx_mlp_problem* p);
  * (_step_size_controller/pid.py:394-567) and SaveAt(ts) (_integrate.py:456-491). */
 int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
                   const float* b2, const float* b3, const dfx_mlp_io* io, void* stream);
+/* Saved rows per trajectory of dfx_mlp_solve: t0 + n_ts + (save_steps ? max_steps /
+ * save_steps : 0) + t1 (the t1 row is dropped when max_steps is a multiple of save_steps),
+ * the allocation rule of _integrate.py:1272-1305. */
+int dfx_mlp_n_out(const dfx_mlp_problem* p);
+/* The dense solution of a dfx_mlp_solve with dense buffers (or its time derivative) at
+ * every trajectory's query times; io->t1 is not read (the problem's t1 is).
+ * Replaces: DenseInterpolation.evaluate / derivative (_global_interpolation.py:315-400)
+ * with the Tsit5 interpolant (_solver/tsit5.py:103-156), for the neural ODE. */
+int dfx_mlp_dense_eval(const dfx_mlp_problem* p, const dfx_erk_dense_io* io, void* stream);
 
 /* ---- neural ODE adjoint ---------------------------------------------------------- */
 typedef struct dfx_mlp_adj_io {

@@ -308,6 +308,33 @@ def test_mlp_sizes_and_argument_checks():
                             max_steps=4096, ckpt_every=64, batch=8192)
         setattr(q, field, bad)
         assert lib.dfx_mlp_solve(ctypes.byref(q), z, z, z, z, ctypes.byref(io), None) == -1
+    # saved rows: t0 + ts + every k-th step + t1 (no t1 row when k divides max_steps)
+    q = _mlp.MlpProblem(d=64, hidden=256, n_ts=5, save_t0=1, save_t1=1, max_steps=100)
+    assert lib.dfx_mlp_n_out(ctypes.byref(q)) == 7
+    q.save_steps = 3
+    assert lib.dfx_mlp_n_out(ctypes.byref(q)) == 7 + 33
+    q.save_steps = 4
+    assert lib.dfx_mlp_n_out(ctypes.byref(q)) == 6 + 25
+    # dense buffers all or none; step saves and dense output are forward only
+    zz = ctypes.c_void_p(8)
+    base = dict(solver=_lib.TSIT5, precision=_lib.F32, d=64, hidden=256, n_ts=32,
+                max_steps=4096, batch=8192)
+    part = _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, dense_ts=zz)
+    assert lib.dfx_mlp_solve(ctypes.byref(_mlp.MlpProblem(**base)), zz, zz, zz, zz,
+                             ctypes.byref(part), None) == -1
+    full = _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, grid=zz, ckpt=zz,
+                      dense_ts=zz, dense_ys=zz, dense_k=zz)
+    assert lib.dfx_mlp_solve(ctypes.byref(_mlp.MlpProblem(ckpt_every=64, **base)), zz, zz, zz,
+                             zz, ctypes.byref(full), None) == -1
+    steps_io = _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, grid=zz, ckpt=zz)
+    assert lib.dfx_mlp_solve(ctypes.byref(_mlp.MlpProblem(ckpt_every=64, save_steps=1, **base)),
+                             zz, zz, zz, zz, ctypes.byref(steps_io), None) == -1
+    dio = _lib.DenseIO(dense_ts=zz, dense_ys=zz, dense_k=zz, stats=zz, tq=zz, n_q=4,
+                       tq_stride=2, out=zz)
+    assert lib.dfx_mlp_dense_eval(ctypes.byref(_mlp.MlpProblem(**base)), ctypes.byref(dio),
+                                  None) == -1       # stride shorter than the query count
+    assert lib.dfx_mlp_dense_eval(ctypes.byref(_mlp.MlpProblem(**dict(base, d=32))),
+                                  ctypes.byref(dio), None) == -1
     adj = _mlp.MlpAdjIO()
     # segment past the last checkpoint, segment longer than the checkpoint spacing
     for seg, steps in ((64, 64), (0, 65)):

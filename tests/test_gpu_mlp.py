@@ -192,3 +192,67 @@ def test_wave_count_gives_the_same_bits(cuda, matmul):
     assert torch.equal(c[0], solve(4, dfx.PIDController(rtol=RTOL, atol=ATOL), 0.1)[0])
     with pytest.raises(ValueError, match="wavefronts"):
         solve(6, dfx.PIDController(rtol=RTOL, atol=ATOL), 0.1)
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+def test_mlp_dense_output_and_step_saves(cuda, matmul):
+    """SaveAt(dense=True) and SaveAt(steps=k) on the neural-ODE kernel.  Dense output
+    evaluated at the SaveAt(ts) times returns the saved rows bit for bit (same interpolant
+    arithmetic), at t0 returns y0, outside [t0, t1] NaN; its derivative at the knots equals
+    the vector field at the knot states to float32 MFMA accuracy (Tsit5's interpolant has
+    the FSAL stage as its end slopes), and in between to the interpolant's accuracy.
+    SaveAt(steps=k) rows are the accepted-step knots of the same solve (every k-th), in
+    time order with the ts rows, with the t1 row dropped when the last step was saved."""
+    W = mlp_weights(9)
+    B = 70
+    y0 = torch.tensor(np.random.default_rng(10).normal(size=(B, 64)), dtype=torch.float32,
+                      device=cuda)
+    ts = np.linspace(0.0, 1.0, 7)
+    term = dfx.ODETerm(_vf(W, matmul))
+    ctrl = dfx.PIDController(rtol=RTOL, atol=ATOL)
+    sol = dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0,
+                          saveat=dfx.SaveAt(ts=ts, dense=True), stepsize_controller=ctrl)
+    assert torch.equal(sol.evaluate(torch.tensor(ts)), sol.ys)
+    assert torch.equal(sol.evaluate(0.0), y0)
+    assert torch.isnan(sol.evaluate(1.5)).all() and torch.isnan(sol.evaluate(-0.1)).all()
+    per = torch.tensor(np.tile(ts[None, 2:5], (B, 1)))
+    assert torch.equal(sol.evaluate(per), sol.ys[:, 2:5])
+    plan = sol.interpolation.plan
+    n = int(sol.stats["num_accepted_steps"].max())
+    knots_t = plan.dense["dense_ts"][:, 1:n + 1]
+    knots_y = plan.dense["dense_ys"][:, 1:n + 1]
+    # derivative at the step ends (left: the step that ends there) = f(knot state)
+    from diffrax_amd._mlp import mlp_eval
+    acc = sol.stats["num_accepted_steps"]
+    dev = sol.derivative(plan.dense["dense_ts"][torch.arange(B), acc].reshape(B, 1),
+                         left=True)[:, 0]
+    f_end = mlp_eval(term.vector_field, plan.dense["dense_ys"][torch.arange(B), acc])
+    assert float((dev - f_end).abs().max()) < 2e-5
+    # in between: against the vector field at the interpolated state
+    mid = torch.tensor(np.tile([0.33, 0.61], (B, 1)))
+    dmid = sol.derivative(mid)
+    fmid = mlp_eval(term.vector_field, sol.evaluate(mid).reshape(-1, 64)).reshape(B, 2, 64)
+    assert float((dmid - fmid).abs().max()) < 1e-3
+    # SaveAt(steps=1): the knots; steps=2: every other one; combined with ts: time order
+    st = dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0,
+                         saveat=dfx.SaveAt(steps=True, t1=True), stepsize_controller=ctrl,
+                         max_steps=64)
+    assert torch.equal(st.stats["num_accepted_steps"], acc)
+    for b in range(B):
+        k = int(acc[b])
+        assert torch.equal(st.ts[b, :k], knots_t[b, :k])
+        assert torch.equal(st.ys[b, :k], knots_y[b, :k])
+        assert torch.isinf(st.ts[b, k:]).all()   # t1 was the last step's row
+    s2 = dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0,
+                         saveat=dfx.SaveAt(steps=2, ts=ts), stepsize_controller=ctrl,
+                         max_steps=64)
+    for b in range(B):
+        row = s2.ts[b][torch.isfinite(s2.ts[b])]
+        assert bool((row[1:] >= row[:-1]).all())
+        k = int(acc[b])
+        want = torch.cat([knots_t[b, 1:k:2], torch.tensor(ts, device=cuda)]).sort().values
+        assert torch.equal(row.sort().values, want)
+    with pytest.raises(NotImplementedError, match="gradients"):
+        y = y0.clone().requires_grad_(True)
+        dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y,
+                        saveat=dfx.SaveAt(ts=ts, dense=True), stepsize_controller=ctrl)
