@@ -115,7 +115,8 @@ def _key_mlp(plan):
     p = plan.prob
     name = torch.cuda.get_device_name(torch.cuda.current_device())
     pid = f"{p.pcoeff}/{p.icoeff}/{p.dcoeff}/{p.error_order}/{p.rtol!r}/{p.atol!r}"
-    return (f"mlp:{p.precision}:{pid}:{int(p.dt0 == p.dt0)}:{p.n_ts}/{p.save_t0}/{p.save_t1}:"
+    return (f"mlp:{p.precision}:{pid}:{int(p.dt0 == p.dt0)}:{p.n_ts}/{int(p.ts_stride != 0)}/"
+            f"{p.save_t0}/{p.save_t1}/{p.save_steps}:"
             f"{p.max_steps}:{p.ckpt_every}:{p.t0!r}/{p.t1!r}:{p.batch}:{name}")
 
 

@@ -30,7 +30,7 @@ class MlpProblem(ctypes.Structure):
                 ("dtmax", ctypes.c_double), ("factormin", ctypes.c_double),
                 ("factormax", ctypes.c_double), ("safety", ctypes.c_double),
                 ("error_order", ctypes.c_double), ("waves", ctypes.c_int32),
-                ("save_steps", ctypes.c_int32)]
+                ("save_steps", ctypes.c_int32), ("ts_stride", ctypes.c_int64)]
 
 
 class MlpAdjIO(ctypes.Structure):
@@ -109,8 +109,9 @@ class MlpPlan:
         W1, b1, W2, b2, W3, b3 = vfo.params
         self.packed, self.wf = pack_weights(W1, W2, W3, vfo.matmul, device)
         self.b = [b.detach().to(device=device, dtype=torch.float32).contiguous() for b in (b1, b2, b3)]
-        n_ts = 0 if ts is None else int(ts.numel())
+        n_ts = 0 if ts is None else int(ts.shape[-1])
         self.ts = (ts.to(device=device, dtype=torch.float64).contiguous() if ts is not None else None)
+        ts_stride = n_ts if ts is not None and ts.ndim == 2 else 0
         nan = float("nan")
         self.prob = MlpProblem(
             solver=_lib.TSIT5, precision=PRECISION[vfo.matmul], d=64, hidden=256, n_ts=n_ts,
@@ -124,7 +125,7 @@ class MlpPlan:
             dtmax=nan if ctrl.dtmax is None else float(ctrl.dtmax),
             factormin=ctrl.factormin, factormax=ctrl.factormax, safety=ctrl.safety,
             error_order=nan if ctrl.error_order is None else float(ctrl.error_order),
-            save_steps=int(sub.steps))
+            save_steps=int(sub.steps), ts_stride=ts_stride)
         self.n_out = int(lib.dfx_mlp_n_out(ctypes.byref(self.prob)))
         self.ys = torch.empty((B, self.n_out, 64), dtype=torch.float32, device=device)
         self.ts_out = torch.empty((B, self.n_out), dtype=torch.float64, device=device)
@@ -218,7 +219,13 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
     if sub.ts is not None:
         # device-resident ts are checked from a pinned copy once the solve is queued
         from ._integrate import _HostCopies
-        ts = torch.as_tensor(sub.ts, dtype=torch.float64).reshape(-1)
+        ts = torch.as_tensor(sub.ts, dtype=torch.float64)
+        if ts.ndim == 2 and ts.shape[0] == y0b.shape[0]:
+            pass   # per-trajectory save times [B, n_ts]
+        elif ts.ndim <= 1:
+            ts = ts.reshape(-1)
+        else:
+            raise ValueError(f"SaveAt(ts) must be [n] or [{y0b.shape[0]}, n]")
         host = _HostCopies()
         host.add("ts", ts)
         host.seal()
@@ -227,7 +234,7 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
         def check():
             dirn = 1.0 if t0f < t1f else -1.0
             tn = host.get("ts") * dirn
-            if len(tn) > 1 and (tn[1:] < tn[:-1]).any():
+            if tn.shape[-1] > 1 and (tn[..., 1:] < tn[..., :-1]).any():
                 raise RuntimeError("saveat.ts must be increasing or decreasing.")
             if ((tn > t1f * dirn) | (tn < t0f * dirn)).any():
                 raise RuntimeError("saveat.ts must lie between t0 and t1.")
@@ -514,14 +521,17 @@ def _split_dw(dw):
 
 class _MlpBacksolve(torch.autograd.Function):
     """ys of a neural-ODE solve with the continuous adjoint (dfx_mlp_backsolve) as its
-    backward."""
+    backward.  The backward solves restart from the returned ys, which are saved with
+    save_for_backward: an in-place change of them makes backward raise."""
 
     @staticmethod
     def forward(ctx, y0, W1, b1, W2, b2, W3, b3, cfg):
         params = _Params([t.detach() for t in (W1, b1, W2, b2, W3, b3)], cfg["matmul"])
         plan = MlpPlan(params, cfg["t0"], cfg["t1"], cfg["dt0"], y0.detach(), cfg["ts"],
                        cfg["ctrl"], cfg["sub"], cfg["max_steps"], 0, cfg["device"])
+        set_waves(plan, cfg.get("launch"))
         plan.launch()
+        ctx.save_for_backward(plan.ys)
         ctx.plan = plan
         ctx.back = backsolve_problem(plan, cfg["adjoint"])
         ctx.throw = cfg["throw"]
@@ -532,6 +542,7 @@ class _MlpBacksolve(torch.autograd.Function):
     @staticmethod
     def backward(ctx, ys_bar, *_):
         from ._solution import RESULTS
+        ctx.saved_tensors   # the version check of the saved ys
         ys_bar = torch.zeros_like(ctx.plan.ys) if ys_bar is None else ys_bar
         grads, result = backsolve_mlp(ctx.plan, ctx.back, ys_bar)
         if ctx.throw:

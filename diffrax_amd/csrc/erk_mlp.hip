@@ -157,6 +157,7 @@ int mlp_check(const dfx_mlp_problem* p) {
   if (p->solver != DFX_TSIT5) return DFX_E_ARG;  // dense output implemented for Tsit5
   if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0 || p->ckpt_every < 0) return DFX_E_ARG;
   if (p->waves != 0 && p->waves != 4 && p->waves != 8) return DFX_E_ARG;
+  if (p->ts_stride != 0 && p->ts_stride != p->n_ts) return DFX_E_ARG;
   return DFX_OK;
 }
 
@@ -260,6 +261,7 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   a.result = io->result; a.grid = io->grid; a.ckpt = io->ckpt;
   a.dense_ts = io->dense_ts; a.dense_ys = io->dense_ys; a.dense_k = io->dense_k;
   a.save_steps = p->save_steps;
+  a.ts_stride = p->ts_stride;
   a.B = p->batch; a.n_ts = p->n_ts; a.n_out = n_out; a.save_t0 = p->save_t0;
   a.save_t1 = p->save_t1; a.max_steps = p->max_steps; a.ckpt_every = p->ckpt_every;
   a.n_ckpt = dfx_mlp_n_ckpt(p);

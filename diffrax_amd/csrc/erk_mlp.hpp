@@ -34,7 +34,7 @@ struct MlpArgs {
   double* dense_ts;  // SaveAt(dense=True): [B, max_steps + 1], [B, max_steps + 1, 64],
   float* dense_ys;   // [B, max_steps, 7, 64] (or null)
   float* dense_k;
-  int64_t B;
+  int64_t B, ts_stride;  // ts_stride: 0 (ts shared) or n_ts (ts [B, n_ts])
   int32_t n_ts, n_out, save_t0, save_t1, max_steps, ckpt_every, n_ckpt, has_dt0, save_steps;
   double t0, t1, dt0, error_order;
   PidParams pid;
@@ -74,7 +74,9 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
   pid.inv = 1.0; pid.pinv = 1.0; pid.at_dtmin = 0;
   int32_t num_steps = 0, num_acc = 0, num_rej = 0, result = DFX_SUCCESSFUL;
   int32_t save_index = 0, ts_index = 0;
-  double next_ts = (a.n_ts > 0) ? a.ts[0] * dir : __builtin_inf();
+  // save times: shared (ts_stride 0) or this trajectory's row (padding rows read row 0)
+  const double* ts_row = a.ts ? a.ts + (valid ? traj : 0) * a.ts_stride : nullptr;
+  double next_ts = (a.n_ts > 0) ? ts_row[0] * dir : __builtin_inf();
   float* ys_row = valid ? a.ys + traj * (int64_t)a.n_out * MLP_D : nullptr;
   double* tso_row = valid ? a.ts_out + traj * (int64_t)a.n_out : nullptr;
   auto write_save = [&](double tn, const float (&v)[CPT]) {
@@ -262,7 +264,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
           }
           write_save(next_ts, v);
           ++ts_index;
-          next_ts = (ts_index < a.n_ts) ? a.ts[ts_index] * dir : __builtin_inf();
+          next_ts = (ts_index < a.n_ts) ? ts_row[ts_index] * dir : __builtin_inf();
         }
 #pragma unroll
         for (int j = 0; j < CPT; ++j) { y[j] = z[j]; fsal[j] = f[j]; }

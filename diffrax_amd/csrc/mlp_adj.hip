@@ -46,7 +46,7 @@ struct AdjArgs {
   const float* ys_bar;
   float* ybar;
   float* zs;
-  int64_t B, B_pad;
+  int64_t B, B_pad, ts_stride;
   int32_t max_steps, n_ckpt, C, Cs, seg, n_ts, n_out, save_t0;
   double t0, t1;
 };
@@ -94,6 +94,7 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
   int steps_here = nacc - n0;
   steps_here = steps_here < 0 ? 0 : (steps_here > a.Cs ? a.Cs : steps_here);
   const int dir = (a.t0 < a.t1) ? 1 : -1;
+  const double* ts_row = a.ts ? a.ts + (valid ? traj : 0) * a.ts_stride : nullptr;
   using TL = TabLit<Tab, float>;  // float32 stage algebra: the coefficients as literals
   float* fs_tile = a.zs + (int64_t)tile * a.Cs * S * MLP_TILE * MLP_D;
   auto fslot = [&](int q, int i) { return fs_tile + (((int64_t)q * S + i) * MLP_TILE + orow) * MLP_D + c0; };
@@ -190,12 +191,12 @@ mlp_adj_segment_kernel(const AdjArgs a, const MlpWeights<P> W, const RowBufs<P> 
       // SaveAt(ts) values interpolated inside this step
       int jlo = 0, jhi = 0;
       for (int s = 0; s < a.n_ts; ++s) {
-        const double tsn = a.ts[s] * dir;
+        const double tsn = ts_row[s] * dir;
         if (n > 0 && tsn <= ta) jlo = s + 1;
         if (tsn <= tb) jhi = s + 1;
       }
       for (int s = jlo; s < jhi; ++s) {
-        const double th = linear_rescale(ta, a.ts[s] * dir, tb);
+        const double th = linear_rescale(ta, ts_row[s] * dir, tb);
         double bb[7];
         tsit5_weights(th, bb);
         const float* g = a.ys_bar + (traj * a.n_out + (a.save_t0 ? 1 : 0) + s) * MLP_D + c0;
@@ -328,6 +329,7 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
   int steps_here = nacc - n0;
   steps_here = steps_here < 0 ? 0 : (steps_here > a.Cs ? a.Cs : steps_here);
   const int dir = (a.t0 < a.t1) ? 1 : -1;
+  const double* ts_row = a.ts ? a.ts + (valid ? traj : 0) * a.ts_stride : nullptr;
   using TL = TabLit<Tab, float>;
   float* fs_tile = a.zs + (int64_t)tile * a.Cs * S * MLP_TILE * MLP_D;
   auto fslot = [&](int q, int i) { return fs_tile + (((int64_t)q * S + i) * MLP_TILE + orow) * MLP_D + c0; };
@@ -426,12 +428,12 @@ mlp_adj_segment3_kernel(const AdjArgs a, const MlpWeights<PolicyBF16x3> W,
       }
       int jlo = 0, jhi = 0;
       for (int s = 0; s < a.n_ts; ++s) {
-        const double tsn = a.ts[s] * dir;
+        const double tsn = ts_row[s] * dir;
         if (n > 0 && tsn <= ta) jlo = s + 1;
         if (tsn <= tb) jhi = s + 1;
       }
       for (int s = jlo; s < jhi; ++s) {
-        const double th = linear_rescale(ta, a.ts[s] * dir, tb);
+        const double th = linear_rescale(ta, ts_row[s] * dir, tb);
         double bb[7];
         tsit5_weights(th, bb);
         const float* g = a.ys_bar + (traj * a.n_out + (a.save_t0 ? 1 : 0) + s) * MLP_D + c0;
@@ -1058,7 +1060,7 @@ int run_segment(Kern kern, int threads, const dfx_mlp_problem* p, const void* pa
   a.ybar = io->ybar; a.zs = io->zs;
   a.B = p->batch; a.B_pad = padded_batch(p->batch);
   a.max_steps = p->max_steps; a.n_ckpt = dfx_mlp_n_ckpt(p); a.C = p->ckpt_every;
-  a.Cs = seg_steps; a.seg = seg; a.n_ts = p->n_ts;
+  a.Cs = seg_steps; a.seg = seg; a.n_ts = p->n_ts; a.ts_stride = p->ts_stride;
   a.n_out = (p->save_t0 ? 1 : 0) + p->n_ts + (p->save_t1 ? 1 : 0);
   a.save_t0 = p->save_t0; a.t0 = p->t0; a.t1 = p->t1;
   const MlpWeights<P> W = mlp_weights_from<P>(packed, P::PIECES, b1, b2, b3);
@@ -1205,6 +1207,7 @@ int dfx_mlp_adj_segment(const dfx_mlp_problem* p, const void* packed, const floa
       !io->partial || !io->dw)
     return DFX_E_ARG;
   if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
+  if (p->ts_stride != 0 && p->ts_stride != p->n_ts) return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   // eight waves per tile, two per SIMD: at four the kernels held 415 (float32) and 512 + 93

@@ -348,11 +348,13 @@ typedef struct dfx_mlp_problem {
                                effect on the results, which are the same bits)          */
   int32_t save_steps;       /* SaveAt(steps=k): k > 0 also saves every k-th accepted step
                                (rows in time order with the ts rows; forward only)      */
+  int64_t ts_stride;        /* 0: ts[n_ts] shared by the batch; n_ts: ts[B, n_ts] per
+                               trajectory (a vmapped SaveAt(ts))                        */
 } dfx_mlp_problem;
 
 typedef struct dfx_mlp_io {
   const float* y0;          /* [B, 64]                                                   */
-  const double* ts;         /* [n_ts]                                                    */
+  const double* ts;         /* [n_ts], or [B, n_ts] with ts_stride = n_ts                */
   float* ys;                /* out [B, n_out, 64]                                        */
   double* ts_out;           /* out [B, n_out]                                            */
   int32_t* stats;           /* out [B, 3]                                                */

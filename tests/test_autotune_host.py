@@ -52,7 +52,8 @@ def test_mlp_key_separates_kernel_variants(monkeypatch):
     variants = [_mlp_plan(precision=_lib.BF16_F32), _mlp_plan(pcoeff=0.3),
                 _mlp_plan(dt0=float("nan")), _mlp_plan(n_ts=8), _mlp_plan(save_t1=1),
                 _mlp_plan(ckpt_every=64), _mlp_plan(t1=2.0), _mlp_plan(batch=1024),
-                _mlp_plan(rtol=1e-6), _mlp_plan(max_steps=100)]
+                _mlp_plan(rtol=1e-6), _mlp_plan(max_steps=100), _mlp_plan(ts_stride=32),
+                _mlp_plan(save_steps=2)]
     keys = [_autotune._key_mlp(v) for v in variants]
     assert base not in keys and len(set(keys)) == len(keys)
     assert _autotune._key_mlp(_mlp_plan(waves=8)) == base

@@ -256,3 +256,51 @@ def test_mlp_dense_output_and_step_saves(cuda, matmul):
         y = y0.clone().requires_grad_(True)
         dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y,
                         saveat=dfx.SaveAt(ts=ts, dense=True), stepsize_controller=ctrl)
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+def test_per_trajectory_save_times(cuda, matmul):
+    """SaveAt(ts=[B, n]) on the neural-ODE kernel (a vmapped SaveAt(ts)): every trajectory of
+    a tile is integrated independently, so each row equals the solve of that trajectory
+    alone with its own ts, bit for bit, forward and y0 gradient (discrete adjoint and
+    BacksolveAdjoint); the weight gradients, summed over trajectories, to float32
+    summation order."""
+    W = mlp_weights(12)
+    B = 40
+    rng = np.random.default_rng(13)
+    y0 = torch.tensor(rng.normal(size=(B, 64)), dtype=torch.float32, device=cuda)
+    ts = np.sort(rng.uniform(0.0, 1.0, size=(B, 6)), axis=1)
+    ts[:, -1] = 1.0
+    ctrl = dfx.PIDController(rtol=RTOL, atol=ATOL)
+    P = [torch.tensor(w, dtype=torch.float32, device=cuda) for w in W]
+    w_out = torch.tensor(rng.normal(size=(B, 6, 64)), dtype=torch.float32, device=cuda)
+
+    def run(y, t, adjoint, rows):
+        Pg = [p.clone().requires_grad_(True) for p in P]
+        yg = y.clone().requires_grad_(True)
+        sol = dfx.diffeqsolve(dfx.ODETerm(dfx.MLPField(*Pg, matmul=matmul)), dfx.Tsit5(), 0.0,
+                              1.0, 0.1, yg, saveat=dfx.SaveAt(ts=t), stepsize_controller=ctrl,
+                              adjoint=adjoint, max_steps=256)
+        (sol.ys * w_out[rows]).sum().backward()
+        return sol.ys.detach(), yg.grad, [p.grad for p in Pg]
+
+    back = dfx.BacksolveAdjoint(stepsize_controller=dfx.PIDController(
+        rtol=1e-6, atol=1e-8, norm=dfx.adjoint_rms_seminorm))
+    for adjoint in (dfx.RecursiveCheckpointAdjoint(), back):
+        ys, gy, gw = run(y0, torch.tensor(ts, device=cuda), adjoint, slice(None))
+        tot = [torch.zeros_like(p) for p in P]
+        for b in range(B):
+            ysb, gyb, gwb = run(y0[b:b + 1], torch.tensor(ts[b:b + 1], device=cuda), adjoint,
+                                slice(b, b + 1))
+            assert torch.equal(ysb, ys[b:b + 1]) and torch.equal(gyb, gy[b:b + 1])
+            tot = [t + g for t, g in zip(tot, gwb)]
+        for t, g in zip(tot, gw):
+            assert float((t - g).abs().max()) <= 1e-5 * float(g.abs().max())
+    # every row the same as the shared ts: the shared solve's bits
+    shared = dfx.diffeqsolve(dfx.ODETerm(dfx.MLPField(*P, matmul=matmul)), dfx.Tsit5(), 0.0,
+                             1.0, 0.1, y0, saveat=dfx.SaveAt(ts=ts[0]),
+                             stepsize_controller=ctrl)
+    rows = dfx.diffeqsolve(dfx.ODETerm(dfx.MLPField(*P, matmul=matmul)), dfx.Tsit5(), 0.0,
+                           1.0, 0.1, y0, saveat=dfx.SaveAt(ts=np.tile(ts[:1], (B, 1))),
+                           stepsize_controller=ctrl)
+    assert torch.equal(shared.ys, rows.ys) and torch.equal(shared.ts, rows.ts)

@@ -139,6 +139,12 @@ def test_backsolve_guards(cuda):
     with pytest.raises(NotImplementedError, match="Tsit5"):
         sol, _, _ = _solve(W, y0, [0.5, 1.0], bad)
         sol.ys.sum().backward()
+    # the backward solves restart from the returned ys: editing them in place raises
+    sol, _, _ = _solve(W, y0, [0.5, 1.0], _seminorm_adjoint())
+    loss = sol.ys.sum()
+    sol.ys.detach().mul_(2.0)   # shares storage and version counter with sol.ys
+    with pytest.raises(RuntimeError, match="modified by an inplace operation"):
+        loss.backward()
 
 
 def test_backsolve_launch_split(cuda):
