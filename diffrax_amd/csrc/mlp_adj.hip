@@ -552,82 +552,101 @@ __host__ __device__ inline DwLayout dw_layout() {
   return l;
 }
 
-// One 32x32 output tile of one chunk of ADJ_CHUNK rows: partial[chunk][tile] = A^T B over
-// the chunk (A = deltas, B = layer inputs), in groups of 8 k-steps of fragments held in two
-// register buffers (the next group loads while the current one feeds the MFMAs).  The row buffers are allocated in whole chunks and rows past the segment's last are
-// zero, so every chunk runs its full length.  Tiles in the first tile column of a matrix
-// also produce the chunk's column sums of A (the bias gradient partials, partial_db:
-// [chunk][db1 | db2 | db3]).
 // Launch order is XCD-aware: workgroups are dealt to the 8 XCDs round-robin by linear id,
-// so block b runs on XCD b % 8; XCD x takes chunks x, x + 8, ... and runs the 96 tiles of a
+// so block b runs on XCD b % 8; XCD x takes chunks x, x + 8, ... and runs the blocks of a
 // chunk back to back, which then read the chunk's rows through one L2 instead of eight.
 constexpr int kXcds = 8;
 
+// partial[chunk][tile] = A^T B over one chunk of ADJ_CHUNK rows (A = deltas, B = layer
+// inputs) for every 32 x 32 output tile, each wave on a 2 x 2 block of tiles (a 64 x 64
+// piece of one matrix): per k-step it loads two delta and two input fragments for four
+// MFMAs (a wave per tile, measured before, loaded twice as much per MFMA and took the
+// backward of 8,192 trajectories from 2.84 to 2.91 ms, the backsolve from 19.0 to 19.6 ms,
+// with the same bits).  The fragments come in groups of 4 k-steps held in two register
+// buffers (the next group loads while the current one feeds the MFMAs).  The row buffers
+// are allocated in whole chunks and rows past the segment's last are zero, so every chunk
+// runs its full length.  The 96 tiles of a chunk form 24 blocks (dW1 4 x 1, dW2 4 x 4,
+// dW3 1 x 4); blocks in the first block column of a matrix also produce the chunk's column
+// sums of A (the bias gradient partials, partial_db: [chunk][db1 | db2 | db3]).
 // SC: the delta rows are multiplied by rscale[row] as they load (the backsolve's per-row
 // weights dt * b_i, zero for rejected steps and idle rows; a row of zero weight is dropped
 // from both operands, so a non-finite value in it contributes nothing).  Float32 rows only.
+__host__ __device__ inline int dw_blocks_of(int m) {
+  const DwLayout l = dw_layout();
+  return (l.n1[m] / 64) * (l.n2[m] / 64);
+}
+
 template <class P, bool SC = false>
 __global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R,
-                                                         float* __restrict__ partial,
-                                                         float* __restrict__ partial_db,
-                                                         int nch,
-                                                         const float* __restrict__ rscale = nullptr) {
+                                                           float* __restrict__ partial,
+                                                           float* __restrict__ partial_db,
+                                                           int nch,
+                                                           const float* __restrict__ rscale = nullptr) {
   static_assert(!SC || P::KA == 1, "row scales need one row per fragment element");
   const DwLayout lay = dw_layout();
-  const int ntile = lay.first[3];
+  const int nblk = dw_blocks_of(0) + dw_blocks_of(1) + dw_blocks_of(2);
   const int xcd = (int)(blockIdx.x % kXcds), slot = (int)(blockIdx.x / kXcds);
-  const int chunk = (slot / ntile) * kXcds + xcd;
-  const int tg = slot - (slot / ntile) * ntile;
+  const int chunk = (slot / nblk) * kXcds + xcd;
+  int bg = slot - (slot / nblk) * nblk;
   if (chunk >= nch) return;
   int m = 0;
-  while (tg >= lay.first[m + 1]) ++m;
-  const int t = tg - lay.first[m];
-  const int ntj = lay.n2[m] / 32;
-  const int ti = t / ntj, tj = t - ti * ntj;
+  while (bg >= dw_blocks_of(m)) { bg -= dw_blocks_of(m); ++m; }
+  const int nbj = lay.n2[m] / 64;
+  const int bi = bg / nbj, bj = bg - bi * nbj;
   const typename P::act_t* A = R.b[2 * m];       // deltas (width n1)
   const typename P::act_t* Bm = R.b[2 * m + 1];  // layer inputs (width n2)
   const int WA = lay.n1[m], WB = lay.n2[m];
   const int lane = threadIdx.x;
-  constexpr int KA = P::KA, KS = P::KSTEP, U = 8;
+  constexpr int KA = P::KA, KS = P::KSTEP, U = 4;
   static_assert(ADJ_CHUNK % (KS * U) == 0, "chunk must hold whole k-step groups");
-  const bool col_sums = (tj == 0);
+  const bool col_sums = (bj == 0);
   const int64_t r_begin = (int64_t)chunk * ADJ_CHUNK;
-  floatx16_t acc = floatx16_t{};
-  float csum = 0.0f;
-  const int ca = ti * 32 + (lane & 31), cb = tj * 32 + (lane & 31);
+  floatx16_t acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x) { acc[x][0] = floatx16_t{}; acc[x][1] = floatx16_t{}; }
+  float csum[2] = {0.0f, 0.0f};
+  const int ca = bi * 64 + (lane & 31), cb = bj * 64 + (lane & 31);
   using frag_t = typename P::frag_t;
-  auto load_group = [&](int64_t r0, frag_t (&fa)[U], frag_t (&fb)[U]) {
+  auto load_group = [&](int64_t r0, frag_t (&fa)[U][2], frag_t (&fb)[U][2]) {
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t rb = (r0 + u * KS) / KA + (lane >> 5);
-      fa[u] = P::load_a(A + (rb * WA + ca) * KA);
-      fb[u] = P::load_a(Bm + (rb * WB + cb) * KA);
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        fa[u][x] = P::load_a(A + (rb * WA + ca + 32 * x) * KA);
+        fb[u][x] = P::load_a(Bm + (rb * WB + cb + 32 * x) * KA);
+      }
       if constexpr (SC) {
         const float w = rscale[rb];
-        fa[u] = (w != 0.0f) ? fa[u] * w : 0.0f;
-        fb[u] = (w != 0.0f) ? fb[u] : 0.0f;
-      }
-    }
-  };
-  auto mma_group = [&](const frag_t (&fa)[U], const frag_t (&fb)[U]) {
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      acc = P::mfma(fa[u], fb[u], acc);
-      if (col_sums) {
-        if constexpr (KA == 1) {
-          csum += fa[u];
-        } else {
-#pragma unroll
-          for (int j = 0; j < KA; ++j) csum += (float)fa[u][j];
+        for (int x = 0; x < 2; ++x) {
+          fa[u][x] = (w != 0.0f) ? fa[u][x] * w : 0.0f;
+          fb[u][x] = (w != 0.0f) ? fb[u][x] : 0.0f;
         }
       }
     }
   };
-  // Two register buffers: group g + 1 is in flight while group g feeds the MFMAs (the last
-  // pass re-loads the final group instead of branching).
+  auto mma_group = [&](const frag_t (&fa)[U][2], const frag_t (&fb)[U][2]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        acc[x][0] = P::mfma(fa[u][x], fb[u][0], acc[x][0]);
+        acc[x][1] = P::mfma(fa[u][x], fb[u][1], acc[x][1]);
+        if (col_sums) {
+          if constexpr (KA == 1) {
+            csum[x] += fa[u][x];
+          } else {
+#pragma unroll
+            for (int j = 0; j < KA; ++j) csum[x] += (float)fa[u][x][j];
+          }
+        }
+      }
+    }
+  };
   constexpr int G = ADJ_CHUNK / (KS * U);
   static_assert(G % 2 == 0, "groups are processed in pairs");
-  frag_t a0[U], b0[U], a1[U], b1[U];
+  frag_t a0[U][2], b0[U][2], a1[U][2], b1[U][2];
   load_group(r_begin, a0, b0);
   for (int g = 0; g < G; g += 2) {
     load_group(r_begin + (int64_t)(g + 1) * KS * U, a1, b1);
@@ -635,13 +654,24 @@ __global__ void __launch_bounds__(64) mlp_dw_gemm_kernel(const RowBufs<P> R,
     load_group(r_begin + (int64_t)min(g + 2, G - 1) * KS * U, a0, b0);
     mma_group(a1, b1);
   }
-  float* o = partial + ((int64_t)chunk * lay.first[3] + tg) * 1024 + lane * 16;
+  const int ntj = lay.n2[m] / 32;
 #pragma unroll
-  for (int v = 0; v < 16; ++v) o[v] = acc[v];
+  for (int x = 0; x < 2; ++x) {
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int tg = lay.first[m] + (2 * bi + x) * ntj + (2 * bj + y);
+      float* o = partial + ((int64_t)chunk * lay.first[3] + tg) * 1024 + lane * 16;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) o[v] = acc[x][y][v];
+    }
+  }
   if (col_sums) {
-    csum += __shfl_xor(csum, 32);
     const int off = (m == 0) ? 0 : (m == 1 ? MLP_H : 2 * MLP_H);
-    if (lane < 32) partial_db[(int64_t)chunk * (2 * MLP_H + MLP_D) + off + ca] = csum;
+#pragma unroll
+    for (int x = 0; x < 2; ++x) {
+      const float c = csum[x] + __shfl_xor(csum[x], 32);
+      if (lane < 32) partial_db[(int64_t)chunk * (2 * MLP_H + MLP_D) + off + ca + 32 * x] = c;
+    }
   }
 }
 
@@ -1046,6 +1076,16 @@ __global__ void bs_reset_kernel(int32_t* counters, float* rscale, int64_t used, 
   for (int64_t r = used + threadIdx.x; r < nrow; r += blockDim.x) rscale[r] = 0.0f;
 }
 
+// The weight-gradient GEMM over nch chunks of float32 rows (SC: per-row weights rscale).
+template <bool SC>
+void launch_dw_gemm(const RowBufs<PolicyF32>& R, float* partial, float* partial_db, int nch,
+                    const float* rscale, hipStream_t s) {
+  const int groups = (nch + kXcds - 1) / kXcds;
+  const int nblk = dw_blocks_of(0) + dw_blocks_of(1) + dw_blocks_of(2);
+  hipLaunchKernelGGL((mlp_dw_gemm_kernel<PolicyF32, SC>), dim3(kXcds * nblk * groups), dim3(64),
+                     0, s, R, partial, partial_db, nch, rscale);
+}
+
 // One segment: the recompute + sweep kernel `kern` with the weights of policy P (float32:
 // mlp_adj_segment_kernel on a float32 buffer; the three-piece bfloat16 form:
 // mlp_adj_segment3_kernel on the three split3 planes of a bf16 buffer), then the float32
@@ -1078,9 +1118,7 @@ int run_segment(Kern kern, int threads, const dfx_mlp_problem* p, const void* pa
   const DwLayout lay = dw_layout();
   float* partial = io->partial;
   float* partial_db = io->partial + (int64_t)nch * lay.first[3] * 1024;
-  const unsigned gemm_blocks = (unsigned)(kXcds * lay.first[3] * ((nch + kXcds - 1) / kXcds));
-  hipLaunchKernelGGL((mlp_dw_gemm_kernel<PolicyF32>), dim3(gemm_blocks), dim3(64), 0, s, R,
-                     partial, partial_db, nch, (const float*)nullptr);
+  launch_dw_gemm<false>(R, partial, partial_db, nch, nullptr, s);
   const int nout = MLP_H * MLP_D + MLP_H * MLP_H + MLP_D * MLP_H + 2 * MLP_H + MLP_D;
   hipLaunchKernelGGL(mlp_dw_reduce_kernel, dim3((nout + 255) / 256), dim3(256), 0, s, partial,
                      partial_db, nch, io->dw);
@@ -1175,9 +1213,7 @@ int dfx_mlp_backsolve(const dfx_mlp_problem* back, const void* packed, const flo
   const DwLayout lay = dw_layout();
   float* partial = io->partial;
   float* partial_db = io->partial + (int64_t)nch * lay.first[3] * 1024;
-  const unsigned gemm_blocks = (unsigned)(kXcds * lay.first[3] * ((nch + kXcds - 1) / kXcds));
-  hipLaunchKernelGGL((mlp_dw_gemm_kernel<P, true>), dim3(gemm_blocks), dim3(64), 0, s, R, partial,
-                     partial_db, nch, (const float*)a.rscale);
+  launch_dw_gemm<true>(R, partial, partial_db, nch, a.rscale, s);
   const int nout = MLP_H * MLP_D + MLP_H * MLP_H + MLP_D * MLP_H + 2 * MLP_H + MLP_D;
   hipLaunchKernelGGL(mlp_dw_reduce_kernel, dim3((nout + 255) / 256), dim3(256), 0, s, partial,
                      partial_db, nch, io->dw);
