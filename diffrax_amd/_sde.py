@@ -16,7 +16,8 @@ class SdeProblem(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("d", ctypes.c_int32),
                 ("paths", ctypes.c_int64), ("num_steps", ctypes.c_int32),
                 ("checkpoint_every", ctypes.c_int32), ("save_t0", ctypes.c_int32),
-                ("save_t1", ctypes.c_int32)]
+                ("save_t1", ctypes.c_int32), ("save_steps", ctypes.c_int32),
+                ("reserved_", ctypes.c_int32)]
 
 
 class SdeIO(ctypes.Structure):
@@ -30,6 +31,7 @@ class SdeGradIO(ctypes.Structure):
 
 
 _SIGS = {
+    "dfx_sde_n_out": ([ctypes.POINTER(SdeProblem)], ctypes.c_int),
     "dfx_sde_n_ckpt": ([ctypes.POINTER(SdeProblem)], ctypes.c_int),
     "dfx_sde_workspace_bytes": ([ctypes.POINTER(SdeProblem), ctypes.POINTER(ctypes.c_size_t)],
                                 ctypes.c_int),
@@ -80,16 +82,18 @@ def _ck_for(num_steps, checkpoints):
 
 
 class _Sde:
-    def __init__(self, y0, theta, mu, sigma, dW, grid, save_t0, save_t1, need_grad, checkpoints):
+    def __init__(self, y0, theta, mu, sigma, dW, grid, save_t0, save_t1, need_grad, checkpoints,
+                 save_steps=0):
         lib = _lib.load()
         self.P, self.d = y0.shape
         self.N = grid.numel() - 1
         prec = _lib.F64 if y0.dtype == torch.float64 else _lib.F32
         ck = _ck_for(self.N, checkpoints) if need_grad else 0
         self.prob = SdeProblem(precision=prec, d=self.d, paths=self.P, num_steps=self.N,
-                               checkpoint_every=ck, save_t0=int(save_t0), save_t1=int(save_t1))
+                               checkpoint_every=ck, save_t0=int(save_t0), save_t1=int(save_t1),
+                               save_steps=int(save_steps))
         n_ck = lib.dfx_sde_n_ckpt(ctypes.byref(self.prob)) if need_grad else 0
-        self.n_out = int(save_t0) + int(save_t1)
+        self.n_out = int(lib.dfx_sde_n_out(ctypes.byref(self.prob)))
         self.ys = torch.empty((self.P, self.n_out, self.d), dtype=y0.dtype, device=y0.device)
         self.ckpt = (torch.empty((max(n_ck, 1), self.P, self.d), dtype=y0.dtype, device=y0.device)
                      if need_grad else None)
@@ -133,10 +137,11 @@ class _Sde:
 
 class _SdeFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, y0, theta, mu, sigma, dW, grid, save_t0, save_t1, checkpoints):
+    def forward(ctx, y0, theta, mu, sigma, dW, grid, save_t0, save_t1, checkpoints,
+                save_steps=0):
         need = any(ctx.needs_input_grad[:4])
         s = _Sde(y0.detach(), theta.detach(), mu.detach(), sigma.detach(), dW, grid, save_t0,
-                 save_t1, need, checkpoints)
+                 save_t1, need, checkpoints, save_steps)
         s.forward()
         ctx.sde = s
         ctx.param_dtypes = (theta.dtype, mu.dtype, sigma.dtype)
@@ -149,7 +154,8 @@ class _SdeFunction(torch.autograd.Function):
         dt, dm, ds = ctx.param_dtypes
         return (y0_bar, thb.to(dt) if ctx.needs_input_grad[1] else None,
                 mub.to(dm) if ctx.needs_input_grad[2] else None,
-                sgb.to(ds) if ctx.needs_input_grad[3] else None, None, None, None, None, None)
+                sgb.to(ds) if ctx.needs_input_grad[3] else None, None, None, None, None, None,
+                None)
 
 
 def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller, adjoint,
@@ -162,8 +168,8 @@ def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
     if not isinstance(stepsize_controller, ConstantStepSize):
         raise NotImplementedError("the SDE path implements ConstantStepSize only")
     sub = saveat.subs
-    if sub is None or sub.ts is not None or sub.steps:
-        raise NotImplementedError("the SDE path saves SaveAt(t0=..., t1=...) only")
+    if sub is None or sub.ts is not None or saveat.dense:
+        raise NotImplementedError("the SDE path saves SaveAt(t0=..., t1=..., steps=...)")
     if dt0 is None:
         raise ValueError("Constant step size solvers cannot select step size automatically; "
                          "please pass a value for `dt0`.")
@@ -192,8 +198,13 @@ def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
     dW = dW.to(device).contiguous()
     grid = torch.tensor(grid_np, dtype=torch.float64, device=device)
     ckpts = adjoint.checkpoints if hasattr(adjoint, "checkpoints") else None
-    ys = _SdeFunction.apply(y0b, theta, mu, sigma, dW, grid, bool(sub.t0), bool(sub.t1), ckpts)
-    ts_list = ([t0f] if sub.t0 else []) + ([t1f] if sub.t1 else [])
+    k = int(sub.steps)
+    ys = _SdeFunction.apply(y0b, theta, mu, sigma, dW, grid, bool(sub.t0), bool(sub.t1), ckpts,
+                            k)
+    # rows: t0, the state after every k-th step, t1 unless the last step was saved
+    ts_list = ([t0f] if sub.t0 else []) + ([float(g) for g in grid_np[k::k]] if k else [])
+    if sub.t1 and not (k and N % k == 0):
+        ts_list.append(t1f)
     ts = torch.tensor(ts_list, dtype=torch.float64, device=device).expand(P, len(ts_list))
     if not batched:
         ys, ts = ys[0], ts[0]

@@ -49,6 +49,7 @@ int sde_check(const dfx_sde_problem* p) {
   if (p->d <= 0 || p->d > 256 || p->paths < 0 || p->num_steps < 0) return DFX_E_ARG;
   const int V = (p->precision == DFX_F32) ? 4 : 2;
   if (p->d % V != 0) return DFX_E_ARG;
+  if (p->save_steps < 0) return DFX_E_ARG;
   if (kBlock % (p->d / V) != 0) return DFX_E_ARG;
   return DFX_OK;
 }
@@ -87,6 +88,14 @@ int static_scratch(void** out) {
 
 extern "C" {
 
+int dfx_sde_n_out(const dfx_sde_problem* p) {
+  if (!p || p->save_steps < 0 || p->num_steps < 0) return DFX_E_ARG;
+  int n = (p->save_t0 ? 1 : 0);
+  if (p->save_steps > 0) n += p->num_steps / p->save_steps;
+  if (p->save_t1 && !(p->save_steps > 0 && p->num_steps % p->save_steps == 0)) n += 1;
+  return n;
+}
+
 int dfx_sde_n_ckpt(const dfx_sde_problem* p) {
   if (!p || p->checkpoint_every <= 0) return 0;
   return (p->num_steps + p->checkpoint_every - 1) / p->checkpoint_every - 1;
@@ -104,7 +113,7 @@ int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* str
   if (rc != DFX_OK) return rc;
   if (!io || !io->y0 || !io->theta || !io->mu || !io->sigma || !io->grid) return DFX_E_ARG;
   if (p->num_steps > 0 && !io->dW) return DFX_E_ARG;
-  if ((p->save_t0 || p->save_t1) && !io->ys) return DFX_E_ARG;
+  if (dfx_sde_n_out(p) > 0 && !io->ys) return DFX_E_ARG;
   if (io->ckpt && p->checkpoint_every <= 0) return DFX_E_ARG;
   if (p->paths == 0) return DFX_OK;
   SdeArgs a;
@@ -112,8 +121,8 @@ int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* str
   a.y0 = io->y0; a.theta = io->theta; a.mu = io->mu; a.sigma = io->sigma;
   a.dW = io->dW; a.grid = io->grid; a.ys = io->ys; a.ckpt = io->ckpt;
   a.P = p->paths; a.d = p->d; a.N = p->num_steps; a.CK = p->checkpoint_every > 0 ? p->checkpoint_every : 1;
-  a.n_out = (p->save_t0 ? 1 : 0) + (p->save_t1 ? 1 : 0);
-  a.save_t0 = p->save_t0; a.save_t1 = p->save_t1;
+  a.n_out = dfx_sde_n_out(p);
+  a.save_t0 = p->save_t0; a.save_t1 = p->save_t1; a.save_steps = p->save_steps;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if ((p->precision == DFX_F32 ? sde_launch_fwd_f32(a, s) : sde_launch_fwd_f64(a, s)) != 0)
     return DFX_E_HIP;
@@ -159,8 +168,8 @@ int sde_adjoint_impl(const dfx_sde_problem* p, const dfx_sde_io* io, const dfx_s
   a.dW = io->dW; a.grid = io->grid; a.ckpt = io->ckpt;
   a.ys_bar = g->ys_bar; a.y0_bar = g->y0_bar;
   a.P = p->paths; a.d = p->d; a.N = p->num_steps; a.CK = p->checkpoint_every;
-  a.n_out = (p->save_t0 ? 1 : 0) + (p->save_t1 ? 1 : 0);
-  a.save_t0 = p->save_t0; a.save_t1 = p->save_t1;
+  a.n_out = dfx_sde_n_out(p);
+  a.save_t0 = p->save_t0; a.save_t1 = p->save_t1; a.save_steps = p->save_steps;
   const int64_t nb = (sde_threads(p, 1) + kBlock - 1) / kBlock;
   a.partial = reinterpret_cast<double*>(workspace);
   const size_t lds = (size_t)3 * kBlock * sizeof(double);

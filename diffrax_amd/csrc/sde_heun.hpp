@@ -46,7 +46,17 @@ struct SdeArgs {
   double* partial;        // [n_blocks, 3, d]
   int64_t P;
   int32_t d, N, CK, n_out, save_t0, save_t1;
+  int32_t save_steps;     // SaveAt(steps=k): rows save_t0 + n / k - 1 hold y after step n
 };
+
+// Row of the t1 save, and whether there is one: SaveAt(steps=k) already saved the last
+// state when k divides N (the _save_t1 rule, _integrate.py:847-877).
+__host__ __device__ inline int sde_t1_row(const SdeArgs& a) {
+  return a.save_t0 + (a.save_steps > 0 ? a.N / a.save_steps : 0);
+}
+__host__ __device__ inline bool sde_has_t1_row(const SdeArgs& a) {
+  return a.save_t1 && !(a.save_steps > 0 && a.N % a.save_steps == 0);
+}
 
 template <class T, int V>
 struct Vec {
@@ -106,6 +116,7 @@ __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
   const float* dwp = a.dW + p * a.d + j0;
   T* ck = reinterpret_cast<T*>(a.ckpt);
   int next_ck = a.CK;  // step count at which the next checkpoint is due
+  int next_save = a.save_steps, save_row = a.save_t0;  // SaveAt(steps=k)
   for (int n0 = 0; n0 < a.N; n0 += U) {
     Vec<T, V> w[U];
 #pragma unroll
@@ -119,6 +130,13 @@ __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
         const T dt = (T)(a.grid[n + 1] - a.grid[n]);
 #pragma unroll
         for (int v = 0; v < V; ++v) y[v] = heun_ou<T>(y[v], dt, th[v], mu[v], sg[v], w[u].v[v]);
+        if (a.save_steps && n + 1 == next_save) {
+          T* o = ys + (int64_t)save_row * a.d;
+#pragma unroll
+          for (int v = 0; v < V; ++v) o[v] = y[v];
+          ++save_row;
+          next_save += a.save_steps;
+        }
         if (ck && n + 1 == next_ck) {
           if (n + 1 < a.N) {
             T* c = ck + ((int64_t)(next_ck / a.CK - 1) * a.P + p) * a.d + j0;
@@ -130,8 +148,8 @@ __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
       }
     }
   }
-  if (a.save_t1) {
-    T* o = ys + (a.save_t0 ? a.d : 0);
+  if (sde_has_t1_row(a)) {
+    T* o = ys + (int64_t)sde_t1_row(a) * a.d;
 #pragma unroll
     for (int v = 0; v < V; ++v) o[v] = y[v];
   }
@@ -160,10 +178,13 @@ __device__ __forceinline__ T heun_ou_vjp(T y, T dt, T th, T mu, T sg, T w, T yb,
 // One checkpoint segment of the adjoint for one component: recompute the cnt (<= CK) steps
 // from the segment's initial state y keeping every step's start state and increment, then
 // walk them backwards.  FULL = (cnt == CK) drops the per-step bounds tests.
+// With SaveAt(steps=k) the cotangent of each saved state (ysb rows) joins yb before the
+// step that produced it is reversed.
 template <class T, int CK, bool FULL>
 __device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, int64_t slab,
                                             int cnt, const T* dts, T th, T mu, T sg, T& yb,
-                                            double& thb, double& mub, double& sgb) {
+                                            double& thb, double& mub, double& sgb,
+                                            const SdeArgs& a, const T* ysb, int nbeg) {
   T yh[CK], w[CK];
 #pragma unroll
   for (int c = 0; c < CK; ++c)
@@ -176,7 +197,13 @@ __device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, i
   T sth = T(0), smu = T(0), ssg = T(0);
 #pragma unroll
   for (int c = CK - 1; c >= 0; --c)
-    if (FULL || c < cnt) yb = heun_ou_vjp<T>(yh[c], dts[c], th, mu, sg, w[c], yb, sth, smu, ssg);
+    if (FULL || c < cnt) {
+      if (a.save_steps) {
+        const int n1 = nbeg + c + 1;  // steps done after this one
+        if (n1 % a.save_steps == 0) yb = yb + ysb[(int64_t)(a.save_t0 + n1 / a.save_steps - 1) * a.d];
+      }
+      yb = heun_ou_vjp<T>(yh[c], dts[c], th, mu, sg, w[c], yb, sth, smu, ssg);
+    }
   thb += (double)sth;
   mub += (double)th * (double)smu;
   sgb += (double)ssg;
@@ -195,7 +222,7 @@ __global__ void __launch_bounds__(256) sde_heun_adj_kernel(const SdeArgs a) {
   const T mu = reinterpret_cast<const T*>(a.mu)[j];
   const T sg = reinterpret_cast<const T*>(a.sigma)[j];
   const T* ysb = reinterpret_cast<const T*>(a.ys_bar) + p * (int64_t)a.n_out * a.d + j;
-  T yb = (live && a.save_t1) ? ysb[a.save_t0 ? a.d : 0] : T(0);
+  T yb = (live && sde_has_t1_row(a)) ? ysb[(int64_t)sde_t1_row(a) * a.d] : T(0);
   const int64_t slab = a.P * a.d;
   const float* dwp = a.dW + (live ? tid : 0);
   const T* ck = reinterpret_cast<const T*>(a.ckpt);
@@ -212,10 +239,10 @@ __global__ void __launch_bounds__(256) sde_heun_adj_kernel(const SdeArgs a) {
       const T y = (s == 0) ? y0[tid] : ck[(int64_t)(s - 1) * slab + tid];
       if (cnt == CK)
         adj_segment<T, CK, true>(y, dwp + (int64_t)nbeg * slab, slab, CK, dts, th, mu, sg, yb,
-                                 thb, mub, sgb);
+                                 thb, mub, sgb, a, ysb, nbeg);
       else
         adj_segment<T, CK, false>(y, dwp + (int64_t)nbeg * slab, slab, cnt, dts, th, mu, sg,
-                                  yb, thb, mub, sgb);
+                                  yb, thb, mub, sgb, a, ysb, nbeg);
     }
   }
   if (live) {

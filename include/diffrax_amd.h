@@ -273,6 +273,9 @@ typedef struct dfx_sde_problem {
   int32_t checkpoint_every; /* CK: states stored every CK steps for the adjoint (8/16/32),
                                0 when no gradient will be taken                          */
   int32_t save_t0, save_t1; /* SaveAt(t0=..., t1=...)                                    */
+  int32_t save_steps;       /* SaveAt(steps=k): k > 0 also saves y after every k-th step;
+                               the t1 row is dropped when k divides num_steps            */
+  int32_t reserved_;        /* 0 */
 } dfx_sde_problem;
 
 typedef struct dfx_sde_io {
@@ -282,7 +285,7 @@ typedef struct dfx_sde_io {
   const void* sigma;        /* [d]                                                       */
   const float* dW;          /* [N, P, d] float32 Brownian increments (step-major)         */
   const double* grid;       /* [N + 1] step times of ConstantStepSize (host-computed)     */
-  void* ys;                 /* out [P, save_t0 + save_t1, d]                              */
+  void* ys;                 /* out [P, dfx_sde_n_out, d]: t0, every k-th step, t1         */
   void* ckpt;               /* out [dfx_sde_n_ckpt, P, d] states at steps CK, 2CK, ...
                                (NULL when no gradient is needed)                          */
 } dfx_sde_io;
@@ -295,6 +298,9 @@ typedef struct dfx_sde_grad_io {
   double* sigma_bar;        /* out [d]                                                     */
 } dfx_sde_grad_io;
 
+/* Saved rows per path: save_t0 + num_steps / save_steps (when save_steps > 0) + save_t1
+ * unless save_steps divides num_steps (the allocation rule of _integrate.py:1272-1305). */
+int dfx_sde_n_out(const dfx_sde_problem* p);
 /* Number of stored checkpoints: ceil(N / CK) - 1. */
 int dfx_sde_n_ckpt(const dfx_sde_problem* p);
 /* Workspace of dfx_sde_heun_adjoint (per-block partial sums), bytes. */

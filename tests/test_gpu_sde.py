@@ -143,3 +143,51 @@ def test_sde_adjoint_f32_vs_f64(cuda):
         rel = np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))
         print(f"f32 adjoint {name}: rel err {rel:.3e}")
         assert rel < 1e-4, name
+
+
+@pytest.mark.parametrize("k", [1, 7, 10])
+def test_sde_step_saves_forward_and_adjoint(cuda, k):
+    """SaveAt(t0=True, steps=k, t1=True) on the SDE path: the rows (t0, the state after every
+    k-th step, t1 unless k divides the step count) equal the FP64 oracle's bit for bit, and
+    the adjoint with cotangents on every row matches torch float64 autograd through the
+    fixed-grid replay to 1e-10."""
+    P, d, N = 24, 16, 100
+    theta, mu, sigma, dW, y0 = _inputs(P=P, d=d, N=N, seed=21)
+    dt0 = 1.0 / N
+    dev = torch.device("cuda", 0)
+    y0t = torch.tensor(y0, dtype=torch.float64, device=dev, requires_grad=True)
+    th = torch.tensor(theta, dtype=torch.float64, device=dev, requires_grad=True)
+    m = torch.tensor(mu, dtype=torch.float64, device=dev, requires_grad=True)
+    sg = torch.tensor(sigma, dtype=torch.float64, device=dev, requires_grad=True)
+    terms = dfx.MultiTerm(dfx.ODETerm(dfx.OUDrift()),
+                          dfx.ControlTerm(dfx.DiagonalNoise(), dfx.IncrementPath(torch.tensor(dW))))
+    sol = dfx.diffeqsolve(terms, dfx.Heun(), 0.0, 1.0, dt0, y0t, (th, m, sg),
+                          saveat=dfx.SaveAt(t0=True, steps=k, t1=True),
+                          stepsize_controller=dfx.ConstantStepSize())
+    n_rows = 1 + N // k + (0 if N % k == 0 else 1)
+    assert tuple(sol.ys.shape) == (P, n_rows, d) and tuple(sol.ts.shape) == (P, n_rows)
+    q = np.concatenate([theta, mu, sigma])
+    ref = oracle.solve(y0, 1.0, q, dW=dW.astype(np.float64), solver="heun", vf="ou",
+                       controller="constant", dt0=dt0, save_t0=True, save_t1=True,
+                       save_steps=k, max_steps=N)
+    assert np.array_equal(sol.ys.detach().cpu().numpy(), ref["ys"][:, :n_rows])
+    assert np.array_equal(sol.ts.cpu().numpy(), ref["ts"][:, :n_rows])
+    w = np.random.default_rng(22).normal(size=(P, n_rows, d))
+    (sol.ys * torch.tensor(w, device=dev)).sum().backward()
+    from diffrax_amd._sde import constant_grid
+    grid = constant_grid(0.0, 1.0, dt0)
+    steps = list(zip(grid[:-1], grid[1:]))
+    Y0 = torch.tensor(y0, dtype=torch.float64, requires_grad=True)
+    TH, MU, SG = (torch.tensor(v, dtype=torch.float64, requires_grad=True)
+                  for v in (theta, mu, sigma))
+    dWt = torch.tensor(dW.astype(np.float64))
+    out = pyoracle.replay(lambda t, y: TH * (MU - y), "heun", steps, Y0, [],
+                          g_dw=lambda n, t, y: SG * dWt[n], save_t1=N % k != 0, save_steps=k)
+    rows = torch.stack([Y0] + out, dim=1)
+    assert rows.shape[1] == n_rows
+    (rows * torch.tensor(w)).sum().backward()
+    for name, got, want in (("y0", y0t.grad, Y0.grad), ("theta", th.grad, TH.grad),
+                            ("mu", m.grad, MU.grad), ("sigma", sg.grad, SG.grad)):
+        got, want = got.detach().cpu().numpy(), want.numpy()
+        rel = np.max(np.abs(got - want)) / np.max(np.abs(want))
+        assert rel < 1e-10, (name, rel)
