@@ -180,7 +180,7 @@ __device__ __forceinline__ T heun_ou_vjp(T y, T dt, T th, T mu, T sg, T w, T yb,
 // walk them backwards.  FULL = (cnt == CK) drops the per-step bounds tests.
 // With SaveAt(steps=k) the cotangent of each saved state (ysb rows) joins yb before the
 // step that produced it is reversed.
-template <class T, int CK, bool FULL>
+template <class T, int CK, bool FULL, bool STEPS>
 __device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, int64_t slab,
                                             int cnt, const T* dts, T th, T mu, T sg, T& yb,
                                             double& thb, double& mub, double& sgb,
@@ -198,7 +198,7 @@ __device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, i
 #pragma unroll
   for (int c = CK - 1; c >= 0; --c)
     if (FULL || c < cnt) {
-      if (a.save_steps) {
+      if (STEPS) {
         const int n1 = nbeg + c + 1;  // steps done after this one
         if (n1 % a.save_steps == 0) yb = yb + ysb[(int64_t)(a.save_t0 + n1 / a.save_steps - 1) * a.d];
       }
@@ -209,7 +209,9 @@ __device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, i
   sgb += (double)ssg;
 }
 
-template <class T, int CK>
+// STEPS: SaveAt(steps=k) rows carry cotangents (a build of its own, so the plain adjoint's
+// unrolled segment loop has no per-step test).
+template <class T, int CK, bool STEPS>
 __global__ void __launch_bounds__(256) sde_heun_adj_kernel(const SdeArgs a) {
   __shared__ T dts[CK];
   extern __shared__ double red[];
@@ -238,11 +240,11 @@ __global__ void __launch_bounds__(256) sde_heun_adj_kernel(const SdeArgs a) {
     if (live) {
       const T y = (s == 0) ? y0[tid] : ck[(int64_t)(s - 1) * slab + tid];
       if (cnt == CK)
-        adj_segment<T, CK, true>(y, dwp + (int64_t)nbeg * slab, slab, CK, dts, th, mu, sg, yb,
-                                 thb, mub, sgb, a, ysb, nbeg);
+        adj_segment<T, CK, true, STEPS>(y, dwp + (int64_t)nbeg * slab, slab, CK, dts, th, mu,
+                                        sg, yb, thb, mub, sgb, a, ysb, nbeg);
       else
-        adj_segment<T, CK, false>(y, dwp + (int64_t)nbeg * slab, slab, cnt, dts, th, mu, sg,
-                                  yb, thb, mub, sgb, a, ysb, nbeg);
+        adj_segment<T, CK, false, STEPS>(y, dwp + (int64_t)nbeg * slab, slab, cnt, dts, th, mu,
+                                         sg, yb, thb, mub, sgb, a, ysb, nbeg);
     }
   }
   if (live) {

@@ -10,12 +10,16 @@ int sde_launch_fwd_f64(const SdeArgs& a, hipStream_t s) {
 }
 
 int sde_launch_adj_f64(const SdeArgs& a, int64_t nb, size_t lds, hipStream_t s) {
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kSdeBlock), lds, s, a);
+  };
+  const bool steps = a.save_steps > 0;
   if (a.CK == 32)
-    hipLaunchKernelGGL((sde_heun_adj_kernel<double, 32>), dim3((unsigned)nb), dim3(kSdeBlock), lds, s, a);
+    steps ? go(sde_heun_adj_kernel<double, 32, true>) : go(sde_heun_adj_kernel<double, 32, false>);
   else if (a.CK == 16)
-    hipLaunchKernelGGL((sde_heun_adj_kernel<double, 16>), dim3((unsigned)nb), dim3(kSdeBlock), lds, s, a);
+    steps ? go(sde_heun_adj_kernel<double, 16, true>) : go(sde_heun_adj_kernel<double, 16, false>);
   else
-    hipLaunchKernelGGL((sde_heun_adj_kernel<double, 8>), dim3((unsigned)nb), dim3(kSdeBlock), lds, s, a);
+    steps ? go(sde_heun_adj_kernel<double, 8, true>) : go(sde_heun_adj_kernel<double, 8, false>);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
