@@ -17,12 +17,12 @@ class SdeProblem(ctypes.Structure):
                 ("paths", ctypes.c_int64), ("num_steps", ctypes.c_int32),
                 ("checkpoint_every", ctypes.c_int32), ("save_t0", ctypes.c_int32),
                 ("save_t1", ctypes.c_int32), ("save_steps", ctypes.c_int32),
-                ("reserved_", ctypes.c_int32)]
+                ("n_ts", ctypes.c_int32)]
 
 
 class SdeIO(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
-                ("y0", "theta", "mu", "sigma", "dW", "grid", "ys", "ckpt")]
+                ("y0", "theta", "mu", "sigma", "dW", "grid", "ys", "ckpt", "ts")]
 
 
 class SdeGradIO(ctypes.Structure):
@@ -83,7 +83,7 @@ def _ck_for(num_steps, checkpoints):
 
 class _Sde:
     def __init__(self, y0, theta, mu, sigma, dW, grid, save_t0, save_t1, need_grad, checkpoints,
-                 save_steps=0):
+                 save_steps=0, ts=None):
         lib = _lib.load()
         self.P, self.d = y0.shape
         self.N = grid.numel() - 1
@@ -91,7 +91,8 @@ class _Sde:
         ck = _ck_for(self.N, checkpoints) if need_grad else 0
         self.prob = SdeProblem(precision=prec, d=self.d, paths=self.P, num_steps=self.N,
                                checkpoint_every=ck, save_t0=int(save_t0), save_t1=int(save_t1),
-                               save_steps=int(save_steps))
+                               save_steps=int(save_steps),
+                               n_ts=0 if ts is None else int(ts.numel()))
         n_ck = lib.dfx_sde_n_ckpt(ctypes.byref(self.prob)) if need_grad else 0
         self.n_out = int(lib.dfx_sde_n_out(ctypes.byref(self.prob)))
         self.ys = torch.empty((self.P, self.n_out, self.d), dtype=y0.dtype, device=y0.device)
@@ -103,6 +104,8 @@ class _Sde:
             setattr(io, n, v.data_ptr())
         io.ys = self.ys.data_ptr()
         io.ckpt = self.ckpt.data_ptr() if self.ckpt is not None else None
+        self.ts = ts
+        io.ts = ts.data_ptr() if ts is not None and ts.numel() else None
         self.io = io
 
     def forward(self):
@@ -138,10 +141,10 @@ class _Sde:
 class _SdeFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, y0, theta, mu, sigma, dW, grid, save_t0, save_t1, checkpoints,
-                save_steps=0):
+                save_steps=0, ts=None):
         need = any(ctx.needs_input_grad[:4])
         s = _Sde(y0.detach(), theta.detach(), mu.detach(), sigma.detach(), dW, grid, save_t0,
-                 save_t1, need, checkpoints, save_steps)
+                 save_t1, need, checkpoints, save_steps, ts)
         s.forward()
         ctx.sde = s
         ctx.param_dtypes = (theta.dtype, mu.dtype, sigma.dtype)
@@ -155,7 +158,7 @@ class _SdeFunction(torch.autograd.Function):
         return (y0_bar, thb.to(dt) if ctx.needs_input_grad[1] else None,
                 mub.to(dm) if ctx.needs_input_grad[2] else None,
                 sgb.to(ds) if ctx.needs_input_grad[3] else None, None, None, None, None, None,
-                None)
+                None, None)
 
 
 def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller, adjoint,
@@ -168,8 +171,8 @@ def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
     if not isinstance(stepsize_controller, ConstantStepSize):
         raise NotImplementedError("the SDE path implements ConstantStepSize only")
     sub = saveat.subs
-    if sub is None or sub.ts is not None or saveat.dense:
-        raise NotImplementedError("the SDE path saves SaveAt(t0=..., t1=..., steps=...)")
+    if sub is None or saveat.dense:
+        raise NotImplementedError("the SDE path saves SaveAt(t0=..., t1=..., ts=..., steps=...)")
     if dt0 is None:
         raise ValueError("Constant step size solvers cannot select step size automatically; "
                          "please pass a value for `dt0`.")
@@ -199,10 +202,28 @@ def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
     grid = torch.tensor(grid_np, dtype=torch.float64, device=device)
     ckpts = adjoint.checkpoints if hasattr(adjoint, "checkpoints") else None
     k = int(sub.steps)
+    ts_np = None
+    ts_dev = None
+    if sub.ts is not None:
+        ts_np = np.asarray(torch.as_tensor(sub.ts, dtype=torch.float64).cpu().numpy()).reshape(-1)
+        if ts_np.size > 1 and (np.diff(ts_np) < 0).any():
+            raise RuntimeError("saveat.ts must be increasing (the SDE path runs forward).")
+        if ts_np.size and (ts_np[0] < t0f or ts_np[-1] > t1f):
+            raise RuntimeError("saveat.ts must lie between t0 and t1.")
+        ts_dev = torch.tensor(ts_np, dtype=torch.float64, device=device)
     ys = _SdeFunction.apply(y0b, theta, mu, sigma, dW, grid, bool(sub.t0), bool(sub.t1), ckpts,
-                            k)
-    # rows: t0, the state after every k-th step, t1 unless the last step was saved
-    ts_list = ([t0f] if sub.t0 else []) + ([float(g) for g in grid_np[k::k]] if k else [])
+                            k, ts_dev)
+    # rows: t0; per step, the ts inside it and then its end when k divides the step count;
+    # t1 unless the last step was saved (the kernels' order)
+    ts_list = [t0f] if sub.t0 else []
+    i = 0
+    n_ts = 0 if ts_np is None else ts_np.size
+    for n in range(N):
+        while i < n_ts and ts_np[i] <= grid_np[n + 1]:
+            ts_list.append(float(ts_np[i]))
+            i += 1
+        if k and (n + 1) % k == 0:
+            ts_list.append(float(grid_np[n + 1]))
     if sub.t1 and not (k and N % k == 0):
         ts_list.append(t1f)
     ts = torch.tensor(ts_list, dtype=torch.float64, device=device).expand(P, len(ts_list))

@@ -49,7 +49,7 @@ int sde_check(const dfx_sde_problem* p) {
   if (p->d <= 0 || p->d > 256 || p->paths < 0 || p->num_steps < 0) return DFX_E_ARG;
   const int V = (p->precision == DFX_F32) ? 4 : 2;
   if (p->d % V != 0) return DFX_E_ARG;
-  if (p->save_steps < 0) return DFX_E_ARG;
+  if (p->save_steps < 0 || p->n_ts < 0) return DFX_E_ARG;
   if (kBlock % (p->d / V) != 0) return DFX_E_ARG;
   return DFX_OK;
 }
@@ -90,7 +90,8 @@ extern "C" {
 
 int dfx_sde_n_out(const dfx_sde_problem* p) {
   if (!p || p->save_steps < 0 || p->num_steps < 0) return DFX_E_ARG;
-  int n = (p->save_t0 ? 1 : 0);
+  if (p->n_ts < 0) return DFX_E_ARG;
+  int n = (p->save_t0 ? 1 : 0) + p->n_ts;
   if (p->save_steps > 0) n += p->num_steps / p->save_steps;
   if (p->save_t1 && !(p->save_steps > 0 && p->num_steps % p->save_steps == 0)) n += 1;
   return n;
@@ -114,6 +115,7 @@ int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* str
   if (!io || !io->y0 || !io->theta || !io->mu || !io->sigma || !io->grid) return DFX_E_ARG;
   if (p->num_steps > 0 && !io->dW) return DFX_E_ARG;
   if (dfx_sde_n_out(p) > 0 && !io->ys) return DFX_E_ARG;
+  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
   if (io->ckpt && p->checkpoint_every <= 0) return DFX_E_ARG;
   if (p->paths == 0) return DFX_OK;
   SdeArgs a;
@@ -123,6 +125,7 @@ int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* str
   a.P = p->paths; a.d = p->d; a.N = p->num_steps; a.CK = p->checkpoint_every > 0 ? p->checkpoint_every : 1;
   a.n_out = dfx_sde_n_out(p);
   a.save_t0 = p->save_t0; a.save_t1 = p->save_t1; a.save_steps = p->save_steps;
+  a.n_ts = p->n_ts; a.ts = io->ts;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if ((p->precision == DFX_F32 ? sde_launch_fwd_f32(a, s) : sde_launch_fwd_f64(a, s)) != 0)
     return DFX_E_HIP;
@@ -159,6 +162,7 @@ int sde_adjoint_impl(const dfx_sde_problem* p, const dfx_sde_io* io, const dfx_s
   if (p->checkpoint_every != 32 && p->checkpoint_every != 16 && p->checkpoint_every != 8)
     return DFX_E_ARG;
   if (dfx_sde_n_ckpt(p) > 0 && !io->ckpt) return DFX_E_ARG;
+  if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
   size_t need = 0;
   dfx_sde_workspace_bytes(p, &need);
   if (!workspace || workspace_bytes < need) return DFX_E_WORKSPACE;
@@ -170,6 +174,7 @@ int sde_adjoint_impl(const dfx_sde_problem* p, const dfx_sde_io* io, const dfx_s
   a.P = p->paths; a.d = p->d; a.N = p->num_steps; a.CK = p->checkpoint_every;
   a.n_out = dfx_sde_n_out(p);
   a.save_t0 = p->save_t0; a.save_t1 = p->save_t1; a.save_steps = p->save_steps;
+  a.n_ts = p->n_ts; a.ts = io->ts;
   const int64_t nb = (sde_threads(p, 1) + kBlock - 1) / kBlock;
   a.partial = reinterpret_cast<double*>(workspace);
   const size_t lds = (size_t)3 * kBlock * sizeof(double);

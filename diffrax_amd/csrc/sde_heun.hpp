@@ -30,6 +30,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "erk_common.hpp"
+
 namespace dfx {
 
 struct SdeArgs {
@@ -46,13 +48,20 @@ struct SdeArgs {
   double* partial;        // [n_blocks, 3, d]
   int64_t P;
   int32_t d, N, CK, n_out, save_t0, save_t1;
-  int32_t save_steps;     // SaveAt(steps=k): rows save_t0 + n / k - 1 hold y after step n
+  int32_t save_steps;     // SaveAt(steps=k): y after every k-th step
+  int32_t n_ts;           // SaveAt(ts): ascending times in [t0, t1], shared by the paths
+  const double* ts;       // [n_ts]
 };
+
+// Saved rows in time order: t0; then per step n the ts inside it (t_n < ts <= t_{n+1}, or
+// ts >= t0 for the first), then y_{n+1} when k divides n + 1; then t1 unless k divides N.
+// The ts with index i, inside step n, is row save_t0 + i + n / k; the step row of step n is
+// save_t0 + (number of ts <= t_{n+1}) + (n + 1) / k - 1.
 
 // Row of the t1 save, and whether there is one: SaveAt(steps=k) already saved the last
 // state when k divides N (the _save_t1 rule, _integrate.py:847-877).
 __host__ __device__ inline int sde_t1_row(const SdeArgs& a) {
-  return a.save_t0 + (a.save_steps > 0 ? a.N / a.save_steps : 0);
+  return a.save_t0 + a.n_ts + (a.save_steps > 0 ? a.N / a.save_steps : 0);
 }
 __host__ __device__ inline bool sde_has_t1_row(const SdeArgs& a) {
   return a.save_t1 && !(a.save_steps > 0 && a.N % a.save_steps == 0);
@@ -93,7 +102,28 @@ __device__ __forceinline__ T heun_ou(T y, T dt, T th, T mu, T sg, T w) {
   return y + (T(0.5) * k0 + T(0.5) * k1);
 }
 
-template <class T, int V, int U>
+// The same step, also returning its two stage increments (the interpolant's end slopes).
+template <class T>
+__device__ __forceinline__ T heun_ou_k(T y, T dt, T th, T mu, T sg, T w, T& k0, T& k1) {
+  k0 = dt * (th * (mu - y)) + sg * w;
+  const T z = y + k0;
+  k1 = dt * (th * (mu - z)) + sg * w;
+  return y + (T(0.5) * k0 + T(0.5) * k1);
+}
+
+// Heun's dense output inside a step at theta (the third-order Hermite polynomial from the
+// end values and the stage increments, ThirdOrderHermitePolynomialInterpolation.from_k,
+// _local_interpolation.py:50-139; the small-system kernel's erk_interp in the same form).
+template <class T>
+__device__ __forceinline__ T heun_interp(T y0, T y1, T k0, T k1, double th) {
+  const T tt = T(th);
+  const T ca = k0 + k1 + T(2) * y0 - T(2) * y1;
+  const T cb = T(-2) * k0 - k1 - T(3) * y0 + T(3) * y1;
+  return ((ca * tt + cb) * tt + k0) * tt + y0;
+}
+
+// SAVES: rows between t0 and t1 (SaveAt(ts) or SaveAt(steps)), a build of its own.
+template <class T, int V, int U, bool SAVES>
 __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
   const int groups = a.d / V;
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -116,7 +146,7 @@ __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
   const float* dwp = a.dW + p * a.d + j0;
   T* ck = reinterpret_cast<T*>(a.ckpt);
   int next_ck = a.CK;  // step count at which the next checkpoint is due
-  int next_save = a.save_steps, save_row = a.save_t0;  // SaveAt(steps=k)
+  int next_save = a.save_steps, save_row = a.save_t0, ts_idx = 0;  // SaveAt(steps=k, ts)
   for (int n0 = 0; n0 < a.N; n0 += U) {
     Vec<T, V> w[U];
 #pragma unroll
@@ -128,9 +158,27 @@ __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
       const int n = n0 + u;
       if (n < a.N) {
         const T dt = (T)(a.grid[n + 1] - a.grid[n]);
+        if constexpr (SAVES) {
+          T k0[V], k1[V], y1[V];
 #pragma unroll
-        for (int v = 0; v < V; ++v) y[v] = heun_ou<T>(y[v], dt, th[v], mu[v], sg[v], w[u].v[v]);
-        if (a.save_steps && n + 1 == next_save) {
+          for (int v = 0; v < V; ++v)
+            y1[v] = heun_ou_k<T>(y[v], dt, th[v], mu[v], sg[v], w[u].v[v], k0[v], k1[v]);
+          const double tn = a.grid[n], tn1 = a.grid[n + 1];
+          while (ts_idx < a.n_ts && a.ts[ts_idx] <= tn1) {
+            const double thv = linear_rescale(tn, a.ts[ts_idx], tn1);
+            T* o = ys + (int64_t)save_row * a.d;
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = heun_interp<T>(y[v], y1[v], k0[v], k1[v], thv);
+            ++save_row;
+            ++ts_idx;
+          }
+#pragma unroll
+          for (int v = 0; v < V; ++v) y[v] = y1[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < V; ++v) y[v] = heun_ou<T>(y[v], dt, th[v], mu[v], sg[v], w[u].v[v]);
+        }
+        if (SAVES && a.save_steps && n + 1 == next_save) {
           T* o = ys + (int64_t)save_row * a.d;
 #pragma unroll
           for (int v = 0; v < V; ++v) o[v] = y[v];
@@ -175,16 +223,37 @@ __device__ __forceinline__ T heun_ou_vjp(T y, T dt, T th, T mu, T sg, T w, T yb,
   return (yb + zb) - c * k0b;    // k0 = dt*(th*(mu - y)) + sg*w
 }
 
+// heun_ou_vjp with further cotangents on the step's start state and stage increments
+// (from the SaveAt(ts) rows interpolated inside the step): yb0 on y, kb0 on k0, kb1 on k1.
+template <class T>
+__device__ __forceinline__ T heun_ou_vjp_k(T y, T dt, T th, T mu, T sg, T w, T yb, T yb0,
+                                           T kb0, T kb1, T& sth, T& smu, T& ssg) {
+  const T k0 = dt * (th * (mu - y)) + sg * w;
+  const T z = y + k0;
+  const T c = dt * th;
+  const T k1b = T(0.5) * yb + kb1;
+  const T zb = -(c * k1b);
+  const T k0b = (T(0.5) * yb + kb0) + zb;
+  const T s = k0b + k1b;
+  sth += dt * ((mu - z) * k1b + (mu - y) * k0b);
+  smu += dt * s;
+  ssg += w * s;
+  return ((yb + yb0) + zb) - c * k0b;
+}
+
 // One checkpoint segment of the adjoint for one component: recompute the cnt (<= CK) steps
 // from the segment's initial state y keeping every step's start state and increment, then
 // walk them backwards.  FULL = (cnt == CK) drops the per-step bounds tests.
-// With SaveAt(steps=k) the cotangent of each saved state (ysb rows) joins yb before the
-// step that produced it is reversed.
-template <class T, int CK, bool FULL, bool STEPS>
+// With saved rows between t0 and t1 (SAVES) the cotangent of each step row joins yb
+// before the step that produced it is reversed, and each ts row inside the step passes
+// its cotangent to the step's end values and stage increments through the interpolant's
+// theta polynomials.  ts_hi: number of ts <= the segment's end time (updated to its start).
+template <class T, int CK, bool FULL, bool SAVES>
 __device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, int64_t slab,
                                             int cnt, const T* dts, T th, T mu, T sg, T& yb,
                                             double& thb, double& mub, double& sgb,
-                                            const SdeArgs& a, const T* ysb, int nbeg) {
+                                            const SdeArgs& a, const T* ysb, int nbeg,
+                                            int& ts_hi) {
   T yh[CK], w[CK];
 #pragma unroll
   for (int c = 0; c < CK; ++c)
@@ -198,20 +267,45 @@ __device__ __forceinline__ void adj_segment(T y, const float* __restrict__ dw, i
 #pragma unroll
   for (int c = CK - 1; c >= 0; --c)
     if (FULL || c < cnt) {
-      if (STEPS) {
-        const int n1 = nbeg + c + 1;  // steps done after this one
-        if (n1 % a.save_steps == 0) yb = yb + ysb[(int64_t)(a.save_t0 + n1 / a.save_steps - 1) * a.d];
+      if constexpr (SAVES) {
+        const int n = nbeg + c;  // this step; n + 1 steps are done after it
+        const int k = a.save_steps;
+        if (k && (n + 1) % k == 0)
+          yb = yb + ysb[(int64_t)(a.save_t0 + ts_hi + (n + 1) / k - 1) * a.d];
+        const double tn = a.grid[n], tn1 = a.grid[n + 1];
+        int lo = ts_hi;
+        while (lo > 0 && (n == 0 ? true : a.ts[lo - 1] > tn)) --lo;
+        if (lo < ts_hi) {
+          T k0, k1;
+          (void)heun_ou_k<T>(yh[c], dts[c], th, mu, sg, w[c], k0, k1);
+          T yb0 = T(0), kb0 = T(0), kb1 = T(0);
+          for (int i = lo; i < ts_hi; ++i) {
+            const T g = ysb[(int64_t)(a.save_t0 + i + (k ? n / k : 0)) * a.d];
+            const T tt = T(linear_rescale(tn, a.ts[i], tn1));
+            const T t2 = tt * tt, t3 = t2 * tt;
+            yb = yb + (T(3) * t2 - T(2) * t3) * g;
+            yb0 = yb0 + ((T(1) - T(3) * t2) + T(2) * t3) * g;
+            kb0 = kb0 + ((tt - T(2) * t2) + t3) * g;
+            kb1 = kb1 + (t3 - t2) * g;
+          }
+          ts_hi = lo;
+          yb = heun_ou_vjp_k<T>(yh[c], dts[c], th, mu, sg, w[c], yb, yb0, kb0, kb1, sth, smu,
+                                ssg);
+        } else {
+          yb = heun_ou_vjp<T>(yh[c], dts[c], th, mu, sg, w[c], yb, sth, smu, ssg);
+        }
+      } else {
+        yb = heun_ou_vjp<T>(yh[c], dts[c], th, mu, sg, w[c], yb, sth, smu, ssg);
       }
-      yb = heun_ou_vjp<T>(yh[c], dts[c], th, mu, sg, w[c], yb, sth, smu, ssg);
     }
   thb += (double)sth;
   mub += (double)th * (double)smu;
   sgb += (double)ssg;
 }
 
-// STEPS: SaveAt(steps=k) rows carry cotangents (a build of its own, so the plain adjoint's
-// unrolled segment loop has no per-step test).
-template <class T, int CK, bool STEPS>
+// SAVES: rows between t0 and t1 (SaveAt(ts) or SaveAt(steps=k)) carry cotangents (a build
+// of its own, so the plain adjoint's unrolled segment loop has no per-step test).
+template <class T, int CK, bool SAVES>
 __global__ void __launch_bounds__(256) sde_heun_adj_kernel(const SdeArgs a) {
   __shared__ T dts[CK];
   extern __shared__ double red[];
@@ -230,6 +324,7 @@ __global__ void __launch_bounds__(256) sde_heun_adj_kernel(const SdeArgs a) {
   const T* ck = reinterpret_cast<const T*>(a.ckpt);
   const T* y0 = reinterpret_cast<const T*>(a.y0);
   const int nseg = (a.N + CK - 1) / CK;
+  int ts_hi = a.n_ts;  // all ts are <= t1
   for (int s = nseg - 1; s >= 0; --s) {
     const int nbeg = s * CK;
     const int cnt = min(CK, a.N - nbeg);
@@ -240,11 +335,11 @@ __global__ void __launch_bounds__(256) sde_heun_adj_kernel(const SdeArgs a) {
     if (live) {
       const T y = (s == 0) ? y0[tid] : ck[(int64_t)(s - 1) * slab + tid];
       if (cnt == CK)
-        adj_segment<T, CK, true, STEPS>(y, dwp + (int64_t)nbeg * slab, slab, CK, dts, th, mu,
-                                        sg, yb, thb, mub, sgb, a, ysb, nbeg);
+        adj_segment<T, CK, true, SAVES>(y, dwp + (int64_t)nbeg * slab, slab, CK, dts, th, mu,
+                                        sg, yb, thb, mub, sgb, a, ysb, nbeg, ts_hi);
       else
-        adj_segment<T, CK, false, STEPS>(y, dwp + (int64_t)nbeg * slab, slab, cnt, dts, th, mu,
-                                         sg, yb, thb, mub, sgb, a, ysb, nbeg);
+        adj_segment<T, CK, false, SAVES>(y, dwp + (int64_t)nbeg * slab, slab, cnt, dts, th, mu,
+                                         sg, yb, thb, mub, sgb, a, ysb, nbeg, ts_hi);
     }
   }
   if (live) {

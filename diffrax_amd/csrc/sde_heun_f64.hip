@@ -5,7 +5,10 @@ namespace dfx {
 
 int sde_launch_fwd_f64(const SdeArgs& a, hipStream_t s) {
   const int64_t nb = (a.P * (a.d / 2) + kSdeBlock - 1) / kSdeBlock;
-  hipLaunchKernelGGL((sde_heun_fwd_kernel<double, 2, 16>), dim3((unsigned)nb), dim3(kSdeBlock), 0, s, a);
+  if (a.n_ts > 0 || a.save_steps > 0)
+    hipLaunchKernelGGL((sde_heun_fwd_kernel<double, 2, 16, true>), dim3((unsigned)nb), dim3(kSdeBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((sde_heun_fwd_kernel<double, 2, 16, false>), dim3((unsigned)nb), dim3(kSdeBlock), 0, s, a);
   return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -13,7 +16,7 @@ int sde_launch_adj_f64(const SdeArgs& a, int64_t nb, size_t lds, hipStream_t s) 
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, dim3((unsigned)nb), dim3(kSdeBlock), lds, s, a);
   };
-  const bool steps = a.save_steps > 0;
+  const bool steps = a.save_steps > 0 || a.n_ts > 0;
   if (a.CK == 32)
     steps ? go(sde_heun_adj_kernel<double, 32, true>) : go(sde_heun_adj_kernel<double, 32, false>);
   else if (a.CK == 16)

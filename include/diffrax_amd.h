@@ -275,7 +275,9 @@ typedef struct dfx_sde_problem {
   int32_t save_t0, save_t1; /* SaveAt(t0=..., t1=...)                                    */
   int32_t save_steps;       /* SaveAt(steps=k): k > 0 also saves y after every k-th step;
                                the t1 row is dropped when k divides num_steps            */
-  int32_t reserved_;        /* 0 */
+  int32_t n_ts;             /* SaveAt(ts): dfx_sde_io.ts holds n_ts ascending times in
+                               [t0, t1], shared by all paths, interpolated inside their
+                               steps by Heun's Hermite polynomial                        */
 } dfx_sde_problem;
 
 typedef struct dfx_sde_io {
@@ -285,21 +287,25 @@ typedef struct dfx_sde_io {
   const void* sigma;        /* [d]                                                       */
   const float* dW;          /* [N, P, d] float32 Brownian increments (step-major)         */
   const double* grid;       /* [N + 1] step times of ConstantStepSize (host-computed)     */
-  void* ys;                 /* out [P, dfx_sde_n_out, d]: t0, every k-th step, t1         */
+  void* ys;                 /* out [P, dfx_sde_n_out, d]: t0, then the ts and every k-th
+                               step in time order (a step's ts rows before its step row),
+                               then t1                                                    */
   void* ckpt;               /* out [dfx_sde_n_ckpt, P, d] states at steps CK, 2CK, ...
                                (NULL when no gradient is needed)                          */
+  const double* ts;         /* [n_ts] SaveAt(ts) times, or NULL when n_ts == 0            */
 } dfx_sde_io;
 
 typedef struct dfx_sde_grad_io {
-  const void* ys_bar;       /* [P, save_t0 + save_t1, d] cotangent of ys                  */
+  const void* ys_bar;       /* [P, dfx_sde_n_out, d] cotangent of ys                       */
   void* y0_bar;             /* out [P, d] or NULL                                          */
   double* theta_bar;        /* out [d] float64                                             */
   double* mu_bar;           /* out [d]                                                     */
   double* sigma_bar;        /* out [d]                                                     */
 } dfx_sde_grad_io;
 
-/* Saved rows per path: save_t0 + num_steps / save_steps (when save_steps > 0) + save_t1
- * unless save_steps divides num_steps (the allocation rule of _integrate.py:1272-1305). */
+/* Saved rows per path: save_t0 + n_ts + num_steps / save_steps (when save_steps > 0) +
+ * save_t1 unless save_steps divides num_steps (the allocation rule of
+ * _integrate.py:1272-1305). */
 int dfx_sde_n_out(const dfx_sde_problem* p);
 /* Number of stored checkpoints: ceil(N / CK) - 1. */
 int dfx_sde_n_ckpt(const dfx_sde_problem* p);

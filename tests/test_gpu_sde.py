@@ -145,13 +145,20 @@ def test_sde_adjoint_f32_vs_f64(cuda):
         assert rel < 1e-4, name
 
 
-@pytest.mark.parametrize("k", [1, 7, 10])
-def test_sde_step_saves_forward_and_adjoint(cuda, k):
-    """SaveAt(t0=True, steps=k, t1=True) on the SDE path: the rows (t0, the state after every
-    k-th step, t1 unless k divides the step count) equal the FP64 oracle's bit for bit, and
-    the adjoint with cotangents on every row matches torch float64 autograd through the
-    fixed-grid replay to 1e-10."""
+SAVE_TS = np.array([0.0, 0.0132, 0.25, 0.2571, 0.5, 0.66, 0.999, 1.0])
+
+
+@pytest.mark.parametrize("k,with_ts", [(1, False), (7, False), (10, False), (0, True),
+                                       (7, True)])
+def test_sde_step_saves_forward_and_adjoint(cuda, k, with_ts):
+    """SaveAt(t0=True, ts=..., steps=k, t1=True) on the SDE path: the rows (t0; per step the
+    ts inside it, interpolated by Heun's Hermite polynomial, then the state when k divides
+    the step count; t1 unless k divides the step count) equal the FP64 oracle's bit for
+    bit, and the adjoint with cotangents on every row matches torch float64 autograd
+    through the fixed-grid replay to 1e-10."""
     P, d, N = 24, 16, 100
+    ts = SAVE_TS if with_ts else None
+    n_ts = 0 if ts is None else ts.size
     theta, mu, sigma, dW, y0 = _inputs(P=P, d=d, N=N, seed=21)
     dt0 = 1.0 / N
     dev = torch.device("cuda", 0)
@@ -162,12 +169,12 @@ def test_sde_step_saves_forward_and_adjoint(cuda, k):
     terms = dfx.MultiTerm(dfx.ODETerm(dfx.OUDrift()),
                           dfx.ControlTerm(dfx.DiagonalNoise(), dfx.IncrementPath(torch.tensor(dW))))
     sol = dfx.diffeqsolve(terms, dfx.Heun(), 0.0, 1.0, dt0, y0t, (th, m, sg),
-                          saveat=dfx.SaveAt(t0=True, steps=k, t1=True),
+                          saveat=dfx.SaveAt(t0=True, ts=ts, steps=k, t1=True),
                           stepsize_controller=dfx.ConstantStepSize())
-    n_rows = 1 + N // k + (0 if N % k == 0 else 1)
+    n_rows = 1 + n_ts + (N // k if k else 0) + (0 if k and N % k == 0 else 1)
     assert tuple(sol.ys.shape) == (P, n_rows, d) and tuple(sol.ts.shape) == (P, n_rows)
     q = np.concatenate([theta, mu, sigma])
-    ref = oracle.solve(y0, 1.0, q, dW=dW.astype(np.float64), solver="heun", vf="ou",
+    ref = oracle.solve(y0, 1.0, q, ts=ts, dW=dW.astype(np.float64), solver="heun", vf="ou",
                        controller="constant", dt0=dt0, save_t0=True, save_t1=True,
                        save_steps=k, max_steps=N)
     assert np.array_equal(sol.ys.detach().cpu().numpy(), ref["ys"][:, :n_rows])
@@ -181,8 +188,9 @@ def test_sde_step_saves_forward_and_adjoint(cuda, k):
     TH, MU, SG = (torch.tensor(v, dtype=torch.float64, requires_grad=True)
                   for v in (theta, mu, sigma))
     dWt = torch.tensor(dW.astype(np.float64))
-    out = pyoracle.replay(lambda t, y: TH * (MU - y), "heun", steps, Y0, [],
-                          g_dw=lambda n, t, y: SG * dWt[n], save_t1=N % k != 0, save_steps=k)
+    out = pyoracle.replay(lambda t, y: TH * (MU - y), "heun", steps, Y0,
+                          [] if ts is None else list(ts), g_dw=lambda n, t, y: SG * dWt[n],
+                          save_t1=not (k and N % k == 0), save_steps=k)
     rows = torch.stack([Y0] + out, dim=1)
     assert rows.shape[1] == n_rows
     (rows * torch.tensor(w)).sum().backward()
