@@ -304,3 +304,26 @@ def test_per_trajectory_save_times(cuda, matmul):
                            1.0, 0.1, y0, saveat=dfx.SaveAt(ts=np.tile(ts[:1], (B, 1))),
                            stepsize_controller=ctrl)
     assert torch.equal(shared.ys, rows.ys) and torch.equal(shared.ts, rows.ts)
+
+
+def test_mlp_save_fn_and_subs(cuda):
+    """SaveAt(fn=...) and SaveAt(subs=[...]) on the neural-ODE path (the host applies fn to
+    the saved rows and runs one solve per SubSaveAt, as for the small-system kernel)."""
+    W = mlp_weights(14)
+    y0 = torch.tensor(np.random.default_rng(15).normal(size=(33, 64)), dtype=torch.float32,
+                      device=cuda)
+    term = dfx.ODETerm(_vf(W, "f32"))
+    ctrl = dfx.PIDController(rtol=RTOL, atol=ATOL)
+    ts = [0.25, 0.5, 1.0]
+    plain = dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0, saveat=dfx.SaveAt(ts=ts),
+                            stepsize_controller=ctrl)
+    sq = dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0,
+                         saveat=dfx.SaveAt(ts=ts, fn=lambda t, y, args: y.square().sum()),
+                         stepsize_controller=ctrl)
+    assert torch.equal(sq.ys, plain.ys.square().sum(-1))
+    subs = dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0,
+                           saveat=dfx.SaveAt(subs=[dfx.SubSaveAt(ts=ts), dfx.SubSaveAt(t1=True)]),
+                           stepsize_controller=ctrl)
+    t1 = dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y0, saveat=dfx.SaveAt(t1=True),
+                         stepsize_controller=ctrl)
+    assert torch.equal(subs.ys[0], plain.ys) and torch.equal(subs.ys[1], t1.ys)
