@@ -688,9 +688,8 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
                                   "for the small-system kernel only")
     if progress_meter is not None and type(progress_meter).__name__ != "NoProgressMeter":
         raise NotImplementedError("progress meters are not provided by diffrax_amd")
-    if event is not None and (_is_ou_sde(terms) or (isinstance(terms, ODETerm) and
-                                                     isinstance(terms.vector_field, MLPField))):
-        raise NotImplementedError("events are provided for the small-system kernel only")
+    if event is not None and _is_ou_sde(terms):
+        raise NotImplementedError("events are provided for the ODE kernels only")
     if _is_ou_sde(terms):
         from ._sde import solve_sde
         device = _require_gpu()
@@ -709,7 +708,7 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
         from . import _mlp
         return _mlp.diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat,
                                     stepsize_controller, adjoint, max_steps, throw,
-                                    _require_gpu(), launch=launch)
+                                    _require_gpu(), launch=launch, event=event)
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                         max_steps, launch, event=event, solver_state=solver_state,
                         controller_state=controller_state, made_jump=made_jump)

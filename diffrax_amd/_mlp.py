@@ -30,7 +30,8 @@ class MlpProblem(ctypes.Structure):
                 ("dtmax", ctypes.c_double), ("factormin", ctypes.c_double),
                 ("factormax", ctypes.c_double), ("safety", ctypes.c_double),
                 ("error_order", ctypes.c_double), ("waves", ctypes.c_int32),
-                ("save_steps", ctypes.c_int32), ("ts_stride", ctypes.c_int64)]
+                ("save_steps", ctypes.c_int32), ("ts_stride", ctypes.c_int64),
+                ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32)]
 
 
 class MlpAdjIO(ctypes.Structure):
@@ -46,7 +47,7 @@ class MlpBacksolveIO(ctypes.Structure):
 class MlpIO(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
                 ("y0", "ts", "ys", "ts_out", "stats", "result", "grid", "ckpt", "dense_ts",
-                 "dense_ys", "dense_k")]
+                 "dense_ys", "dense_k", "event_coeffs", "event_flags", "event_mask")]
 
 
 _P = ctypes.POINTER
@@ -101,7 +102,7 @@ class MlpPlan:
     """Prepared batched neural-ODE solve (launch() enqueues without host sync)."""
 
     def __init__(self, vfo, t0, t1, dt0, y0, ts, ctrl, sub, max_steps, ckpt_every, device,
-                 dense=False):
+                 dense=False, event=None):
         lib = _lib.load()
         self.vfo = vfo
         B = y0.shape[0]
@@ -144,6 +145,19 @@ class MlpPlan:
         io.stats, io.result = self.stats.data_ptr(), self.result.data_ptr()
         io.grid = self.grid.data_ptr() if self.grid is not None else None
         io.ckpt = self.ckpt.data_ptr() if self.ckpt is not None else None
+        self.event = event
+        self.event_mask = None
+        if event is not None:
+            self.prob.n_events = len(event.conds)
+            self.prob.event_root_find = int(event.root_finder is not None)
+            self.event_t = dict(
+                coeffs=torch.tensor([c.coeffs(64) for c in event.conds], dtype=torch.float64,
+                                    device=device),
+                flags=torch.tensor(event.flags(), dtype=torch.int32, device=device))
+            self.event_mask = torch.zeros((B, len(event.conds)), dtype=torch.int32, device=device)
+            io.event_coeffs = self.event_t["coeffs"].data_ptr()
+            io.event_flags = self.event_t["flags"].data_ptr()
+            io.event_mask = self.event_mask.data_ptr()
         self.dense = None
         if dense:
             M = int(max_steps)
@@ -184,7 +198,7 @@ class _Probe:
 
 
 def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max_steps,
-                device, ckpt_every=0, allocate=True):
+                device, ckpt_every=0, allocate=True, event=None):
     from ._solver import Tsit5
     from ._step_size_controller import PIDController
     if not isinstance(solver, Tsit5):
@@ -248,7 +262,7 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
         probe.sub = sub
         return probe
     plan = MlpPlan(terms.vector_field, t0, t1, dt0, y0b, ts, c, sub, max_steps, ckpt_every, device,
-                   dense=saveat.dense)
+                   dense=saveat.dense, event=event)
     plan.batched = batched
     plan.deferred_checks = check
     return plan
@@ -269,9 +283,16 @@ def _solution(plan, t0, t1, max_steps, throw, ys):
         ys, ts, result = ys[0], ts[0], result[0]
         stats = {k: (v[0] if isinstance(v, torch.Tensor) else v) for k, v in stats.items()}
     if throw:
-        bad = plan.result != 0
+        bad = (plan.result != 0) & (plan.result != int(RESULTS.event_occurred))   # is_okay
         if bool(bad.any()):
             raise RuntimeError(RESULTS[int(plan.result[bad][0].item())])
+    event_mask = None
+    if getattr(plan, "event_mask", None) is not None:
+        m = plan.event_mask.bool()
+        if not plan.batched:
+            m = m[0]
+        event_mask = ([m[..., i] for i in range(m.shape[-1])] if plan.event.is_list
+                      else m[..., 0])
     interp = None
     if getattr(plan, "dense", None) is not None:
         interp = MlpDenseInterpolation(plan)
@@ -279,7 +300,7 @@ def _solution(plan, t0, t1, max_steps, throw, ys):
         if p.n_ts == 0 and not (p.save_t0 or p.save_t1 or p.save_steps):
             ts = ys = None   # SaveAt(dense=True) alone saves no rows
     return Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=result,
-                    interpolation=interp)
+                    interpolation=interp, event_mask=event_mask)
 
 
 def save_y_fn():
@@ -581,14 +602,17 @@ def set_waves(plan, launch):
 
 
 def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, adjoint,
-                    max_steps, throw, device, launch=None):
+                    max_steps, throw, device, launch=None, event=None):
     vfo = terms.vector_field
     y0t = torch.as_tensor(y0)
     needs_grad = torch.is_grad_enabled() and (
         y0t.requires_grad or any(p.requires_grad for p in vfo.params))
+    if event is not None and needs_grad:
+        raise NotImplementedError("gradients through a neural-ODE solve with events are not "
+                                  "provided")
     if not needs_grad:
         plan = prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
-                           max_steps, device)
+                           max_steps, device, event=event)
         set_waves(plan, launch)
         plan.launch()
         return _solution(plan, t0, t1, max_steps, throw, plan.ys)

@@ -158,6 +158,7 @@ int mlp_check(const dfx_mlp_problem* p) {
   if (p->batch < 0 || p->n_ts < 0 || p->max_steps < 0 || p->ckpt_every < 0) return DFX_E_ARG;
   if (p->waves != 0 && p->waves != 4 && p->waves != 8) return DFX_E_ARG;
   if (p->ts_stride != 0 && p->ts_stride != p->n_ts) return DFX_E_ARG;
+  if (p->n_events < 0 || p->n_events > kMaxEvents) return DFX_E_ARG;
   return DFX_OK;
 }
 
@@ -246,8 +247,10 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   if (p->save_steps < 0) return DFX_E_ARG;
   const int dense = (io->dense_ts != nullptr) + (io->dense_ys != nullptr) + (io->dense_k != nullptr);
   if (dense != 0 && dense != 3) return DFX_E_ARG;
-  // the adjoints replay the ts saves only
-  if (p->ckpt_every > 0 && (dense || p->save_steps > 0)) return DFX_E_ARG;
+  // the adjoints replay the ts saves only, without events
+  if (p->ckpt_every > 0 && (dense || p->save_steps > 0 || p->n_events > 0)) return DFX_E_ARG;
+  if (p->n_events > 0 && (!io->event_coeffs || !io->event_flags || !io->event_mask))
+    return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
   const int n_out = dfx_mlp_n_out(p);
   if (!io->y0 || !io->stats || !io->result) return DFX_E_ARG;
@@ -262,6 +265,11 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   a.dense_ts = io->dense_ts; a.dense_ys = io->dense_ys; a.dense_k = io->dense_k;
   a.save_steps = p->save_steps;
   a.ts_stride = p->ts_stride;
+  a.n_events = p->n_events;
+  a.event_root_find = p->event_root_find ? 1 : 0;
+  a.event_coeffs = io->event_coeffs;
+  a.event_flags = io->event_flags;
+  a.event_mask = io->event_mask;
   a.B = p->batch; a.n_ts = p->n_ts; a.n_out = n_out; a.save_t0 = p->save_t0;
   a.save_t1 = p->save_t1; a.max_steps = p->max_steps; a.ckpt_every = p->ckpt_every;
   a.n_ckpt = dfx_mlp_n_ckpt(p);

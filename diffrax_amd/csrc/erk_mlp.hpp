@@ -34,16 +34,22 @@ struct MlpArgs {
   double* dense_ts;  // SaveAt(dense=True): [B, max_steps + 1], [B, max_steps + 1, 64],
   float* dense_ys;   // [B, max_steps, 7, 64] (or null)
   float* dense_k;
+  const double* event_coeffs;  // Event: [n_events, 66] affine conditions (c_t, c_y[64], c_0)
+  const int32_t* event_flags;  // bit 0 boolean, bits 1-2 direction (erk_small.hpp)
+  int32_t* event_mask;         // [B, n_events] out
   int64_t B, ts_stride;  // ts_stride: 0 (ts shared) or n_ts (ts [B, n_ts])
   int32_t n_ts, n_out, save_t0, save_t1, max_steps, ckpt_every, n_ckpt, has_dt0, save_steps;
+  int32_t n_events, event_root_find;
   double t0, t1, dt0, error_order;
   PidParams pid;
 };
 
 // Ctl: PID_INTEGRAL (the integral controller, the default) or PID_GENERIC (any p, i, d
 // coefficients).  NW: waves per tile (4 or 8; MlpWaves); each trajectory has 64 / CPT owner
-// threads of CPT components each.  The results do not depend on NW.
-template <class P, class Tab, bool Grad, int Ctl, int NW>
+// threads of CPT components each.  The results do not depend on NW.  Ev: terminating
+// Events (_event.py:13-118) with affine conditions, as the small-system kernel's (their
+// own build).
+template <class P, class Tab, bool Grad, int Ctl, int NW, bool Ev = false>
 __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, const MlpWeights<P> W) {
   extern __shared__ __align__(16) unsigned char smem[];
   MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
@@ -86,6 +92,42 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
     ++save_index;
   };
   if (valid && a.save_t0) write_save(T0, y);
+  // ---- Event: affine conditions c_t t + c_y . y + c_0, summed in component order (the
+  // owner group passes the partial sum from owner to owner), one fma per component, so
+  // that every wave count rounds them alike ----
+  [[maybe_unused]] double ev_old[kMaxEvents];
+  [[maybe_unused]] int ev_mask = 0;
+  [[maybe_unused]] double tfinal = 0.0;
+  [[maybe_unused]] float yfinal[CPT];
+  auto event_value = [&](int i, double t, const float (&v)[CPT]) {
+    const double* c = a.event_coeffs + i * (MLP_D + 2);
+    double part = c[0] * t;
+    for (int q = 0; q < GRP; ++q) {
+      double mine = part;
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) mine = __builtin_fma(c[1 + c0 + j], (double)v[j], mine);
+      part = __shfl(mine, gbase + q);
+    }
+    return part + c[1 + MLP_D];
+  };
+  if constexpr (Ev) {
+    if (valid) {
+      // values at (t0, y0); a boolean condition that holds ends the solve here
+      // (_integrate.py:1333-1426)
+      for (int i = 0; i < a.n_events; ++i) {
+        const double v = event_value(i, T0, y);
+        const bool boolean = a.event_flags[i] & 1;
+        ev_old[i] = boolean ? (double)(v > 0.0) : v;
+        if (boolean && v > 0.0 && ev_mask == 0) ev_mask = 1 << i;
+      }
+      if (ev_mask) {
+        result = DFX_EVENT_OCCURRED;
+        tfinal = T0;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) yfinal[j] = y[j];
+      }
+    }
+  }
   if (valid && a.dense_ts) {  // the dense output's first knot: (t0, y0)
     const int64_t row = traj * ((int64_t)a.max_steps + 1);
     if (lead) a.dense_ts[row] = T0;
@@ -222,6 +264,68 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
       ++num_steps;
       if (keep) ++num_acc; else ++num_rej;
       if (result == DFX_SUCCESSFUL) result = code;
+      bool event_now = false;
+      if constexpr (Ev) {
+        // Event check at the new state (_integrate.py:542-633); with a root finder the time
+        // is located on this step's interpolant by bisection down to adjacent floats, as
+        // the small-system kernel does (event_root), and saves after it are dropped
+        int mask = 0, fired = -1;
+        for (int i = 0; i < a.n_events; ++i) {
+          float yn[CPT];
+#pragma unroll
+          for (int j = 0; j < CPT; ++j) yn[j] = keep ? z[j] : y[j];
+          const double v = event_value(i, ntprev, yn);
+          const int flags = a.event_flags[i];
+          const bool trig = event_triggered(flags, ev_old[i], v);
+          if (trig && mask == 0) { mask = 1 << i; fired = i; }
+          ev_old[i] = (flags & 1) ? (double)(v > 0.0) : v;
+        }
+        ev_mask = mask;
+        if (mask) {
+          event_now = true;
+          result = DFX_EVENT_OCCURRED;
+          auto interp_at = [&](double t, float (&v)[CPT]) {
+            const double th = linear_rescale(ts0, t, ts1);
+            double bb[7];
+            tsit5_weights(th, bb);
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) {
+              float acc = (float)bb[0] * k[0][j];
+#pragma unroll
+              for (int m = 1; m < 7; ++m) acc = acc + (float)bb[m] * k[m][j];
+              v[j] = y[j] + acc;
+            }
+          };
+          if (a.event_root_find) {
+            double troot = ts1;
+            if (!(a.event_flags[fired] & 1)) {
+              float ya[CPT], yb[CPT], ym[CPT];
+              interp_at(ts0, ya);
+              interp_at(ts1, yb);
+              double ga = event_value(fired, ts0, ya), gb = event_value(fired, ts1, yb);
+              if (event_sign(ga) != event_sign(gb) && ga == ga && gb == gb) {
+                double lo = ts0, hi = ts1;
+                troot = -1.0;
+                for (int it = 0; it < 2100; ++it) {
+                  const double mid = lo + 0.5 * (hi - lo);
+                  if (!(mid > lo && mid < hi)) break;
+                  interp_at(mid, ym);
+                  const double gm = event_value(fired, mid, ym);
+                  if (gm == 0.0) { troot = mid; break; }
+                  if (event_sign(gm) == event_sign(ga)) { lo = mid; ga = gm; } else { hi = mid; gb = gm; }
+                }
+                if (troot < 0.0) troot = (fabs(ga) <= fabs(gb)) ? lo : hi;
+              }
+            }
+            tfinal = troot;
+            interp_at(tfinal, yfinal);
+          } else {
+            tfinal = ntprev;
+#pragma unroll
+            for (int j = 0; j < CPT; ++j) yfinal[j] = keep ? z[j] : y[j];
+          }
+        }
+      }
       if (keep) {
         if constexpr (Grad) {
           const int n = num_acc - 1;
@@ -250,7 +354,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
             for (int j = 0; j < CPT; ++j) dk[m * MLP_D + j] = k[m][j];
           }
         }
-        while (next_ts <= ts1) {
+        while (next_ts <= ts1 && !(Ev && event_now && next_ts > tfinal)) {
           const double th = linear_rescale(ts0, next_ts, ts1);
           double bb[7];
           tsit5_weights(th, bb);
@@ -269,7 +373,8 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
 #pragma unroll
         for (int j = 0; j < CPT; ++j) { y[j] = z[j]; fsal[j] = f[j]; }
         // SaveAt(steps=k): every k-th accepted step (_integrate.py:493-527)
-        if (a.save_steps && num_acc % a.save_steps == 0) write_save(ntprev, y);
+        if (a.save_steps && num_acc % a.save_steps == 0 && !(Ev && event_now && ntprev > tfinal))
+          write_save(ntprev, y);
       }
       tprev = ntprev;
       tnext = ntnext;
@@ -279,14 +384,25 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
   if (T0 == T1) {
     for (int r = 0; r < a.n_ts; ++r) write_save(T0, y);
   }
-  // _save_t1 (_integrate.py:847-877): no t1 row when the last step was saved via steps
+  if (Ev && result == DFX_EVENT_OCCURRED) {  // the solve ends at the event
+    tprev = tfinal;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) y[j] = yfinal[j];
+  }
+  // _save_t1 (_integrate.py:847-877): no t1 row when the last step was saved via steps,
+  // except with an event root finder (that row, after the event time, was removed)
   const bool via_steps = a.save_steps && num_acc % a.save_steps == 0;
-  if (a.save_t1 && !via_steps && save_index < a.n_out) write_save(tprev, y);
+  const bool root = Ev && a.n_events > 0 && a.event_root_find;
+  const bool t1_pred = root ? (a.save_t1 || via_steps) : (a.save_t1 && !via_steps);
+  if (t1_pred && save_index < a.n_out) write_save(tprev, y);
   if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
   for (int s = save_index; s < a.n_out; ++s) {
 #pragma unroll
     for (int j = 0; j < CPT; ++j) ys_row[s * MLP_D + c0 + j] = __builtin_inff();
     if (lead) tso_row[s] = __builtin_inf();  // (dir * inf) * dir: _integrate.py:1296, 1481
+  }
+  if (Ev && lead) {
+    for (int i = 0; i < a.n_events; ++i) a.event_mask[traj * a.n_events + i] = (ev_mask >> i) & 1;
   }
   if (lead) {
     a.stats[traj * 3 + 0] = num_steps;
@@ -307,7 +423,10 @@ void launch_erk_mlp(const MlpArgs& a, const MlpWeights<P>& W, bool grad, bool in
                               (int)sizeof(MlpLds<P>));
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64 * NW), sizeof(MlpLds<P>), s, a, W);
   };
-  if (grad) {
+  if (a.n_events > 0) {  // events: forward only
+    if (integral) go(&erk_mlp_kernel<P, Tsit5Tab, false, PID_INTEGRAL, NW, true>);
+    else go(&erk_mlp_kernel<P, Tsit5Tab, false, PID_GENERIC, NW, true>);
+  } else if (grad) {
     if (integral) go(&erk_mlp_kernel<P, Tsit5Tab, true, PID_INTEGRAL, NW>);
     else go(&erk_mlp_kernel<P, Tsit5Tab, true, PID_GENERIC, NW>);
   } else {

@@ -362,6 +362,9 @@ typedef struct dfx_mlp_problem {
                                (rows in time order with the ts rows; forward only)      */
   int64_t ts_stride;        /* 0: ts[n_ts] shared by the batch; n_ts: ts[B, n_ts] per
                                trajectory (a vmapped SaveAt(ts))                        */
+  int32_t n_events;         /* terminating Events (_event.py:13-56; <= 4 affine
+                               conditions, as for dfx_erk_problem; forward only)         */
+  int32_t event_root_find;  /* 1: locate the event time on the step's interpolant       */
 } dfx_mlp_problem;
 
 typedef struct dfx_mlp_io {
@@ -377,6 +380,9 @@ typedef struct dfx_mlp_io {
   double* dense_ts;         /* out [B, max_steps + 1]: t0, then each accepted step's end  */
   float* dense_ys;          /* out [B, max_steps + 1, 64]: the states at dense_ts         */
   float* dense_k;           /* out [B, max_steps, 7, 64]: each accepted step's k_i        */
+  const double* event_coeffs; /* [n_events, 66]: (c_t, c_y[64], c_0) per condition         */
+  const int32_t* event_flags; /* [n_events]: bit 0 boolean, bits 1-2 direction             */
+  int32_t* event_mask;        /* out [B, n_events]: the condition that ended the solve     */
 } dfx_mlp_io;
 
 /* Bytes of the packed-weight buffer for a precision. */

@@ -329,6 +329,19 @@ def test_mlp_sizes_and_argument_checks():
     steps_io = _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, grid=zz, ckpt=zz)
     assert lib.dfx_mlp_solve(ctypes.byref(_mlp.MlpProblem(ckpt_every=64, save_steps=1, **base)),
                              zz, zz, zz, zz, ctypes.byref(steps_io), None) == -1
+    # events: at most 4 conditions, all three event buffers given, forward only
+    ev_io = _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, event_coeffs=zz,
+                       event_flags=zz, event_mask=zz)
+    no_mask = _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, event_coeffs=zz,
+                         event_flags=zz)
+    for prob, io_ in ((_mlp.MlpProblem(n_events=5, **base), ev_io),
+                      (_mlp.MlpProblem(n_events=-1, **base), ev_io),
+                      (_mlp.MlpProblem(n_events=1, **base), no_mask),
+                      (_mlp.MlpProblem(n_events=1, ckpt_every=64, **base),
+                       _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, grid=zz,
+                                  ckpt=zz, event_coeffs=zz, event_flags=zz, event_mask=zz))):
+        assert lib.dfx_mlp_solve(ctypes.byref(prob), zz, zz, zz, zz, ctypes.byref(io_),
+                                 None) == -1
     dio = _lib.DenseIO(dense_ts=zz, dense_ys=zz, dense_k=zz, stats=zz, tq=zz, n_q=4,
                        tq_stride=2, out=zz)
     assert lib.dfx_mlp_dense_eval(ctypes.byref(_mlp.MlpProblem(**base)), ctypes.byref(dio),
