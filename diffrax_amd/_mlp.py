@@ -31,7 +31,8 @@ class MlpProblem(ctypes.Structure):
                 ("factormax", ctypes.c_double), ("safety", ctypes.c_double),
                 ("error_order", ctypes.c_double), ("waves", ctypes.c_int32),
                 ("save_steps", ctypes.c_int32), ("ts_stride", ctypes.c_int64),
-                ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32)]
+                ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
+                ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32)]
 
 
 class MlpAdjIO(ctypes.Structure):
@@ -47,7 +48,8 @@ class MlpBacksolveIO(ctypes.Structure):
 class MlpIO(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
                 ("y0", "ts", "ys", "ts_out", "stats", "result", "grid", "ckpt", "dense_ts",
-                 "dense_ys", "dense_k", "event_coeffs", "event_flags", "event_mask")]
+                 "dense_ys", "dense_k", "event_coeffs", "event_flags", "event_mask", "step_ts",
+                 "jump_ts")]
 
 
 _P = ctypes.POINTER
@@ -102,7 +104,7 @@ class MlpPlan:
     """Prepared batched neural-ODE solve (launch() enqueues without host sync)."""
 
     def __init__(self, vfo, t0, t1, dt0, y0, ts, ctrl, sub, max_steps, ckpt_every, device,
-                 dense=False, event=None):
+                 dense=False, event=None, clip=None):
         lib = _lib.load()
         self.vfo = vfo
         B = y0.shape[0]
@@ -158,6 +160,13 @@ class MlpPlan:
             io.event_coeffs = self.event_t["coeffs"].data_ptr()
             io.event_flags = self.event_t["flags"].data_ptr()
             io.event_mask = self.event_mask.data_ptr()
+        self.clip_t = {}
+        for name in ("step_ts", "jump_ts"):   # ClipStepSizeController's times, sorted
+            arr = None if clip is None else getattr(clip, name)
+            if arr is not None and arr.size > 0:
+                self.clip_t[name] = torch.tensor(arr, dtype=torch.float64, device=device)
+                setattr(self.prob, "n_" + name, int(arr.size))
+                setattr(io, name, self.clip_t[name].data_ptr())
         self.dense = None
         if dense:
             M = int(max_steps)
@@ -197,12 +206,28 @@ class _Probe:
     pass
 
 
+def _unclip(ctrl):
+    """(ClipStepSizeController or None, the PIDController inside)."""
+    from ._step_size_controller import ClipStepSizeController, PIDController
+    if not isinstance(ctrl, ClipStepSizeController):
+        return None, ctrl
+    if not isinstance(ctrl.controller, PIDController):
+        raise NotImplementedError("ClipStepSizeController is provided around a PIDController only")
+    if ctrl.store_rejected_steps is not None or ctrl.callback_on_reject is not None:
+        raise NotImplementedError("store_rejected_steps is not provided by diffrax_amd")
+    return ctrl, ctrl.controller
+
+
 def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max_steps,
                 device, ckpt_every=0, allocate=True, event=None):
     from ._solver import Tsit5
     from ._step_size_controller import PIDController
     if not isinstance(solver, Tsit5):
         raise NotImplementedError("the neural-ODE kernel implements Tsit5")
+    clip, stepsize_controller = _unclip(stepsize_controller)
+    if clip is not None and ckpt_every > 0:
+        raise NotImplementedError("gradients through ClipStepSizeController are not provided "
+                                  "on the neural-ODE kernel")
     if not isinstance(stepsize_controller, PIDController):
         raise NotImplementedError("the neural-ODE kernel implements PIDController")
     c = stepsize_controller
@@ -262,7 +287,7 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
         probe.sub = sub
         return probe
     plan = MlpPlan(terms.vector_field, t0, t1, dt0, y0b, ts, c, sub, max_steps, ckpt_every, device,
-                   dense=saveat.dense, event=event)
+                   dense=saveat.dense, event=event, clip=clip)
     plan.batched = batched
     plan.deferred_checks = check
     return plan
@@ -610,6 +635,9 @@ def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
     if event is not None and needs_grad:
         raise NotImplementedError("gradients through a neural-ODE solve with events are not "
                                   "provided")
+    if needs_grad and _unclip(stepsize_controller)[0] is not None:
+        raise NotImplementedError("gradients through ClipStepSizeController are not provided "
+                                  "on the neural-ODE kernel")
     if not needs_grad:
         plan = prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
                            max_steps, device, event=event)

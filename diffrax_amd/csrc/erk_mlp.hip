@@ -159,6 +159,7 @@ int mlp_check(const dfx_mlp_problem* p) {
   if (p->waves != 0 && p->waves != 4 && p->waves != 8) return DFX_E_ARG;
   if (p->ts_stride != 0 && p->ts_stride != p->n_ts) return DFX_E_ARG;
   if (p->n_events < 0 || p->n_events > kMaxEvents) return DFX_E_ARG;
+  if (p->n_step_ts < 0 || p->n_jump_ts < 0) return DFX_E_ARG;
   return DFX_OK;
 }
 
@@ -248,7 +249,10 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   const int dense = (io->dense_ts != nullptr) + (io->dense_ys != nullptr) + (io->dense_k != nullptr);
   if (dense != 0 && dense != 3) return DFX_E_ARG;
   // the adjoints replay the ts saves only, without events
-  if (p->ckpt_every > 0 && (dense || p->save_steps > 0 || p->n_events > 0)) return DFX_E_ARG;
+  if (p->ckpt_every > 0 && (dense || p->save_steps > 0 || p->n_events > 0 || p->n_step_ts > 0 ||
+                            p->n_jump_ts > 0))
+    return DFX_E_ARG;
+  if ((p->n_step_ts > 0 && !io->step_ts) || (p->n_jump_ts > 0 && !io->jump_ts)) return DFX_E_ARG;
   if (p->n_events > 0 && (!io->event_coeffs || !io->event_flags || !io->event_mask))
     return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
@@ -270,6 +274,10 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   a.event_coeffs = io->event_coeffs;
   a.event_flags = io->event_flags;
   a.event_mask = io->event_mask;
+  a.n_step_ts = p->n_step_ts;
+  a.n_jump_ts = p->n_jump_ts;
+  a.step_ts = io->step_ts;
+  a.jump_ts = io->jump_ts;
   a.B = p->batch; a.n_ts = p->n_ts; a.n_out = n_out; a.save_t0 = p->save_t0;
   a.save_t1 = p->save_t1; a.max_steps = p->max_steps; a.ckpt_every = p->ckpt_every;
   a.n_ckpt = dfx_mlp_n_ckpt(p);

@@ -37,19 +37,25 @@ struct MlpArgs {
   const double* event_coeffs;  // Event: [n_events, 66] affine conditions (c_t, c_y[64], c_0)
   const int32_t* event_flags;  // bit 0 boolean, bits 1-2 direction (erk_small.hpp)
   int32_t* event_mask;         // [B, n_events] out
+  const double* step_ts;       // ClipStepSizeController: sorted ascending (or null)
+  const double* jump_ts;
   int64_t B, ts_stride;  // ts_stride: 0 (ts shared) or n_ts (ts [B, n_ts])
   int32_t n_ts, n_out, save_t0, save_t1, max_steps, ckpt_every, n_ckpt, has_dt0, save_steps;
-  int32_t n_events, event_root_find;
+  int32_t n_events, event_root_find, n_step_ts, n_jump_ts;
   double t0, t1, dt0, error_order;
   PidParams pid;
 };
 
 // Ctl: PID_INTEGRAL (the integral controller, the default) or PID_GENERIC (any p, i, d
 // coefficients).  NW: waves per tile (4 or 8; MlpWaves); each trajectory has 64 / CPT owner
-// threads of CPT components each.  The results do not depend on NW.  Ev: terminating
-// Events (_event.py:13-118) with affine conditions, as the small-system kernel's (their
-// own build).
-template <class P, class Tab, bool Grad, int Ctl, int NW, bool Ev = false>
+// threads of CPT components each.  The results do not depend on NW.  Ext: the forward-only
+// extras in a build of their own: terminating Events (_event.py:13-118) with affine
+// conditions and ClipStepSizeController's step_ts / jump_ts (clip.py:120-428), as the
+// small-system kernel's.  A jump's made_jump re-evaluates the next step's first stage in
+// the reference (runge_kutta.py:684-690); here the FSAL stage is kept, which is the same
+// value bit for bit: the last stage is evaluated at the new state itself and the field is
+// autonomous.
+template <class P, class Tab, bool Grad, int Ctl, int NW, bool Ext = false>
 __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, const MlpWeights<P> W) {
   extern __shared__ __align__(16) unsigned char smem[];
   MlpLds<P>& L = *reinterpret_cast<MlpLds<P>*>(smem);
@@ -110,7 +116,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
     }
     return part + c[1 + MLP_D];
   };
-  if constexpr (Ev) {
+  if constexpr (Ext) {
     if (valid) {
       // values at (t0, y0); a boolean condition that holds ends the solve here
       // (_integrate.py:1333-1426)
@@ -198,7 +204,18 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
   }
   if (a.pid.has_dtmax) dt0v = jmin(dt0v, a.pid.dtmax);
   if (a.pid.has_dtmin) dt0v = jmax(dt0v, a.pid.dtmin);
-  double tnext = jmin(T0 + dt0v, T1);
+  double tnext = T0 + dt0v;
+  if constexpr (Ext) {  // ClipStepSizeController.init (clip.py:247-302)
+    if (a.n_step_ts > 0) {
+      const ClipTimes c{a.step_ts, a.n_step_ts, dir};
+      tnext = c.clip(tnext, c.find(T0), false);
+    }
+    if (a.n_jump_ts > 0) {
+      const ClipTimes c{a.jump_ts, a.n_jump_ts, dir};
+      tnext = c.clip(tnext, c.find(T0), true);
+    }
+  }
+  tnext = jmin(tnext, T1);
 
   auto running = [&]() {
     return valid && tprev < T1 && result == DFX_SUCCESSFUL && num_steps < a.max_steps;
@@ -257,6 +274,20 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
     int code = 0;
     PidState npid = pid;
     bool keep = pid_adapt<Ctl>(a.pid, scaled, ts0, ts1, npid, ntprev, ntnext, code);
+    if constexpr (Ext) {  // ClipStepSizeController.adapt_step_size (clip.py:304-428)
+      if (a.n_step_ts > 0) {
+        const ClipTimes c{a.step_ts, a.n_step_ts, dir};
+        bool unused;
+        ntnext = c.clip(ntnext, c.find(c.bump(ntprev, unused)), false);
+      }
+      if (a.n_jump_ts > 0) {
+        const ClipTimes c{a.jump_ts, a.n_jump_ts, dir};
+        bool made_jump;
+        ntprev = c.bump(ntprev, made_jump);
+        ntnext = jmax(nextup(ntprev), ntnext);
+        ntnext = c.clip(ntnext, c.find(ntprev), true);
+      }
+    }
     ntprev = jmin(ntprev, T1);
     if (ntnext > floor_t1) ntnext = keep ? T1 : ntprev + 0.5 * (T1 - ntprev);
     if (run) {
@@ -265,7 +296,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
       if (keep) ++num_acc; else ++num_rej;
       if (result == DFX_SUCCESSFUL) result = code;
       bool event_now = false;
-      if constexpr (Ev) {
+      if constexpr (Ext) {
         // Event check at the new state (_integrate.py:542-633); with a root finder the time
         // is located on this step's interpolant by bisection down to adjacent floats, as
         // the small-system kernel does (event_root), and saves after it are dropped
@@ -354,7 +385,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
             for (int j = 0; j < CPT; ++j) dk[m * MLP_D + j] = k[m][j];
           }
         }
-        while (next_ts <= ts1 && !(Ev && event_now && next_ts > tfinal)) {
+        while (next_ts <= ts1 && !(Ext && event_now && next_ts > tfinal)) {
           const double th = linear_rescale(ts0, next_ts, ts1);
           double bb[7];
           tsit5_weights(th, bb);
@@ -373,7 +404,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
 #pragma unroll
         for (int j = 0; j < CPT; ++j) { y[j] = z[j]; fsal[j] = f[j]; }
         // SaveAt(steps=k): every k-th accepted step (_integrate.py:493-527)
-        if (a.save_steps && num_acc % a.save_steps == 0 && !(Ev && event_now && ntprev > tfinal))
+        if (a.save_steps && num_acc % a.save_steps == 0 && !(Ext && event_now && ntprev > tfinal))
           write_save(ntprev, y);
       }
       tprev = ntprev;
@@ -384,7 +415,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
   if (T0 == T1) {
     for (int r = 0; r < a.n_ts; ++r) write_save(T0, y);
   }
-  if (Ev && result == DFX_EVENT_OCCURRED) {  // the solve ends at the event
+  if (Ext && result == DFX_EVENT_OCCURRED) {  // the solve ends at the event
     tprev = tfinal;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) y[j] = yfinal[j];
@@ -392,7 +423,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
   // _save_t1 (_integrate.py:847-877): no t1 row when the last step was saved via steps,
   // except with an event root finder (that row, after the event time, was removed)
   const bool via_steps = a.save_steps && num_acc % a.save_steps == 0;
-  const bool root = Ev && a.n_events > 0 && a.event_root_find;
+  const bool root = Ext && a.n_events > 0 && a.event_root_find;
   const bool t1_pred = root ? (a.save_t1 || via_steps) : (a.save_t1 && !via_steps);
   if (t1_pred && save_index < a.n_out) write_save(tprev, y);
   if (tprev < T1 && result == DFX_SUCCESSFUL) result = DFX_MAX_STEPS_REACHED;
@@ -401,7 +432,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
     for (int j = 0; j < CPT; ++j) ys_row[s * MLP_D + c0 + j] = __builtin_inff();
     if (lead) tso_row[s] = __builtin_inf();  // (dir * inf) * dir: _integrate.py:1296, 1481
   }
-  if (Ev && lead) {
+  if (Ext && lead) {
     for (int i = 0; i < a.n_events; ++i) a.event_mask[traj * a.n_events + i] = (ev_mask >> i) & 1;
   }
   if (lead) {
@@ -423,7 +454,7 @@ void launch_erk_mlp(const MlpArgs& a, const MlpWeights<P>& W, bool grad, bool in
                               (int)sizeof(MlpLds<P>));
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64 * NW), sizeof(MlpLds<P>), s, a, W);
   };
-  if (a.n_events > 0) {  // events: forward only
+  if (a.n_events > 0 || a.n_step_ts > 0 || a.n_jump_ts > 0) {  // the extras: forward only
     if (integral) go(&erk_mlp_kernel<P, Tsit5Tab, false, PID_INTEGRAL, NW, true>);
     else go(&erk_mlp_kernel<P, Tsit5Tab, false, PID_GENERIC, NW, true>);
   } else if (grad) {

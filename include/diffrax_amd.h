@@ -365,6 +365,9 @@ typedef struct dfx_mlp_problem {
   int32_t n_events;         /* terminating Events (_event.py:13-56; <= 4 affine
                                conditions, as for dfx_erk_problem; forward only)         */
   int32_t event_root_find;  /* 1: locate the event time on the step's interpolant       */
+  int32_t n_step_ts;        /* ClipStepSizeController (clip.py:120-428) counts, as for
+                               dfx_erk_problem; forward only                             */
+  int32_t n_jump_ts;
 } dfx_mlp_problem;
 
 typedef struct dfx_mlp_io {
@@ -383,6 +386,8 @@ typedef struct dfx_mlp_io {
   const double* event_coeffs; /* [n_events, 66]: (c_t, c_y[64], c_0) per condition         */
   const int32_t* event_flags; /* [n_events]: bit 0 boolean, bits 1-2 direction             */
   int32_t* event_mask;        /* out [B, n_events]: the condition that ended the solve     */
+  const double* step_ts;      /* [n_step_ts] ascending, shared by the batch, or NULL       */
+  const double* jump_ts;      /* [n_jump_ts] ascending, or NULL                            */
 } dfx_mlp_io;
 
 /* Bytes of the packed-weight buffer for a precision. */

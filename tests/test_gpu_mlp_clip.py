@@ -1,0 +1,110 @@
+"""ClipStepSizeController(PIDController, step_ts, jump_ts) on the neural-ODE kernel
+(clip.py:120-428; PIDController(step_ts=, jump_ts=) builds the same), forward only.
+
+The kernel integrates in float32, so its steps are not the FP64 oracle's (module docstring
+of test_gpu_mlp.py).  Bars:
+- the clipping itself, exactly: every step_ts time inside (t0, t1) is an accepted step's
+  end, and after every jump_ts time j the next step starts at nextup(j) (the saved step
+  time, clip.py:82-101); times outside (t0, t1) change nothing, bit for bit;
+- accuracy at matched tolerance: the rtol-scaled error against a tight FP64 solve at most
+  the FP64 oracle's with the same clipping, plus the kernel's float32 floor (the largest
+  of its errors at rtol 1e-7, 1e-8, 1e-9), plus one unit;
+- four and eight wavefronts, and both ways of building the controller, give the same bits;
+  gradients raise."""
+import numpy as np
+import pytest
+import torch
+
+import diffrax_amd as dfx
+import oracle
+from tests.test_gpu_mlp import ATOL, RTOL, _scaled, mlp_weights
+
+pytestmark = pytest.mark.gpu
+
+B = 70
+STEP_TS = [0.25, 0.5, 0.75]
+JUMP_TS = [0.4]
+
+
+def _setup(matmul, seed=50):
+    W = mlp_weights(seed)
+    y0 = np.random.default_rng(seed + 1).normal(size=(B, 64))
+    term = dfx.ODETerm(dfx.MLPField(*(torch.tensor(w, dtype=torch.float32) for w in W),
+                                    matmul=matmul))
+    return W, y0, term
+
+
+def _ctrl(rtol=RTOL, atol=ATOL, step_ts=STEP_TS, jump_ts=JUMP_TS):
+    return dfx.ClipStepSizeController(dfx.PIDController(rtol=rtol, atol=atol), step_ts=step_ts,
+                                      jump_ts=jump_ts)
+
+
+def _solve(term, y0, saveat, ctrl, launch=None, dt0=0.1):
+    return dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, dt0,
+                           torch.as_tensor(y0, dtype=torch.float32), saveat=saveat,
+                           stepsize_controller=ctrl, max_steps=4096, launch=launch)
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+def test_clipped_steps_land_on_the_times(cuda, matmul):
+    W, y0, term = _setup(matmul)
+    for dt0 in (0.1, None):
+        st = _solve(term, y0, dfx.SaveAt(steps=True), _ctrl(), dt0=dt0)
+        torch.cuda.synchronize()
+        acc = st.stats["num_accepted_steps"].cpu().numpy()
+        kt = st.ts.cpu().numpy()
+        for b in range(B):
+            knots = set(kt[b, :acc[b]].tolist())
+            for s in STEP_TS:
+                assert s in knots, (b, s)
+            for j in JUMP_TS:
+                assert np.nextafter(j, np.inf) in knots and j not in knots, (b, j)
+            assert kt[b, acc[b] - 1] == 1.0
+    # times outside (t0, t1) leave the solve as it was, bit for bit
+    plain = _solve(term, y0, dfx.SaveAt(ts=np.linspace(0, 1, 5)),
+                   dfx.PIDController(rtol=RTOL, atol=ATOL))
+    out = _solve(term, y0, dfx.SaveAt(ts=np.linspace(0, 1, 5)),
+                 _ctrl(step_ts=[-1.0, 2.0], jump_ts=[3.0]))
+    torch.cuda.synchronize()
+    assert torch.equal(plain.ys, out.ys)
+    assert torch.equal(plain.stats["num_steps"], out.stats["num_steps"])
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+def test_clipped_accuracy_matched_tolerance(cuda, matmul):
+    W, y0, term = _setup(matmul, seed=52)
+    ts = np.linspace(0.0, 1.0, 9)
+    sol = _solve(term, y0, dfx.SaveAt(ts=ts), _ctrl())
+    q = oracle.mlp_pack(*W)
+    y0m = y0.astype(np.float32).astype(np.float64)
+    kw = dict(ts=ts, solver="tsit5", vf="mlp", hidden=256, dt0=0.1)
+    true = oracle.solve(y0m, 1.0, q, rtol=1e-12, atol=1e-12, max_steps=100000, **kw)["ys"]
+    ref = oracle.solve(y0m, 1.0, q, rtol=RTOL, atol=ATOL, step_ts=STEP_TS, jump_ts=JUMP_TS, **kw)
+    e_floor = 0.0
+    for rt in (1e-7, 1e-8, 1e-9):
+        fl = _solve(term, y0, dfx.SaveAt(ts=ts), _ctrl(rtol=rt, atol=rt / 100))
+        e_floor = max(e_floor, _scaled(fl.ys.double().cpu().numpy(), true))
+    e_dev = _scaled(sol.ys.double().cpu().numpy(), true)
+    e_ora = _scaled(ref["ys"], true)
+    st = sol.stats["num_steps"].cpu().numpy()
+    print(f"{matmul}: rtol-scaled err {e_dev:.3f} (oracle {e_ora:.3f}, floor {e_floor:.3f}); "
+          f"steps {st.min()}-{st.max()} (oracle {ref['stats'][:, 0].min()}-"
+          f"{ref['stats'][:, 0].max()})")
+    assert int(sol.result.abs().max()) == 0
+    assert e_dev <= e_ora + e_floor + 1.0
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+def test_clip_waves_forms_and_gradients(cuda, matmul):
+    W, y0, term = _setup(matmul, seed=54)
+    ts = np.linspace(0.0, 1.0, 6)
+    a = _solve(term, y0, dfx.SaveAt(ts=ts, t1=True), _ctrl(), launch=4)
+    b = _solve(term, y0, dfx.SaveAt(ts=ts, t1=True), _ctrl(), launch=8)
+    c = _solve(term, y0, dfx.SaveAt(ts=ts, t1=True),
+               dfx.PIDController(rtol=RTOL, atol=ATOL, step_ts=STEP_TS, jump_ts=JUMP_TS))
+    torch.cuda.synchronize()
+    assert torch.equal(a.ys, b.ys) and torch.equal(a.ys, c.ys)
+    assert torch.equal(a.stats["num_steps"], b.stats["num_steps"])
+    y = torch.tensor(y0, dtype=torch.float32).requires_grad_(True)
+    with pytest.raises(NotImplementedError, match="ClipStepSizeController"):
+        _solve(term, y, dfx.SaveAt(t1=True), _ctrl())
