@@ -225,9 +225,6 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
     if not isinstance(solver, Tsit5):
         raise NotImplementedError("the neural-ODE kernel implements Tsit5")
     clip, stepsize_controller = _unclip(stepsize_controller)
-    if clip is not None and ckpt_every > 0:
-        raise NotImplementedError("gradients through ClipStepSizeController are not provided "
-                                  "on the neural-ODE kernel")
     if not isinstance(stepsize_controller, PIDController):
         raise NotImplementedError("the neural-ODE kernel implements PIDController")
     c = stepsize_controller
@@ -443,9 +440,10 @@ class _MlpSolve(torch.autograd.Function):
     def forward(ctx, y0, W1, b1, W2, b2, W3, b3, cfg):
         params = _Params([t.detach() for t in (W1, b1, W2, b2, W3, b3)], cfg["matmul"])
         need = any(ctx.needs_input_grad[:7])
+        clip, ctrl = _unclip(cfg["ctrl"])
         plan = MlpPlan(params, cfg["t0"], cfg["t1"], cfg["dt0"], y0.detach(), cfg["ts"],
-                       cfg["ctrl"], cfg["sub"], cfg["max_steps"],
-                       cfg["ckpt_every"] if need else 0, cfg["device"])
+                       ctrl, cfg["sub"], cfg["max_steps"],
+                       cfg["ckpt_every"] if need else 0, cfg["device"], clip=clip)
         set_waves(plan, cfg.get("launch"))
         plan.launch()
         plan.note_max_accepted()
@@ -635,9 +633,10 @@ def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
     if event is not None and needs_grad:
         raise NotImplementedError("gradients through a neural-ODE solve with events are not "
                                   "provided")
-    if needs_grad and _unclip(stepsize_controller)[0] is not None:
-        raise NotImplementedError("gradients through ClipStepSizeController are not provided "
-                                  "on the neural-ODE kernel")
+    from ._adjoint import BacksolveAdjoint
+    if (needs_grad and isinstance(adjoint, BacksolveAdjoint)
+            and _unclip(stepsize_controller)[0] is not None):
+        raise NotImplementedError("BacksolveAdjoint with ClipStepSizeController is not provided")
     if not needs_grad:
         plan = prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
                            max_steps, device, event=event)

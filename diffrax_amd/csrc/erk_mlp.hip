@@ -249,9 +249,7 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   const int dense = (io->dense_ts != nullptr) + (io->dense_ys != nullptr) + (io->dense_k != nullptr);
   if (dense != 0 && dense != 3) return DFX_E_ARG;
   // the adjoints replay the ts saves only, without events
-  if (p->ckpt_every > 0 && (dense || p->save_steps > 0 || p->n_events > 0 || p->n_step_ts > 0 ||
-                            p->n_jump_ts > 0))
-    return DFX_E_ARG;
+  if (p->ckpt_every > 0 && (dense || p->save_steps > 0 || p->n_events > 0)) return DFX_E_ARG;
   if ((p->n_step_ts > 0 && !io->step_ts) || (p->n_jump_ts > 0 && !io->jump_ts)) return DFX_E_ARG;
   if (p->n_events > 0 && (!io->event_coeffs || !io->event_flags || !io->event_mask))
     return DFX_E_ARG;

@@ -48,10 +48,10 @@ struct MlpArgs {
 
 // Ctl: PID_INTEGRAL (the integral controller, the default) or PID_GENERIC (any p, i, d
 // coefficients).  NW: waves per tile (4 or 8; MlpWaves); each trajectory has 64 / CPT owner
-// threads of CPT components each.  The results do not depend on NW.  Ext: the forward-only
-// extras in a build of their own: terminating Events (_event.py:13-118) with affine
-// conditions and ClipStepSizeController's step_ts / jump_ts (clip.py:120-428), as the
-// small-system kernel's.  A jump's made_jump re-evaluates the next step's first stage in
+// threads of CPT components each.  The results do not depend on NW.  Ext: the extras in a
+// build of their own: terminating Events (_event.py:13-118, forward only) with affine
+// conditions and ClipStepSizeController's step_ts / jump_ts (clip.py:120-428; the
+// gradient build records the clipped grid like any other), as the small-system kernel's.  A jump's made_jump re-evaluates the next step's first stage in
 // the reference (runge_kutta.py:684-690); here the FSAL stage is kept, which is the same
 // value bit for bit: the last stage is evaluated at the new state itself and the field is
 // autonomous.
@@ -454,9 +454,14 @@ void launch_erk_mlp(const MlpArgs& a, const MlpWeights<P>& W, bool grad, bool in
                               (int)sizeof(MlpLds<P>));
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64 * NW), sizeof(MlpLds<P>), s, a, W);
   };
-  if (a.n_events > 0 || a.n_step_ts > 0 || a.n_jump_ts > 0) {  // the extras: forward only
-    if (integral) go(&erk_mlp_kernel<P, Tsit5Tab, false, PID_INTEGRAL, NW, true>);
-    else go(&erk_mlp_kernel<P, Tsit5Tab, false, PID_GENERIC, NW, true>);
+  if (a.n_events > 0 || a.n_step_ts > 0 || a.n_jump_ts > 0) {  // the extras
+    if (grad) {  // step clipping only: events are forward only
+      if (integral) go(&erk_mlp_kernel<P, Tsit5Tab, true, PID_INTEGRAL, NW, true>);
+      else go(&erk_mlp_kernel<P, Tsit5Tab, true, PID_GENERIC, NW, true>);
+    } else {
+      if (integral) go(&erk_mlp_kernel<P, Tsit5Tab, false, PID_INTEGRAL, NW, true>);
+      else go(&erk_mlp_kernel<P, Tsit5Tab, false, PID_GENERIC, NW, true>);
+    }
   } else if (grad) {
     if (integral) go(&erk_mlp_kernel<P, Tsit5Tab, true, PID_INTEGRAL, NW>);
     else go(&erk_mlp_kernel<P, Tsit5Tab, true, PID_GENERIC, NW>);

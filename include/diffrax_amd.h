@@ -366,7 +366,7 @@ typedef struct dfx_mlp_problem {
                                conditions, as for dfx_erk_problem; forward only)         */
   int32_t event_root_find;  /* 1: locate the event time on the step's interpolant       */
   int32_t n_step_ts;        /* ClipStepSizeController (clip.py:120-428) counts, as for
-                               dfx_erk_problem; forward only                             */
+                               dfx_erk_problem (with checkpoints too)                    */
   int32_t n_jump_ts;
 } dfx_mlp_problem;
 

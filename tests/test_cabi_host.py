@@ -342,14 +342,11 @@ def test_mlp_sizes_and_argument_checks():
                                   ckpt=zz, event_coeffs=zz, event_flags=zz, event_mask=zz))):
         assert lib.dfx_mlp_solve(ctypes.byref(prob), zz, zz, zz, zz, ctypes.byref(io_),
                                  None) == -1
-    # step clipping: counts not negative, times given, forward only
+    # step clipping: counts not negative, times given
     clip_io = _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, step_ts=zz,
                          jump_ts=zz)
     for prob, io_ in ((_mlp.MlpProblem(n_step_ts=-1, **base), clip_io),
-                      (_mlp.MlpProblem(n_jump_ts=2, **base), no_mask),
-                      (_mlp.MlpProblem(n_step_ts=2, ckpt_every=64, **base),
-                       _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, grid=zz,
-                                  ckpt=zz, step_ts=zz))):
+                      (_mlp.MlpProblem(n_jump_ts=2, **base), no_mask)):
         assert lib.dfx_mlp_solve(ctypes.byref(prob), zz, zz, zz, zz, ctypes.byref(io_),
                                  None) == -1
     dio = _lib.DenseIO(dense_ts=zz, dense_ys=zz, dense_k=zz, stats=zz, tq=zz, n_q=4,

@@ -1,5 +1,5 @@
 """ClipStepSizeController(PIDController, step_ts, jump_ts) on the neural-ODE kernel
-(clip.py:120-428; PIDController(step_ts=, jump_ts=) builds the same), forward only.
+(clip.py:120-428; PIDController(step_ts=, jump_ts=) builds the same).
 
 The kernel integrates in float32, so its steps are not the FP64 oracle's (module docstring
 of test_gpu_mlp.py).  Bars:
@@ -10,7 +10,9 @@ of test_gpu_mlp.py).  Bars:
   the FP64 oracle's with the same clipping, plus the kernel's float32 floor (the largest
   of its errors at rtol 1e-7, 1e-8, 1e-9), plus one unit;
 - four and eight wavefronts, and both ways of building the controller, give the same bits;
-  gradients raise."""
+- gradients (RecursiveCheckpointAdjoint) against torch float64 autograd through a replay of
+  the same clipped accepted-step grid, 1e-4 relative as for the unclipped solve
+  (test_gpu_mlp_grad.py); BacksolveAdjoint with clipping raises."""
 import numpy as np
 import pytest
 import torch
@@ -107,4 +109,45 @@ def test_clip_waves_forms_and_gradients(cuda, matmul):
     assert torch.equal(a.stats["num_steps"], b.stats["num_steps"])
     y = torch.tensor(y0, dtype=torch.float32).requires_grad_(True)
     with pytest.raises(NotImplementedError, match="ClipStepSizeController"):
-        _solve(term, y, dfx.SaveAt(t1=True), _ctrl())
+        dfx.diffeqsolve(term, dfx.Tsit5(), 0.0, 1.0, 0.1, y, saveat=dfx.SaveAt(t1=True),
+                        stepsize_controller=_ctrl(), adjoint=dfx.BacksolveAdjoint())
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+def test_clipped_gradient_against_fp64_replay(cuda, matmul):
+    from diffrax_amd import _mlp
+    from tests.test_gpu_mlp_grad import _replay_grads, _weights
+    C, max_steps = 2, 32
+    Wn = _weights(5)
+    rng = np.random.default_rng(56)
+    y0 = rng.normal(size=(24, 64)).astype(np.float32)
+    ts = np.linspace(0.0, 1.0, 8)
+    wloss = rng.normal(size=(24, 8, 64)).astype(np.float32)
+    dev = torch.device("cuda", 0)
+    P = [torch.tensor(w, dtype=torch.float32, device=dev, requires_grad=True) for w in Wn]
+    y0t = torch.tensor(y0, device=dev, requires_grad=True)
+    sol = dfx.diffeqsolve(dfx.ODETerm(dfx.MLPField(*P, matmul=matmul)), dfx.Tsit5(), 0.0, 1.0,
+                          0.1, y0t, saveat=dfx.SaveAt(ts=ts), max_steps=max_steps,
+                          stepsize_controller=_ctrl(),
+                          adjoint=dfx.RecursiveCheckpointAdjoint(checkpoints=max_steps // C))
+    (sol.ys * torch.tensor(wloss, device=dev)).sum().backward()
+    # the clipped accepted grid of the same solve
+    plan = _mlp.prepare_mlp(dfx.ODETerm(dfx.MLPField(*[p.detach() for p in P], matmul=matmul)),
+                            dfx.Tsit5(), 0.0, 1.0, 0.1, torch.tensor(y0), dfx.SaveAt(ts=ts),
+                            _ctrl(), max_steps, dev, ckpt_every=C)
+    plan.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(plan.ys, sol.ys.detach())
+    grid = plan.grid.cpu().numpy()
+    nacc = plan.stats[:, 1].cpu().numpy()
+    assert nacc.max() > C
+    for b in range(len(y0)):   # the jump: a step ends just before 0.4, the next starts after
+        ends = set(grid[b, :nacc[b], 1].tolist())
+        starts = set(grid[b, :nacc[b], 0].tolist())
+        assert np.nextafter(0.4, -np.inf) in ends and np.nextafter(0.4, np.inf) in starts
+    ref, _ = _replay_grads(Wn, y0, ts, wloss, grid, nacc)
+    grads = [y0t.grad.double().cpu()] + [p.grad.double().cpu() for p in P]
+    worst = max(float((g - r).abs().max() / r.abs().max()) for g, r in zip(grads, ref))
+    print(f"{matmul}: clipped gradient rel err {worst:.2e}; accepted steps "
+          f"{nacc.min()}-{nacc.max()}")
+    assert worst < 1e-4
