@@ -682,10 +682,9 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
                      controller_state, made_jump, launch):
     resuming = solver_state is not None or controller_state is not None or made_jump is not None
     if (resuming or saveat.solver_state or saveat.controller_state or saveat.made_jump) and (
-            _is_ou_sde(terms) or (isinstance(terms, ODETerm) and
-                                  isinstance(terms.vector_field, MLPField))):
+            _is_ou_sde(terms)):
         raise NotImplementedError("solver/controller state export and resume are provided "
-                                  "for the small-system kernel only")
+                                  "for the ODE kernels only")
     if progress_meter is not None and type(progress_meter).__name__ != "NoProgressMeter":
         raise NotImplementedError("progress meters are not provided by diffrax_amd")
     if event is not None and _is_ou_sde(terms):
@@ -708,7 +707,9 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
         from . import _mlp
         return _mlp.diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat,
                                     stepsize_controller, adjoint, max_steps, throw,
-                                    _require_gpu(), launch=launch, event=event)
+                                    _require_gpu(), launch=launch, event=event,
+                                    solver_state=solver_state,
+                                    controller_state=controller_state, made_jump=made_jump)
     plan = _prepare_erk(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
                         max_steps, launch, event=event, solver_state=solver_state,
                         controller_state=controller_state, made_jump=made_jump)

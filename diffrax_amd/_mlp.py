@@ -32,7 +32,8 @@ class MlpProblem(ctypes.Structure):
                 ("error_order", ctypes.c_double), ("waves", ctypes.c_int32),
                 ("save_steps", ctypes.c_int32), ("ts_stride", ctypes.c_int64),
                 ("n_events", ctypes.c_int32), ("event_root_find", ctypes.c_int32),
-                ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32)]
+                ("n_step_ts", ctypes.c_int32), ("n_jump_ts", ctypes.c_int32),
+                ("resume", ctypes.c_int32)]
 
 
 class MlpAdjIO(ctypes.Structure):
@@ -49,7 +50,7 @@ class MlpIO(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
                 ("y0", "ts", "ys", "ts_out", "stats", "result", "grid", "ckpt", "dense_ts",
                  "dense_ys", "dense_k", "event_coeffs", "event_flags", "event_mask", "step_ts",
-                 "jump_ts")]
+                 "jump_ts", "state_in", "state_out")]
 
 
 _P = ctypes.POINTER
@@ -104,7 +105,7 @@ class MlpPlan:
     """Prepared batched neural-ODE solve (launch() enqueues without host sync)."""
 
     def __init__(self, vfo, t0, t1, dt0, y0, ts, ctrl, sub, max_steps, ckpt_every, device,
-                 dense=False, event=None, clip=None):
+                 dense=False, event=None, clip=None, state=None):
         lib = _lib.load()
         self.vfo = vfo
         B = y0.shape[0]
@@ -167,6 +168,16 @@ class MlpPlan:
                 self.clip_t[name] = torch.tensor(arr, dtype=torch.float64, device=device)
                 setattr(self.prob, "n_" + name, int(arr.size))
                 setattr(io, name, self.clip_t[name].data_ptr())
+        self.state_out = None
+        if state is not None:   # (resume mask, [B, 73] records or None, export wanted)
+            mask, rec, want = state
+            if mask:
+                self.prob.resume = int(mask)
+                self.state_in = rec.to(device=device, dtype=torch.float64).contiguous()
+                io.state_in = self.state_in.data_ptr()
+            if want:
+                self.state_out = torch.empty((B, 64 + 9), dtype=torch.float64, device=device)
+                io.state_out = self.state_out.data_ptr()
         self.dense = None
         if dense:
             M = int(max_steps)
@@ -219,7 +230,7 @@ def _unclip(ctrl):
 
 
 def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max_steps,
-                device, ckpt_every=0, allocate=True, event=None):
+                device, ckpt_every=0, allocate=True, event=None, state=None):
     from ._solver import Tsit5
     from ._step_size_controller import PIDController
     if not isinstance(solver, Tsit5):
@@ -284,7 +295,7 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
         probe.sub = sub
         return probe
     plan = MlpPlan(terms.vector_field, t0, t1, dt0, y0b, ts, c, sub, max_steps, ckpt_every, device,
-                   dense=saveat.dense, event=event, clip=clip)
+                   dense=saveat.dense, event=event, clip=clip, state=state)
     plan.batched = batched
     plan.deferred_checks = check
     return plan
@@ -293,7 +304,25 @@ def prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, max
 _ = (math, np)
 
 
-def _solution(plan, t0, t1, max_steps, throw, ys):
+def _state_outputs(plan, saveat):
+    """Solution.solver_state / controller_state / made_jump from the final state records
+    (_integrate.py:1489-1500; the layout of DFX_MLP_STATE_WORDS), batch leading."""
+    rec = getattr(plan, "state_out", None)
+    if rec is None or saveat is None:
+        return None, None, None
+    d = 64
+    pick = (lambda v: v) if plan.batched else (lambda v: v[0])
+    solver_state = controller_state = made_jump = None
+    if saveat.solver_state:
+        solver_state = (pick(rec[:, d] != 0), pick(rec[:, :d].to(torch.float32)))
+    if saveat.controller_state:
+        controller_state = (pick(rec[:, d + 2]), pick(rec[:, d + 3]), pick(rec[:, d + 4] != 0))
+    if saveat.made_jump:
+        made_jump = pick(rec[:, d + 1] != 0)
+    return solver_state, controller_state, made_jump
+
+
+def _solution(plan, t0, t1, max_steps, throw, ys, saveat=None):
     from ._solution import RESULTS, Solution
     if getattr(plan, "deferred_checks", None) is not None:
         plan.deferred_checks()
@@ -321,8 +350,10 @@ def _solution(plan, t0, t1, max_steps, throw, ys):
         p = plan.prob
         if p.n_ts == 0 and not (p.save_t0 or p.save_t1 or p.save_steps):
             ts = ys = None   # SaveAt(dense=True) alone saves no rows
+    solver_state, controller_state, made_jump = _state_outputs(plan, saveat)
     return Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats, result=result,
-                    interpolation=interp, event_mask=event_mask)
+                    interpolation=interp, event_mask=event_mask, solver_state=solver_state,
+                    controller_state=controller_state, made_jump=made_jump)
 
 
 def save_y_fn():
@@ -625,7 +656,8 @@ def set_waves(plan, launch):
 
 
 def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller, adjoint,
-                    max_steps, throw, device, launch=None, event=None):
+                    max_steps, throw, device, launch=None, event=None, solver_state=None,
+                    controller_state=None, made_jump=None):
     vfo = terms.vector_field
     y0t = torch.as_tensor(y0)
     needs_grad = torch.is_grad_enabled() and (
@@ -637,12 +669,29 @@ def diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
     if (needs_grad and isinstance(adjoint, BacksolveAdjoint)
             and _unclip(stepsize_controller)[0] is not None):
         raise NotImplementedError("BacksolveAdjoint with ClipStepSizeController is not provided")
+    resuming = solver_state is not None or controller_state is not None or made_jump is not None
+    want_state = bool(saveat.solver_state or saveat.controller_state or saveat.made_jump)
+    state = None
+    if resuming or want_state:
+        if needs_grad:
+            raise NotImplementedError("gradients through solver / controller state export or "
+                                      "resume are not provided on the neural-ODE kernel")
+        clip, ctrl = _unclip(stepsize_controller)
+        if resuming and clip is not None:
+            raise NotImplementedError("resuming a ClipStepSizeController solve is not provided")
+        from ._integrate import _state_record
+        B = y0t.shape[0] if y0t.ndim == 2 else 1
+        mask, rec = (0, None)
+        if resuming:
+            mask, rec = _state_record(solver, ctrl, B, 64, y0t.ndim == 2, solver_state,
+                                      controller_state, made_jump)
+        state = (mask, rec, want_state)
     if not needs_grad:
         plan = prepare_mlp(terms, solver, t0, t1, dt0, y0, saveat, stepsize_controller,
-                           max_steps, device, event=event)
+                           max_steps, device, event=event, state=state)
         set_waves(plan, launch)
         plan.launch()
-        return _solution(plan, t0, t1, max_steps, throw, plan.ys)
+        return _solution(plan, t0, t1, max_steps, throw, plan.ys, saveat)
     from ._adjoint import BacksolveAdjoint, RecursiveCheckpointAdjoint
     if not isinstance(adjoint, (RecursiveCheckpointAdjoint, BacksolveAdjoint)):
         raise NotImplementedError("gradients are provided through RecursiveCheckpointAdjoint "

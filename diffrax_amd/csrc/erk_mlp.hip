@@ -160,6 +160,8 @@ int mlp_check(const dfx_mlp_problem* p) {
   if (p->ts_stride != 0 && p->ts_stride != p->n_ts) return DFX_E_ARG;
   if (p->n_events < 0 || p->n_events > kMaxEvents) return DFX_E_ARG;
   if (p->n_step_ts < 0 || p->n_jump_ts < 0) return DFX_E_ARG;
+  if (p->resume & ~(DFX_RESUME_SOLVER | DFX_RESUME_CONTROLLER | DFX_RESUME_MADE_JUMP))
+    return DFX_E_ARG;
   return DFX_OK;
 }
 
@@ -251,6 +253,10 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   // the adjoints replay the ts saves only, without events
   if (p->ckpt_every > 0 && (dense || p->save_steps > 0 || p->n_events > 0)) return DFX_E_ARG;
   if ((p->n_step_ts > 0 && !io->step_ts) || (p->n_jump_ts > 0 && !io->jump_ts)) return DFX_E_ARG;
+  // resume: state_in given; forward only; the clipped controller state is not carried
+  if (p->resume && (!io->state_in || p->ckpt_every > 0 || p->n_step_ts > 0 || p->n_jump_ts > 0))
+    return DFX_E_ARG;
+  if (io->state_out && p->ckpt_every > 0) return DFX_E_ARG;
   if (p->n_events > 0 && (!io->event_coeffs || !io->event_flags || !io->event_mask))
     return DFX_E_ARG;
   if (p->batch == 0) return DFX_OK;
@@ -276,6 +282,9 @@ int dfx_mlp_solve(const dfx_mlp_problem* p, const void* packed, const float* b1,
   a.n_jump_ts = p->n_jump_ts;
   a.step_ts = io->step_ts;
   a.jump_ts = io->jump_ts;
+  a.resume = p->resume;
+  a.state_in = io->state_in;
+  a.state_out = io->state_out;
   a.B = p->batch; a.n_ts = p->n_ts; a.n_out = n_out; a.save_t0 = p->save_t0;
   a.save_t1 = p->save_t1; a.max_steps = p->max_steps; a.ckpt_every = p->ckpt_every;
   a.n_ckpt = dfx_mlp_n_ckpt(p);

@@ -22,6 +22,8 @@
 
 namespace dfx {
 
+constexpr int MLP_STATE_WORDS = DFX_ERK_STATE_WORDS(MLP_D);
+
 struct MlpArgs {
   const float* y0;
   const double* ts;
@@ -39,9 +41,11 @@ struct MlpArgs {
   int32_t* event_mask;         // [B, n_events] out
   const double* step_ts;       // ClipStepSizeController: sorted ascending (or null)
   const double* jump_ts;
+  const double* state_in;      // [B, MLP_STATE_WORDS] solver / controller state to resume
+  double* state_out;           // [B, MLP_STATE_WORDS] out (or null)
   int64_t B, ts_stride;  // ts_stride: 0 (ts shared) or n_ts (ts [B, n_ts])
   int32_t n_ts, n_out, save_t0, save_t1, max_steps, ckpt_every, n_ckpt, has_dt0, save_steps;
-  int32_t n_events, event_root_find, n_step_ts, n_jump_ts;
+  int32_t n_events, event_root_find, n_step_ts, n_jump_ts, resume;
   double t0, t1, dt0, error_order;
   PidParams pid;
 };
@@ -50,8 +54,9 @@ struct MlpArgs {
 // coefficients).  NW: waves per tile (4 or 8; MlpWaves); each trajectory has 64 / CPT owner
 // threads of CPT components each.  The results do not depend on NW.  Ext: the extras in a
 // build of their own: terminating Events (_event.py:13-118, forward only) with affine
-// conditions and ClipStepSizeController's step_ts / jump_ts (clip.py:120-428; the
-// gradient build records the clipped grid like any other), as the small-system kernel's.  A jump's made_jump re-evaluates the next step's first stage in
+// conditions, ClipStepSizeController's step_ts / jump_ts (clip.py:120-428; the gradient
+// build records the clipped grid like any other) and the solver / controller state export
+// and resume (forward only), as the small-system kernel's.  A jump's made_jump re-evaluates the next step's first stage in
 // the reference (runge_kutta.py:684-690); here the FSAL stage is kept, which is the same
 // value bit for bit: the last stage is evaluated at the new state itself and the field is
 // autonomous.
@@ -216,6 +221,30 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
     }
   }
   tnext = jmin(tnext, T1);
+  // Resumed solver / controller state (_integrate.py:1252-1270; the record layout of
+  // DFX_ERK_STATE_WORDS).  The FSAL stage is the one given unless the first stage is
+  // evaluated afresh (first_step or made_jump, runge_kutta.py:684-690), which the seed
+  // evaluation above already did.
+  [[maybe_unused]] bool first = true, made_jump = false;
+  if constexpr (Ext) {
+    if (valid && a.resume) {
+      const double* r = a.state_in + traj * MLP_STATE_WORDS;
+      if (a.resume & DFX_RESUME_MADE_JUMP) made_jump = r[MLP_D + 1] != 0.0;
+      if (a.resume & DFX_RESUME_SOLVER) {
+        first = r[MLP_D] != 0.0;
+        if (!first && !made_jump) {
+#pragma unroll
+          for (int j = 0; j < CPT; ++j) fsal[j] = (float)r[c0 + j];
+        }
+      }
+      if (a.resume & DFX_RESUME_CONTROLLER) {
+        pid.inv = r[MLP_D + 2];
+        pid.pinv = r[MLP_D + 3];
+        pid.at_dtmin = r[MLP_D + 4] != 0.0;
+        if (a.has_dt0) tnext = jmin(T0 + a.dt0 * dir, T1);  // the controller's init skipped
+      }
+    }
+  }
 
   auto running = [&]() {
     return valid && tprev < T1 && result == DFX_SUCCESSFUL && num_steps < a.max_steps;
@@ -274,6 +303,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
     int code = 0;
     PidState npid = pid;
     bool keep = pid_adapt<Ctl>(a.pid, scaled, ts0, ts1, npid, ntprev, ntnext, code);
+    [[maybe_unused]] bool mj = false;  // made_jump of this step: only a jump time sets it
     if constexpr (Ext) {  // ClipStepSizeController.adapt_step_size (clip.py:304-428)
       if (a.n_step_ts > 0) {
         const ClipTimes c{a.step_ts, a.n_step_ts, dir};
@@ -282,8 +312,7 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
       }
       if (a.n_jump_ts > 0) {
         const ClipTimes c{a.jump_ts, a.n_jump_ts, dir};
-        bool made_jump;
-        ntprev = c.bump(ntprev, made_jump);
+        ntprev = c.bump(ntprev, mj);
         ntnext = jmax(nextup(ntprev), ntnext);
         ntnext = c.clip(ntnext, c.find(ntprev), true);
       }
@@ -403,6 +432,10 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
         }
 #pragma unroll
         for (int j = 0; j < CPT; ++j) { y[j] = z[j]; fsal[j] = f[j]; }
+        if constexpr (Ext) {  // an accepted step takes the step's made_jump (_integrate.py:425)
+          first = false;
+          made_jump = mj;
+        }
         // SaveAt(steps=k): every k-th accepted step (_integrate.py:493-527)
         if (a.save_steps && num_acc % a.save_steps == 0 && !(Ext && event_now && ntprev > tfinal))
           write_save(ntprev, y);
@@ -412,6 +445,24 @@ __global__ void __launch_bounds__(64 * NW, 1) erk_mlp_kernel(const MlpArgs a, co
     }
   }
   if (!valid) return;
+  if constexpr (Ext) {
+    if (a.state_out) {  // the final states (_integrate.py:1489-1500), as erk_small.hpp writes them
+      double* r = a.state_out + traj * MLP_STATE_WORDS;
+      const bool resumed_f0 = (a.resume & DFX_RESUME_SOLVER) != 0;
+#pragma unroll
+      for (int j = 0; j < CPT; ++j)
+        r[c0 + j] = !first ? (double)fsal[j]
+                           : (resumed_f0 ? a.state_in[traj * MLP_STATE_WORDS + c0 + j] : 0.0);
+      if (lead) {
+        r[MLP_D] = first ? 1.0 : 0.0;
+        r[MLP_D + 1] = made_jump ? 1.0 : 0.0;
+        r[MLP_D + 2] = pid.inv;
+        r[MLP_D + 3] = pid.pinv;
+        r[MLP_D + 4] = pid.at_dtmin ? 1.0 : 0.0;
+        for (int w = MLP_D + 5; w < MLP_STATE_WORDS; ++w) r[w] = 0.0;  // ConstantStepSize's
+      }
+    }
+  }
   if (T0 == T1) {
     for (int r = 0; r < a.n_ts; ++r) write_save(T0, y);
   }
@@ -454,7 +505,7 @@ void launch_erk_mlp(const MlpArgs& a, const MlpWeights<P>& W, bool grad, bool in
                               (int)sizeof(MlpLds<P>));
     hipLaunchKernelGGL(kern, dim3(nblocks), dim3(64 * NW), sizeof(MlpLds<P>), s, a, W);
   };
-  if (a.n_events > 0 || a.n_step_ts > 0 || a.n_jump_ts > 0) {  // the extras
+  if (a.n_events > 0 || a.n_step_ts > 0 || a.n_jump_ts > 0 || a.resume || a.state_out) {
     if (grad) {  // step clipping only: events are forward only
       if (integral) go(&erk_mlp_kernel<P, Tsit5Tab, true, PID_INTEGRAL, NW, true>);
       else go(&erk_mlp_kernel<P, Tsit5Tab, true, PID_GENERIC, NW, true>);

@@ -368,6 +368,8 @@ typedef struct dfx_mlp_problem {
   int32_t n_step_ts;        /* ClipStepSizeController (clip.py:120-428) counts, as for
                                dfx_erk_problem (with checkpoints too)                    */
   int32_t n_jump_ts;
+  int32_t resume;           /* DFX_RESUME_* mask of dfx_mlp_io.state_in, as
+                               dfx_erk_problem.resume (forward only, not with clipping)  */
 } dfx_mlp_problem;
 
 typedef struct dfx_mlp_io {
@@ -388,7 +390,13 @@ typedef struct dfx_mlp_io {
   int32_t* event_mask;        /* out [B, n_events]: the condition that ended the solve     */
   const double* step_ts;      /* [n_step_ts] ascending, shared by the batch, or NULL       */
   const double* jump_ts;      /* [n_jump_ts] ascending, or NULL                            */
+  const double* state_in;     /* [B, DFX_MLP_STATE_WORDS] when problem.resume != 0         */
+  double* state_out;          /* optional out [B, DFX_MLP_STATE_WORDS]: the final states   */
 } dfx_mlp_io;
+
+/* The neural-ODE solver / controller state record: the DFX_ERK_STATE_WORDS layout at
+ * d = 64 (the ConstantStepSize words are written as zero). */
+#define DFX_MLP_STATE_WORDS DFX_ERK_STATE_WORDS(64)
 
 /* Bytes of the packed-weight buffer for a precision. */
 int dfx_mlp_pack_bytes(int32_t precision, size_t* bytes);

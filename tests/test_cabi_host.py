@@ -115,7 +115,7 @@ def test_resume_host_validation():
         _solve(stepsize_controller=dfx.ClipStepSizeController(
             dfx.PIDController(rtol=1e-6, atol=1e-8), step_ts=[0.5]),
             saveat=dfx.SaveAt(t1=True, controller_state=True))
-    with pytest.raises(NotImplementedError, match="small-system kernel only"):
+    with pytest.raises(NotImplementedError, match="ODE kernels only"):
         dfx.diffeqsolve(dfx.MultiTerm(dfx.ODETerm(dfx.OUDrift()), dfx.ControlTerm(
             dfx.DiagonalNoise(), dfx.IncrementPath(torch.zeros(1, 4, 2)))), dfx.Heun(), 0.0,
             1.0, 0.25, torch.zeros(1, 2), (torch.ones(2), torch.zeros(2), torch.ones(2)),
@@ -342,11 +342,24 @@ def test_mlp_sizes_and_argument_checks():
                                   ckpt=zz, event_coeffs=zz, event_flags=zz, event_mask=zz))):
         assert lib.dfx_mlp_solve(ctypes.byref(prob), zz, zz, zz, zz, ctypes.byref(io_),
                                  None) == -1
-    # step clipping: counts not negative, times given
+    # step clipping (counts not negative, times given) and resume
     clip_io = _mlp.MlpIO(y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, step_ts=zz,
                          jump_ts=zz)
     for prob, io_ in ((_mlp.MlpProblem(n_step_ts=-1, **base), clip_io),
-                      (_mlp.MlpProblem(n_jump_ts=2, **base), no_mask)):
+                      (_mlp.MlpProblem(n_jump_ts=2, **base), no_mask),
+                      # resume: known bits, state_in given, not with checkpoints or clipping
+                      (_mlp.MlpProblem(resume=8, **base), _mlp.MlpIO(
+                          y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, state_in=zz)),
+                      (_mlp.MlpProblem(resume=2, **base), no_mask),
+                      (_mlp.MlpProblem(resume=2, n_step_ts=1, **base), _mlp.MlpIO(
+                          y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, state_in=zz,
+                          step_ts=zz)),
+                      (_mlp.MlpProblem(resume=1, ckpt_every=64, **base), _mlp.MlpIO(
+                          y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, grid=zz,
+                          ckpt=zz, state_in=zz)),
+                      (_mlp.MlpProblem(ckpt_every=64, **base), _mlp.MlpIO(
+                          y0=zz, ts=zz, ys=zz, ts_out=zz, stats=zz, result=zz, grid=zz,
+                          ckpt=zz, state_out=zz))):
         assert lib.dfx_mlp_solve(ctypes.byref(prob), zz, zz, zz, zz, ctypes.byref(io_),
                                  None) == -1
     dio = _lib.DenseIO(dense_ts=zz, dense_ys=zz, dense_k=zz, stats=zz, tq=zz, n_q=4,
