@@ -692,17 +692,19 @@ def _diffeqsolve_one(terms, solver, t0, t1, dt0, y0, args, *, saveat, stepsize_c
     if _is_ou_sde(terms):
         from ._sde import solve_sde
         device = _require_gpu()
-        ts, ys, n = solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat,
-                              stepsize_controller, adjoint, device)
+        ts, ys, n, interp = solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat,
+                                      stepsize_controller, adjoint, device)
         if n > max_steps:
             if throw:
                 raise RuntimeError(RESULTS[RESULTS.max_steps_reached])
-        shape = ys.shape[:-2]
-        full = lambda v: torch.full(shape, v, dtype=torch.int32, device=ys.device)
+        shape = interp.run.ys.shape[:1] if ys is None else ys.shape[:-2]
+        if ys is None and not interp.batched:
+            shape = ()
+        full = lambda v: torch.full(shape, v, dtype=torch.int32, device=device)
         stats = {"num_steps": full(n), "num_accepted_steps": full(n),
                  "num_rejected_steps": full(0), "max_steps": max_steps}
         return Solution(t0=t0, t1=t1, ts=ts, ys=ys, stats=stats,
-                        result=full(0))
+                        result=full(0), interpolation=interp)
     if isinstance(terms, ODETerm) and isinstance(terms.vector_field, MLPField):
         from . import _mlp
         return _mlp.diffeqsolve_mlp(terms, solver, t0, t1, dt0, y0, saveat,

@@ -22,7 +22,15 @@ class SdeProblem(ctypes.Structure):
 
 class SdeIO(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in
-                ("y0", "theta", "mu", "sigma", "dW", "grid", "ys", "ckpt", "ts")]
+                ("y0", "theta", "mu", "sigma", "dW", "grid", "ys", "ckpt", "ts", "dense_ys",
+                 "dense_k")]
+
+
+class SdeDenseIO(ctypes.Structure):
+    _fields_ = [("dense_ys", ctypes.c_void_p), ("dense_k", ctypes.c_void_p),
+                ("grid", ctypes.c_void_p), ("tq", ctypes.c_void_p), ("n_q", ctypes.c_int64),
+                ("tq_stride", ctypes.c_int64), ("left", ctypes.c_int32),
+                ("derivative", ctypes.c_int32), ("out", ctypes.c_void_p)]
 
 
 class SdeGradIO(ctypes.Structure):
@@ -42,6 +50,8 @@ _SIGS = {
                               ctypes.c_void_p], ctypes.c_int),
     "dfx_sde_heun_adjoint_static": ([ctypes.POINTER(SdeProblem), ctypes.POINTER(SdeIO),
                                      ctypes.POINTER(SdeGradIO), ctypes.c_void_p], ctypes.c_int),
+    "dfx_sde_dense_eval": ([ctypes.POINTER(SdeProblem), ctypes.POINTER(SdeDenseIO),
+                            ctypes.c_void_p], ctypes.c_int),
 }
 # DFX_SDE_STATIC_SCRATCH_BYTES: the static partial-sum array of dfx_sde_heun_adjoint_static
 SDE_STATIC_SCRATCH_BYTES = 8 << 20
@@ -83,7 +93,7 @@ def _ck_for(num_steps, checkpoints):
 
 class _Sde:
     def __init__(self, y0, theta, mu, sigma, dW, grid, save_t0, save_t1, need_grad, checkpoints,
-                 save_steps=0, ts=None):
+                 save_steps=0, ts=None, dense=False):
         lib = _lib.load()
         self.P, self.d = y0.shape
         self.N = grid.numel() - 1
@@ -106,6 +116,13 @@ class _Sde:
         io.ckpt = self.ckpt.data_ptr() if self.ckpt is not None else None
         self.ts = ts
         io.ts = ts.data_ptr() if ts is not None and ts.numel() else None
+        self.dense = None
+        if dense:   # SaveAt(dense=True): step-end states and stage increments
+            self.dense = dict(
+                dense_ys=torch.empty((self.P, self.N + 1, self.d), dtype=y0.dtype, device=y0.device),
+                dense_k=torch.empty((self.P, self.N, 2, self.d), dtype=y0.dtype, device=y0.device))
+            io.dense_ys = self.dense["dense_ys"].data_ptr()
+            io.dense_k = self.dense["dense_k"].data_ptr()
         self.io = io
 
     def forward(self):
@@ -171,8 +188,8 @@ def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
     if not isinstance(stepsize_controller, ConstantStepSize):
         raise NotImplementedError("the SDE path implements ConstantStepSize only")
     sub = saveat.subs
-    if sub is None or saveat.dense:
-        raise NotImplementedError("the SDE path saves SaveAt(t0=..., t1=..., ts=..., steps=...)")
+    if sub is None and not saveat.dense:
+        raise ValueError("SaveAt must save something")
     if dt0 is None:
         raise ValueError("Constant step size solvers cannot select step size automatically; "
                          "please pass a value for `dt0`.")
@@ -201,6 +218,12 @@ def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
     dW = dW.to(device).contiguous()
     grid = torch.tensor(grid_np, dtype=torch.float64, device=device)
     ckpts = adjoint.checkpoints if hasattr(adjoint, "checkpoints") else None
+    sub_none = sub is None
+    if sub_none:   # dense output only: no saved rows
+        from ._saveat import SubSaveAt
+        sub = SubSaveAt.__new__(SubSaveAt)
+        sub.t0 = sub.t1 = False
+        sub.ts, sub.steps = None, 0
     k = int(sub.steps)
     ts_np = None
     ts_dev = None
@@ -211,8 +234,21 @@ def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
         if ts_np.size and (ts_np[0] < t0f or ts_np[-1] > t1f):
             raise RuntimeError("saveat.ts must lie between t0 and t1.")
         ts_dev = torch.tensor(ts_np, dtype=torch.float64, device=device)
-    ys = _SdeFunction.apply(y0b, theta, mu, sigma, dW, grid, bool(sub.t0), bool(sub.t1), ckpts,
-                            k, ts_dev)
+    if saveat.dense:
+        if torch.is_grad_enabled() and (y0b.requires_grad or any(
+                a.requires_grad for a in (theta, mu, sigma))):
+            raise NotImplementedError("gradients through SaveAt(dense=True) are not provided "
+                                      "on the SDE kernel")
+        run = _Sde(y0b.detach(), theta.detach(), mu.detach(), sigma.detach(), dW, grid,
+                   bool(sub.t0) if sub else False, bool(sub.t1) if sub else False, False, None,
+                   k, ts_dev, dense=True)
+        run.forward()
+        ys = run.ys
+        interp = SdeDenseInterpolation(run, batched)
+    else:
+        ys = _SdeFunction.apply(y0b, theta, mu, sigma, dW, grid, bool(sub.t0), bool(sub.t1),
+                                ckpts, k, ts_dev)
+        interp = None
     # rows: t0; per step, the ts inside it and then its end when k divides the step count;
     # t1 unless the last step was saved (the kernels' order)
     ts_list = [t0f] if sub.t0 else []
@@ -229,4 +265,53 @@ def solve_sde(terms, solver, t0, t1, dt0, y0, args, saveat, stepsize_controller,
     ts = torch.tensor(ts_list, dtype=torch.float64, device=device).expand(P, len(ts_list))
     if not batched:
         ys, ts = ys[0], ts[0]
-    return ts, ys, N
+    if sub_none:
+        ts = ys = None   # SaveAt(dense=True) alone saves no rows
+    return ts, ys, N, interp
+
+
+class SdeDenseInterpolation:
+    """Dense output of an SDE SaveAt(dense=True) solve (DenseInterpolation,
+    _global_interpolation.py:315-400, with Heun's Hermite polynomial), evaluated on the GPU
+    by dfx_sde_dense_eval.  ``evaluate(t0, t1=None, left=True)`` and
+    ``derivative(t, left=True)`` take a scalar time, [n] times shared by the paths or
+    [P, n] per-path times."""
+
+    def __init__(self, run, batched):
+        self.run, self.batched = run, batched
+
+    def evaluate(self, t0, t1=None, left: bool = True):
+        if t1 is not None:
+            return self.evaluate(t1, left=left) - self.evaluate(t0, left=left)
+        return self._query(t0, False, left)
+
+    def derivative(self, t, left: bool = True):
+        return self._query(t, True, left)
+
+    def _query(self, t, derivative, left):
+        run = self.run
+        dev = run.ys.device
+        tq = torch.as_tensor(t, dtype=torch.float64)
+        scalar = tq.ndim == 0
+        if scalar:
+            tq = tq.reshape(1)
+        if tq.ndim == 1:
+            n_q, stride = tq.shape[0], 0
+        elif tq.ndim == 2 and tq.shape[0] == run.P:
+            n_q, stride = tq.shape[1], tq.shape[1]
+        else:
+            raise ValueError(f"evaluation times must be a scalar, [n] or [{run.P}, n]")
+        tq = tq.to(dev).contiguous()
+        out = torch.empty((run.P, n_q, run.d), dtype=run.ys.dtype, device=dev)
+        io = SdeDenseIO(dense_ys=run.dense["dense_ys"].data_ptr(),
+                        dense_k=run.dense["dense_k"].data_ptr(), grid=run.keep[5].data_ptr(),
+                        tq=tq.data_ptr(), n_q=n_q, tq_stride=stride, left=int(bool(left)),
+                        derivative=int(bool(derivative)), out=out.data_ptr())
+        _lib.check(_lib.load().dfx_sde_dense_eval(
+            ctypes.byref(run.prob), ctypes.byref(io),
+            ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)), "dfx_sde_dense_eval")
+        if scalar:
+            out = out[:, 0]
+        if not self.batched:
+            out = out[0]
+        return out

@@ -117,6 +117,8 @@ int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* str
   if (dfx_sde_n_out(p) > 0 && !io->ys) return DFX_E_ARG;
   if (p->n_ts > 0 && !io->ts) return DFX_E_ARG;
   if (io->ckpt && p->checkpoint_every <= 0) return DFX_E_ARG;
+  if ((io->dense_ys == nullptr) != (io->dense_k == nullptr)) return DFX_E_ARG;
+  if (io->dense_ys && (p->checkpoint_every > 0 || io->ckpt || p->num_steps < 1)) return DFX_E_ARG;
   if (p->paths == 0) return DFX_OK;
   SdeArgs a;
   memset(&a, 0, sizeof(a));
@@ -126,8 +128,28 @@ int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* str
   a.n_out = dfx_sde_n_out(p);
   a.save_t0 = p->save_t0; a.save_t1 = p->save_t1; a.save_steps = p->save_steps;
   a.n_ts = p->n_ts; a.ts = io->ts;
+  a.dense_ys = io->dense_ys; a.dense_k = io->dense_k;
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   if ((p->precision == DFX_F32 ? sde_launch_fwd_f32(a, s) : sde_launch_fwd_f64(a, s)) != 0)
+    return DFX_E_HIP;
+  return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
+}
+
+int dfx_sde_dense_eval(const dfx_sde_problem* p, const dfx_sde_dense_io* io, void* stream) {
+  int rc = sde_check(p);
+  if (rc != DFX_OK) return rc;
+  if (!io || !io->dense_ys || !io->dense_k || !io->grid || p->num_steps < 1 || io->n_q < 0)
+    return DFX_E_ARG;
+  if (io->tq_stride != 0 && io->tq_stride < io->n_q) return DFX_E_ARG;
+  if (p->paths == 0 || io->n_q == 0) return DFX_OK;
+  if (!io->tq || !io->out) return DFX_E_ARG;
+  SdeDenseArgs a;
+  a.dense_ys = io->dense_ys; a.dense_k = io->dense_k; a.grid = io->grid; a.tq = io->tq;
+  a.out = io->out; a.P = p->paths; a.n_q = io->n_q; a.tq_stride = io->tq_stride;
+  a.d = p->d; a.N = p->num_steps; a.left = io->left ? 1 : 0;
+  a.derivative = io->derivative ? 1 : 0;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  if ((p->precision == DFX_F32 ? sde_launch_dense_f32(a, s) : sde_launch_dense_f64(a, s)) != 0)
     return DFX_E_HIP;
   return hipGetLastError() == hipSuccess ? DFX_OK : DFX_E_HIP;
 }

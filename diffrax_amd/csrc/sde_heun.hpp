@@ -51,6 +51,19 @@ struct SdeArgs {
   int32_t save_steps;     // SaveAt(steps=k): y after every k-th step
   int32_t n_ts;           // SaveAt(ts): ascending times in [t0, t1], shared by the paths
   const double* ts;       // [n_ts]
+  void* dense_ys;         // SaveAt(dense=True): [P, N + 1, d] step-end states (or null)
+  void* dense_k;          // [P, N, 2, d] each step's two stage increments
+};
+
+// Query side of the SDE dense output: one thread per (path, query, component).
+struct SdeDenseArgs {
+  const void* dense_ys;
+  const void* dense_k;
+  const double* grid;     // [N + 1]
+  const double* tq;       // [P * tq_stride + n_q]
+  void* out;              // [P, n_q, d]
+  int64_t P, n_q, tq_stride;
+  int32_t d, N, left, derivative;
 };
 
 // Saved rows in time order: t0; then per step n the ts inside it (t_n < ts <= t_{n+1}, or
@@ -142,6 +155,14 @@ __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
 #pragma unroll
     for (int v = 0; v < V; ++v) ys[v] = y[v];
   }
+  T* dys = SAVES && a.dense_ys ? reinterpret_cast<T*>(a.dense_ys) + p * (int64_t)(a.N + 1) * a.d + j0
+                               : nullptr;
+  T* dks = SAVES && a.dense_ys ? reinterpret_cast<T*>(a.dense_k) + p * (int64_t)a.N * 2 * a.d + j0
+                               : nullptr;
+  if (dys) {  // the dense output's first knot (_integrate.py:529-540)
+#pragma unroll
+    for (int v = 0; v < V; ++v) dys[v] = y[v];
+  }
   const int64_t slab = a.P * a.d;  // elements per step
   const float* dwp = a.dW + p * a.d + j0;
   T* ck = reinterpret_cast<T*>(a.ckpt);
@@ -172,6 +193,12 @@ __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
             ++save_row;
             ++ts_idx;
           }
+          if (dys) {  // the step's end state and stage increments
+            T* oy = dys + (int64_t)(n + 1) * a.d;
+            T* ok = dks + (int64_t)n * 2 * a.d;
+#pragma unroll
+            for (int v = 0; v < V; ++v) { oy[v] = y1[v]; ok[v] = k0[v]; ok[a.d + v] = k1[v]; }
+          }
 #pragma unroll
           for (int v = 0; v < V; ++v) y[v] = y1[v];
         } else {
@@ -200,6 +227,44 @@ __global__ void __launch_bounds__(256) sde_heun_fwd_kernel(const SdeArgs a) {
     T* o = ys + (int64_t)sde_t1_row(a) * a.d;
 #pragma unroll
     for (int v = 0; v < V; ++v) o[v] = y[v];
+  }
+}
+
+// SaveAt(dense=True) evaluation (DenseInterpolation.evaluate / derivative,
+// _global_interpolation.py:315-400): the step holding t on the shared grid (left: a time
+// on a boundary belongs to the earlier step), then the step's Hermite polynomial as the
+// in-kernel saves evaluate it, or its time derivative.  Outside [t0, t1]: NaN.
+template <class T>
+__global__ void __launch_bounds__(256) sde_dense_eval_kernel(const SdeDenseArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.P * a.n_q * a.d) return;
+  const int j = (int)(i % a.d);
+  const int64_t pq = i / a.d;
+  const int64_t q = pq % a.n_q, p = pq / a.n_q;
+  const double t = a.tq[p * a.tq_stride + q];
+  const double* g = a.grid;
+  T* o = reinterpret_cast<T*>(a.out) + i;
+  if (!(t >= g[0] && t <= g[a.N])) {
+    *o = T(__builtin_nan(""));
+    return;
+  }
+  int lo = 0, hi = a.N - 1;  // first step whose end is at or after t (left) / after t
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const bool after = a.left ? (g[mid + 1] >= t) : (g[mid + 1] > t);
+    if (after) hi = mid; else lo = mid + 1;
+  }
+  const int n = lo;
+  const T* ys = reinterpret_cast<const T*>(a.dense_ys) + (p * (int64_t)(a.N + 1) + n) * a.d + j;
+  const T* ks = reinterpret_cast<const T*>(a.dense_k) + (p * (int64_t)a.N + n) * 2 * a.d + j;
+  const double th = linear_rescale(g[n], t, g[n + 1]);
+  if (!a.derivative) {
+    *o = heun_interp<T>(ys[0], ys[a.d], ks[0], ks[a.d], th);
+  } else {
+    const T y0 = ys[0], y1 = ys[a.d], k0 = ks[0], k1 = ks[a.d], tt = T(th);
+    const T ca = k0 + k1 + T(2) * y0 - T(2) * y1;
+    const T cb = T(-2) * k0 - k1 - T(3) * y0 + T(3) * y1;
+    *o = ((T(3) * ca * tt + T(2) * cb) * tt + k0) / T(g[n + 1] - g[n]);
   }
 }
 
@@ -367,6 +432,8 @@ int sde_launch_fwd_f32(const SdeArgs& a, hipStream_t s);
 int sde_launch_fwd_f64(const SdeArgs& a, hipStream_t s);
 int sde_launch_adj_f32(const SdeArgs& a, int64_t blocks, size_t lds, hipStream_t s);
 int sde_launch_adj_f64(const SdeArgs& a, int64_t blocks, size_t lds, hipStream_t s);
+int sde_launch_dense_f32(const SdeDenseArgs& a, hipStream_t s);
+int sde_launch_dense_f64(const SdeDenseArgs& a, hipStream_t s);
 constexpr int kSdeBlock = 256;
 
 }  // namespace dfx

@@ -293,7 +293,23 @@ typedef struct dfx_sde_io {
   void* ckpt;               /* out [dfx_sde_n_ckpt, P, d] states at steps CK, 2CK, ...
                                (NULL when no gradient is needed)                          */
   const double* ts;         /* [n_ts] SaveAt(ts) times, or NULL when n_ts == 0            */
+  void* dense_ys;           /* SaveAt(dense=True), both or neither, forward only (no
+                               checkpoints): out [P, N + 1, d] the step-end states         */
+  void* dense_k;            /* out [P, N, 2, d] each step's two stage increments          */
 } dfx_sde_io;
+
+/* Query side of the SDE's SaveAt(dense=True): buffers dfx_sde_heun_solve wrote. */
+typedef struct dfx_sde_dense_io {
+  const void* dense_ys;     /* [P, N + 1, d] from dfx_sde_heun_solve                       */
+  const void* dense_k;      /* [P, N, 2, d]                                                */
+  const double* grid;       /* [N + 1] as given to dfx_sde_heun_solve                      */
+  const double* tq;         /* query times [P * tq_stride + n_q]                           */
+  int64_t n_q;              /* queries per path                                            */
+  int64_t tq_stride;        /* 0: the same n_q times for every path; n_q: per path         */
+  int32_t left;             /* 1: a time on a step boundary belongs to the earlier step    */
+  int32_t derivative;       /* 0: y(t); 1: dy/dt                                           */
+  void* out;                /* out [P, n_q, d] state dtype                                 */
+} dfx_sde_dense_io;
 
 typedef struct dfx_sde_grad_io {
   const void* ys_bar;       /* [P, dfx_sde_n_out, d] cotangent of ys                       */
@@ -316,6 +332,11 @@ int dfx_sde_workspace_bytes(const dfx_sde_problem* p, size_t* bytes);
  * _solver/runge_kutta.py:847-1203), MultiTerm(ODETerm, ControlTerm).vf_prod
  * (_term.py:711-722, 420-427) and ConstantStepSize (_step_size_controller/constant.py:30-104). */
 int dfx_sde_heun_solve(const dfx_sde_problem* p, const dfx_sde_io* io, void* stream);
+/* Evaluate the dense solution (or its time derivative) of every path at its query times:
+ * Heun's Hermite polynomial in the step holding t, NaN outside [t0, t1].
+ * Replaces: DenseInterpolation.evaluate / derivative (_global_interpolation.py:315-400)
+ * with ThirdOrderHermitePolynomialInterpolation (_local_interpolation.py:50-139). */
+int dfx_sde_dense_eval(const dfx_sde_problem* p, const dfx_sde_dense_io* io, void* stream);
 /* Discrete adjoint of dfx_sde_heun_solve: from ys_bar, the cotangents of y0, theta, mu,
  * sigma, replaying each CK-step segment from its checkpoint.
  * Replaces: reverse-mode through RecursiveCheckpointAdjoint.loop (_adjoint.py:264-312). */

@@ -283,6 +283,16 @@ def test_sde_sizes_and_argument_checks():
                             checkpoint_every=p.checkpoint_every, save_t0=0, save_t1=1)
         setattr(q, field, bad)
         assert lib.dfx_sde_heun_solve(ctypes.byref(q), ctypes.byref(io), None) == -1
+    # dense output: both buffers, forward only (no checkpoints)
+    z = ctypes.c_void_p(8)
+    full = dict(y0=z, theta=z, mu=z, sigma=z, dW=z, grid=z, ys=z)
+    q = _sde.SdeProblem(precision=_lib.F32, d=16, paths=1000, num_steps=1000, save_t1=1)
+    assert lib.dfx_sde_heun_solve(ctypes.byref(q), ctypes.byref(_sde.SdeIO(dense_ys=z, **full)),
+                                  None) == -1
+    assert lib.dfx_sde_heun_solve(ctypes.byref(p), ctypes.byref(
+        _sde.SdeIO(dense_ys=z, dense_k=z, ckpt=z, **full)), None) == -1
+    dio = _sde.SdeDenseIO(dense_ys=z, dense_k=z, grid=z, tq=z, n_q=4, tq_stride=2, out=z)
+    assert lib.dfx_sde_dense_eval(ctypes.byref(q), ctypes.byref(dio), None) == -1
     p.checkpoint_every = 12
     assert lib.dfx_sde_heun_adjoint(ctypes.byref(p), ctypes.byref(io), ctypes.byref(g), None, 0,
                                     None) == -1

@@ -199,3 +199,49 @@ def test_sde_step_saves_forward_and_adjoint(cuda, k, with_ts):
         got, want = got.detach().cpu().numpy(), want.numpy()
         rel = np.max(np.abs(got - want)) / np.max(np.abs(want))
         assert rel < 1e-10, (name, rel)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_sde_dense_output(cuda, dtype):
+    """SaveAt(dense=True) on the SDE path (dfx_sde_dense_eval): evaluated at the SaveAt(ts)
+    times it returns the saved rows bit for bit (the same Hermite arithmetic, which the
+    FP64 oracle pins in test_sde_step_saves_forward_and_adjoint), shared or per path; at
+    t0 exactly y0; at the step ends the SaveAt(steps=True) states from either side, to the
+    rounding of the polynomial's sum at theta = 0 or 1 (as the reference's polyval); outside
+    [t0, t1] NaN.  Its derivative matches central differences of the evaluation inside a
+    step (1e-6 relative, float64).  SaveAt(dense=True) alone saves no rows; gradients
+    through a dense solve raise."""
+    P, d, N = 24, 16, 100
+    theta, mu, sigma, dW, y0 = _inputs(P=P, d=d, N=N, seed=31)
+    dev = torch.device("cuda", 0)
+    args = tuple(torch.tensor(v, dtype=dtype, device=dev) for v in (theta, mu, sigma))
+    terms = dfx.MultiTerm(dfx.ODETerm(dfx.OUDrift()),
+                          dfx.ControlTerm(dfx.DiagonalNoise(), dfx.IncrementPath(torch.tensor(dW))))
+    y0t = torch.tensor(y0, dtype=dtype, device=dev)
+    solve = lambda saveat: dfx.diffeqsolve(terms, dfx.Heun(), 0.0, 1.0, 1.0 / N, y0t, args,
+                                           saveat=saveat,
+                                           stepsize_controller=dfx.ConstantStepSize())
+    sol = solve(dfx.SaveAt(ts=SAVE_TS, dense=True))
+    steps = solve(dfx.SaveAt(steps=True))
+    assert torch.equal(sol.evaluate(torch.tensor(SAVE_TS)), sol.ys)
+    grid = steps.ts[0].cpu().numpy()
+    tol = 1e-12 if dtype == torch.float64 else 2e-5
+    for left, rows in ((True, slice(None)), (False, slice(0, -1))):
+        got = sol.evaluate(torch.tensor(grid[rows]), left=left)
+        assert float((got - steps.ys[:, rows]).abs().max()) <= tol
+    assert torch.equal(sol.evaluate(0.0), y0t)
+    assert torch.isnan(sol.evaluate(1.5)).all() and torch.isnan(sol.evaluate(-0.1)).all()
+    per = torch.tensor(np.tile(SAVE_TS[None, 1:3], (P, 1)))
+    assert torch.equal(sol.evaluate(per), sol.ys[:, 1:3])
+    only = solve(dfx.SaveAt(dense=True))
+    assert only.ys is None and torch.equal(only.evaluate(torch.tensor(SAVE_TS)), sol.ys)
+    if dtype == torch.float64:
+        t, h = 0.4537, 1e-6
+        fd = (sol.evaluate(t + h) - sol.evaluate(t - h)) / (2 * h)
+        der = sol.derivative(t)
+        assert float((der - fd).abs().max()) <= 1e-6 * float(der.abs().max())
+    with pytest.raises(NotImplementedError, match="dense"):
+        g = y0t.clone().requires_grad_(True)
+        dfx.diffeqsolve(terms, dfx.Heun(), 0.0, 1.0, 1.0 / N, g, args,
+                        saveat=dfx.SaveAt(t1=True, dense=True),
+                        stepsize_controller=dfx.ConstantStepSize())
