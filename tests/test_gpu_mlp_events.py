@@ -185,3 +185,27 @@ def test_boolean_condition(cuda):
     with pytest.raises(NotImplementedError, match="events"):
         _solve(term, torch.tensor(y0, dtype=torch.float32).requires_grad_(True),
                dfx.SaveAt(t1=True), dfx.Event(_cond(E0)))
+
+
+@pytest.mark.parametrize("matmul", ["f32", "bf16"])
+def test_event_in_reverse_time(cuda, matmul):
+    """t0 = 1 > t1 = 0: conditions see direction-normalised time, as the small-system kernel
+    and the oracle; the root-found event time within 1e-6 of a tight FP64 oracle solve's."""
+    W, y0, term = _setup(matmul, seed=70)
+    cond = (-1.0, [0.1] + [0.0] * 63, -0.5)   # -t_norm - 0.5 + 0.1 y0 = t - 0.5 + 0.1 y0
+    sol = dfx.diffeqsolve(term, dfx.Tsit5(), 1.0, 0.0, -0.1,
+                          torch.tensor(y0, dtype=torch.float32), saveat=dfx.SaveAt(t1=True),
+                          stepsize_controller=dfx.PIDController(rtol=RTOL, atol=ATOL),
+                          event=dfx.Event(_cond(cond), dfx.Newton()), max_steps=64)
+    torch.cuda.synchronize()
+    assert (sol.result.cpu().numpy() == int(dfx.RESULTS.event_occurred)).all()
+    q = oracle.mlp_pack(*W)
+    y0m = y0.astype(np.float32).astype(np.float64)
+    true = oracle.solve(y0m, 0.0, q, t0=1.0, solver="tsit5", vf="mlp", hidden=256, dt0=-0.1,
+                        save_t1=True, rtol=1e-12, atol=1e-12, max_steps=100000,
+                        events=[(*cond, False, None)], event_root_find=True)
+    assert true["event_mask"].all()
+    t_dev = sol.ts[:, -1].cpu().numpy()
+    err = np.max(np.abs(t_dev - true["ts"][:, -1]))
+    print(f"{matmul}: reverse-time event time error {err:.2e}")
+    assert np.all((t_dev > 0.0) & (t_dev < 1.0)) and err < 1e-6
