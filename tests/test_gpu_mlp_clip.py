@@ -151,3 +151,23 @@ def test_clipped_gradient_against_fp64_replay(cuda, matmul):
     print(f"{matmul}: clipped gradient rel err {worst:.2e}; accepted steps "
           f"{nacc.min()}-{nacc.max()}")
     assert worst < 1e-4
+
+
+def test_clipped_steps_in_reverse_time(cuda):
+    """t0 = 1 > t1 = 0: the clipping works on the direction-scaled times (clip.py:231-241),
+    so step_ts are still step ends and after a jump the next step starts just past it in
+    the direction of integration, at nextafter(j, -inf)."""
+    W, y0, term = _setup("f32", seed=58)
+    st = dfx.diffeqsolve(term, dfx.Tsit5(), 1.0, 0.0, -0.1,
+                         torch.as_tensor(y0, dtype=torch.float32), saveat=dfx.SaveAt(steps=True),
+                         stepsize_controller=_ctrl(), max_steps=4096)
+    torch.cuda.synchronize()
+    acc = st.stats["num_accepted_steps"].cpu().numpy()
+    kt = st.ts.cpu().numpy()
+    for b in range(B):
+        knots = set(kt[b, :acc[b]].tolist())
+        for s in STEP_TS:
+            assert s in knots, (b, s)
+        for j in JUMP_TS:
+            assert np.nextafter(j, -np.inf) in knots and j not in knots, (b, j)
+        assert kt[b, acc[b] - 1] == 0.0
