@@ -209,3 +209,27 @@ def test_event_in_reverse_time(cuda, matmul):
     err = np.max(np.abs(t_dev - true["ts"][:, -1]))
     print(f"{matmul}: reverse-time event time error {err:.2e}")
     assert np.all((t_dev > 0.0) & (t_dev < 1.0)) and err < 1e-6
+
+
+def test_event_with_step_saves(cuda):
+    """SaveAt(steps=True, t1=True) with an event: without a root finder the rows are the
+    knots up to and including the crossing step's, with no separate t1 row (the last step
+    was saved, _integrate.py:847-877); with one the crossing step's knot, after the event
+    time, is dropped and the t1 row records the event state (_integrate.py:781-806)."""
+    W, y0, term = _setup("f32", seed=80)
+    y0f = y0.astype(np.float32)
+    steps = _solve(term, y0, dfx.SaveAt(steps=True, t1=True))
+    plain = _solve(term, y0, dfx.SaveAt(steps=True, t1=True), dfx.Event(_cond(E0)))
+    root = _solve(term, y0, dfx.SaveAt(steps=True, t1=True), dfx.Event(_cond(E0), dfx.Newton()))
+    one = _solve(term, y0, dfx.SaveAt(t1=True), dfx.Event(_cond(E0), dfx.Newton()))
+    torch.cuda.synchronize()
+    acc = steps.stats["num_accepted_steps"].cpu().numpy()
+    kt, ky = steps.ts.cpu().numpy(), steps.ys.cpu().numpy()
+    for b in range(B):
+        k = _first_crossing(E0, y0f[b], kt[b, :acc[b]], ky[b, :acc[b]])
+        assert torch.equal(plain.ts[b, :k + 1], steps.ts[b, :k + 1])
+        assert torch.equal(plain.ys[b, :k + 1], steps.ys[b, :k + 1])
+        assert torch.isinf(plain.ts[b, k + 1:]).all()
+        assert torch.equal(root.ts[b, :k], steps.ts[b, :k])
+        assert torch.equal(root.ts[b, k], one.ts[b, 0]) and torch.equal(root.ys[b, k], one.ys[b, 0])
+        assert torch.isinf(root.ts[b, k + 1:]).all()
